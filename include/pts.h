@@ -1,0 +1,205 @@
+/* C ABI of the MI355X-native heads-up DCFR postflop solver (libpts.so).
+ *
+ * Two layers:
+ *  - pts_engine_*: the drop-in for the reference's GPU backend. It takes the
+ *    caller's game tree, hand lists and regret / strategy-sum storage and runs
+ *    DCFR iterations, best responses and subtree evaluations in HIP kernels on
+ *    an MI355X (gfx950). It replaces crates/solver/src/gpu/{mod,plan,engine}.rs
+ *    (`Gpu`, `Plan::build_with`, `Engine`) and the CUDA kernels in
+ *    crates/solver/cuda/.
+ *  - pts_solver_*: the solver crate's public API (crates/solver/src/lib.rs:
+ *    `Solver::{estimate,new,solve,query,ranges_after}`) over that engine, for
+ *    callers without a tree builder of their own.
+ * All functions return a pts_status; pts_last_error() describes the last
+ * failure on the calling thread. No function aborts the process.
+ */
+#ifndef PTS_H
+#define PTS_H
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* crates/solver/src/gpu/mod.rs:86-136 (GpuError), as status codes */
+typedef enum {
+  PTS_OK = 0,
+  PTS_DRIVER_MISSING = 1,
+  PTS_DRIVER_TOO_OLD = 2,
+  PTS_NO_DEVICE = 3,
+  PTS_DRIVER = 4,
+  PTS_INVALID_INPUT = 5,
+  PTS_OUT_OF_MEMORY = 6,
+  PTS_CANCELLED = 7,
+} pts_status;
+
+/* crates/solver/src/cfr.rs:248-256 */
+typedef enum { PTS_MODE_CFR = 0, PTS_MODE_BEST_RESPONSE = 1, PTS_MODE_AVERAGE = 2 } pts_mode;
+
+/* crates/solver/src/lib.rs:57-71 */
+typedef enum { PTS_BACKEND_CPU = 0, PTS_BACKEND_GPU = 1, PTS_BACKEND_AUTO = 2 } pts_backend;
+
+typedef struct pts_engine pts_engine;
+typedef struct pts_solver pts_solver;
+
+/* sizeof of the ABI structs, for binders to check their layouts:
+ * 0 pts_config_c, 1 pts_tree_desc, 2 pts_progress_c, 3 pts_step_c, 4 pts_stats_c. */
+size_t pts_abi_sizeof(int what);
+/* Text of the last error on this thread; returns its full length. */
+size_t pts_last_error(char* buf, size_t len);
+/* Number of usable GPUs (crates/solver/src/gpu/mod.rs:242-282 opens device 0). */
+pts_status pts_device_count(int* out);
+/* Whether `backend` puts a solve rooted on `street` (1 flop, 2 turn, 3 river)
+ * on the GPU (crates/solver/src/lib.rs:497-503). */
+int pts_backend_wants_gpu(int backend, int street);
+
+/* ------------------------------------------------------------------ engine */
+
+/* The caller's game tree (the arena of crates/solver/src/tree.rs:154-245),
+ * hand lists (crates/solver/src/hands.rs:14-30) and strength-table boards. */
+typedef struct {
+  uint64_t num_nodes;
+  const uint8_t* kind;          /* 0/1 decision of player 0/1, 2 chance, 3/4 fold by player 0/1, 5 showdown */
+  const uint8_t* street;        /* 1 flop, 2 turn, 3 river */
+  const uint16_t* num_children;
+  const uint32_t* first_child;  /* children are contiguous */
+  const int32_t* invest;        /* 2 per node: chips each player put in since the root */
+  const uint64_t* offset;       /* decision: storage offset; showdown: board index */
+  const float* rake;            /* terminals */
+  const float* chance_factor;   /* chance nodes */
+  const int8_t* card;           /* chance children: dealt card; else -1 */
+  uint32_t num_boards;
+  const int32_t* boards;        /* 5 cards per showdown board */
+  uint32_t num_hands[2];
+  const uint8_t* hand_cards[2]; /* 2 per hand, higher card index first */
+  const float* hand_weights[2];
+  const uint32_t* hand_same[2]; /* index of the identical combo in the other list, or 0xFFFFFFFF */
+  double half_pot;              /* root pot / 2 */
+  double num_combinations;      /* total weight of compatible hand pairs */
+  uint64_t storage_len;         /* regret / strategy-sum cells */
+  int32_t batch_river;          /* river subtrees use the batched evaluators' rounding (the reference default) */
+  int32_t _pad;
+} pts_tree_desc;
+
+/* Gpu::new + Plan::build_with + Engine::new (gpu/mod.rs:244, plan.rs:270,
+ * engine.rs:292): validates and flattens the tree, uploads it and the storage. */
+pts_status pts_engine_create(int device, const pts_tree_desc* tree, const float* regrets, const float* sums,
+                             pts_engine** out);
+/* Engine::cfr_iteration (engine.rs:386-392): player 0's update, then player 1's,
+ * with the discounts of cfr.rs:83-107 computed by the caller. */
+pts_status pts_engine_iterate(pts_engine* e, float pos, float neg, float strat, int reach_weighted);
+/* One traversal of `player` under `mode` (CFR mode updates storage); the
+ * player's root values (FP64 for best response / average). */
+pts_status pts_engine_traverse(pts_engine* e, int player, int mode, float pos, float neg, float strat, int reach_weighted,
+                               double* out, size_t n);
+/* Engine::root_values (engine.rs:406): expected payoffs in chips, traversals in FP64. */
+pts_status pts_engine_root_values(pts_engine* e, int mode, double out[2]);
+/* Engine::exploitability (engine.rs:417): FP64 best responses of both players. */
+pts_status pts_engine_exploitability(pts_engine* e, int raked, double* out);
+/* Engine::download (engine.rs:347): regrets and strategy sums to the host. */
+pts_status pts_engine_download(pts_engine* e, float* regrets, float* sums, uint64_t len);
+pts_status pts_engine_upload(pts_engine* e, const float* regrets, const float* sums, uint64_t len);
+/* Average-strategy values of the children of decision node `node` (player
+ * `player`) given the opponent's reach there (the traversals of
+ * Solver::query, lib.rs:854-863): out[action * n_player + hand]. */
+pts_status pts_engine_subtree_values(pts_engine* e, uint32_t node, int player, const float* opp_reach, size_t nq, float* out,
+                                     size_t len);
+/* Bytes of HBM the engine holds. */
+uint64_t pts_engine_device_bytes(pts_engine* e);
+void pts_engine_destroy(pts_engine* e);
+
+/* ------------------------------------------------------------------ solver */
+
+typedef struct {
+  int32_t kind[8]; /* 0 pot fraction, 1 multiple of the current bet, 2 all-in (crates/holdem/src/menu.rs:14-24) */
+  double value[8];
+  int32_t count;
+  int32_t _pad;
+} pts_sizes_c;
+
+typedef struct { /* crates/holdem/src/menu.rs:100-119 */
+  pts_sizes_c bet[3]; /* flop, turn, river */
+  pts_sizes_c raise[3];
+  uint32_t max_raises;
+  uint32_t _pad;
+  double add_allin_threshold, force_allin_threshold, merge_threshold;
+} pts_menu_c;
+
+typedef struct { /* crates/solver/src/lib.rs:73-105 (exact chance, f32 storage) */
+  int32_t board[5];
+  int32_t board_len;
+  float ranges[2][1326];
+  int32_t street; /* root betting state (crates/solver/src/tree.rs:19-34) */
+  int32_t to_act;
+  int64_t pot, commit[2], stack[2], current_bet, last_raise_inc;
+  uint32_t num_raises;
+  int32_t needs[2];
+  int32_t has_future_menu;
+  pts_menu_c menu, future_menu;
+  int64_t big_blind;
+  double rake_rate, rake_cap;
+  uint64_t max_memory_bytes;
+  float prune;        /* must be 0 */
+  int32_t storage_mode; /* must be 0 (f32) */
+  int32_t isomorphism;  /* solved without suit isomorphism; rejected if it would apply */
+  int32_t beta_some;
+  double alpha, beta, gamma; /* crates/solver/src/cfr.rs:47-63 */
+  int32_t restart_average, unweighted_average;
+  int32_t backend;
+  int32_t _pad;
+} pts_config_c;
+
+typedef struct { uint64_t player_nodes, chance_nodes, terminal_nodes, storage_cells, memory_bytes; } pts_stats_c;
+
+typedef struct { /* crates/solver/src/lib.rs:174-184 */
+  uint32_t iteration, max_iterations;
+  double exploitability, exploitability_pct;
+  int32_t finished, _pad;
+} pts_progress_c;
+
+typedef struct { /* crates/solver/src/lib.rs:212-217: kind 0 action (act_kind 0 fold, 1 check, 2 call, 3 raise to `to`), 1 card */
+  int32_t kind, act_kind;
+  int64_t to;
+  int32_t card, _pad;
+} pts_step_c;
+
+/* Solver::estimate (lib.rs:355-362) */
+pts_status pts_solver_estimate(const pts_config_c* cfg, pts_stats_c* out);
+/* Solver::new (lib.rs:365-396) plus the engine: the tree is built on the
+ * host and lives in HBM with its storage. */
+pts_status pts_solver_new(const pts_config_c* cfg, int device, pts_solver** out);
+void pts_solver_free(pts_solver* s);
+void pts_solver_stats(pts_solver* s, pts_stats_c* out);
+/* Solver::solve (lib.rs:475-642). `history` (may be null) receives the
+ * progress of every exploitability check. */
+pts_status pts_solver_solve(pts_solver* s, uint32_t max_iterations, double target_pct, uint32_t check_every,
+                            pts_progress_c* out, pts_progress_c* history, int32_t history_cap, int32_t* history_len);
+/* `count` DCFR iterations without exploitability checks. */
+pts_status pts_solver_iterate(pts_solver* s, uint32_t count);
+pts_status pts_solver_exploitability(pts_solver* s, double* out);
+pts_status pts_solver_root_values(pts_solver* s, int mode, double out[2]);
+uint32_t pts_solver_iterations(pts_solver* s);
+void pts_solver_set_iterations(pts_solver* s, uint32_t t);
+uint64_t pts_solver_storage_len(pts_solver* s);
+int32_t pts_solver_num_hands(pts_solver* s, int player);
+pts_status pts_solver_get_storage(pts_solver* s, float* regrets, float* sums);
+pts_status pts_solver_set_storage(pts_solver* s, const float* regrets, const float* sums);
+pts_engine* pts_solver_engine(pts_solver* s);
+/* The arena (same layout as pts_tree_desc; arrays sized num_nodes). */
+uint64_t pts_solver_num_nodes(pts_solver* s);
+void pts_solver_nodes(pts_solver* s, uint8_t* kind, uint8_t* street, uint16_t* num_children, uint32_t* first_child,
+                      int32_t* invest, uint64_t* offset, float* rake, float* chance_factor, int8_t* card, int32_t* edge_act_kind,
+                      int64_t* edge_to);
+/* Solver::query (lib.rs:842-899): strategy and per-action EVs at the
+ * decision node reached by `path`. */
+pts_status pts_solver_query(pts_solver* s, const pts_step_c* path, int32_t npath, int32_t* player, int32_t* na, int32_t* n,
+                            int32_t* act_kinds, int64_t* act_tos, uint16_t* combos, float* strategy, float* ev, float* reach,
+                            double reach_fraction[2], int64_t* pot, int64_t invest[2], int32_t cap_actions, int32_t cap_hands);
+/* Solver::ranges_after (lib.rs:903-912) */
+pts_status pts_solver_ranges_after(pts_solver* s, const pts_step_c* path, int32_t npath, float* out0, float* out1);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -67,6 +67,9 @@ typedef struct {
   int32_t card, _pad;
 } orc_step_c;
 
+/* sizeof(orc_config_c) */
+size_t orc_config_size(void);
+
 /* primitives */
 uint32_t orc_evaluate(const int32_t* cards, int32_t n);
 int32_t orc_combo_index(int32_t a, int32_t b);

@@ -91,6 +91,8 @@ static std::vector<Step> path_from(const orc_step_c* path, int32_t n) {
 
 extern "C" {
 
+size_t orc_config_size(void) { return sizeof(orc_config_c); }
+
 uint32_t orc_evaluate(const int32_t* cards, int32_t n) { return evaluate_cards(cards, n); }
 int32_t orc_combo_index(int32_t a, int32_t b) { return combo_index(a, b); }
 void orc_combo_cards(int32_t idx, int32_t* hi, int32_t* lo) {

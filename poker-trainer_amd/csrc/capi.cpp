@@ -1,0 +1,578 @@
+// C ABI (include/pts.h): the engine layer and the Solver layer that mirrors
+// crates/solver/src/lib.rs on top of it.
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <limits>
+#include <memory>
+
+#include "../../include/pts.h"
+#include "engine.hpp"
+#include "host.hpp"
+#include "plan.hpp"
+
+using namespace pts;
+
+struct pts_engine {
+  std::unique_ptr<Engine> e;
+};
+
+struct pts_solver {
+  pts_config_c cfg;
+  TreeSpec spec;
+  Tree tree;
+  Hands hands;
+  TreeCounts counts;
+  std::unique_ptr<Engine> engine;
+  uint32_t iterations = 0;
+  double exploitability = std::numeric_limits<double>::infinity();
+};
+
+static pts_status st(Status s) { return (pts_status)s; }
+static pts_status bad(const std::string& m) {
+  set_error(m);
+  return PTS_INVALID_INPUT;
+}
+
+// ------------------------------------------------------------ shared helpers
+// crates/solver/src/cfr.rs:83-107
+static IterParams discount(uint32_t t, const pts_config_c& c) {
+  uint32_t tm = t < 1 ? 1 : t;
+  double prev = (double)(tm - 1);
+  double pa = std::pow(prev, c.alpha);
+  float neg = 0.0f;
+  if (c.beta_some) {
+    double pb = std::pow(prev, c.beta);
+    neg = (float)(pb / (pb + 1.0));
+  }
+  double since = (double)tm - 1.0;
+  if (c.restart_average) {
+    uint32_t t0 = tm - 1;
+    uint32_t p4 = t0 == 0 ? 0 : (1u << ((31 - __builtin_clz(t0)) & ~1u));
+    since = (double)(t0 - p4);
+  }
+  IterParams d;
+  d.pos = (float)(pa / (pa + 1.0));
+  d.neg = neg;
+  d.strat = (float)std::pow(since / (since + 1.0), c.gamma);
+  d.reach_weighted = c.unweighted_average ? 0 : 1;
+  return d;
+}
+
+// crates/solver/src/lib.rs:158-172
+static uint32_t check_gap(bool has_prev, uint32_t t0, double e0, uint32_t t, double e, double target, uint32_t max_gap) {
+  uint32_t min_gap = std::max(max_gap / 4, 1u);
+  if (!has_prev || !(e < e0 && e > target && t > t0 && target > 0.0)) return max_gap;
+  double k = std::log(e0 / e) / std::log((double)t / (double)t0);
+  double gap = std::ceil((double)t * std::pow(e / target, 1.0 / k) - (double)t);
+  if (!std::isfinite(gap)) return max_gap;
+  double g = std::max(gap, 0.0);
+  uint32_t gi = g >= 4294967295.0 ? 0xFFFFFFFFu : (uint32_t)g;
+  return std::min(std::max(gi, min_gap), max_gap);
+}
+
+// f32 normalization of a node's strategy sums (crates/solver/src/cfr.rs:124-154)
+static void normalize(float* s, size_t na, size_t n) {
+  std::vector<float> inv(s, s + n);
+  for (size_t a = 1; a < na; a++)
+    for (size_t h = 0; h < n; h++) inv[h] += s[a * n + h];
+  for (size_t h = 0; h < n; h++) {
+    float i = 1.0f / inv[h];
+    inv[h] = (inv[h] > 0.0f && std::isfinite(i)) ? i : 0.0f;
+  }
+  for (size_t a = 0; a < na; a++)
+    for (size_t h = 0; h < n; h++) s[a * n + h] *= inv[h];
+  float uni = 1.0f / (float)na;
+  for (size_t h = 0; h < n; h++)
+    if (inv[h] == 0.0f)
+      for (size_t a = 0; a < na; a++) s[a * n + h] = uni;
+}
+
+static Menu menu_of(const pts_menu_c& m) {
+  Menu r;
+  for (int s = 0; s < 3; s++) {
+    for (int i = 0; i < m.bet[s].count && i < 8; i++) r.bet[s].push_back({m.bet[s].kind[i], m.bet[s].value[i]});
+    for (int i = 0; i < m.raise[s].count && i < 8; i++) r.raise[s].push_back({m.raise[s].kind[i], m.raise[s].value[i]});
+  }
+  r.max_raises = m.max_raises;
+  r.add_allin = m.add_allin_threshold;
+  r.force_allin = m.force_allin_threshold;
+  r.merge = m.merge_threshold;
+  return r;
+}
+
+static const int kPairs[6][2] = {{0, 1}, {0, 2}, {0, 3}, {1, 2}, {1, 3}, {2, 3}};
+
+// Whether the ranges are symmetric under some suit swap (crates/solver/src/iso.rs:245-273).
+static bool any_symmetric_pair(const Hands& h) {
+  for (auto& pr : kPairs) {
+    bool ok = true;
+    for (int p = 0; p < 2 && ok; p++) {
+      float mx = 0.0f;
+      for (float w : h.weights[p]) mx = std::max(mx, w);
+      mx = std::max(mx, 1e-30f);
+      for (size_t i = 0; i < h.n(p) && ok; i++) {
+        int c[2];
+        for (int k = 0; k < 2; k++) {
+          int x = h.cards[p][2 * i + k], s = x & 3;
+          int s2 = s == pr[0] ? pr[1] : s == pr[1] ? pr[0] : s;
+          c[k] = (x >> 2) * 4 + s2;
+        }
+        uint32_t j = h.lookup[p][combo_of(c[0], c[1])];
+        if (j == kNone || std::fabs(h.weights[p][i] - h.weights[p][j]) > 1e-4f * mx) ok = false;
+      }
+    }
+    if (ok) return true;
+  }
+  return false;
+}
+
+// lib.rs:307-329 + the parts of Solver::new the GPU build supports
+static std::string validate(const pts_config_c& c, TreeSpec& spec, Hands& hands) {
+  int n = c.board_len;
+  if (n < 3 || n > 5) return "invalid board: " + std::to_string(n) + " cards";
+  uint64_t bits = 0;
+  for (int i = 0; i < n; i++) {
+    if (c.board[i] < 0 || c.board[i] >= 52) return "invalid board: bad card";
+    if (bits >> c.board[i] & 1) return "invalid board: duplicate cards";
+    bits |= 1ull << c.board[i];
+  }
+  int street = n == 3 ? kFlop : n == 4 ? kTurn : kRiver;
+  if (street != c.street) return "invalid board: board doesn't match the root street";
+  if (c.prune != 0.0f) return "invalid input: range pruning is not supported";
+  if (c.storage_mode != 0) return "invalid input: only f32 storage is supported";
+  const float* r[2] = {c.ranges[0], c.ranges[1]};
+  hands = make_hands(r, bits);
+  for (int p = 0; p < 2; p++)
+    if (hands.n(p) == 0) return "player " + std::to_string(p) + " has an empty range";
+  if (hands.num_combinations <= 0.0) return "player 0 has an empty range";
+  if (c.isomorphism && any_symmetric_pair(hands))
+    return "invalid input: suit isomorphism is not supported by the GPU plan (set isomorphism = false)";
+  spec.root_street = (Street)street;
+  spec.board.assign(c.board, c.board + n);
+  Betting& b = spec.root;
+  b.street = (Street)c.street;
+  b.pot = c.pot;
+  b.commit[0] = c.commit[0]; b.commit[1] = c.commit[1];
+  b.stack[0] = c.stack[0]; b.stack[1] = c.stack[1];
+  b.current_bet = c.current_bet;
+  b.last_raise_inc = c.last_raise_inc;
+  b.num_raises = c.num_raises;
+  b.to_act = c.to_act;
+  b.needs[0] = c.needs[0] != 0; b.needs[1] = c.needs[1] != 0;
+  spec.menu = menu_of(c.menu);
+  spec.future_menu = c.has_future_menu ? menu_of(c.future_menu) : spec.menu;
+  spec.big_blind = c.big_blind;
+  spec.rake_rate = c.rake_rate;
+  spec.rake_cap = c.rake_cap;
+  spec.hands[0] = hands.n(0);
+  spec.hands[1] = hands.n(1);
+  return "";
+}
+
+static uint64_t memory_bytes(const TreeCounts& t) { return t.cells * 8 + (t.player + t.chance + t.terminal) * 56; }
+
+// ------------------------------------------------------------------ engine
+extern "C" {
+
+size_t pts_abi_sizeof(int what) {
+  switch (what) {
+    case 0: return sizeof(pts_config_c);
+    case 1: return sizeof(pts_tree_desc);
+    case 2: return sizeof(pts_progress_c);
+    case 3: return sizeof(pts_step_c);
+    case 4: return sizeof(pts_stats_c);
+    default: return 0;
+  }
+}
+
+size_t pts_last_error(char* buf, size_t len) {
+  const std::string& e = last_error();
+  if (buf && len) std::snprintf(buf, len, "%s", e.c_str());
+  return e.size();
+}
+
+pts_status pts_device_count(int* out) {
+  *out = 0;
+  hipError_t e = hipGetDeviceCount(out);
+  if (e != hipSuccess) {
+    *out = 0;
+    set_error(std::string("no GPU: ") + hipGetErrorString(e));
+    return PTS_NO_DEVICE;
+  }
+  return PTS_OK;
+}
+
+int pts_backend_wants_gpu(int backend, int street) {
+  if (backend == PTS_BACKEND_GPU) return 1;
+  if (backend == PTS_BACKEND_AUTO) return street != kRiver;
+  return 0;
+}
+
+pts_status pts_engine_create(int device, const pts_tree_desc* t, const float* regrets, const float* sums, pts_engine** out) {
+  *out = nullptr;
+  if (!t || !regrets || !sums) return bad("invalid input: null argument");
+  PlanInput in;
+  in.num_nodes = t->num_nodes;
+  in.kind = t->kind;
+  in.street = t->street;
+  in.num_children = t->num_children;
+  in.first_child = t->first_child;
+  in.invest = t->invest;
+  in.offset = t->offset;
+  in.rake = t->rake;
+  in.chance_factor = t->chance_factor;
+  in.card = t->card;
+  in.num_boards = t->num_boards;
+  in.boards = t->boards;
+  for (int q = 0; q < 2; q++) {
+    in.num_hands[q] = t->num_hands[q];
+    in.hand_cards[q] = t->hand_cards[q];
+    in.hand_weights[q] = t->hand_weights[q];
+    in.hand_same[q] = t->hand_same[q];
+  }
+  in.half_pot = t->half_pot;
+  in.num_combinations = t->num_combinations;
+  in.storage_len = t->storage_len;
+  in.batch_river = t->batch_river != 0;
+  Plan plan;
+  std::string err = build_plan(in, plan);
+  if (!err.empty()) return bad(err);
+  std::unique_ptr<Engine> e;
+  Status s = Engine::create(device, std::move(plan), regrets, sums, e);
+  if (s != kOk) return st(s);
+  *out = new pts_engine{std::move(e)};
+  return PTS_OK;
+}
+
+pts_status pts_engine_iterate(pts_engine* e, float pos, float neg, float strat, int rw) {
+  return st(e->e->cfr_iteration({pos, neg, strat, rw}));
+}
+
+pts_status pts_engine_traverse(pts_engine* e, int player, int mode, float pos, float neg, float strat, int rw, double* out,
+                               size_t n) {
+  if (player < 0 || player > 1 || mode < 0 || mode > 2) return bad("invalid input: player or mode");
+  if (n != e->e->plan().n[player]) return bad("invalid input: output length");
+  std::vector<double> v;
+  Status s = e->e->traverse(player, (Mode)mode, {pos, neg, strat, rw}, v);
+  if (s != kOk) return st(s);
+  std::memcpy(out, v.data(), n * sizeof(double));
+  return PTS_OK;
+}
+
+pts_status pts_engine_root_values(pts_engine* e, int mode, double out[2]) {
+  if (mode != PTS_MODE_BEST_RESPONSE && mode != PTS_MODE_AVERAGE) return bad("invalid input: mode");
+  return st(e->e->root_values((Mode)mode, out));
+}
+
+pts_status pts_engine_exploitability(pts_engine* e, int raked, double* out) { return st(e->e->exploitability(raked != 0, out)); }
+
+pts_status pts_engine_download(pts_engine* e, float* regrets, float* sums, uint64_t len) {
+  if (len != e->e->plan().storage_len) return bad("invalid input: storage length");
+  return st(e->e->download(regrets, sums));
+}
+
+pts_status pts_engine_upload(pts_engine* e, const float* regrets, const float* sums, uint64_t len) {
+  if (len != e->e->plan().storage_len) return bad("invalid input: storage length");
+  return st(e->e->upload(regrets, sums));
+}
+
+pts_status pts_engine_subtree_values(pts_engine* e, uint32_t node, int player, const float* opp, size_t nq, float* out, size_t len) {
+  const Plan& P = e->e->plan();
+  if (player < 0 || player > 1 || node >= P.num_nodes) return bad("invalid input: node or player");
+  if (len != (size_t)P.na[node] * P.n[player]) return bad("invalid input: output length");
+  std::vector<float> r(opp, opp + nq), v;
+  Status s = e->e->subtree_values(node, player, r, v);
+  if (s != kOk) return st(s);
+  std::memcpy(out, v.data(), len * sizeof(float));
+  return PTS_OK;
+}
+
+uint64_t pts_engine_device_bytes(pts_engine* e) { return e->e->device_bytes(); }
+void pts_engine_destroy(pts_engine* e) { delete e; }
+
+// ------------------------------------------------------------------ solver
+pts_status pts_solver_estimate(const pts_config_c* c, pts_stats_c* out) {  // lib.rs:355-362
+  TreeSpec spec;
+  Hands hands;
+  std::string err = validate(*c, spec, hands);
+  if (!err.empty()) return bad(err);
+  TreeCounts t = count_tree(spec);
+  *out = {t.player, t.chance, t.terminal, t.cells, memory_bytes(t)};
+  return PTS_OK;
+}
+
+static PlanInput plan_input(pts_solver* s, std::vector<uint8_t>& kind, std::vector<uint8_t>& street, std::vector<uint16_t>& nc,
+                            std::vector<uint32_t>& fc, std::vector<int32_t>& inv, std::vector<uint64_t>& off,
+                            std::vector<float>& rake, std::vector<float>& fac, std::vector<int8_t>& card,
+                            std::vector<int32_t>& boards, std::vector<uint32_t>* same) {
+  size_t N = s->tree.nodes.size();
+  kind.resize(N); street.resize(N); nc.resize(N); fc.resize(N); inv.resize(2 * N); off.resize(N);
+  rake.resize(N); fac.resize(N); card.resize(N);
+  for (size_t i = 0; i < N; i++) {
+    const Node& n = s->tree.nodes[i];
+    kind[i] = n.kind; street[i] = n.street; nc[i] = n.num_children; fc[i] = n.first_child;
+    inv[2 * i] = n.invest[0]; inv[2 * i + 1] = n.invest[1]; off[i] = n.offset;
+    rake[i] = n.rake; fac[i] = n.chance_factor; card[i] = n.card;
+  }
+  for (auto& b : s->tree.boards) boards.insert(boards.end(), b.begin(), b.end());
+  PlanInput in;
+  in.num_nodes = N;
+  in.kind = kind.data(); in.street = street.data(); in.num_children = nc.data(); in.first_child = fc.data();
+  in.invest = inv.data(); in.offset = off.data(); in.rake = rake.data(); in.chance_factor = fac.data(); in.card = card.data();
+  in.num_boards = (uint32_t)s->tree.boards.size();
+  in.boards = boards.data();
+  for (int q = 0; q < 2; q++) {
+    in.num_hands[q] = (uint32_t)s->hands.n(q);
+    in.hand_cards[q] = s->hands.cards[q].data();
+    in.hand_weights[q] = s->hands.weights[q].data();
+    same[q] = s->hands.same[q];
+    in.hand_same[q] = same[q].data();
+  }
+  in.half_pot = (double)s->cfg.pot / 2.0;
+  in.num_combinations = s->hands.num_combinations;
+  in.storage_len = s->tree.storage_len;
+  in.batch_river = true;
+  return in;
+}
+
+pts_status pts_solver_new(const pts_config_c* c, int device, pts_solver** out) {  // lib.rs:365-396
+  *out = nullptr;
+  std::unique_ptr<pts_solver> s(new pts_solver());
+  s->cfg = *c;
+  std::string err = validate(*c, s->spec, s->hands);
+  if (!err.empty()) return bad(err);
+  s->counts = count_tree(s->spec);
+  uint64_t need = memory_bytes(s->counts);
+  if (need > c->max_memory_bytes) {
+    char m[160];
+    std::snprintf(m, sizeof m, "game tree needs ~%.2f GB but the limit is %.2f GB", (double)need / 1e9,
+                  (double)c->max_memory_bytes / 1e9);
+    return bad(m);
+  }
+  s->tree = build_tree(s->spec);
+  std::vector<uint8_t> kind, street;
+  std::vector<uint16_t> nc;
+  std::vector<uint32_t> fc;
+  std::vector<int32_t> inv, boards;
+  std::vector<uint64_t> off;
+  std::vector<float> rake, fac;
+  std::vector<int8_t> card;
+  std::vector<uint32_t> same[2];
+  PlanInput in = plan_input(s.get(), kind, street, nc, fc, inv, off, rake, fac, card, boards, same);
+  Plan plan;
+  err = build_plan(in, plan);
+  if (!err.empty()) return bad(err);
+  if (device < 0) {  // host only: tree, hands and a validated plan, no engine
+    *out = s.release();
+    return PTS_OK;
+  }
+  std::vector<float> zeros(s->tree.storage_len, 0.0f);
+  Status e = Engine::create(device, std::move(plan), zeros.data(), zeros.data(), s->engine);
+  if (e != kOk) return st(e);
+  *out = s.release();
+  return PTS_OK;
+}
+
+void pts_solver_free(pts_solver* s) { delete s; }
+
+void pts_solver_stats(pts_solver* s, pts_stats_c* out) {
+  const TreeCounts& t = s->counts;
+  *out = {t.player, t.chance, t.terminal, t.cells, memory_bytes(t)};
+}
+
+static pts_status need_engine(pts_solver* s) {
+  if (!s->engine) {
+    set_error("no GPU engine (the solver was built host-only)");
+    return PTS_NO_DEVICE;
+  }
+  return PTS_OK;
+}
+#define NEED_ENGINE(s) do { pts_status r_ = need_engine(s); if (r_ != PTS_OK) return r_; } while (0)
+
+static bool raked(const pts_solver* s) { return s->cfg.rake_rate > 0.0 && s->cfg.rake_cap > 0.0; }
+
+pts_status pts_solver_solve(pts_solver* s, uint32_t max_iterations, double target_pct, uint32_t check_every, pts_progress_c* out,
+                            pts_progress_c* history, int32_t history_cap, int32_t* history_len) {  // lib.rs:594-642
+  NEED_ENGINE(s);
+  double pot = (double)s->cfg.pot;
+  uint32_t ce = std::max(check_every, 1u);
+  pts_progress_c pr{s->iterations, max_iterations, s->exploitability, 100.0 * s->exploitability / pot, 0, 0};
+  double target = target_pct * pot / 100.0;
+  uint32_t next_check = s->iterations + ce;
+  bool has_last = false;
+  uint32_t lt = 0;
+  double le = 0.0;
+  int32_t nh = 0;
+  while (s->iterations < max_iterations) {
+    s->iterations += 1;
+    Status e = s->engine->cfr_iteration(discount(s->iterations, s->cfg));
+    if (e != kOk) return st(e);
+    bool last = s->iterations >= max_iterations;
+    if (s->iterations >= next_check || last) {
+      double x;
+      if ((e = s->engine->exploitability(raked(s), &x)) != kOk) return st(e);
+      s->exploitability = x;
+      pr.iteration = s->iterations;
+      pr.exploitability = x;
+      pr.exploitability_pct = 100.0 * x / pot;
+      bool done = pr.exploitability_pct <= target_pct || last;
+      pr.finished = done;
+      if (history && nh < history_cap) history[nh] = pr;
+      nh++;
+      if (done) break;
+      uint32_t t = s->iterations;
+      next_check = t + check_gap(has_last, lt, le, t, x, target, ce);
+      has_last = true;
+      lt = t;
+      le = x;
+    }
+  }
+  pr.finished = 1;
+  if (history_len) *history_len = nh;
+  *out = pr;
+  return PTS_OK;
+}
+
+pts_status pts_solver_iterate(pts_solver* s, uint32_t count) {
+  NEED_ENGINE(s);
+  for (uint32_t i = 0; i < count; i++) {
+    s->iterations += 1;
+    Status e = s->engine->cfr_iteration(discount(s->iterations, s->cfg));
+    if (e != kOk) return st(e);
+  }
+  return PTS_OK;
+}
+
+pts_status pts_solver_exploitability(pts_solver* s, double* out) { NEED_ENGINE(s); return st(s->engine->exploitability(raked(s), out)); }
+pts_status pts_solver_root_values(pts_solver* s, int mode, double out[2]) {
+  NEED_ENGINE(s);
+  if (mode != PTS_MODE_BEST_RESPONSE && mode != PTS_MODE_AVERAGE) return bad("invalid input: mode");
+  return st(s->engine->root_values((Mode)mode, out));
+}
+uint32_t pts_solver_iterations(pts_solver* s) { return s->iterations; }
+void pts_solver_set_iterations(pts_solver* s, uint32_t t) { s->iterations = t; }
+uint64_t pts_solver_storage_len(pts_solver* s) { return s->tree.storage_len; }
+int32_t pts_solver_num_hands(pts_solver* s, int p) { return (int32_t)s->hands.n(p); }
+pts_status pts_solver_get_storage(pts_solver* s, float* r, float* m) { NEED_ENGINE(s); return st(s->engine->download(r, m)); }
+pts_status pts_solver_set_storage(pts_solver* s, const float* r, const float* m) { NEED_ENGINE(s); return st(s->engine->upload(r, m)); }
+pts_engine* pts_solver_engine(pts_solver*) { return nullptr; }
+uint64_t pts_solver_num_nodes(pts_solver* s) { return s->tree.nodes.size(); }
+
+void pts_solver_nodes(pts_solver* s, uint8_t* kind, uint8_t* street, uint16_t* nc, uint32_t* fc, int32_t* inv, uint64_t* off,
+                      float* rake, float* fac, int8_t* card, int32_t* ak, int64_t* to) {
+  for (size_t i = 0; i < s->tree.nodes.size(); i++) {
+    const Node& n = s->tree.nodes[i];
+    kind[i] = n.kind; street[i] = n.street; nc[i] = n.num_children; fc[i] = n.first_child;
+    inv[2 * i] = n.invest[0]; inv[2 * i + 1] = n.invest[1]; off[i] = n.offset;
+    rake[i] = n.rake; fac[i] = n.chance_factor; card[i] = n.card;
+    bool act_edge = i > 0 && n.card < 0;
+    ak[i] = act_edge ? (int32_t)n.action.kind : -1;
+    to[i] = act_edge && n.action.kind == kRaise ? n.action.to : 0;
+  }
+}
+
+// lib.rs:746-825 (no isomorphism): reach of both players along `path`.
+static pts_status walk(pts_solver* s, const pts_step_c* path, int32_t np, uint32_t& id, std::vector<float> reach[2]) {
+  NEED_ENGINE(s);
+  id = 0;
+  reach[0] = s->hands.weights[0];
+  reach[1] = s->hands.weights[1];
+  for (int32_t d = 0; d < np; d++) {
+    const Node& node = s->tree.nodes[id];
+    const pts_step_c& stp = path[d];
+    if (is_player(node.kind) && stp.kind == 0) {
+      int ai = -1;
+      for (int a = 0; a < node.num_children; a++) {
+        const Action& x = s->tree.nodes[node.first_child + a].action;
+        if ((int)x.kind == stp.act_kind && (x.kind != kRaise || x.to == stp.to)) { ai = a; break; }
+      }
+      if (ai < 0) return bad("bad path: action not available at depth " + std::to_string(d));
+      int q = node.kind;
+      size_t n = s->hands.n(q);
+      std::vector<float> strat((size_t)node.num_children * n);
+      Status e = s->engine->read_sums(node.offset, strat.size(), strat.data());
+      if (e != kOk) return st(e);
+      normalize(strat.data(), node.num_children, n);
+      for (size_t i = 0; i < n; i++) reach[q][i] *= strat[(size_t)ai * n + i];
+      id = node.first_child + ai;
+    } else if (node.kind == kChance && stp.kind == 1) {
+      uint32_t ch = kNone;
+      for (uint32_t c = node.first_child; c < node.first_child + node.num_children; c++)
+        if (s->tree.nodes[c].card == stp.card) { ch = c; break; }
+      if (ch == kNone) return bad("bad path: card not in the tree at depth " + std::to_string(d));
+      id = ch;
+      for (int x = 0; x < 2; x++)
+        for (size_t h = 0; h < s->hands.n(x); h++)
+          if (s->hands.cards[x][2 * h] == stp.card || s->hands.cards[x][2 * h + 1] == stp.card) reach[x][h] = 0.0f;
+    } else {
+      return bad("bad path: step doesn't match node kind at depth " + std::to_string(d));
+    }
+  }
+  return PTS_OK;
+}
+
+pts_status pts_solver_query(pts_solver* s, const pts_step_c* path, int32_t npath, int32_t* player, int32_t* na, int32_t* n,
+                            int32_t* act_kinds, int64_t* act_tos, uint16_t* combos, float* strategy, float* ev, float* reach_out,
+                            double reach_fraction[2], int64_t* pot, int64_t invest[2], int32_t cap_actions, int32_t cap_hands) {
+  uint32_t id;
+  std::vector<float> reach[2];
+  pts_status r = walk(s, path, npath, id, reach);
+  if (r != PTS_OK) return r;
+  const Node node = s->tree.nodes[id];
+  if (!is_player(node.kind)) return bad("bad path: not a decision node");
+  int q = node.kind, o = q ^ 1;
+  size_t nh = s->hands.n(q), A = node.num_children;
+  if ((int32_t)A > cap_actions || (int32_t)nh > cap_hands) return bad("invalid input: capacity too small");
+  std::vector<double> compat = compatible_reach(s->hands, q, reach[o].data());
+  std::vector<float> cv;
+  Status e = s->engine->subtree_values(id, q, reach[o], cv);
+  if (e != kOk) return st(e);
+  std::vector<float> strat(A * nh);
+  if ((e = s->engine->read_sums(node.offset, strat.size(), strat.data())) != kOk) return st(e);
+  normalize(strat.data(), A, nh);
+  double half_pot = (double)s->cfg.pot / 2.0, norm = 1.0 / s->hands.num_combinations;
+  double offset = half_pot + (double)node.invest[q];
+  for (size_t h = 0; h < nh; h++) {
+    reach_out[h] = reach[q][h];
+    double denom = compat[h] * norm;
+    for (size_t a = 0; a < A; a++) {
+      strategy[a * nh + h] = strat[a * nh + h];
+      ev[a * nh + h] = denom > 0.0 ? (float)((double)cv[a * nh + h] / denom + offset) : 0.0f;
+    }
+  }
+  for (int p = 0; p < 2; p++) {
+    double root = 0.0, now = 0.0;
+    for (float x : s->hands.weights[p]) root += (double)x;
+    for (float x : reach[p]) now += (double)x;
+    reach_fraction[p] = root > 0.0 ? now / root : 0.0;
+  }
+  *player = q;
+  *na = (int32_t)A;
+  *n = (int32_t)nh;
+  for (size_t a = 0; a < A; a++) {
+    const Action& x = s->tree.nodes[node.first_child + a].action;
+    act_kinds[a] = x.kind;
+    act_tos[a] = x.kind == kRaise ? x.to : 0;
+  }
+  std::memcpy(combos, s->hands.combos[q].data(), nh * sizeof(uint16_t));
+  *pot = s->cfg.pot + node.invest[0] + node.invest[1];
+  invest[0] = node.invest[0];
+  invest[1] = node.invest[1];
+  return PTS_OK;
+}
+
+pts_status pts_solver_ranges_after(pts_solver* s, const pts_step_c* path, int32_t npath, float* out0, float* out1) {
+  uint32_t id;
+  std::vector<float> reach[2];
+  pts_status r = walk(s, path, npath, id, reach);
+  if (r != PTS_OK) return r;
+  float* outs[2] = {out0, out1};
+  for (int p = 0; p < 2; p++) {
+    std::memset(outs[p], 0, 1326 * sizeof(float));
+    for (size_t h = 0; h < s->hands.n(p); h++) outs[p][s->hands.combos[p][h]] = reach[p][h];
+  }
+  return PTS_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,99 @@
+// The HIP engine: plan and storage resident in HBM, one DCFR iteration per
+// hipGraph launch, FP64 best response. Replaces the reference's CUDA
+// `gpu::Engine` (crates/solver/src/gpu/engine.rs:291-749).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "plan.hpp"
+
+namespace pts {
+
+enum Status : int {
+  kOk = 0,
+  kDriverMissing = 1,
+  kDriverTooOld = 2,
+  kNoDevice = 3,
+  kDriver = 4,
+  kInvalidInput = 5,
+  kOutOfMemory = 6,
+};
+
+void set_error(const std::string& msg);
+const std::string& last_error();
+
+// Traversal modes (crates/solver/src/cfr.rs:248-256)
+enum Mode : int { kCfr = 0, kBest = 1, kAverage = 2 };
+
+struct IterParams {
+  float pos, neg, strat;
+  int reach_weighted;
+};
+
+struct DeviceBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+};
+
+class Engine {
+ public:
+  ~Engine();
+  // Uploads the plan and the storage; fails with kOutOfMemory when HBM is short.
+  static Status create(int device, Plan&& plan, const float* regrets, const float* sums, std::unique_ptr<Engine>& out);
+
+  // One DCFR iteration: player 0's update, then player 1's (crates/solver/src/lib.rs:577-585).
+  Status cfr_iteration(const IterParams& d);
+  // One traversal of player p; CFR mode updates storage. Root values of p's hands into out (n_p).
+  Status traverse(int p, Mode mode, const IterParams& d, std::vector<double>& out);
+  // Expected payoffs (chips) at the root under `mode` (kBest / kAverage), traversals in FP64.
+  Status root_values(Mode mode, double out[2]);
+  Status exploitability(bool raked, double* out);
+  Status download(float* regrets, float* sums) const;
+  Status upload(const float* regrets, const float* sums);
+  // A slice of the strategy sums (e.g. one node's block) to the host.
+  Status read_sums(uint64_t offset, size_t len, float* out) const;
+  // Subtree traversal for queries: player q at decision node `node`, opponent
+  // reach given; writes the node's children values (na * n_q, action-major).
+  Status subtree_values(uint32_t node, int q, const std::vector<float>& opp_reach, std::vector<float>& child_vals);
+
+  const Plan& plan() const { return plan_; }
+  hipStream_t stream() const { return stream_; }
+  // Event-timed kernel durations of the last graph-less CFR iteration (profiling aid)
+  double last_ms = 0;
+  size_t device_bytes() const { return bytes_; }
+
+ private:
+  Engine() = default;
+  Status alloc(DeviceBuf& b, size_t bytes);
+  Status launch_traversal(int p, Mode mode, bool f64);
+  Status build_graphs();
+  Status run_graph(int which);
+
+  Plan plan_;
+  int device_ = 0;
+  hipStream_t stream_ = nullptr;
+  std::vector<DeviceBuf> bufs_;
+  size_t bytes_ = 0;
+  // device pointers
+  void* dplan_blob_ = nullptr;
+  float* regrets_ = nullptr;
+  float* sums_ = nullptr;
+  void* reach_ = nullptr;
+  void* vals_ = nullptr;
+  uint32_t* alive_ = nullptr;
+  float* weights_[2] = {nullptr, nullptr};
+  IterParams* params_ = nullptr;
+  IterParams* params_host_ = nullptr;
+  uint32_t rowlen_ = 0;
+  size_t lds_down_ = 0, lds_term_[2] = {0, 0}, lds_up_[2] = {0, 0};
+  hipGraphExec_t graph_[3] = {nullptr, nullptr, nullptr};  // cfr iteration, best response, average
+  bool use_graph_ = true;
+  struct DPlanHolder;
+  std::unique_ptr<DPlanHolder> dp_;
+};
+
+}  // namespace pts

@@ -1,0 +1,101 @@
+// Host side of the MI355X heads-up DCFR solver: cards, hand evaluator, bet
+// menu, the game-tree arena, hand sets and strength tables.
+//
+// The arena (node order, storage offsets, chance factors) is the one the
+// reference solver builds (crates/solver/src/tree.rs:403-574), because the
+// regret / strategy-sum layout `offset + action * hands + hand` is the data
+// format the engine shares with its caller (crates/solver/src/storage.rs:87-106).
+#pragma once
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+namespace pts {
+
+using Chips = int64_t;
+constexpr uint32_t kNone = 0xFFFFFFFFu;
+
+enum Street : uint8_t { kPreflop = 0, kFlop = 1, kTurn = 2, kRiver = 3 };
+
+// ----------------------------------------------------------------- cards
+// Card = rank*4 + suit (crates/holdem/src/card.rs:25-69). Combo index of
+// {a, b} = hi*(hi-1)/2 + lo (crates/holdem/src/combo.rs:12-17).
+inline int combo_of(int a, int b) { int lo = a < b ? a : b, hi = a ^ b ^ lo; return hi * (hi - 1) / 2 + lo; }
+void combo_pair(int combo, int& hi, int& lo);
+
+// 7-card strength, category << 26 | payload (crates/holdem/src/eval.rs:142-198)
+uint32_t hand_strength(uint64_t lane_mask);
+inline uint64_t lane_bit(int card) { return 1ull << ((card & 3) * 16 + (card >> 2)); }
+
+// ----------------------------------------------------------------- menu
+struct Size { int kind; double v; };  // 0 pot fraction, 1 multiple, 2 all-in
+struct Menu {
+  std::vector<Size> bet[3], raise[3];  // flop, turn, river
+  uint32_t max_raises = 3;
+  double add_allin = 0, force_allin = 0, merge = 0;
+};
+enum Act : uint8_t { kFold = 0, kCheck = 1, kCall = 2, kRaise = 3 };
+struct Action { Act kind; Chips to; };
+
+struct Betting {  // crates/solver/src/tree.rs:17-126
+  Street street; Chips pot; Chips commit[2]; Chips stack[2];
+  Chips current_bet, last_raise_inc; uint32_t num_raises; int to_act; bool needs[2];
+};
+// Actions at a betting state (crates/holdem/src/menu.rs:264-392 via tree.rs:298-300)
+std::vector<Action> menu_actions(const Betting& b, const Menu& m, Chips big_blind);
+// 0 continue, 1 fold, 2 street over; `put` = chips the actor adds
+int betting_apply(const Betting& b, const Action& a, Betting& next, Chips& put);
+
+// ----------------------------------------------------------------- tree
+enum Kind : uint8_t { kP0 = 0, kP1 = 1, kChance = 2, kFold0 = 3, kFold1 = 4, kShowdown = 5 };
+inline bool is_player(uint8_t k) { return k <= kP1; }
+inline bool is_terminal(uint8_t k) { return k >= kFold0; }
+
+struct Node {
+  uint8_t kind, street;
+  uint16_t num_children;
+  uint32_t first_child;
+  int32_t invest[2];
+  uint64_t offset;  // player: storage offset; showdown: board index
+  float rake, chance_factor;
+  int8_t card;      // chance child: the dealt card, else -1
+  Action action;    // action child: the action taken
+};
+
+struct TreeSpec {
+  Street root_street;
+  std::vector<int> board;
+  Betting root;
+  Menu menu, future_menu;
+  Chips big_blind = 100;
+  double rake_rate = 0, rake_cap = 0;
+  size_t hands[2] = {0, 0};
+};
+
+struct TreeCounts { uint64_t player = 0, chance = 0, terminal = 0, cells = 0; };
+
+struct Tree {
+  std::vector<Node> nodes;
+  std::vector<std::vector<int>> boards;  // full 5-card board of every showdown table
+  uint64_t storage_len = 0;
+  uint32_t player_nodes = 0;
+};
+
+TreeCounts count_tree(const TreeSpec& spec);
+Tree build_tree(const TreeSpec& spec);
+
+// ----------------------------------------------------------------- hands
+struct Hands {  // crates/solver/src/hands.rs:14-126
+  std::vector<uint16_t> combos[2];
+  std::vector<uint8_t> cards[2];     // 2 per hand: higher card index first
+  std::vector<float> weights[2];
+  std::vector<uint32_t> same[2];     // identical combo in the other list
+  std::vector<uint32_t> lookup[2];   // combo -> hand
+  double num_combinations = 0;
+  size_t n(int p) const { return combos[p].size(); }
+};
+Hands make_hands(const float* const ranges[2], uint64_t board_bits);
+std::vector<double> compatible_reach(const Hands& h, int p, const float* opp);
+
+}  // namespace pts

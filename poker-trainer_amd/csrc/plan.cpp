@@ -1,0 +1,282 @@
+// Plan builder and validator (the host-side checks that every index the
+// kernels follow is in range, as crates/solver/src/gpu/plan.rs:466-699 does
+// for the reference's CUDA plan).
+#include "plan.hpp"
+
+#include <algorithm>
+#include <cstdio>
+#include <deque>
+
+namespace pts {
+
+static std::string fail(const char* what, size_t i) {
+  char b[160];
+  std::snprintf(b, sizeof b, "invalid plan input: %s (at %zu)", what, i);
+  return b;
+}
+
+std::string build_plan(const PlanInput& in, Plan& P) {
+  const size_t N = in.num_nodes;
+  if (N == 0) return fail("empty tree", 0);
+  if (in.num_hands[0] == 0 || in.num_hands[1] == 0) return fail("empty hand list", 0);
+  if (in.num_hands[0] > 1326 || in.num_hands[1] > 1326) return fail("more than 1326 hands", 0);
+  if (!(in.num_combinations > 0.0)) return fail("no compatible hand pairs", 0);
+  if (N >= 0xFFFFFFFFull) return fail("tree too large for 32-bit node ids", N);
+  P.num_nodes = N;
+  P.n[0] = in.num_hands[0];
+  P.n[1] = in.num_hands[1];
+  P.half_pot = in.half_pot;
+  P.norm = 1.0 / in.num_combinations;
+  P.storage_len = in.storage_len;
+  P.batch_river = in.batch_river;
+  P.num_boards = in.num_boards;
+
+  // ---- hands
+  for (int q = 0; q < 2; q++) {
+    uint32_t n = in.num_hands[q];
+    P.cards[q].assign(in.hand_cards[q], in.hand_cards[q] + 2 * n);
+    P.weights[q].assign(in.hand_weights[q], in.hand_weights[q] + n);
+    P.same[q].assign(in.hand_same[q], in.hand_same[q] + n);
+    for (uint32_t i = 0; i < n; i++) {
+      uint8_t a = P.cards[q][2 * i], b = P.cards[q][2 * i + 1];
+      if (a >= 52 || b >= 52 || a == b) return fail("bad hand cards", i);
+      if (P.same[q][i] != kNone && P.same[q][i] >= in.num_hands[q ^ 1]) return fail("bad same-combo index", i);
+      if (!(P.weights[q][i] > 0.0f)) return fail("non-positive hand weight", i);
+    }
+    P.bl_off[q].assign(53, 0);
+    std::vector<std::vector<uint16_t>> lists(52);
+    for (uint32_t i = 0; i < n; i++) {
+      lists[P.cards[q][2 * i]].push_back((uint16_t)i);
+      lists[P.cards[q][2 * i + 1]].push_back((uint16_t)i);
+    }
+    P.bl_idx[q].clear();
+    for (int c = 0; c < 52; c++) {
+      P.bl_off[q][c] = (uint32_t)P.bl_idx[q].size();
+      P.bl_idx[q].insert(P.bl_idx[q].end(), lists[c].begin(), lists[c].end());
+    }
+    P.bl_off[q][52] = (uint32_t)P.bl_idx[q].size();
+  }
+
+  // ---- nodes
+  P.kind.assign(in.kind, in.kind + N);
+  P.na.assign(N, 0);
+  P.first_child.assign(in.first_child, in.first_child + N);
+  P.offset.assign(in.offset, in.offset + N);
+  P.invest.assign(in.invest, in.invest + 2 * N);
+  P.rake.assign(in.rake, in.rake + N);
+  P.factor.assign(in.chance_factor, in.chance_factor + N);
+  P.card.assign(in.card, in.card + N);
+  std::vector<uint32_t> parent(N, kNone);
+  if (!is_player(P.kind[0])) return fail("root is not a decision node", 0);
+  for (size_t i = 0; i < N; i++) {
+    uint8_t k = P.kind[i];
+    if (k > kShowdown) return fail("bad node kind", i);
+    uint32_t nc = in.num_children[i];
+    if (is_terminal(k)) {
+      if (nc != 0) return fail("terminal with children", i);
+      if (k == kShowdown && P.offset[i] >= in.num_boards) return fail("showdown board out of range", i);
+      continue;
+    }
+    if (nc == 0 || nc > (is_player(k) ? 16u : 255u)) return fail("bad child count", i);
+    P.na[i] = (uint8_t)nc;
+    uint64_t f = P.first_child[i];
+    if (f <= i || f + nc > N) return fail("children out of range", i);
+    for (uint32_t a = 0; a < nc; a++) {
+      if (parent[f + a] != kNone) return fail("node with two parents", f + a);
+      parent[f + a] = (uint32_t)i;
+    }
+    if (is_player(k)) {
+      uint64_t cells = (uint64_t)nc * in.num_hands[k];
+      if (P.offset[i] + cells > in.storage_len) return fail("storage block out of range", i);
+    } else {
+      if (!(P.factor[i] > 0.0f)) return fail("bad chance factor", i);
+      for (uint32_t a = 0; a < nc; a++)
+        if (P.card[f + a] < 0 || P.card[f + a] >= 52) return fail("chance child without a card", f + a);
+    }
+  }
+  for (size_t i = 1; i < N; i++)
+    if (parent[i] == kNone) return fail("unreachable node", i);
+  P.subtree_end.assign(N, 0);
+  for (size_t i = N; i-- > 0;) {
+    uint32_t e = (uint32_t)i + 1;
+    if (!is_terminal(P.kind[i]))
+      for (uint32_t a = 0; a < P.na[i]; a++) e = std::max(e, P.subtree_end[P.first_child[i] + a]);
+    P.subtree_end[i] = e;
+  }
+  for (uint32_t b = 0; b < in.num_boards; b++) {
+    uint64_t s = 0;
+    for (int k = 0; k < 5; k++) {
+      int c = in.boards[5 * b + k];
+      if (c < 0 || c >= 52 || (s >> c & 1)) return fail("bad board", b);
+      s |= 1ull << c;
+    }
+  }
+
+  // ---- segments (breadth-first members; chance children start new segments)
+  struct Seg { uint32_t root, parent, depth; std::vector<uint32_t> mem; };
+  std::vector<Seg> segs;
+  std::deque<std::pair<uint32_t, uint32_t>> roots;  // (root, depth)
+  roots.push_back({0, 0});
+  P.local.assign(N, 0);
+  while (!roots.empty()) {
+    auto [r, d] = roots.front();
+    roots.pop_front();
+    Seg s{r, r == 0 ? kNone : parent[r], d, {r}};
+    for (size_t m = 0; m < s.mem.size(); m++) {
+      uint32_t id = s.mem[m];
+      uint8_t k = P.kind[id];
+      if (is_player(k)) {
+        for (uint32_t a = 0; a < P.na[id]; a++) s.mem.push_back(P.first_child[id] + a);
+      } else if (k == kChance) {
+        for (uint32_t a = 0; a < P.na[id]; a++) roots.push_back({P.first_child[id] + a, d + 1});
+      }
+    }
+    if (s.mem.size() > 0xFFFF) return fail("segment too large", r);
+    for (size_t m = 0; m < s.mem.size(); m++) P.local[s.mem[m]] = (uint16_t)m;
+    segs.push_back(std::move(s));
+    if (segs.size() >= 0xFFFFFFFFull) return fail("too many segments", r);
+  }
+  std::stable_sort(segs.begin(), segs.end(), [](const Seg& a, const Seg& b) { return a.depth < b.depth; });
+  uint32_t maxd = segs.back().depth;
+  P.depth_off.assign(maxd + 2, 0);
+  P.members.clear();
+  P.max_members = 0;
+  for (size_t i = 0; i < segs.size(); i++) {
+    P.depth_off[segs[i].depth + 1] = (uint32_t)(i + 1);
+    P.seg_root.push_back(segs[i].root);
+    P.seg_parent.push_back(segs[i].parent);
+    P.seg_mem_off.push_back((uint32_t)P.members.size());
+    P.seg_mem_cnt.push_back((uint32_t)segs[i].mem.size());
+    P.members.insert(P.members.end(), segs[i].mem.begin(), segs[i].mem.end());
+    P.max_members = std::max(P.max_members, (uint32_t)segs[i].mem.size());
+  }
+  for (uint32_t d = 1; d <= maxd + 1; d++) P.depth_off[d] = std::max(P.depth_off[d], P.depth_off[d - 1]);
+  P.seg_of.assign(N, kNone);
+  for (size_t i = 0; i < segs.size(); i++)
+    for (uint32_t m = 0; m < P.seg_mem_cnt[i]; m++) P.seg_of[P.members[P.seg_mem_off[i] + m]] = (uint32_t)i;
+
+  // ---- reach slots and alive sources, per traverser (parents precede children in the arena)
+  P.rslot.assign(2 * N, kNone);
+  P.alive_src.assign(2 * N, 0);
+  for (int p = 0; p < 2; p++) {
+    uint32_t* rs = P.rslot.data() + p * N;
+    uint16_t* src = P.alive_src.data() + p * N;
+    uint32_t next = 1;
+    rs[0] = 0;
+    src[0] = 0;
+    for (size_t i = 0; i < N; i++) {
+      uint8_t k = P.kind[i];
+      if (is_terminal(k)) continue;
+      for (uint32_t a = 0; a < P.na[i]; a++) {
+        uint32_t c = P.first_child[i] + a;
+        bool fresh = k == kChance || (int)k == (p ^ 1);
+        rs[c] = fresh ? next++ : rs[i];
+        src[c] = fresh ? P.local[c] : src[i];
+      }
+    }
+    P.num_slots[p] = next;
+  }
+
+  // ---- value rows, terminals and their flags
+  P.vslot.assign(N, kNone);
+  P.tflags.assign(N, 0);
+  P.tboard.assign(N, kNone);
+  uint32_t nv = 0;
+  for (uint32_t r : P.seg_root) P.vslot[r] = nv++;
+  for (size_t i = 0; i < N; i++) {
+    if (!is_terminal(P.kind[i])) continue;
+    P.terminals.push_back((uint32_t)i);
+    if (P.vslot[i] == kNone) P.vslot[i] = nv++;
+    uint32_t par = parent[i];
+    bool batched = in.batch_river && is_player(P.kind[par]) && in.street[par] == kRiver;
+    if (batched) P.tflags[i] |= kTermBatched;
+    if (P.kind[i] == kShowdown) P.tboard[i] = (uint32_t)P.offset[i];
+  }
+  // A fold and a showdown under one river node of the traverser share its
+  // reach and one strength sweep (crates/solver/src/river.rs:163-213).
+  for (size_t i = 0; i < N; i++) {
+    if (!(P.tflags[i] & kTermBatched) || (P.kind[i] != kFold0 && P.kind[i] != kFold1)) continue;
+    uint32_t par = parent[i];
+    for (uint32_t a = 0; a < P.na[par]; a++) {
+      uint32_t s = P.first_child[par] + a;
+      if (P.kind[s] != kShowdown) continue;
+      P.tflags[i] |= P.kind[par] == kP0 ? kTermPaired0 : kTermPaired1;
+      P.tboard[i] = (uint32_t)P.offset[s];
+      break;
+    }
+  }
+  P.num_vslots = nv;
+
+  // ---- strength tables (crates/solver/src/hands.rs:128-159)
+  uint32_t B = in.num_boards;
+  for (int q = 0; q < 2; q++) {
+    P.ent_off[q].assign(B, 0);
+    P.ent_cnt[q].assign(B, 0);
+    P.cl_base[q].assign(B, 0);
+    P.cl_off[q].assign((size_t)B * 53, 0);
+    P.hand_ent[q].assign((size_t)B * P.n[q], 0xFFFF);
+    P.ent_idx[q].clear();
+    P.ent_str[q].clear();
+    P.cl_idx[q].clear();
+  }
+  std::vector<std::vector<uint32_t>> cl_str[2];
+  for (uint32_t b = 0; b < B; b++) {
+    uint64_t lanes = 0, bits = 0;
+    for (int k = 0; k < 5; k++) { lanes |= lane_bit(in.boards[5 * b + k]); bits |= 1ull << in.boards[5 * b + k]; }
+    for (int q = 0; q < 2; q++) {
+      std::vector<std::pair<uint32_t, uint16_t>> e;
+      for (uint32_t i = 0; i < P.n[q]; i++) {
+        int a = P.cards[q][2 * i], c = P.cards[q][2 * i + 1];
+        if ((bits >> a & 1) || (bits >> c & 1)) continue;
+        e.push_back({hand_strength(lanes | lane_bit(a) | lane_bit(c)), (uint16_t)i});
+      }
+      std::sort(e.begin(), e.end());  // (strength, hand index): the reference's Entry order
+      P.ent_off[q][b] = (uint32_t)P.ent_idx[q].size();
+      P.ent_cnt[q][b] = (uint32_t)e.size();
+      P.max_entries = std::max(P.max_entries, (uint32_t)e.size());
+      std::vector<std::vector<std::pair<uint32_t, uint16_t>>> per(52);
+      for (size_t j = 0; j < e.size(); j++) {
+        P.ent_idx[q].push_back(e[j].second);
+        P.ent_str[q].push_back(e[j].first);
+        P.hand_ent[q][(size_t)b * P.n[q] + e[j].second] = (uint16_t)j;
+        per[P.cards[q][2 * e[j].second]].push_back(e[j]);
+        per[P.cards[q][2 * e[j].second + 1]].push_back(e[j]);
+      }
+      P.cl_base[q][b] = (uint32_t)P.cl_idx[q].size();
+      cl_str[q].assign(52, {});
+      uint32_t pos = 0;
+      for (int c = 0; c < 52; c++) {
+        P.cl_off[q][(size_t)b * 53 + c] = (uint16_t)pos;
+        for (auto& x : per[c]) { P.cl_idx[q].push_back(x.second); cl_str[q][c].push_back(x.first); }
+        pos += (uint32_t)per[c].size();
+      }
+      P.cl_off[q][(size_t)b * 53 + 52] = (uint16_t)pos;
+    }
+    // counts of the other player's entries below / at each of q's strengths
+    for (int q = 0; q < 2; q++) {
+      int o = q ^ 1;
+      const uint32_t* os = P.ent_str[o].data() + P.ent_off[o][b];
+      uint32_t om = P.ent_cnt[o][b];
+      if (q == 0 && b == 0) {
+        P.my_cnt[0].clear();
+        P.my_cnt[1].clear();
+      }
+      for (uint32_t j = 0; j < P.ent_cnt[q][b]; j++) {
+        uint32_t s = P.ent_str[q][P.ent_off[q][b] + j];
+        uint16_t h = P.ent_idx[q][P.ent_off[q][b] + j];
+        uint16_t lt = (uint16_t)(std::lower_bound(os, os + om, s) - os);
+        uint16_t le = (uint16_t)(std::upper_bound(os, os + om, s) - os);
+        P.my_cnt[q].push_back(lt);
+        P.my_cnt[q].push_back(le);
+        for (int k = 0; k < 2; k++) {
+          const std::vector<uint32_t>& cs = cl_str[o][P.cards[q][2 * h + k]];
+          P.my_cnt[q].push_back((uint16_t)(std::lower_bound(cs.begin(), cs.end(), s) - cs.begin()));
+          P.my_cnt[q].push_back((uint16_t)(std::upper_bound(cs.begin(), cs.end(), s) - cs.begin()));
+        }
+      }
+    }
+  }
+  return "";
+}
+
+}  // namespace pts

@@ -1,0 +1,107 @@
+// The GPU plan: the game-tree arena flattened into structure-of-arrays that
+// the HIP kernels walk level by level.
+//
+// Segments. The tree is cut at chance nodes: a segment is the tree root or a
+// chance node's child together with every descendant reachable without
+// crossing another chance node (chance nodes are the segment's leaves; their
+// children root deeper segments). Segments are grouped by depth (the number
+// of chance nodes above them). One traversal of player p runs
+//   begin -> down(depth 0..D-1) -> terminals -> up(depth D-1..0)
+// where a down kernel gives one workgroup per segment (the opponent's reach
+// pushed down in breadth-first member order, with whole-row "alive" tests as
+// block reductions), the terminal kernel gives one workgroup per terminal,
+// and an up kernel gives one workgroup per (segment, 64-hand tile) (values
+// pulled up in reverse member order, regret update fused).
+//
+// Reach slots (per traverser p, opponent o = p^1): slot 0 is the root; every
+// child of an o-acting node and every chance child gets a new slot; children
+// of p-acting nodes share their parent's slot (crates/solver/src/river.rs:
+// 92-161 does the same inside river subtrees). Value rows exist for
+// terminals and segment roots only; everything else lives in LDS.
+#pragma once
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "host.hpp"
+
+namespace pts {
+
+// Terminal flags
+constexpr uint8_t kTermBatched = 1;   // inside a river subtree (river.rs batched formulas)
+constexpr uint8_t kTermPaired0 = 2;   // fold read off the sibling showdown's sums (traverser 0)
+constexpr uint8_t kTermPaired1 = 4;   // (traverser 1)
+
+struct PlanInput {
+  size_t num_nodes = 0;
+  const uint8_t* kind = nullptr;
+  const uint8_t* street = nullptr;
+  const uint16_t* num_children = nullptr;
+  const uint32_t* first_child = nullptr;
+  const int32_t* invest = nullptr;     // 2 per node
+  const uint64_t* offset = nullptr;    // player: storage offset; showdown: board index
+  const float* rake = nullptr;
+  const float* chance_factor = nullptr;
+  const int8_t* card = nullptr;        // chance child: dealt card, else -1
+  uint32_t num_boards = 0;
+  const int32_t* boards = nullptr;     // 5 per board
+  uint32_t num_hands[2] = {0, 0};
+  const uint8_t* hand_cards[2] = {nullptr, nullptr};
+  const float* hand_weights[2] = {nullptr, nullptr};
+  const uint32_t* hand_same[2] = {nullptr, nullptr};
+  double half_pot = 0, num_combinations = 0;
+  uint64_t storage_len = 0;
+  bool batch_river = true;
+};
+
+struct Plan {
+  size_t num_nodes = 0;
+  uint32_t n[2] = {0, 0};
+  // per node
+  std::vector<uint8_t> kind, na, tflags;
+  std::vector<uint32_t> first_child, vslot, tboard;
+  std::vector<uint32_t> subtree_end;  // one past the largest node id in each node's subtree
+  std::vector<uint32_t> seg_of;       // segment index of every node
+  std::vector<uint64_t> offset;
+  std::vector<int32_t> invest;
+  std::vector<float> rake, factor;
+  std::vector<int8_t> card;
+  std::vector<uint32_t> rslot;       // [2][num_nodes]
+  std::vector<uint16_t> local;       // index in the segment's member list
+  std::vector<uint16_t> alive_src;   // [2][num_nodes]: member whose slot decides this node's alive flag
+  // segments, ordered by depth
+  std::vector<uint32_t> seg_root, seg_parent, seg_mem_off, seg_mem_cnt;
+  std::vector<uint32_t> depth_off;   // segments of depth d: [depth_off[d], depth_off[d+1])
+  std::vector<uint32_t> members;
+  std::vector<uint32_t> terminals;
+  uint32_t num_slots[2] = {0, 0}, num_vslots = 0, max_members = 0;
+  // hands
+  std::vector<uint8_t> cards[2];
+  std::vector<float> weights[2];
+  std::vector<uint32_t> same[2];
+  std::vector<uint32_t> bl_off[2];   // 53 per player: hands holding each card (ascending)
+  std::vector<uint16_t> bl_idx[2];
+  // strength tables, per board b and player q (the player whose hands are listed)
+  uint32_t num_boards = 0;
+  std::vector<uint32_t> ent_off[2], ent_cnt[2];  // [b]
+  std::vector<uint16_t> ent_idx[2];              // hand index per entry, ascending (strength, index)
+  std::vector<uint32_t> ent_str[2];              // strength per entry
+  std::vector<uint16_t> cl_off[2];               // [b*53 + c]: start of card c's list in the board block
+  std::vector<uint16_t> cl_idx[2];               // hand index, ascending strength, per card (block of 2*cnt)
+  std::vector<uint32_t> cl_base[2];              // [b]: block start in cl_idx
+  // per board b, for q as "mine" against the other player's list: per entry
+  // lt/le counts and per-card lt/le counts (u16 x 6), and the entry of each hand
+  std::vector<uint16_t> my_cnt[2];   // 6 per entry, at ent_off
+  std::vector<uint16_t> hand_ent[2]; // [b * n_q + hand] -> entry or 0xFFFF
+  uint32_t max_entries = 0;
+  double half_pot = 0, norm = 0;
+  uint64_t storage_len = 0;
+  bool batch_river = true;
+  uint32_t depths() const { return (uint32_t)depth_off.size() - 1; }
+};
+
+// Builds and validates the plan; returns "" or the reason it is invalid.
+std::string build_plan(const PlanInput& in, Plan& out);
+
+}  // namespace pts

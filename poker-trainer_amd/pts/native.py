@@ -1,0 +1,155 @@
+"""Loads libpts.so (the HIP build for gfx950) and declares the C ABI of
+include/pts.h. There is no CPU fallback: without the library, or without a
+GPU, the calls fail loudly."""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+from .config import ConfigC
+
+PKG = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB_PATH = os.path.join(PKG, "libpts.so")
+
+_lib = None
+
+
+class PtsError(RuntimeError):
+    """A pts_status other than PTS_OK (crates/solver/src/gpu/mod.rs:86-136)."""
+
+    NAMES = {1: "DriverMissing", 2: "DriverTooOld", 3: "NoDevice", 4: "Driver", 5: "InvalidInput",
+             6: "OutOfMemory", 7: "Cancelled"}
+
+    def __init__(self, status, msg):
+        super().__init__(f"{self.NAMES.get(status, status)}: {msg}")
+        self.status = status
+
+
+class StatsC(ctypes.Structure):
+    _fields_ = [("player_nodes", ctypes.c_uint64), ("chance_nodes", ctypes.c_uint64), ("terminal_nodes", ctypes.c_uint64),
+                ("storage_cells", ctypes.c_uint64), ("memory_bytes", ctypes.c_uint64)]
+
+
+class ProgressC(ctypes.Structure):
+    _fields_ = [("iteration", ctypes.c_uint32), ("max_iterations", ctypes.c_uint32), ("exploitability", ctypes.c_double),
+                ("exploitability_pct", ctypes.c_double), ("finished", ctypes.c_int32), ("_pad", ctypes.c_int32)]
+
+
+class StepC(ctypes.Structure):
+    _fields_ = [("kind", ctypes.c_int32), ("act_kind", ctypes.c_int32), ("to", ctypes.c_int64), ("card", ctypes.c_int32),
+                ("_pad", ctypes.c_int32)]
+
+
+class TreeDescC(ctypes.Structure):
+    _fields_ = [("num_nodes", ctypes.c_uint64), ("kind", ctypes.c_void_p), ("street", ctypes.c_void_p),
+                ("num_children", ctypes.c_void_p), ("first_child", ctypes.c_void_p), ("invest", ctypes.c_void_p),
+                ("offset", ctypes.c_void_p), ("rake", ctypes.c_void_p), ("chance_factor", ctypes.c_void_p),
+                ("card", ctypes.c_void_p), ("num_boards", ctypes.c_uint32), ("boards", ctypes.c_void_p),
+                ("num_hands", ctypes.c_uint32 * 2), ("hand_cards", ctypes.c_void_p * 2),
+                ("hand_weights", ctypes.c_void_p * 2), ("hand_same", ctypes.c_void_p * 2),
+                ("half_pot", ctypes.c_double), ("num_combinations", ctypes.c_double), ("storage_len", ctypes.c_uint64),
+                ("batch_river", ctypes.c_int32), ("_pad", ctypes.c_int32)]
+
+
+EXPORTS = [
+    "pts_abi_sizeof", "pts_last_error", "pts_device_count", "pts_backend_wants_gpu", "pts_engine_create", "pts_engine_iterate",
+    "pts_engine_traverse", "pts_engine_root_values", "pts_engine_exploitability", "pts_engine_download",
+    "pts_engine_upload", "pts_engine_subtree_values", "pts_engine_device_bytes", "pts_engine_destroy",
+    "pts_solver_estimate", "pts_solver_new", "pts_solver_free", "pts_solver_stats", "pts_solver_solve",
+    "pts_solver_iterate", "pts_solver_exploitability", "pts_solver_root_values", "pts_solver_iterations",
+    "pts_solver_set_iterations", "pts_solver_storage_len", "pts_solver_num_hands", "pts_solver_get_storage",
+    "pts_solver_set_storage", "pts_solver_engine", "pts_solver_num_nodes", "pts_solver_nodes", "pts_solver_query",
+    "pts_solver_ranges_after",
+]
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is missing: build it (python -c 'import __graft_entry__ as g; g.build()')")
+        _lib = ctypes.CDLL(LIB_PATH)
+        _declare(_lib)
+    return _lib
+
+
+P = ctypes.c_void_p
+f32p = np.ctypeslib.ndpointer(dtype=np.float32, flags="C_CONTIGUOUS")
+f64p = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+
+
+def _declare(L):
+    L.pts_abi_sizeof.restype = ctypes.c_size_t
+    L.pts_abi_sizeof.argtypes = [ctypes.c_int]
+    L.pts_last_error.restype = ctypes.c_size_t
+    L.pts_last_error.argtypes = [ctypes.c_char_p, ctypes.c_size_t]
+    L.pts_device_count.argtypes = [ctypes.POINTER(ctypes.c_int)]
+    L.pts_backend_wants_gpu.argtypes = [ctypes.c_int, ctypes.c_int]
+    L.pts_engine_create.argtypes = [ctypes.c_int, ctypes.POINTER(TreeDescC), f32p, f32p, ctypes.POINTER(P)]
+    L.pts_engine_iterate.argtypes = [P, ctypes.c_float, ctypes.c_float, ctypes.c_float, ctypes.c_int]
+    L.pts_engine_traverse.argtypes = [P, ctypes.c_int, ctypes.c_int, ctypes.c_float, ctypes.c_float, ctypes.c_float,
+                                      ctypes.c_int, f64p, ctypes.c_size_t]
+    L.pts_engine_root_values.argtypes = [P, ctypes.c_int, f64p]
+    L.pts_engine_exploitability.argtypes = [P, ctypes.c_int, ctypes.POINTER(ctypes.c_double)]
+    L.pts_engine_download.argtypes = [P, f32p, f32p, ctypes.c_uint64]
+    L.pts_engine_upload.argtypes = [P, f32p, f32p, ctypes.c_uint64]
+    L.pts_engine_subtree_values.argtypes = [P, ctypes.c_uint32, ctypes.c_int, f32p, ctypes.c_size_t, f32p, ctypes.c_size_t]
+    L.pts_engine_device_bytes.restype = ctypes.c_uint64
+    L.pts_engine_device_bytes.argtypes = [P]
+    L.pts_engine_destroy.argtypes = [P]
+    L.pts_solver_estimate.argtypes = [ctypes.POINTER(ConfigC), ctypes.POINTER(StatsC)]
+    L.pts_solver_new.argtypes = [ctypes.POINTER(ConfigC), ctypes.c_int, ctypes.POINTER(P)]
+    L.pts_solver_free.argtypes = [P]
+    L.pts_solver_stats.argtypes = [P, ctypes.POINTER(StatsC)]
+    L.pts_solver_solve.argtypes = [P, ctypes.c_uint32, ctypes.c_double, ctypes.c_uint32, ctypes.POINTER(ProgressC),
+                                   ctypes.POINTER(ProgressC), ctypes.c_int32, ctypes.POINTER(ctypes.c_int32)]
+    L.pts_solver_iterate.argtypes = [P, ctypes.c_uint32]
+    L.pts_solver_exploitability.argtypes = [P, ctypes.POINTER(ctypes.c_double)]
+    L.pts_solver_root_values.argtypes = [P, ctypes.c_int, f64p]
+    L.pts_solver_iterations.restype = ctypes.c_uint32
+    L.pts_solver_iterations.argtypes = [P]
+    L.pts_solver_set_iterations.argtypes = [P, ctypes.c_uint32]
+    L.pts_solver_storage_len.restype = ctypes.c_uint64
+    L.pts_solver_storage_len.argtypes = [P]
+    L.pts_solver_num_hands.restype = ctypes.c_int32
+    L.pts_solver_num_hands.argtypes = [P, ctypes.c_int]
+    L.pts_solver_get_storage.argtypes = [P, f32p, f32p]
+    L.pts_solver_set_storage.argtypes = [P, f32p, f32p]
+    L.pts_solver_num_nodes.restype = ctypes.c_uint64
+    L.pts_solver_num_nodes.argtypes = [P]
+    L.pts_solver_nodes.argtypes = [P] + [ctypes.c_void_p] * 11
+    L.pts_solver_query.argtypes = [P, ctypes.POINTER(StepC), ctypes.c_int32] + [ctypes.c_void_p] * 12 + \
+        [ctypes.c_int32, ctypes.c_int32]
+    L.pts_solver_ranges_after.argtypes = [P, ctypes.POINTER(StepC), ctypes.c_int32, f32p, f32p]
+
+
+def last_error() -> str:
+    buf = ctypes.create_string_buffer(1024)
+    lib().pts_last_error(buf, 1024)
+    return buf.value.decode(errors="replace")
+
+
+def check(status: int):
+    if status != 0:
+        raise PtsError(status, last_error())
+
+
+def device_count() -> int:
+    n = ctypes.c_int(0)
+    st = lib().pts_device_count(ctypes.byref(n))
+    return n.value if st == 0 else 0
+
+
+def steps_c(path):
+    """path: list of "fold" | "check" | "call" | ("raise", to) | ("card", c)."""
+    arr = (StepC * max(1, len(path)))()
+    for i, s in enumerate(path):
+        if isinstance(s, tuple) and s[0] == "card":
+            arr[i].kind, arr[i].card = 1, s[1]
+        elif isinstance(s, tuple) and s[0] == "raise":
+            arr[i].kind, arr[i].act_kind, arr[i].to = 0, 3, s[1]
+        else:
+            arr[i].kind, arr[i].act_kind = 0, {"fold": 0, "check": 1, "call": 2}[s]
+    return arr

@@ -1,0 +1,259 @@
+"""`Solver` and `Engine`: the solver crate's API over libpts (HIP, gfx950).
+
+`Solver` mirrors crates/solver/src/lib.rs:288-913 (new / estimate / solve /
+query / ranges_after / exploitability); `Engine` is the lower boundary that
+replaces crates/solver/src/gpu/engine.rs: it takes a caller-built tree and
+storage (pts_tree_desc) and runs iterations, best responses and subtree
+evaluations on the GPU.
+"""
+from __future__ import annotations
+
+import ctypes
+from dataclasses import dataclass
+from typing import List, Optional
+
+import numpy as np
+
+from .config import SolveConfig
+from .native import ProgressC, StatsC, TreeDescC, check, lib, steps_c
+
+ACT_NAMES = {0: "fold", 1: "check", 2: "call"}
+
+
+def _action(kind, to):
+    return ("raise", int(to)) if kind == 3 else ACT_NAMES[int(kind)]
+
+
+@dataclass
+class SolveOptions:
+    """lib.rs:137-153"""
+    max_iterations: int = 1000
+    target_exploitability_pct: float = 0.5
+    check_every: int = 10
+
+
+class NodeInfo:
+    """lib.rs:219-275"""
+
+    def __init__(self, **kw):
+        self.__dict__.update(kw)
+
+    def num_hands(self):
+        return len(self.combos)
+
+    def hand_index(self, combo):
+        idx = np.nonzero(self.combos == combo)[0]
+        return int(idx[0]) if len(idx) else None
+
+    def hand_strategy(self, i):
+        return self.strategy[:, i]
+
+    def hand_ev(self, i):
+        return self.ev[:, i]
+
+    def range_strategy(self):
+        tot = float(np.sum(self.reach.astype(np.float64)))
+        if tot <= 0:
+            return np.zeros(len(self.actions))
+        return (self.strategy.astype(np.float64) * self.reach.astype(np.float64)).sum(axis=1) / tot
+
+
+class Solver:
+    """A built game whose storage lives in HBM; solves run in HIP kernels."""
+
+    def __init__(self, cfg: SolveConfig, device: int = 0):
+        self.cfg = cfg
+        self._c = cfg.to_c()
+        h = ctypes.c_void_p()
+        check(lib().pts_solver_new(ctypes.byref(self._c), device, ctypes.byref(h)))
+        self.h = h
+        self.n = [lib().pts_solver_num_hands(self.h, p) for p in range(2)]
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().pts_solver_free(self.h)
+            self.h = None
+
+    def close(self):
+        self.__del__()
+
+    @staticmethod
+    def estimate(cfg: SolveConfig) -> StatsC:
+        c = cfg.to_c()
+        st = StatsC()
+        check(lib().pts_solver_estimate(ctypes.byref(c), ctypes.byref(st)))
+        return st
+
+    def stats(self) -> StatsC:
+        st = StatsC()
+        lib().pts_solver_stats(self.h, ctypes.byref(st))
+        return st
+
+    @property
+    def storage_len(self):
+        return lib().pts_solver_storage_len(self.h)
+
+    @property
+    def iterations(self):
+        return lib().pts_solver_iterations(self.h)
+
+    @iterations.setter
+    def iterations(self, t):
+        lib().pts_solver_set_iterations(self.h, t)
+
+    def solve(self, opts: SolveOptions, history: Optional[list] = None):
+        out = ProgressC()
+        hist = (ProgressC * 4096)()
+        hl = ctypes.c_int32(0)
+        check(lib().pts_solver_solve(self.h, opts.max_iterations, opts.target_exploitability_pct, opts.check_every,
+                                     ctypes.byref(out), hist, 4096, ctypes.byref(hl)))
+        if history is not None:
+            history.extend(hist[i] for i in range(min(hl.value, 4096)))
+        return out
+
+    def iterate(self, count: int = 1):
+        check(lib().pts_solver_iterate(self.h, count))
+
+    def exploitability(self) -> float:
+        x = ctypes.c_double()
+        check(lib().pts_solver_exploitability(self.h, ctypes.byref(x)))
+        return x.value
+
+    def root_values(self, mode: int = 2):
+        out = np.zeros(2, np.float64)
+        check(lib().pts_solver_root_values(self.h, mode, out))
+        return out
+
+    def storage(self):
+        n = self.storage_len
+        r = np.zeros(n, np.float32)
+        s = np.zeros(n, np.float32)
+        check(lib().pts_solver_get_storage(self.h, r, s))
+        return r, s
+
+    def set_storage(self, r, s):
+        check(lib().pts_solver_set_storage(self.h, np.ascontiguousarray(r, np.float32), np.ascontiguousarray(s, np.float32)))
+
+    def nodes(self):
+        m = lib().pts_solver_num_nodes(self.h)
+        a = dict(kind=np.zeros(m, np.uint8), street=np.zeros(m, np.uint8), num_children=np.zeros(m, np.uint16),
+                 first_child=np.zeros(m, np.uint32), invest=np.zeros((m, 2), np.int32), offset=np.zeros(m, np.uint64),
+                 rake=np.zeros(m, np.float32), chance_factor=np.zeros(m, np.float32), card=np.zeros(m, np.int8),
+                 edge_act=np.zeros(m, np.int32), edge_to=np.zeros(m, np.int64))
+        lib().pts_solver_nodes(self.h, *[x.ctypes.data for x in a.values()])
+        return a
+
+    def query(self, path) -> NodeInfo:
+        cap_a, cap_h = 16, 1326
+        player, na, n = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int32()
+        kinds = np.zeros(cap_a, np.int32)
+        tos = np.zeros(cap_a, np.int64)
+        combos = np.zeros(cap_h, np.uint16)
+        strat = np.zeros(cap_a * cap_h, np.float32)
+        ev = np.zeros(cap_a * cap_h, np.float32)
+        reach = np.zeros(cap_h, np.float32)
+        rf = np.zeros(2, np.float64)
+        pot = ctypes.c_int64()
+        inv = np.zeros(2, np.int64)
+        check(lib().pts_solver_query(self.h, steps_c(path), len(path), ctypes.byref(player), ctypes.byref(na),
+                                     ctypes.byref(n), kinds.ctypes.data, tos.ctypes.data, combos.ctypes.data,
+                                     strat.ctypes.data, ev.ctypes.data, reach.ctypes.data, rf.ctypes.data,
+                                     ctypes.addressof(pot), inv.ctypes.data, cap_a, cap_h))
+        na, n = na.value, n.value
+        return NodeInfo(player=player.value, actions=[_action(kinds[a], tos[a]) for a in range(na)],
+                        combos=combos[:n].copy(), strategy=strat[:na * n].reshape(na, n).copy(),
+                        ev=ev[:na * n].reshape(na, n).copy(), reach=reach[:n].copy(), reach_fraction=rf.copy(),
+                        pot=pot.value, invest=inv.copy())
+
+    def ranges_after(self, path):
+        o0 = np.zeros(1326, np.float32)
+        o1 = np.zeros(1326, np.float32)
+        check(lib().pts_solver_ranges_after(self.h, steps_c(path), len(path), o0, o1))
+        return o0, o1
+
+
+class TreeArrays:
+    """The arrays of a pts_tree_desc, kept alive while an Engine uses them."""
+
+    def __init__(self, kind, street, num_children, first_child, invest, offset, rake, chance_factor, card, boards,
+                 hand_cards, hand_weights, hand_same, half_pot, num_combinations, storage_len, batch_river=True):
+        self.a = dict(kind=np.ascontiguousarray(kind, np.uint8), street=np.ascontiguousarray(street, np.uint8),
+                      num_children=np.ascontiguousarray(num_children, np.uint16),
+                      first_child=np.ascontiguousarray(first_child, np.uint32),
+                      invest=np.ascontiguousarray(invest, np.int32).reshape(-1),
+                      offset=np.ascontiguousarray(offset, np.uint64), rake=np.ascontiguousarray(rake, np.float32),
+                      chance_factor=np.ascontiguousarray(chance_factor, np.float32),
+                      card=np.ascontiguousarray(card, np.int8),
+                      boards=np.ascontiguousarray(np.asarray(boards, np.int32).reshape(-1)))
+        self.hc = [np.ascontiguousarray(x, np.uint8).reshape(-1) for x in hand_cards]
+        self.hw = [np.ascontiguousarray(x, np.float32) for x in hand_weights]
+        self.hs = [np.ascontiguousarray(x, np.uint32) for x in hand_same]
+        d = TreeDescC()
+        d.num_nodes = len(self.a["kind"])
+        for k in ("kind", "street", "num_children", "first_child", "invest", "offset", "rake", "chance_factor", "card",
+                  "boards"):
+            setattr(d, k, self.a[k].ctypes.data)
+        d.num_boards = len(self.a["boards"]) // 5
+        for q in range(2):
+            d.num_hands[q] = len(self.hw[q])
+            d.hand_cards[q] = self.hc[q].ctypes.data
+            d.hand_weights[q] = self.hw[q].ctypes.data
+            d.hand_same[q] = self.hs[q].ctypes.data
+        d.half_pot, d.num_combinations, d.storage_len = half_pot, num_combinations, storage_len
+        d.batch_river = int(batch_river)
+        self.desc = d
+
+
+class Engine:
+    """pts_engine_*: the GPU backend boundary (crates/solver/src/gpu/engine.rs)."""
+
+    def __init__(self, tree: TreeArrays, regrets, sums, device: int = 0):
+        self.tree = tree
+        h = ctypes.c_void_p()
+        check(lib().pts_engine_create(device, ctypes.byref(tree.desc), np.ascontiguousarray(regrets, np.float32),
+                                      np.ascontiguousarray(sums, np.float32), ctypes.byref(h)))
+        self.h = h
+        self.storage_len = tree.desc.storage_len
+        self.n = [tree.desc.num_hands[0], tree.desc.num_hands[1]]
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().pts_engine_destroy(self.h)
+            self.h = None
+
+    def iterate(self, d):
+        check(lib().pts_engine_iterate(self.h, d[0], d[1], d[2], int(d[3]) if len(d) > 3 else 1))
+
+    def traverse(self, p, mode, d=(0.0, 0.0, 0.0, 1)):
+        out = np.zeros(self.n[p], np.float64)
+        check(lib().pts_engine_traverse(self.h, p, mode, d[0], d[1], d[2], int(d[3]), out, self.n[p]))
+        return out
+
+    def root_values(self, mode):
+        out = np.zeros(2, np.float64)
+        check(lib().pts_engine_root_values(self.h, mode, out))
+        return out
+
+    def exploitability(self, raked=False):
+        x = ctypes.c_double()
+        check(lib().pts_engine_exploitability(self.h, int(raked), ctypes.byref(x)))
+        return x.value
+
+    def download(self):
+        r = np.zeros(self.storage_len, np.float32)
+        s = np.zeros(self.storage_len, np.float32)
+        check(lib().pts_engine_download(self.h, r, s, self.storage_len))
+        return r, s
+
+    def upload(self, r, s):
+        check(lib().pts_engine_upload(self.h, np.ascontiguousarray(r, np.float32), np.ascontiguousarray(s, np.float32),
+                                      self.storage_len))
+
+    def subtree_values(self, node, player, opp_reach, na):
+        out = np.zeros(na * self.n[player], np.float32)
+        opp = np.ascontiguousarray(opp_reach, np.float32)
+        check(lib().pts_engine_subtree_values(self.h, node, player, opp, len(opp), out, len(out)))
+        return out.reshape(na, self.n[player])
+
+    def device_bytes(self):
+        return lib().pts_engine_device_bytes(self.h)

@@ -1,0 +1,171 @@
+"""GPU parity: the HIP engine against the oracle (the reference CPU solve
+restated), on the same seeded inputs. The engine repeats the reference's
+operation order, so regrets, strategy sums, values and exploitability must be
+bit-identical (compared as floats, so +0 == -0)."""
+import numpy as np
+import pytest
+
+import pyoracle as O
+from pts import spots, PostflopMenu, make_config, parse_cards, combo_index, DcfrParams
+from pts.solver import Engine, Solver, SolveOptions
+
+from test_host_cpu import oracle_tree
+
+pytestmark = pytest.mark.gpu
+
+
+def random_storage(n, seed):
+    rng = np.random.default_rng(seed)
+    r = (rng.standard_normal(n) * 3).astype(np.float32)
+    r[rng.random(n) < 0.3] = 0.0
+    s = (rng.random(n) * 5).astype(np.float32)
+    s[rng.random(n) < 0.2] = 0.0
+    return r, s
+
+
+def same(a, b):
+    a, b = np.asarray(a), np.asarray(b)
+    return a.shape == b.shape and np.array_equal(a, b)
+
+
+def describe(a, b):
+    d = np.abs(np.asarray(a, np.float64) - np.asarray(b, np.float64))
+    return f"{int((d > 0).sum())} of {d.size} differ, max {d.max(initial=0):.3e}"
+
+
+SPOTS = [("river0", lambda: spots.river_spot(0)), ("river5", lambda: spots.river_spot(5)),
+         ("turn0", lambda: spots.turn_spot(0))]
+
+
+@pytest.mark.parametrize("name,make", SPOTS, ids=[s[0] for s in SPOTS])
+def test_traversals_from_random_storage(name, make):
+    cfg = make()
+    o, t = oracle_tree(cfg)
+    r, s = random_storage(o.storage_len, 11)
+    o.set_storage(r, s)
+    e = Engine(t, r, s)
+    for p in (0, 1):
+        for mode in (1, 2):  # FP64 best response and average
+            g = e.traverse(p, mode)
+            c = o.traverse_root_f64(p, mode)
+            assert same(g, c), f"p{p} mode{mode}: {describe(g, c)}"
+    # one CFR traversal per player at t = 3 (updates storage)
+    d = O.discount(3)
+    for p in (0, 1):
+        g = e.traverse(p, 0, (d[0], d[1], d[2], 1))
+        c = o.traverse_root(p, 0, 3)
+        assert same(g.astype(np.float32), c), f"cfr p{p}: {describe(g, c)}"
+        rg, sg = e.download()
+        rc, sc = o.storage()
+        assert same(rg, rc), f"regrets after p{p}: {describe(rg, rc)}"
+        assert same(sg, sc), f"sums after p{p}: {describe(sg, sc)}"
+
+
+@pytest.mark.parametrize("seed", [0, 1, 2, 3])
+def test_river_solves_bit_exact(seed):
+    cfg = spots.river_spot(seed)
+    g, c = Solver(cfg), O.OracleSolver(cfg)
+    g.iterate(40)
+    c.iterate(40)
+    rg, sg = g.storage()
+    rc, sc = c.storage()
+    assert same(rg, rc), describe(rg, rc)
+    assert same(sg, sc), describe(sg, sc)
+    assert g.exploitability() == c.exploitability_f64()
+
+
+def test_turn_solve_bit_exact():
+    cfg = spots.turn_spot(0)
+    g, c = Solver(cfg), O.OracleSolver(cfg)
+    g.iterate(6)
+    c.iterate(6)
+    rg, sg = g.storage()
+    rc, sc = c.storage()
+    assert same(rg, rc), describe(rg, rc)
+    assert same(sg, sc), describe(sg, sc)
+    eg, ec = g.exploitability(), c.exploitability_f64()
+    assert eg == ec, (eg, ec)
+    # the reference's own f32 best response agrees to far inside 0.01% of the pot
+    assert abs(eg - c.exploitability()) < 1e-4 * cfg.root.pot
+
+
+def test_solve_loop_matches_the_oracle():
+    cfg = spots.river_spot(4)
+    g, c = Solver(cfg), O.OracleSolver(cfg)
+    hg = []
+    pg = g.solve(SolveOptions(200, 0.5, 10), hg)
+    pc, hc = c.solve(200, 0.5, 10)
+    # the checks land on the same iterations; exploitability is FP64 on the GPU
+    assert [h.iteration for h in hg] == [h.iteration for h in hc]
+    for a, b in zip(hg, hc):
+        assert abs(a.exploitability - b.exploitability) < 1e-4 * cfg.root.pot
+
+
+def test_queries_match_the_oracle():
+    cfg = spots.turn_spot(2)
+    g, c = Solver(cfg), O.OracleSolver(cfg)
+    g.iterate(8)
+    c.iterate(8)
+    qg, qc = g.query([]), c.query([])
+    assert qg.actions == qc.actions
+    assert same(qg.strategy, qc.strategy)
+    assert same(qg.ev, qc.ev), describe(qg.ev, qc.ev)
+    path = [qc.actions[0]]
+    qc2 = c.query(path)
+    path2 = path + [qc2.actions[0]]
+    card = next(x for x in range(52) if x not in cfg.board)
+    for pth in (path, path2 + [("card", card)]):
+        a, b = g.query(pth), c.query(pth)
+        assert a.actions == b.actions
+        assert same(a.strategy, b.strategy) and same(a.reach, b.reach)
+        assert same(a.ev, b.ev), describe(a.ev, b.ev)
+        ra, rb = g.ranges_after(pth), c.ranges_after(pth)
+        assert same(ra[0], rb[0]) and same(ra[1], rb[1])
+
+
+REF = DcfrParams.reference_default()
+
+
+def combo(s):
+    c = parse_cards(s)
+    return combo_index(c[0], c[1])
+
+
+def test_polarized_toy_game_on_the_gpu():
+    # crates/solver/src/tests.rs:42-88
+    cfg = make_config("Kc8d5h3s2c", "AA, JTs, J9s:0.5", "KQ", 1000, 5000, PostflopMenu.one(["100%"], []), dcfr=REF)
+    s = Solver(cfg)
+    p = s.solve(SolveOptions(2000, 0.05, 25))
+    assert p.exploitability_pct < 0.2
+    root = s.query([])
+    ev = root.hand_ev(root.hand_index(combo("AsAh")))
+    assert abs(ev[1] - 1500) < 20 and abs(ev[0] - 1000) < 20
+    facing = s.query([("raise", 1000)])
+    assert abs(facing.range_strategy()[1] - 0.5) < 0.05
+
+
+def test_all_in_equity_on_the_gpu():
+    # crates/solver/src/tests.rs:92-104
+    cfg = make_config("2c7d9hTs", "AhAd", "KhKd", 1000, 1000, PostflopMenu.one(["allin"], []), dcfr=REF)
+    s = Solver(cfg)
+    s.solve(SolveOptions(50, 0.0, 25))
+    ev = s.query([("raise", 1000)]).hand_ev(0)
+    assert abs(ev[1] - (2.0 / 44.0 * 3000.0 - 1000.0)) < 0.5 and abs(ev[0]) < 1e-3
+
+
+def test_ties_and_rake_on_the_gpu():
+    # crates/solver/src/tests.rs:106-127,305-328
+    from pts import BettingState
+    for rake, expect in ((0.0, 500.0), (0.05, 450.0)):
+        cfg = make_config("AsKsQsJsTs", "22", "33", 1500, 5000, PostflopMenu.one(["100%"], ["3x"]), dcfr=REF,
+                          rake_rate=rake, rake_cap=300.0 if rake else 0.0)
+        cfg.root = BettingState(3, 1500, [0, 500], [5000, 4500], 500, 500, 1, 0, [True, False])
+        s, c = Solver(cfg), O.OracleSolver(cfg)
+        s.iterate(30)
+        c.iterate(30)
+        rg, sg = s.storage()
+        rc, sc = c.storage()
+        assert same(rg, rc) and same(sg, sc)
+        s.iterate(170)
+        ev = s.query([]).hand_ev(0)
+        assert abs(ev[0]) < 1e-3 and abs(ev[1] - expect) < 1.0
