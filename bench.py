@@ -1,0 +1,150 @@
+"""Benchmark: time to 0.5%-of-pot exploitability of the exact HU DCFR solve on
+one MI355X, with the reference CPU solve (the oracle) timed on the host.
+
+A step is one DCFR iteration (both players' traversals) over the whole tree.
+`value` is the wall time of a fresh solve from zero storage until the first
+FP64 best-response check at or below the target (checks included; tree build
+and upload excluded and reported as build_s). `ms_per_step` is the mean of K
+timed iterations after W warmup iterations. `roofline` is for the kernel class
+with the largest share of an iteration's event time: algorithmic bytes per
+launch (down: 12 B per storage cell of the opponent's nodes; up: 8 B per cell
+of the traverser's nodes; terminal: 4 B per opponent hand per terminal) over
+its mean launch duration, both measured with HIP events on the engine's
+stream.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload turn|river]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "poker-trainer_amd"))
+
+METRIC = "time to 0.5%-pot exploitability, exact vs value-net DCFR, HU flop, 1 MI355X"
+HBM_PEAK_GBS = 8000.0
+
+WORKLOADS = {
+    "turn": ("turn HU exact DCFR: seeded 4-card board, pot 20bb, stacks 90bb, bets 33%/75%/all-in, raises "
+             "2.5x/all-in, one raise (cap 2), 48 river cards; ranges U(0,1) per combo with 30% zeroed; f32 regrets "
+             "and sums, alpha 1.5 beta 0 gamma 2", "turn_spot", 500),
+    "river": ("river HU exact DCFR: seeded 5-card board, pot 10bb, stacks 40bb, bets 50%/100%/all-in, raise 2.5x, "
+              "3 bets; ranges U(0,1) per combo with 40% zeroed; f32, alpha 1.5 beta 0 gamma 2", "river_spot", 1000),
+}
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def cpu_baseline(cfg, iters_to_target, checks, seconds):
+    """The oracle (C++ restatement of the reference CPU solve, parallel over
+    chance children like the reference's rayon split) on the host cores: a
+    bounded sample of iterations plus one exploitability check, scaled to the
+    GPU run's iteration and check counts."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or os.cpu_count()
+    o = pyoracle.OracleSolver(cfg, parallel=True, threads=threads)
+    n = 0
+    t0 = time.perf_counter()
+    while True:
+        o.iterate(1)
+        n += 1
+        if time.perf_counter() - t0 > seconds or n >= iters_to_target:
+            break
+    t_iter = (time.perf_counter() - t0) / n
+    t1 = time.perf_counter()
+    o.exploitability()
+    t_check = time.perf_counter() - t1
+    est = iters_to_target * t_iter + checks * t_check
+    return {"value": round(est, 4), "unit": "s", "cores": threads, "kind": "port",
+            "sample": f"{n} DCFR iterations ({t_iter * 1e3:.1f} ms each) and one best-response check "
+                      f"({t_check * 1e3:.1f} ms) of the same spot, scaled to the GPU run's {iters_to_target} "
+                      f"iterations and {checks} checks"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--workload", default="turn", choices=sorted(WORKLOADS))
+    ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--target", type=float, default=0.5)
+    ap.add_argument("--check-every", type=int, default=10)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only (this build has no multi-GPU path); got --gpus {args.gpus}",
+              file=sys.stderr)
+        sys.exit(2)
+
+    from pts import spots
+    from pts.solver import Solver, SolveOptions
+
+    desc, maker, max_iters = WORKLOADS[args.workload]
+    cfg = getattr(spots, maker)(args.seed)
+
+    # steps: W warmup iterations, then K timed ones
+    t0 = time.perf_counter()
+    s = Solver(cfg)
+    build_s = time.perf_counter() - t0
+    st = s.stats()
+    n0, n1 = s.n
+    log(f"[bench] {args.workload}: hands {n0}/{n1}, {st.player_nodes} decision nodes, {st.terminal_nodes} terminals, "
+        f"{st.storage_cells} cells; build+upload {build_s:.2f}s")
+    s.iterate(args.warmup)
+    t0 = time.perf_counter()
+    s.iterate(args.steps)
+    step_s = (time.perf_counter() - t0) / max(args.steps, 1)
+    log(f"[bench] {args.steps} timed iterations: {step_s * 1e3:.3f} ms each")
+    ms, launches, wall = s.profile_iteration()
+    log(f"[bench] profiled iteration: {ms} launches {launches} wall {wall:.3f} ms")
+    del s
+
+    # the metric: a fresh solve to the target
+    s = Solver(cfg)
+    hist = []
+    t0 = time.perf_counter()
+    pr = s.solve(SolveOptions(max_iters, args.target, args.check_every), hist)
+    solve_s = time.perf_counter() - t0
+    log(f"[bench] solve: {pr.iteration} iterations, {pr.exploitability_pct:.4f}% pot, {solve_s:.3f}s, "
+        f"{len(hist)} checks")
+
+    cells = st.storage_cells
+    bytes_cls = {"down": 12.0 * cells, "up": 8.0 * cells, "terminal": 4.0 * st.terminal_nodes * (n0 + n1)}
+    dom = max(("down", "terminal", "up"), key=lambda k: ms[k])
+    achieved = bytes_cls[dom] / (ms[dom] * 1e-3) / 1e9 if ms[dom] > 0 else 0.0
+    roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None, "kernel": dom,
+                "launches_per_iteration": launches[dom], "avg_launch_ms": round(ms[dom] / max(launches[dom], 1), 5),
+                "algorithmic_bytes_per_launch": round(bytes_cls[dom] / max(launches[dom], 1)),
+                "iteration_ms_by_class": {k: round(v, 4) for k, v in ms.items()}}
+
+    reached = pr.exploitability_pct <= args.target
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(cfg, pr.iteration, len(hist), args.cpu_seconds)
+        log(f"[bench] cpu baseline: {cpu}")
+
+    line = {
+        "metric": METRIC, "value": round(solve_s, 4), "unit": "s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(step_s * 1e3, 4), "higher_is_better": False, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": desc, "seed": args.seed, "hands": [n0, n1], "decision_nodes": st.player_nodes,
+                   "terminals": st.terminal_nodes, "storage_cells": cells, "target_pct": args.target,
+                   "check_every": args.check_every, "iterations_to_target": pr.iteration,
+                   "target_reached": reached, "final_exploitability_pct": round(pr.exploitability_pct, 5),
+                   "iterations_per_s": round(1.0 / step_s, 2), "build_s": round(build_s, 3),
+                   "parallelism": "1 GPU"},
+        "roofline": roofline, "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

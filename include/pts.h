@@ -105,6 +105,13 @@ pts_status pts_engine_upload(pts_engine* e, const float* regrets, const float* s
  * Solver::query, lib.rs:854-863): out[action * n_player + hand]. */
 pts_status pts_engine_subtree_values(pts_engine* e, uint32_t node, int player, const float* opp_reach, size_t nq, float* out,
                                      size_t len);
+/* One DCFR iteration launched kernel by kernel with HIP events around every
+ * launch on the engine's stream: summed milliseconds and launch counts per
+ * kernel class [0 begin, 1 down (reach push-down + strategy sums), 2 terminal
+ * (fold / showdown values), 3 up (value pull-up + regret update)], and the
+ * wall time from the first event to the last. */
+pts_status pts_engine_profile_iteration(pts_engine* e, float pos, float neg, float strat, int reach_weighted, double ms[4],
+                                        uint32_t launches[4], double* wall_ms);
 /* Bytes of HBM the engine holds. */
 uint64_t pts_engine_device_bytes(pts_engine* e);
 void pts_engine_destroy(pts_engine* e);
@@ -185,7 +192,9 @@ uint64_t pts_solver_storage_len(pts_solver* s);
 int32_t pts_solver_num_hands(pts_solver* s, int player);
 pts_status pts_solver_get_storage(pts_solver* s, float* regrets, float* sums);
 pts_status pts_solver_set_storage(pts_solver* s, const float* regrets, const float* sums);
-pts_engine* pts_solver_engine(pts_solver* s);
+/* The next DCFR iteration (advancing the iteration count), profiled as
+ * pts_engine_profile_iteration. */
+pts_status pts_solver_profile_iteration(pts_solver* s, double ms[4], uint32_t launches[4], double* wall_ms);
 /* The arena (same layout as pts_tree_desc; arrays sized num_nodes). */
 uint64_t pts_solver_num_nodes(pts_solver* s);
 void pts_solver_nodes(pts_solver* s, uint8_t* kind, uint8_t* street, uint16_t* num_children, uint32_t* first_child,

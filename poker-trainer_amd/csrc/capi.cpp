@@ -289,6 +289,19 @@ pts_status pts_engine_subtree_values(pts_engine* e, uint32_t node, int player, c
 }
 
 uint64_t pts_engine_device_bytes(pts_engine* e) { return e->e->device_bytes(); }
+
+pts_status pts_engine_profile_iteration(pts_engine* e, float pos, float neg, float strat, int rw, double ms[4],
+                                        uint32_t launches[4], double* wall_ms) {
+  Profile p;
+  Status s = e->e->profile_iteration({pos, neg, strat, rw}, p);
+  if (s != kOk) return st(s);
+  for (int k = 0; k < 4; k++) {
+    ms[k] = p.ms[k];
+    launches[k] = p.launches[k];
+  }
+  if (wall_ms) *wall_ms = p.wall_ms;
+  return PTS_OK;
+}
 void pts_engine_destroy(pts_engine* e) { delete e; }
 
 // ------------------------------------------------------------------ solver
@@ -456,7 +469,20 @@ uint64_t pts_solver_storage_len(pts_solver* s) { return s->tree.storage_len; }
 int32_t pts_solver_num_hands(pts_solver* s, int p) { return (int32_t)s->hands.n(p); }
 pts_status pts_solver_get_storage(pts_solver* s, float* r, float* m) { NEED_ENGINE(s); return st(s->engine->download(r, m)); }
 pts_status pts_solver_set_storage(pts_solver* s, const float* r, const float* m) { NEED_ENGINE(s); return st(s->engine->upload(r, m)); }
-pts_engine* pts_solver_engine(pts_solver*) { return nullptr; }
+pts_status pts_solver_profile_iteration(pts_solver* s, double ms[4], uint32_t launches[4], double* wall_ms) {
+  NEED_ENGINE(s);
+  s->iterations += 1;
+  IterParams d = discount(s->iterations, s->cfg);
+  Profile p;
+  Status e = s->engine->profile_iteration(d, p);
+  if (e != kOk) return st(e);
+  for (int k = 0; k < 4; k++) {
+    ms[k] = p.ms[k];
+    launches[k] = p.launches[k];
+  }
+  if (wall_ms) *wall_ms = p.wall_ms;
+  return PTS_OK;
+}
 uint64_t pts_solver_num_nodes(pts_solver* s) { return s->tree.nodes.size(); }
 
 void pts_solver_nodes(pts_solver* s, uint8_t* kind, uint8_t* street, uint16_t* nc, uint32_t* fc, int32_t* inv, uint64_t* off,

@@ -708,40 +708,74 @@ Status Engine::read_sums(uint64_t offset, size_t len, float* out) const {
 // The root-value row of traverser p is kept past the other traversal.
 static inline size_t result_offset(int p, uint32_t rowlen) { return sizeof(IterParams) + (size_t)p * rowlen * 8; }
 
+struct ProfRec {
+  hipStream_t st;
+  std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> ev;
+  void start(int cls) {
+    hipEvent_t a, b;
+    (void)hipEventCreate(&a);
+    (void)hipEventCreate(&b);
+    (void)hipEventRecord(a, st);
+    ev.push_back({cls, {a, b}});
+  }
+  void stop() { (void)hipEventRecord(ev.back().second.second, st); }
+  ~ProfRec() {
+    for (auto& e : ev) {
+      (void)hipEventDestroy(e.second.first);
+      (void)hipEventDestroy(e.second.second);
+    }
+  }
+};
+#define PROF_START(c) \
+  if (prof) prof->start(c)
+#define PROF_STOP() \
+  if (prof) prof->stop()
+
 template <typename V, int DMODE, int UMODE>
 static void launch_span(const DPlan& d, const Plan& P, hipStream_t st, int p, Span span, uint32_t first_depth, void* reach,
                         uint32_t* alive, float* regrets, float* sums, const IterParams* prm, void* vals, uint32_t rowlen,
-                        size_t lds_down, size_t lds_term, size_t lds_up, V* qout) {
+                        size_t lds_down, size_t lds_term, size_t lds_up, V* qout, ProfRec* prof = nullptr) {
   const uint32_t D = P.depths();
   for (uint32_t dd = first_depth; dd < D; dd++) {
     uint32_t a = P.depth_off[dd], b = P.depth_off[dd + 1];
-    if (b > a)
+    if (b > a) {
+      PROF_START(kKDown);
       hipLaunchKernelGGL((down_kernel<V, DMODE>), dim3(b - a), dim3(DOWN_T), lds_down, st, d, p, a, span, (V*)reach, alive,
                          regrets, sums, prm);
+      PROF_STOP();
+    }
   }
-  if (!P.terminals.empty())
+  if (!P.terminals.empty()) {
+    PROF_START(kKTerminal);
     hipLaunchKernelGGL((terminal_kernel<V>), dim3((uint32_t)P.terminals.size()), dim3(TERM_T), lds_term, st, d, p, span,
                        (const V*)reach, (const uint32_t*)alive, (V*)vals, rowlen);
+    PROF_STOP();
+  }
   const uint32_t tiles = (P.n[p] + UP_T - 1) / UP_T;
   for (uint32_t dd = D; dd-- > first_depth;) {
     uint32_t a = P.depth_off[dd], b = P.depth_off[dd + 1];
-    if (b > a)
+    if (b > a) {
+      PROF_START(kKUp);
       hipLaunchKernelGGL((up_kernel<V, UMODE>), dim3(b - a, tiles), dim3(UP_T), lds_up, st, d, p, a, span, (V*)vals, rowlen,
                          (const uint32_t*)alive, regrets, (const float*)sums, prm, qout);
+      PROF_STOP();
+    }
   }
 }
 
-Status Engine::launch_traversal(int p, Mode mode, bool f64) {
+Status Engine::launch_traversal(int p, Mode mode, bool f64, ProfRec* prof) {
   const Plan& P = plan_;
   const DPlan& d = dp_->d;
   const int o = p ^ 1;
   Span span{0, 1, (uint32_t)P.num_nodes, 0};
   const uint32_t nq = P.n[o];
+  PROF_START(kKBegin);
   if (f64) hipLaunchKernelGGL((begin_kernel<double>), dim3((nq + 255) / 256), dim3(256), 0, stream_, nq, weights_[o], (double*)reach_, 0u, alive_, 1u);
   else hipLaunchKernelGGL((begin_kernel<float>), dim3((nq + 255) / 256), dim3(256), 0, stream_, nq, weights_[o], (float*)reach_, 0u, alive_, 1u);
+  PROF_STOP();
   if (mode == kCfr)
     launch_span<float, kCfr, kCfr>(d, P, stream_, p, span, 0, reach_, alive_, regrets_, sums_, params_, vals_, rowlen_, lds_down_,
-                                   lds_term_[0], lds_up_[0], (float*)nullptr);
+                                   lds_term_[0], lds_up_[0], (float*)nullptr, prof);
   else if (mode == kBest)
     launch_span<double, kAverage, kBest>(d, P, stream_, p, span, 0, reach_, alive_, regrets_, sums_, params_, vals_, rowlen_,
                                          lds_down_, lds_term_[1], lds_up_[1], (double*)nullptr);
@@ -793,6 +827,31 @@ Status Engine::cfr_iteration(const IterParams& dsc) {
   Status st = run_graph(0);
   if (st != kOk) return st;
   HIPCHK(hipStreamSynchronize(stream_));
+  return kOk;
+}
+
+Status Engine::profile_iteration(const IterParams& dsc, Profile& out) {
+  *params_host_ = dsc;
+  HIPCHK(hipMemcpyAsync(params_, params_host_, sizeof(IterParams), hipMemcpyHostToDevice, stream_));
+  ProfRec rec;
+  rec.st = stream_;
+  for (int p = 0; p < 2; p++) {
+    Status st = launch_traversal(p, kCfr, false, &rec);
+    if (st != kOk) return st;
+  }
+  HIPCHK(hipStreamSynchronize(stream_));
+  out = Profile();
+  for (auto& e : rec.ev) {
+    float ms = 0.0f;
+    HIPCHK(hipEventElapsedTime(&ms, e.second.first, e.second.second));
+    out.ms[e.first] += ms;
+    out.launches[e.first] += 1;
+  }
+  if (!rec.ev.empty()) {
+    float w = 0.0f;
+    HIPCHK(hipEventElapsedTime(&w, rec.ev.front().second.first, rec.ev.back().second.second));
+    out.wall_ms = w;
+  }
   return kOk;
 }
 

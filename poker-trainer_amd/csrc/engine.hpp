@@ -34,6 +34,14 @@ struct IterParams {
   int reach_weighted;
 };
 
+// Kernel classes timed by Engine::profile_iteration
+enum KernelClass : int { kKBegin = 0, kKDown = 1, kKTerminal = 2, kKUp = 3, kKClasses = 4 };
+struct Profile {
+  double ms[kKClasses] = {0, 0, 0, 0};      // summed event time per class
+  uint32_t launches[kKClasses] = {0, 0, 0, 0};
+  double wall_ms = 0;                       // first event to last
+};
+
 struct DeviceBuf {
   void* p = nullptr;
   size_t bytes = 0;
@@ -47,6 +55,9 @@ class Engine {
 
   // One DCFR iteration: player 0's update, then player 1's (crates/solver/src/lib.rs:577-585).
   Status cfr_iteration(const IterParams& d);
+  // The same iteration launched kernel by kernel with HIP events around every
+  // launch (on the engine's stream): per-class durations for roofline figures.
+  Status profile_iteration(const IterParams& d, Profile& out);
   // One traversal of player p; CFR mode updates storage. Root values of p's hands into out (n_p).
   Status traverse(int p, Mode mode, const IterParams& d, std::vector<double>& out);
   // Expected payoffs (chips) at the root under `mode` (kBest / kAverage), traversals in FP64.
@@ -69,7 +80,7 @@ class Engine {
  private:
   Engine() = default;
   Status alloc(DeviceBuf& b, size_t bytes);
-  Status launch_traversal(int p, Mode mode, bool f64);
+  Status launch_traversal(int p, Mode mode, bool f64, struct ProfRec* prof = nullptr);
   Status build_graphs();
   Status run_graph(int which);
 

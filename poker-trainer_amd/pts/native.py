@@ -60,7 +60,7 @@ EXPORTS = [
     "pts_solver_estimate", "pts_solver_new", "pts_solver_free", "pts_solver_stats", "pts_solver_solve",
     "pts_solver_iterate", "pts_solver_exploitability", "pts_solver_root_values", "pts_solver_iterations",
     "pts_solver_set_iterations", "pts_solver_storage_len", "pts_solver_num_hands", "pts_solver_get_storage",
-    "pts_solver_set_storage", "pts_solver_engine", "pts_solver_num_nodes", "pts_solver_nodes", "pts_solver_query",
+    "pts_solver_set_storage", "pts_solver_profile_iteration", "pts_engine_profile_iteration", "pts_solver_num_nodes", "pts_solver_nodes", "pts_solver_query",
     "pts_solver_ranges_after",
 ]
 
@@ -117,6 +117,9 @@ def _declare(L):
     L.pts_solver_num_hands.argtypes = [P, ctypes.c_int]
     L.pts_solver_get_storage.argtypes = [P, f32p, f32p]
     L.pts_solver_set_storage.argtypes = [P, f32p, f32p]
+    L.pts_solver_profile_iteration.argtypes = [P, f64p, ctypes.POINTER(ctypes.c_uint32), ctypes.POINTER(ctypes.c_double)]
+    L.pts_engine_profile_iteration.argtypes = [P, ctypes.c_float, ctypes.c_float, ctypes.c_float, ctypes.c_int, f64p,
+                                               ctypes.POINTER(ctypes.c_uint32), ctypes.POINTER(ctypes.c_double)]
     L.pts_solver_num_nodes.restype = ctypes.c_uint64
     L.pts_solver_num_nodes.argtypes = [P]
     L.pts_solver_nodes.argtypes = [P] + [ctypes.c_void_p] * 11

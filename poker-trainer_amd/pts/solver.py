@@ -124,6 +124,17 @@ class Solver:
         check(lib().pts_solver_root_values(self.h, mode, out))
         return out
 
+    def profile_iteration(self):
+        """The next DCFR iteration with HIP events around every kernel launch:
+        ({class: ms}, {class: launches}, wall_ms) for the classes begin, down,
+        terminal, up."""
+        ms = np.zeros(4, np.float64)
+        ln = (ctypes.c_uint32 * 4)()
+        wall = ctypes.c_double()
+        check(lib().pts_solver_profile_iteration(self.h, ms, ln, ctypes.byref(wall)))
+        names = ("begin", "down", "terminal", "up")
+        return dict(zip(names, ms.tolist())), dict(zip(names, list(ln))), wall.value
+
     def storage(self):
         n = self.storage_len
         r = np.zeros(n, np.float32)
