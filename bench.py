@@ -77,6 +77,9 @@ def main():
     ap.add_argument("--check-every", type=int, default=10)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--precision", default="fast", choices=["fast", "exact"],
+                    help="fast: block-scan terminal sums (default, as the reference's GPU default); exact: the "
+                         "reference's summation order, bit-identical to the CPU solve")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (this build has no multi-GPU path); got --gpus {args.gpus}",
@@ -91,7 +94,8 @@ def main():
 
     # steps: W warmup iterations, then K timed ones
     t0 = time.perf_counter()
-    s = Solver(cfg)
+    prec = 1 if args.precision == "exact" else 0
+    s = Solver(cfg, precision=prec)
     build_s = time.perf_counter() - t0
     st = s.stats()
     n0, n1 = s.n
@@ -107,7 +111,7 @@ def main():
     del s
 
     # the metric: a fresh solve to the target
-    s = Solver(cfg)
+    s = Solver(cfg, precision=prec)
     hist = []
     t0 = time.perf_counter()
     pr = s.solve(SolveOptions(max_iters, args.target, args.check_every), hist)
@@ -137,7 +141,7 @@ def main():
         "vs_baseline": None, "dtype": "f32", "data": "synthetic",
         "config": {"workload": desc, "seed": args.seed, "hands": [n0, n1], "decision_nodes": st.player_nodes,
                    "terminals": st.terminal_nodes, "storage_cells": cells, "target_pct": args.target,
-                   "check_every": args.check_every, "iterations_to_target": pr.iteration,
+                   "check_every": args.check_every, "precision": args.precision, "iterations_to_target": pr.iteration,
                    "target_reached": reached, "final_exploitability_pct": round(pr.exploitability_pct, 5),
                    "iterations_per_s": round(1.0 / step_s, 2), "build_s": round(build_s, 3),
                    "parallelism": "1 GPU"},

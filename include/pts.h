@@ -40,6 +40,12 @@ typedef enum { PTS_MODE_CFR = 0, PTS_MODE_BEST_RESPONSE = 1, PTS_MODE_AVERAGE = 
 /* crates/solver/src/lib.rs:57-71 */
 typedef enum { PTS_BACKEND_CPU = 0, PTS_BACKEND_GPU = 1, PTS_BACKEND_AUTO = 2 } pts_backend;
 
+/* How terminal sums are accumulated (crates/solver/src/gpu/mod.rs:51-63):
+ * FAST adds the long strength sweeps as f64 block scans (the values differ
+ * from the CPU solve's by rare one-rounding-step f32 differences); EXACT keeps
+ * the CPU's sequential f64 order and is bit-identical to it. */
+typedef enum { PTS_PRECISION_FAST = 0, PTS_PRECISION_EXACT = 1 } pts_precision;
+
 typedef struct pts_engine pts_engine;
 typedef struct pts_solver pts_solver;
 
@@ -84,7 +90,7 @@ typedef struct {
 
 /* Gpu::new + Plan::build_with + Engine::new (gpu/mod.rs:244, plan.rs:270,
  * engine.rs:292): validates and flattens the tree, uploads it and the storage. */
-pts_status pts_engine_create(int device, const pts_tree_desc* tree, const float* regrets, const float* sums,
+pts_status pts_engine_create(int device, const pts_tree_desc* tree, const float* regrets, const float* sums, int precision,
                              pts_engine** out);
 /* Engine::cfr_iteration (engine.rs:386-392): player 0's update, then player 1's,
  * with the discounts of cfr.rs:83-107 computed by the caller. */
@@ -173,9 +179,10 @@ typedef struct { /* crates/solver/src/lib.rs:212-217: kind 0 action (act_kind 0 
 
 /* Solver::estimate (lib.rs:355-362) */
 pts_status pts_solver_estimate(const pts_config_c* cfg, pts_stats_c* out);
-/* Solver::new (lib.rs:365-396) plus the engine: the tree is built on the
- * host and lives in HBM with its storage. */
-pts_status pts_solver_new(const pts_config_c* cfg, int device, pts_solver** out);
+/* Solver::new (lib.rs:365-396) plus the engine (Solver::use_gpu,
+ * lib.rs:656-660): the tree is built on the host and lives in HBM with its
+ * storage. device -1 builds and validates on the host only. */
+pts_status pts_solver_new(const pts_config_c* cfg, int device, int precision, pts_solver** out);
 void pts_solver_free(pts_solver* s);
 void pts_solver_stats(pts_solver* s, pts_stats_c* out);
 /* Solver::solve (lib.rs:475-642). `history` (may be null) receives the

@@ -209,7 +209,8 @@ int pts_backend_wants_gpu(int backend, int street) {
   return 0;
 }
 
-pts_status pts_engine_create(int device, const pts_tree_desc* t, const float* regrets, const float* sums, pts_engine** out) {
+pts_status pts_engine_create(int device, const pts_tree_desc* t, const float* regrets, const float* sums, int precision,
+                             pts_engine** out) {
   *out = nullptr;
   if (!t || !regrets || !sums) return bad("invalid input: null argument");
   PlanInput in;
@@ -239,7 +240,7 @@ pts_status pts_engine_create(int device, const pts_tree_desc* t, const float* re
   std::string err = build_plan(in, plan);
   if (!err.empty()) return bad(err);
   std::unique_ptr<Engine> e;
-  Status s = Engine::create(device, std::move(plan), regrets, sums, e);
+  Status s = Engine::create(device, std::move(plan), regrets, sums, precision, e);
   if (s != kOk) return st(s);
   *out = new pts_engine{std::move(e)};
   return PTS_OK;
@@ -349,7 +350,7 @@ static PlanInput plan_input(pts_solver* s, std::vector<uint8_t>& kind, std::vect
   return in;
 }
 
-pts_status pts_solver_new(const pts_config_c* c, int device, pts_solver** out) {  // lib.rs:365-396
+pts_status pts_solver_new(const pts_config_c* c, int device, int precision, pts_solver** out) {  // lib.rs:365-396
   *out = nullptr;
   std::unique_ptr<pts_solver> s(new pts_solver());
   s->cfg = *c;
@@ -381,7 +382,7 @@ pts_status pts_solver_new(const pts_config_c* c, int device, pts_solver** out) {
     return PTS_OK;
   }
   std::vector<float> zeros(s->tree.storage_len, 0.0f);
-  Status e = Engine::create(device, std::move(plan), zeros.data(), zeros.data(), s->engine);
+  Status e = Engine::create(device, std::move(plan), zeros.data(), zeros.data(), precision, s->engine);
   if (e != kOk) return st(e);
   *out = s.release();
   return PTS_OK;

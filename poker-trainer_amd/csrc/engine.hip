@@ -40,29 +40,29 @@ constexpr uint16_t NONE16 = 0xFFFF;
 constexpr int MAXA = 8;       // actions per decision node (checked at create)
 constexpr int TERM_T = 64;    // terminal workgroup: one wave
 constexpr int UP_T = 64;      // up workgroup: one wave, one hand per lane
-constexpr int DOWN_T = 256;   // down workgroup
+constexpr int DOWN_T = 512;   // down workgroup
 constexpr int MAX_K = 21;     // ceil(1326 / 64): hands per lane in the terminal kernel
 
 // Device view of the plan (all pointers into one HBM blob).
 struct DPlan {
-  uint32_t num_nodes;
+  uint32_t num_nodes, num_segs;
   uint32_t n[2];
-  uint32_t max_entries;
+  uint32_t max_entries, max_members;
   double half_pot, norm;
-  const uint8_t *kind, *na, *tflags;
-  const uint32_t *first_child, *vslot, *tboard, *rslot;
-  const uint64_t* offset;
-  const int32_t* invest;
-  const float *rake, *factor;
   const int8_t* card;
-  const uint16_t *local, *alive_src;
-  const uint32_t *seg_root, *seg_parent, *seg_mem_off, *seg_mem_cnt, *members, *terminals;
+  const uint32_t *seg_mem_off, *seg_mem_cnt, *seg_pslot;
+  const float* seg_factor;
+  const MemRec* mrec;
+  const TermRec* trec;
+  const MemRec* crec;
+  const uint32_t* chance_child_vslot;
   const uint8_t* cards[2];
   const uint32_t* same[2];
   const uint32_t* bl_off[2];
   const uint16_t* bl_idx[2];
   const uint32_t *ent_off[2], *ent_cnt[2], *cl_base[2];
   const uint16_t *ent_idx[2], *cl_off[2], *cl_idx[2], *my_cnt[2], *hand_ent[2];
+  const uint4* hand_cnt[2];
 };
 
 // The nodes a traversal covers: {root} and [lo, hi) (the whole tree, or the
@@ -90,6 +90,13 @@ __device__ __forceinline__ uint32_t block_or(uint32_t bits, uint32_t* red) {
   return *red;
 }
 
+// Copies a segment's member records into LDS (all threads of the block).
+__device__ __forceinline__ void stage_records(const MemRec* __restrict__ g, uint32_t cnt, MemRec* lds) {
+  const uint4* src = (const uint4*)g;
+  uint4* dst = (uint4*)lds;
+  for (uint32_t i = threadIdx.x; i < cnt * 3; i += blockDim.x) dst[i] = src[i];
+}
+
 // ------------------------------------------------------------------ begin
 template <typename V>
 __global__ void begin_kernel(uint32_t nq, const float* __restrict__ w, V* __restrict__ reach, uint32_t slot,
@@ -105,31 +112,42 @@ __global__ void begin_kernel(uint32_t nq, const float* __restrict__ w, V* __rest
 // the strategy-sum update (cfr.rs:156-207,389-419; river.rs:116-160).
 // Otherwise the normalized average strategy (cfr.rs:124-154).
 template <typename V, int MODE>
-__global__ void __launch_bounds__(DOWN_T) down_kernel(DPlan P, int p, uint32_t seg0, Span span, V* __restrict__ reach,
+__global__ void __launch_bounds__(1024) down_kernel(DPlan P, int p, uint32_t seg0, Span span, V* __restrict__ reach,
                                                       uint32_t* __restrict__ alive, const float* __restrict__ regrets,
                                                       float* __restrict__ sums, const IterParams* __restrict__ prm) {
-  extern __shared__ uint8_t m_alive[];
+  extern __shared__ uint4 smem_down[];
   __shared__ uint32_t red;
   const uint32_t seg = seg0 + blockIdx.x;
   const int o = p ^ 1;
-  const uint32_t nq = P.n[o], N = P.num_nodes, tid = threadIdx.x;
-  const uint32_t* rs = P.rslot + (size_t)p * N;
-  const uint16_t* src = P.alive_src + (size_t)p * N;
-  const uint32_t root = P.seg_root[seg];
+  const uint32_t nq = P.n[o], tid = threadIdx.x;
+  const uint32_t moff = P.seg_mem_off[seg], mcnt = P.seg_mem_cnt[seg];
+  const uint32_t root = P.mrec[moff].node;
   if (!span.has(root) && seg != span.seg) return;
+  MemRec* rec = (MemRec*)smem_down;
+  uint8_t* m_alive = (uint8_t*)(rec + P.max_members);
+  stage_records(P.mrec + moff, mcnt, rec);
+  __syncthreads();
   if (seg == span.seg) {
-    if (tid == 0) m_alive[src[span.root]] = (uint8_t)(alive[rs[span.root]] != 0);
+    if (tid == 0) {
+      // the span root's row was written by begin_kernel
+      const uint32_t l = span.root == root ? 0u : 0xFFFFu;
+      uint32_t rl = l;
+      if (l != 0)
+        for (uint32_t m = 0; m < mcnt; m++)
+          if (rec[m].node == span.root) { rl = m; break; }
+      m_alive[rec[rl].src[p]] = (uint8_t)(alive[rec[rl].slot[p]] != 0);
+    }
   } else {
     // Chance child: the parent's reach times the chance factor, opponent
     // hands holding the dealt card zeroed (cfr.rs:449-456).
-    const uint32_t par = P.seg_parent[seg], sp = rs[par], sm = rs[root];
+    const uint32_t sp = P.seg_pslot[(size_t)p * P.num_segs + seg], sm = rec[0].slot[p];
     uint32_t any = 0;
     if (alive[sp]) {
-      const V f = (V)P.factor[par];
-      const int c = P.card[root];
+      const V f = (V)P.seg_factor[seg];
+      const int c = rec[0].card;
       const V* pr = reach + (size_t)sp * nq;
       V* mr = reach + (size_t)sm * nq;
-      for (uint32_t h = tid; h < nq; h += DOWN_T) {
+      for (uint32_t h = tid; h < nq; h += blockDim.x) {
         V v = pr[h] * f;
         if (holds(P.cards[o], h, c)) v = V(0);
         mr[h] = v;
@@ -143,7 +161,6 @@ __global__ void __launch_bounds__(DOWN_T) down_kernel(DPlan P, int p, uint32_t s
     }
   }
   __syncthreads();
-  const uint32_t moff = P.seg_mem_off[seg], mcnt = P.seg_mem_cnt[seg];
   float g = 0.0f;
   bool weighted = false;
   if (MODE == kCfr) {
@@ -151,23 +168,24 @@ __global__ void __launch_bounds__(DOWN_T) down_kernel(DPlan P, int p, uint32_t s
     weighted = prm->reach_weighted != 0;
   }
   for (uint32_t m = 0; m < mcnt; m++) {
-    const uint32_t node = P.members[moff + m];
-    if ((int)P.kind[node] != o || !span.has(node)) continue;
-    const uint32_t na = P.na[node], fc = P.first_child[node];
-    if (!m_alive[src[node]]) {
+    const MemRec& r = rec[m];
+    if ((int)r.kind != o || !span.has(r.node)) continue;
+    const uint32_t na = r.na, cs = r.cslot[p], cl = r.clocal;
+    if (!m_alive[r.src[p]]) {
       // A dead node's children are dead (cfr.rs:271-274, river.rs:103-107).
       if (tid < na) {
-        alive[rs[fc + tid]] = 0;
-        m_alive[P.local[fc + tid]] = 0;
+        alive[cs + tid] = 0;
+        m_alive[cl + tid] = 0;
       }
       __syncthreads();
       continue;
     }
-    const V* rrow = reach + (size_t)rs[node] * nq;
-    const uint64_t off = P.offset[node];
+    const V* rrow = reach + (size_t)r.slot[p] * nq;
+    V* crow = reach + (size_t)cs * nq;
+    const uint64_t off = r.offset;
     uint32_t bits = 0;
-    for (uint32_t h = tid; h < nq; h += DOWN_T) {
-      const V r = rrow[h];
+    for (uint32_t h = tid; h < nq; h += blockDim.x) {
+      const V rr = rrow[h];
       V sa[MAXA];
       if (MODE == kCfr) {
         float rg[MAXA];
@@ -182,10 +200,10 @@ __global__ void __launch_bounds__(DOWN_T) down_kernel(DPlan P, int p, uint32_t s
 #pragma unroll
         for (int a = 0; a < MAXA; a++)
           if (a < (int)na) {
-            const float s = inv == 0.0f ? uni : pos0(rg[a]) * inv;
+            const float sv = inv == 0.0f ? uni : pos0(rg[a]) * inv;
             float* x = sums + off + (size_t)a * nq + h;
-            *x = weighted ? *x * g + s * (float)r : *x * g + s;
-            sa[a] = (V)s;
+            *x = weighted ? *x * g + sv * (float)rr : *x * g + sv;
+            sa[a] = (V)sv;
           }
       } else {
         V sv[MAXA];
@@ -204,16 +222,16 @@ __global__ void __launch_bounds__(DOWN_T) down_kernel(DPlan P, int p, uint32_t s
 #pragma unroll
       for (int a = 0; a < MAXA; a++)
         if (a < (int)na) {
-          const V v = sa[a] * r;
-          reach[(size_t)rs[fc + a] * nq + h] = v;
+          const V v = sa[a] * rr;
+          crow[(size_t)a * nq + h] = v;
           if (v != V(0)) bits |= 1u << a;
         }
     }
     bits = block_or(bits, &red);
     if (tid < na) {
       const uint32_t b = (bits >> tid) & 1u;
-      alive[rs[fc + tid]] = b;
-      m_alive[P.local[fc + tid]] = (uint8_t)b;
+      alive[cs + tid] = b;
+      m_alive[cl + tid] = (uint8_t)b;
     }
     __syncthreads();
   }
@@ -228,71 +246,156 @@ __global__ void __launch_bounds__(DOWN_T) down_kernel(DPlan P, int p, uint32_t s
 //   showdowns (eval.rs:69-148,214-344): ascending and descending strength
 //     sweeps, kept as prefix sums so every hand reads the running sums the
 //     reference's sweep would hold at its strength.
+// The values a chain reads are first gathered into LDS in chain order by the
+// whole wave; each chain then runs over consecutive LDS words, in place.
+// Sequential f64 chains over x[a..b), 16 values loaded ahead of the
+// dependent adds (LDS latency overlaps the add chain).
+template <bool STORE, bool FWD>
+__device__ __forceinline__ double chain(double* x, uint32_t a, uint32_t b) {
+  double acc = 0.0;
+  const uint32_t n = b - a;
+  uint32_t i = 0;
+  auto at = [&](uint32_t k) -> double* { return FWD ? x + a + k : x + b - 1 - k; };
+  if (n >= 16) {
+    double cur[16];
+#pragma unroll
+    for (int k = 0; k < 16; k++) cur[k] = *at(k);
+    for (; i + 32 <= n; i += 16) {
+      double nxt[16];
+#pragma unroll
+      for (int k = 0; k < 16; k++) nxt[k] = *at(i + 16 + k);
+#pragma unroll
+      for (int k = 0; k < 16; k++) {
+        acc += cur[k];
+        if (STORE) *at(i + k) = acc;
+      }
+#pragma unroll
+      for (int k = 0; k < 16; k++) cur[k] = nxt[k];
+    }
+#pragma unroll
+    for (int k = 0; k < 16; k++) {
+      acc += cur[k];
+      if (STORE) *at(i + k) = acc;
+    }
+    i += 16;
+  }
+  for (; i < n; i++) {
+    acc += *at(i);
+    if (STORE) *at(i) = acc;
+  }
+  return acc;
+}
+__device__ __forceinline__ void chain_fwd(double* x, uint32_t a, uint32_t b) { chain<true, true>(x, a, b); }
+__device__ __forceinline__ void chain_bwd(double* x, uint32_t a, uint32_t b) { chain<true, false>(x, a, b); }
+__device__ __forceinline__ double chain_sum(double* x, uint32_t a, uint32_t b) { return chain<false, true>(x, a, b); }
+
+// Gathers x[j] = (double)rv[idx[j]] for j < n, 8 index loads in flight per lane.
+template <typename V>
+__device__ __forceinline__ void gather(double* x, const V* rv, const uint16_t* __restrict__ idx, uint32_t n) {
+  if (n == 0) return;
+  for (uint32_t j0 = 0; j0 < n; j0 += TERM_T * 8) {
+    uint32_t ix[8];
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+      const uint32_t j = min(j0 + k * TERM_T + threadIdx.x, n - 1);
+      ix[k] = idx[j];
+    }
+    double v[8];
+#pragma unroll
+    for (int k = 0; k < 8; k++) v[k] = (double)rv[ix[k]];
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+      const uint32_t j = j0 + k * TERM_T + threadIdx.x;
+      if (j < n) x[j] = v[k];
+    }
+  }
+}
+
 template <typename V>
 __global__ void __launch_bounds__(TERM_T) terminal_kernel(DPlan P, int p, Span span, const V* __restrict__ reach,
                                                           const uint32_t* __restrict__ alive, V* __restrict__ vals,
                                                           uint32_t rowlen) {
   extern __shared__ double sh[];
-  const uint32_t t = P.terminals[blockIdx.x];
-  if (!span.has(t)) return;
-  const uint32_t N = P.num_nodes;
-  const uint32_t s = P.rslot[(size_t)p * N + t];
+  __shared__ double tot[53];
+  const TermRec tr = P.trec[blockIdx.x];
+  if (!span.has(tr.node)) return;
+  const uint32_t s = tr.slot[p];
   if (!alive[s]) return;
   const int o = p ^ 1;
   const uint32_t nq = P.n[o], np = P.n[p], tid = threadIdx.x;
-  const uint32_t mmax = P.max_entries;
-  double* Pa = sh;                                  // [mmax]
-  double* Ca = sh + mmax;                           // [max(2*mmax, 53)]
-  V* rv = (V*)(Ca + (2 * mmax > 53 ? 2 * mmax : 53));  // [nq]
+  double* Pa = sh;               // [rowlen]
+  double* Ca = sh + rowlen;      // [2 * rowlen]
+  V* rv = (V*)(sh + 3 * rowlen); // [rowlen]: the opponent's reach row, hand order
   const V* R = reach + (size_t)s * nq;
-  V* out = vals + (size_t)P.vslot[t] * rowlen;
-  for (uint32_t h = tid; h < nq; h += TERM_T) rv[h] = R[h];
+  V* out = vals + (size_t)tr.vslot * rowlen;
+  for (uint32_t j0 = 0; j0 < nq; j0 += TERM_T * 8) {
+    V x[8];
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+      const uint32_t j = j0 + k * TERM_T + tid;
+      x[k] = j < nq ? R[j] : V(0);
+    }
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+      const uint32_t j = j0 + k * TERM_T + tid;
+      if (j < nq) rv[j] = x[k];
+    }
+  }
   __syncthreads();
-  const uint8_t k = P.kind[t], fl = P.tflags[t];
-  const bool batched = (fl & kTermBatched) != 0;
-  const double hp = P.half_pot, norm = P.norm, rake = (double)P.rake[t];
-  const int32_t inv_p = P.invest[2 * t + p], inv_o = P.invest[2 * t + o];
+  const bool batched = (tr.tflags & kTermBatched) != 0;
+  const double hp = P.half_pot, norm = P.norm, rake = (double)tr.rake;
+  const int32_t inv_p = tr.invest[p], inv_o = tr.invest[o];
   const uint8_t* cp = P.cards[p];
   const uint8_t* co = P.cards[o];
   const uint32_t* same_p = P.same[p];
 
-  if (k != kShowdown) {
-    const int f = k - kFold0;
+  if (tr.kind != kShowdown) {
+    const int f = tr.kind - kFold0;
     // cfr.rs:224-235
-    const double pay = (f == p ? -(hp + (double)inv_p) : hp + (double)P.invest[2 * t + f] - rake) * norm;
-    const bool paired = (fl & (p == 0 ? kTermPaired0 : kTermPaired1)) != 0;
-    if (tid <= 52) {
-      double acc = 0.0;
-      if (paired) {
-        const uint32_t b = P.tboard[t];
-        if (tid == 52) {
-          const uint32_t eo = P.ent_off[o][b], cnt = P.ent_cnt[o][b];
-          for (uint32_t j = 0; j < cnt; j++) acc += (double)rv[P.ent_idx[o][eo + j]];
-        } else {
-          const uint16_t* clo = P.cl_off[o] + (size_t)b * 53;
-          const uint16_t* ci = P.cl_idx[o] + P.cl_base[o][b];
-          for (uint32_t x = clo[tid]; x < clo[tid + 1]; x++) acc += (double)rv[ci[x]];
-        }
-      } else {
-        if (tid == 52) {
-          for (uint32_t h = 0; h < nq; h++) acc += (double)rv[h];
-        } else {
-          const uint16_t* bi = P.bl_idx[o];
-          for (uint32_t x = P.bl_off[o][tid]; x < P.bl_off[o][tid + 1]; x++) acc += (double)rv[bi[x]];
-        }
+    const double pay = (f == p ? -(hp + (double)inv_p) : hp + (double)tr.invest[f] - rake) * norm;
+    const bool paired = (tr.tflags & (p == 0 ? kTermPaired0 : kTermPaired1)) != 0;
+    if (paired) {  // strength order of the sibling showdown's board
+      const uint32_t b = tr.board;
+      const uint32_t cnt = P.ent_cnt[o][b];
+      const uint16_t* clo = P.cl_off[o] + (size_t)b * 53;
+      gather(Pa, rv, P.ent_idx[o] + P.ent_off[o][b], cnt);
+      gather(Ca, rv, P.cl_idx[o] + P.cl_base[o][b], 2 * cnt);
+      __syncthreads();
+      if (tid == 52) tot[52] = chain_sum(Pa, 0, cnt);
+      else if (tid < 52) tot[tid] = chain_sum(Ca, clo[tid], clo[tid + 1]);
+    } else {  // hand order
+      const uint32_t* bo = P.bl_off[o];
+      gather(Ca, rv, P.bl_idx[o], bo[52]);
+      __syncthreads();
+      for (uint32_t j = tid; j < nq; j += TERM_T) Pa[j] = (double)rv[j];
+      __syncthreads();
+      if (tid == 52) {
+        tot[52] = chain_sum(Pa, 0, nq);
+      } else if (tid < 52) {
+        tot[tid] = chain_sum(Ca, bo[tid], bo[tid + 1]);
       }
-      Ca[tid] = acc;
     }
     __syncthreads();
-    const double total = Ca[52];
-    for (uint32_t i = tid; i < np; i += TERM_T) {
-      const int a = cp[2 * i], b = cp[2 * i + 1];
-      const uint32_t sm = same_p[i];
-      const double sw = sm == NONE ? 0.0 : (double)rv[sm];
-      V v;
-      if (batched) v = (V)(pay * (total - Ca[a] - Ca[b] + sw));
-      else v = total == 0.0 ? V(0) : (V)(pay * (total - (Ca[a] + Ca[b]) + sw));
-      out[i] = v;
+    const double total = tot[52];
+    const uint16_t* cp2 = (const uint16_t*)cp;
+    for (uint32_t i0 = 0; i0 < np; i0 += TERM_T * 8) {
+      uint32_t cc[8], sm[8];
+#pragma unroll
+      for (int k = 0; k < 8; k++) {
+        const uint32_t i = min(i0 + k * TERM_T + tid, np - 1);
+        cc[k] = cp2[i];
+        sm[k] = same_p[i];
+      }
+#pragma unroll
+      for (int k = 0; k < 8; k++) {
+        const uint32_t i = i0 + k * TERM_T + tid;
+        const int a = cc[k] & 0xFF, b = cc[k] >> 8;
+        const double sw = sm[k] == NONE ? 0.0 : (double)rv[sm[k]];
+        V v;
+        if (batched) v = (V)(pay * (total - tot[a] - tot[b] + sw));
+        else v = total == 0.0 ? V(0) : (V)(pay * (total - (tot[a] + tot[b]) + sw));
+        if (i < np) out[i] = v;
+      }
     }
     return;
   }
@@ -301,150 +404,380 @@ __global__ void __launch_bounds__(TERM_T) terminal_kernel(DPlan P, int p, Span s
   const double win = (hp + (double)inv_o - rake) * norm;
   const double lose = -(hp + (double)inv_p) * norm;
   const double tie = 0.5 * ((double)(inv_o - inv_p) - rake) * norm;
-  const uint32_t b = P.tboard[t];
-  const uint32_t eo = P.ent_off[o][b], mo = P.ent_cnt[o][b];
-  const uint16_t* eidx = P.ent_idx[o] + eo;
+  const uint32_t b = tr.board;
+  const uint32_t mo = P.ent_cnt[o][b];
+  const uint16_t* eidx = P.ent_idx[o] + P.ent_off[o][b];
   const uint16_t* clo = P.cl_off[o] + (size_t)b * 53;
   const uint16_t* ci = P.cl_idx[o] + P.cl_base[o][b];
-  const uint32_t ep = P.ent_off[p][b];
-  const uint16_t* hent = P.hand_ent[p] + (size_t)b * np;
-  const uint16_t* c6 = P.my_cnt[p];
+  const uint4* hcnt = P.hand_cnt[p] + (size_t)b * np;
+  const uint16_t* cp2 = (const uint16_t*)cp;
 
   // ascending sweep: weaker opponent hands
-  if (tid == 52) {
-    double acc = 0.0;
-    for (uint32_t j = 0; j < mo; j++) { acc += (double)rv[eidx[j]]; Pa[j] = acc; }
-  } else if (tid < 52) {
-    double acc = 0.0;
-    for (uint32_t x = clo[tid]; x < clo[tid + 1]; x++) { acc += (double)rv[ci[x]]; Ca[x] = acc; }
-  }
+  gather(Pa, rv, eidx, mo);
+  gather(Ca, rv, ci, 2 * mo);
   __syncthreads();
+  if (tid == 52) chain_fwd(Pa, 0, mo);
+  else if (tid < 52) chain_fwd(Ca, clo[tid], clo[tid + 1]);
+  __syncthreads();
+  // per-hand counts and cards for this board, loaded once (clamped indices:
+  // no load waits behind a branch)
+  uint4 hc[MAX_K];
+  uint32_t hcards[MAX_K];
+#pragma unroll
+  for (int kk = 0; kk < MAX_K; kk++) {
+    const uint32_t i = min(tid + kk * TERM_T, np - 1);
+    hc[kk] = hcnt[i];
+    hcards[kk] = cp2[i];
+  }
   double W[MAX_K];
 #pragma unroll
   for (int kk = 0; kk < MAX_K; kk++) {
-    W[kk] = 0.0;
-    const uint32_t i = tid + kk * TERM_T;
-    if (i < np) {
-      const uint16_t e = hent[i];
-      if (e != NONE16) {
-        const uint16_t* c = c6 + (size_t)(ep + e) * 6;
-        const int c1 = cp[2 * i], c2 = cp[2 * i + 1];
-        const double Pv = c[0] ? Pa[c[0] - 1] : 0.0;
-        const double P1 = c[2] ? Ca[clo[c1] + c[2] - 1] : 0.0;
-        const double P2 = c[4] ? Ca[clo[c2] + c[4] - 1] : 0.0;
-        W[kk] = batched ? win * (Pv - P1 - P2) : win * (Pv - (P1 + P2));
-      }
-    }
+    const uint32_t lt = hc[kk].x & 0xFFFF;
+    const uint32_t l1 = hc[kk].y & 0xFFFF, l2 = hc[kk].z & 0xFFFF;
+    const int c1 = hcards[kk] & 0xFF, c2 = hcards[kk] >> 8;
+    const double Pv = (lt != 0xFFFF && lt) ? Pa[lt - 1] : 0.0;
+    const double P1 = l1 ? Ca[clo[c1] + l1 - 1] : 0.0;
+    const double P2 = l2 ? Ca[clo[c2] + l2 - 1] : 0.0;
+    W[kk] = batched ? win * (Pv - P1 - P2) : win * (Pv - (P1 + P2));
   }
   __syncthreads();
   // descending sweep: stronger opponent hands
-  if (tid == 52) {
-    double acc = 0.0;
-    for (uint32_t j = mo; j-- > 0;) { acc += (double)rv[eidx[j]]; Pa[j] = acc; }
-  } else if (tid < 52) {
-    double acc = 0.0;
-    for (uint32_t x = clo[tid + 1]; x-- > clo[tid];) { acc += (double)rv[ci[x]]; Ca[x] = acc; }
-  }
+  gather(Pa, rv, eidx, mo);
+  gather(Ca, rv, ci, 2 * mo);
+  __syncthreads();
+  if (tid == 52) chain_bwd(Pa, 0, mo);
+  else if (tid < 52) chain_bwd(Ca, clo[tid], clo[tid + 1]);
   __syncthreads();
 #pragma unroll
   for (int kk = 0; kk < MAX_K; kk++) {
     const uint32_t i = tid + kk * TERM_T;
-    if (i < np) {
-      V v = V(0);
-      const uint16_t e = hent[i];
-      if (e != NONE16) {
-        const uint16_t* c = c6 + (size_t)(ep + e) * 6;
-        const int c1 = cp[2 * i], c2 = cp[2 * i + 1];
-        const uint32_t len1 = clo[c1 + 1] - clo[c1], len2 = clo[c2 + 1] - clo[c2];
-        const double Sv = c[1] < mo ? Pa[c[1]] : 0.0;
-        const double S1 = c[3] < len1 ? Ca[clo[c1] + c[3]] : 0.0;
-        const double S2 = c[5] < len2 ? Ca[clo[c2] + c[5]] : 0.0;
-        if (batched) {
-          const double r = W[kk] + lose * (Sv - S1 - S2);
-          v = (V)r;
-        } else {
-          v = (V)W[kk];
-          v = v + (V)(lose * (Sv - (S1 + S2)));
-        }
-        if (tie != 0.0) {  // eval.rs:347-379
-          double es = 0.0, m1 = 0.0, m2 = 0.0;
-          for (uint32_t j = c[0]; j < c[1]; j++) {
-            const uint32_t idx = eidx[j];
-            const double w = (double)rv[idx];
-            es += w;
-            if (holds(co, idx, c1)) m1 += w;
-            if (holds(co, idx, c2)) m2 += w;
-          }
-          const uint32_t sm = same_p[i];
-          const double sw = sm == NONE ? 0.0 : (double)rv[sm];
-          const double eq = es - (m1 + m2) + sw;
-          v = v + (V)(tie * eq);
-        }
+    const uint32_t lt = hc[kk].x & 0xFFFF, le = hc[kk].x >> 16;
+    const uint32_t m1 = hc[kk].y >> 16, m2 = hc[kk].z >> 16;
+    const int c1 = hcards[kk] & 0xFF, c2 = hcards[kk] >> 8;
+    V v = V(0);
+    if (lt != 0xFFFF) {
+      const uint32_t len1 = clo[c1 + 1] - clo[c1], len2 = clo[c2 + 1] - clo[c2];
+      const double Sv = le < mo ? Pa[le] : 0.0;
+      const double S1 = m1 < len1 ? Ca[clo[c1] + m1] : 0.0;
+      const double S2 = m2 < len2 ? Ca[clo[c2] + m2] : 0.0;
+      if (batched) {
+        const double r = W[kk] + lose * (Sv - S1 - S2);
+        v = (V)r;
+      } else {
+        v = (V)W[kk];
+        v = v + (V)(lose * (Sv - (S1 + S2)));
       }
+      if (tie != 0.0 && i < np) {  // eval.rs:347-379
+        double es = 0.0, e1 = 0.0, e2 = 0.0;
+        for (uint32_t j = lt; j < le; j++) {
+          const uint32_t idx = eidx[j];
+          const double w = (double)rv[idx];
+          es += w;
+          if (holds(co, idx, c1)) e1 += w;
+          if (holds(co, idx, c2)) e2 += w;
+        }
+        const uint32_t sm = same_p[i];
+        const double sw = sm == NONE ? 0.0 : (double)rv[sm];
+        const double eq = es - (e1 + e2) + sw;
+        v = v + (V)(tie * eq);
+      }
+    }
+    if (i < np) out[i] = v;
+  }
+}
+
+// ------------------------------------------------------------------ fast terminals
+// The same terminal values with the two long strength sweeps (and the fold
+// totals) computed as 256-thread block scans in f64 instead of one lane's
+// sequential chain: the sums differ from the reference's only in summation
+// order (a few f64 ulps), so an f32 value occasionally differs by one
+// rounding step, as the reference's own `Fast` precision does
+// (crates/solver/src/gpu/mod.rs:51-63). The per-card sums (at most 49
+// terms) stay sequential.
+constexpr int FT = 256;  // fast terminal workgroup
+
+// Inclusive scan of x[0..n) in place (fwd) or suffix sums (bwd), block-wide.
+template <bool FWD>
+__device__ __forceinline__ void block_scan(double* x, uint32_t n, double* wsum) {
+  const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const uint32_t E = (n + FT - 1) / FT;
+  const uint32_t b = tid * E;
+  auto at = [&](uint32_t k) -> double& { return FWD ? x[k] : x[n - 1 - k]; };
+  double loc = 0.0;
+  for (uint32_t k = 0; k < E; k++)
+    if (b + k < n) loc += at(b + k);
+  // wave-inclusive scan of thread totals
+  double inc = loc;
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const double y = __shfl_up(inc, d, 64);
+    if ((int)lane >= d) inc += y;
+  }
+  if (lane == 63) wsum[wave] = inc;
+  __syncthreads();
+  double off = 0.0;
+  for (uint32_t w = 0; w < wave; w++) off += wsum[w];
+  double ex = __shfl_up(inc, 1, 64);
+  if (lane == 0) ex = 0.0;
+  double acc = off + ex;
+  for (uint32_t k = 0; k < E; k++)
+    if (b + k < n) {
+      acc += at(b + k);
+      at(b + k) = acc;
+    }
+  __syncthreads();
+}
+
+__device__ __forceinline__ double block_sum(const double* x, uint32_t n, double* wsum) {
+  const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  double s = 0.0;
+  for (uint32_t k = tid; k < n; k += FT) s += x[k];
+#pragma unroll
+  for (int d = 32; d > 0; d >>= 1) s += __shfl_xor(s, d, 64);
+  if (lane == 0) wsum[wave] = s;
+  __syncthreads();
+  double t = 0.0;
+  for (uint32_t w = 0; w < FT / 64; w++) t += wsum[w];
+  __syncthreads();
+  return t;
+}
+
+template <typename V>
+__device__ __forceinline__ void gather_b(double* x, const V* rv, const uint16_t* __restrict__ idx, uint32_t n) {
+  for (uint32_t j = threadIdx.x; j < n; j += FT) x[j] = (double)rv[idx[j]];
+}
+
+template <typename V>
+__global__ void __launch_bounds__(FT) terminal_fast_kernel(DPlan P, int p, Span span, const V* __restrict__ reach,
+                                                           const uint32_t* __restrict__ alive, V* __restrict__ vals,
+                                                           uint32_t rowlen) {
+  extern __shared__ double sh[];
+  __shared__ double tot[53];
+  __shared__ double wsum[FT / 64];
+  const TermRec tr = P.trec[blockIdx.x];
+  if (!span.has(tr.node)) return;
+  const uint32_t s = tr.slot[p];
+  if (!alive[s]) return;
+  const int o = p ^ 1;
+  const uint32_t nq = P.n[o], np = P.n[p], tid = threadIdx.x;
+  double* Pa = sh;                   // [rowlen] ascending prefix
+  double* Sa = sh + rowlen;          // [rowlen] descending suffix
+  double* Ca = sh + 2 * rowlen;      // [2 rowlen] per-card ascending
+  double* Cd = sh + 4 * rowlen;      // [2 rowlen] per-card descending
+  V* rv = (V*)(sh + 6 * rowlen);     // [rowlen]
+  const V* R = reach + (size_t)s * nq;
+  V* out = vals + (size_t)tr.vslot * rowlen;
+  for (uint32_t j = tid; j < nq; j += FT) rv[j] = R[j];
+  __syncthreads();
+  const bool batched = (tr.tflags & kTermBatched) != 0;
+  const double hp = P.half_pot, norm = P.norm, rake = (double)tr.rake;
+  const int32_t inv_p = tr.invest[p], inv_o = tr.invest[o];
+  const uint8_t* cp = P.cards[p];
+  const uint8_t* co = P.cards[o];
+  const uint32_t* same_p = P.same[p];
+
+  if (tr.kind != kShowdown) {
+    const int f = tr.kind - kFold0;
+    const double pay = (f == p ? -(hp + (double)inv_p) : hp + (double)tr.invest[f] - rake) * norm;
+    const bool paired = (tr.tflags & (p == 0 ? kTermPaired0 : kTermPaired1)) != 0;
+    uint32_t cnt;
+    if (paired) {
+      const uint32_t b = tr.board;
+      cnt = P.ent_cnt[o][b];
+      const uint16_t* clo = P.cl_off[o] + (size_t)b * 53;
+      gather_b(Pa, rv, P.ent_idx[o] + P.ent_off[o][b], cnt);
+      gather_b(Ca, rv, P.cl_idx[o] + P.cl_base[o][b], 2 * cnt);
+      __syncthreads();
+      if (tid < 52) tot[tid] = chain_sum(Ca, clo[tid], clo[tid + 1]);
+    } else {
+      cnt = nq;
+      const uint32_t* bo = P.bl_off[o];
+      for (uint32_t j = tid; j < nq; j += FT) Pa[j] = (double)rv[j];
+      gather_b(Ca, rv, P.bl_idx[o], bo[52]);
+      __syncthreads();
+      if (tid < 52) tot[tid] = chain_sum(Ca, bo[tid], bo[tid + 1]);
+    }
+    const double total = block_sum(Pa, cnt, wsum);
+    for (uint32_t i = tid; i < np; i += FT) {
+      const int a = cp[2 * i], b = cp[2 * i + 1];
+      const uint32_t sm = same_p[i];
+      const double sw = sm == NONE ? 0.0 : (double)rv[sm];
+      V v;
+      if (batched) v = (V)(pay * (total - tot[a] - tot[b] + sw));
+      else v = total == 0.0 ? V(0) : (V)(pay * (total - (tot[a] + tot[b]) + sw));
       out[i] = v;
     }
+    return;
   }
+
+  const double win = (hp + (double)inv_o - rake) * norm;
+  const double lose = -(hp + (double)inv_p) * norm;
+  const double tie = 0.5 * ((double)(inv_o - inv_p) - rake) * norm;
+  const uint32_t b = tr.board;
+  const uint32_t mo = P.ent_cnt[o][b];
+  const uint16_t* eidx = P.ent_idx[o] + P.ent_off[o][b];
+  const uint16_t* clo = P.cl_off[o] + (size_t)b * 53;
+  const uint16_t* ci = P.cl_idx[o] + P.cl_base[o][b];
+  const uint4* hcnt = P.hand_cnt[p] + (size_t)b * np;
+  gather_b(Pa, rv, eidx, mo);
+  for (uint32_t j = tid; j < mo; j += FT) Sa[j] = Pa[j];
+  gather_b(Ca, rv, ci, 2 * mo);
+  for (uint32_t j = tid; j < 2 * mo; j += FT) Cd[j] = Ca[j];
+  __syncthreads();
+  if (tid < 52) chain_fwd(Ca, clo[tid], clo[tid + 1]);
+  else if (tid >= 64 && tid < 116) chain_bwd(Cd, clo[tid - 64], clo[tid - 63]);
+  block_scan<true>(Pa, mo, wsum);
+  block_scan<false>(Sa, mo, wsum);
+  for (uint32_t i = tid; i < np; i += FT) {
+    const uint4 hc = hcnt[i];
+    const uint32_t lt = hc.x & 0xFFFF, le = hc.x >> 16;
+    V v = V(0);
+    if (lt != 0xFFFF) {
+      const int c1 = cp[2 * i], c2 = cp[2 * i + 1];
+      const uint32_t l1 = hc.y & 0xFFFF, m1 = hc.y >> 16, l2 = hc.z & 0xFFFF, m2 = hc.z >> 16;
+      const uint32_t len1 = clo[c1 + 1] - clo[c1], len2 = clo[c2 + 1] - clo[c2];
+      const double Pv = lt ? Pa[lt - 1] : 0.0;
+      const double P1 = l1 ? Ca[clo[c1] + l1 - 1] : 0.0;
+      const double P2 = l2 ? Ca[clo[c2] + l2 - 1] : 0.0;
+      const double Sv = le < mo ? Sa[le] : 0.0;
+      const double S1 = m1 < len1 ? Cd[clo[c1] + m1] : 0.0;
+      const double S2 = m2 < len2 ? Cd[clo[c2] + m2] : 0.0;
+      if (batched) {
+        const double r = win * (Pv - P1 - P2) + lose * (Sv - S1 - S2);
+        v = (V)r;
+      } else {
+        v = (V)(win * (Pv - (P1 + P2)));
+        v = v + (V)(lose * (Sv - (S1 + S2)));
+      }
+      if (tie != 0.0) {
+        double es = 0.0, e1 = 0.0, e2 = 0.0;
+        for (uint32_t j = lt; j < le; j++) {
+          const uint32_t idx = eidx[j];
+          const double w = (double)rv[idx];
+          es += w;
+          if (holds(co, idx, c1)) e1 += w;
+          if (holds(co, idx, c2)) e2 += w;
+        }
+        const uint32_t sm = same_p[i];
+        const double sw = sm == NONE ? 0.0 : (double)rv[sm];
+        v = v + (V)(tie * (es - (e1 + e2) + sw));
+      }
+    }
+    out[i] = v;
+  }
+}
+
+// ------------------------------------------------------------------ chance
+// Chance nodes of one segment depth, after the deeper segments' up pass:
+// the f64 sum over dealt cards of the child rows, zero for own hands holding
+// the card (cfr.rs:457-478); dead nodes get a zero row. One lane per hand,
+// 16 child rows in flight per lane.
+template <typename V>
+__global__ void __launch_bounds__(UP_T) chance_kernel(DPlan P, int p, uint32_t c0, Span span, V* __restrict__ vals,
+                                                      uint32_t rowlen, const uint32_t* __restrict__ alive) {
+  const uint32_t ci = c0 + blockIdx.x;
+  const MemRec& r = P.crec[ci];
+  const uint32_t np = P.n[p];
+  const uint32_t h = blockIdx.y * UP_T + threadIdx.x;
+  if (!span.has(r.node) || h >= np) return;
+  V* out = vals + (size_t)r.vslot * rowlen + h;
+  if (!alive[r.slot[p]]) {
+    *out = V(0);
+    return;
+  }
+  const uint32_t na = r.na;
+  const V* row = vals + (size_t)P.chance_child_vslot[ci] * rowlen + h;
+  const int8_t* cards = P.card + r.offset;
+  const int c1 = P.cards[p][2 * h], c2 = P.cards[p][2 * h + 1];
+  double acc = 0.0;
+  uint32_t a = 0;
+  for (; a + 16 <= na; a += 16) {
+    V x[16];
+#pragma unroll
+    for (int k = 0; k < 16; k++) x[k] = row[(size_t)(a + k) * rowlen];
+#pragma unroll
+    for (int k = 0; k < 16; k++) {
+      const int c = cards[a + k];
+      acc += (c == c1 || c == c2) ? 0.0 : (double)x[k];
+    }
+  }
+  for (; a < na; a++) {
+    const int c = cards[a];
+    const V x = row[(size_t)a * rowlen];
+    acc += (c == c1 || c == c2) ? 0.0 : (double)x;
+  }
+  *out = (V)acc;
 }
 
 // ------------------------------------------------------------------ up
 // One wave per (segment, 64-hand tile), one hand per lane: values pulled up
-// the segment's members in reverse breadth-first order (no barriers: every
-// lane owns its hand). Own nodes: cfr.rs:315-382 (regret update in CFR
-// mode); opponent nodes: sum of the live children (cfr.rs:421-432,
-// river.rs:264-271); chance nodes: f64 sum over dealt cards, own hands
-// holding the card skipped (cfr.rs:457-478).
+// the segment's members in reverse breadth-first order (no barriers after
+// staging: every lane owns its hand). Own nodes: cfr.rs:315-382 (regret
+// update in CFR mode); opponent nodes: sum of the children, dead ones adding
+// an exact zero (cfr.rs:421-432, river.rs:264-271); chance nodes: f64 sum
+// over dealt cards, zero for own hands holding the card (cfr.rs:457-478).
+// Every segment root's row is written (zeros when dead), so a chance node
+// reads its children's rows unconditionally.
 template <typename V, int MODE>
 __global__ void __launch_bounds__(UP_T) up_kernel(DPlan P, int p, uint32_t seg0, Span span, V* __restrict__ vals,
                                                   uint32_t rowlen, const uint32_t* __restrict__ alive,
                                                   float* __restrict__ regrets, const float* __restrict__ sums,
                                                   const IterParams* __restrict__ prm, V* __restrict__ qout) {
-  extern __shared__ unsigned char lds_raw[];
-  V* lv = (V*)lds_raw;
+  extern __shared__ uint4 smem_up[];
   const uint32_t seg = seg0 + blockIdx.x;
-  const uint32_t np = P.n[p], N = P.num_nodes, tid = threadIdx.x;
+  const uint32_t np = P.n[p], tid = threadIdx.x;
   const uint32_t h = blockIdx.y * UP_T + tid;
-  const uint32_t root = P.seg_root[seg];
-  if (!span.has(root) && seg != span.seg) return;
-  if (h >= np) return;
-  const uint32_t* rs = P.rslot + (size_t)p * N;
   const uint32_t moff = P.seg_mem_off[seg], mcnt = P.seg_mem_cnt[seg];
-  const uint8_t* cp = P.cards[p];
+  const uint32_t root = P.mrec[moff].node;
+  if (!span.has(root) && seg != span.seg) return;
+  MemRec* rec = (MemRec*)smem_up;
+  uint8_t* live = (uint8_t*)(rec + P.max_members);
+  V* lv = (V*)(live + ((P.max_members + 15) & ~15u));
+  stage_records(P.mrec + moff, mcnt, rec);
+  __syncthreads();
+  for (uint32_t m = tid; m < mcnt; m += UP_T) live[m] = alive[rec[m].slot[p]] != 0;
+  __syncthreads();
+  if (h >= np) return;
+  // leaves first: terminal and chance values, 8 rows in flight per lane
+  for (uint32_t m0 = 0; m0 < mcnt; m0 += 8) {
+    V x[8];
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+      const uint32_t m = m0 + k;
+      x[k] = V(0);
+      if (m < mcnt && rec[m].kind >= kChance && live[m] && span.has(rec[m].node))
+        x[k] = vals[(size_t)rec[m].vslot * rowlen + h];
+    }
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+      const uint32_t m = m0 + k;
+      if (m < mcnt && rec[m].kind >= kChance) lv[(size_t)m * UP_T + tid] = x[k];
+    }
+  }
   float pos = 0.0f, neg = 0.0f;
   if (MODE == kCfr) {
     pos = prm->pos;
     neg = prm->neg;
   }
   for (uint32_t m = mcnt; m-- > 0;) {
-    const uint32_t node = P.members[moff + m];
-    if (!span.has(node)) continue;
-    const uint8_t k = P.kind[node];
-    const uint32_t na = P.na[node], fc = P.first_child[node];
-    if (qout != nullptr && node == span.root) {
+    const MemRec& r = rec[m];
+    if (!span.has(r.node)) continue;
+    const uint8_t k = r.kind;
+    const uint32_t na = r.na, cl = r.clocal;
+    if (qout != nullptr && r.node == span.root) {
       // query: the values of the decision node's children (lib.rs:854-863)
-      for (uint32_t a = 0; a < na; a++) qout[(size_t)a * np + h] = lv[(size_t)P.local[fc + a] * UP_T + tid];
+      for (uint32_t a = 0; a < na; a++) qout[(size_t)a * np + h] = lv[(size_t)(cl + a) * UP_T + tid];
       return;
     }
-    const bool live = alive[rs[node]] != 0;
+    if (k >= kChance) continue;  // prefetched
     V v = V(0);
-    if (k >= kFold0) {
-      if (live) v = vals[(size_t)P.vslot[node] * rowlen + h];
-    } else if (k == kChance) {
-      if (live) {
-        double acc = 0.0;
-        for (uint32_t a = 0; a < na; a++) {
-          const uint32_t c = fc + a;
-          if (holds(cp, h, P.card[c]) || !alive[rs[c]]) continue;
-          acc += (double)vals[(size_t)P.vslot[c] * rowlen + h];
-        }
-        v = (V)acc;
-      }
-    } else if ((int)k == p) {
-      if (live) {
+    if (live[m]) {
+      if ((int)k == p) {
         V cv[MAXA];
 #pragma unroll
         for (int a = 0; a < MAXA; a++)
-          if (a < (int)na) cv[a] = lv[(size_t)P.local[fc + a] * UP_T + tid];
-        const uint64_t off = P.offset[node];
+          if (a < (int)na) cv[a] = lv[(size_t)(cl + a) * UP_T + tid];
+        const uint64_t off = r.offset;
         if (MODE == kCfr) {
           float rg[MAXA];
 #pragma unroll
@@ -497,16 +830,13 @@ __global__ void __launch_bounds__(UP_T) up_kernel(DPlan P, int p, uint32_t seg0,
             if (a < (int)na) acc += (inv == V(0) ? uni : sv[a] * inv) * cv[a];
           v = acc;
         }
-      }
-    } else {
-      if (live) {
-        for (uint32_t a = 0; a < na; a++)
-          if (alive[rs[fc + a]]) v += lv[(size_t)P.local[fc + a] * UP_T + tid];
+      } else {
+        for (uint32_t a = 0; a < na; a++) v += lv[(size_t)(cl + a) * UP_T + tid];
       }
     }
     lv[(size_t)m * UP_T + tid] = v;
   }
-  vals[(size_t)P.vslot[root] * rowlen + h] = lv[tid];
+  vals[(size_t)rec[0].vslot * rowlen + h] = lv[tid];
 }
 
 // ================================================================== engine
@@ -546,8 +876,14 @@ static size_t put(std::vector<unsigned char>& blob, const std::vector<T>& v) {
   return off;
 }
 
-Status Engine::create(int device, Plan&& plan, const float* regrets, const float* sums, std::unique_ptr<Engine>& out) {
+Status Engine::create(int device, Plan&& plan, const float* regrets, const float* sums, int precision,
+                      std::unique_ptr<Engine>& out) {
   std::unique_ptr<Engine> e(new Engine());
+  if (precision != kFast && precision != kExact) {
+    set_error("invalid input: precision");
+    return kInvalidInput;
+  }
+  e->precision_ = precision;
   e->plan_ = std::move(plan);
   e->device_ = device;
   Plan& P = e->plan_;
@@ -567,6 +903,8 @@ Status Engine::create(int device, Plan&& plan, const float* regrets, const float
   }
   HIPCHK(hipSetDevice(device));
   HIPCHK(hipStreamCreateWithFlags(&e->stream_, hipStreamNonBlocking));
+  // the records are copied into LDS as 16-byte words
+  static_assert(sizeof(MemRec) % 16 == 0, "MemRec staging");
 
   // sizes and the HBM budget (crates/solver/src/gpu/engine.rs:261-315 declines with a 512 MiB margin)
   const uint32_t rowlen = std::max(P.n[0], P.n[1]);
@@ -585,14 +923,10 @@ Status Engine::create(int device, Plan&& plan, const float* regrets, const float
 
   // plan blob
   std::vector<unsigned char> blob;
-  struct Off { size_t v; };
-  size_t o_kind = put(blob, P.kind), o_na = put(blob, P.na), o_tfl = put(blob, P.tflags);
-  size_t o_fc = put(blob, P.first_child), o_vs = put(blob, P.vslot), o_tb = put(blob, P.tboard), o_rs = put(blob, P.rslot);
-  size_t o_off = put(blob, P.offset), o_inv = put(blob, P.invest), o_rake = put(blob, P.rake), o_fac = put(blob, P.factor);
-  size_t o_card = put(blob, P.card), o_loc = put(blob, P.local), o_src = put(blob, P.alive_src);
-  size_t o_sr = put(blob, P.seg_root), o_sp = put(blob, P.seg_parent), o_smo = put(blob, P.seg_mem_off);
-  size_t o_smc = put(blob, P.seg_mem_cnt), o_mem = put(blob, P.members), o_term = put(blob, P.terminals);
-  size_t o_cards[2], o_same[2], o_bo[2], o_bi[2], o_eo[2], o_ec[2], o_cb[2], o_ei[2], o_co[2], o_ci[2], o_mc[2], o_he[2], o_w[2];
+  size_t o_card = put(blob, P.card), o_smo = put(blob, P.seg_mem_off), o_smc = put(blob, P.seg_mem_cnt);
+  size_t o_sps = put(blob, P.seg_pslot), o_sf = put(blob, P.seg_factor), o_mr = put(blob, P.mrec), o_tr = put(blob, P.trec);
+  size_t o_cr = put(blob, P.crec), o_ccv = put(blob, P.chance_child_vslot);
+  size_t o_cards[2], o_same[2], o_bo[2], o_bi[2], o_eo[2], o_ec[2], o_cb[2], o_ei[2], o_co[2], o_ci[2], o_mc[2], o_he[2], o_w[2], o_hc[2];
   for (int q = 0; q < 2; q++) {
     o_cards[q] = put(blob, P.cards[q]);
     o_same[q] = put(blob, P.same[q]);
@@ -606,6 +940,7 @@ Status Engine::create(int device, Plan&& plan, const float* regrets, const float
     o_ci[q] = put(blob, P.cl_idx[q]);
     o_mc[q] = put(blob, P.my_cnt[q]);
     o_he[q] = put(blob, P.hand_ent[q]);
+    o_hc[q] = put(blob, P.hand_cnt[q]);
     o_w[q] = put(blob, P.weights[q]);
   }
   DeviceBuf b;
@@ -616,24 +951,25 @@ Status Engine::create(int device, Plan&& plan, const float* regrets, const float
   e->dp_.reset(new DPlanHolder());
   DPlan& d = e->dp_->d;
   d.num_nodes = (uint32_t)P.num_nodes;
+  d.num_segs = (uint32_t)P.seg_root.size();
   d.n[0] = P.n[0];
   d.n[1] = P.n[1];
   d.max_entries = std::max(P.max_entries, 1u);
+  d.max_members = std::max(P.max_members, 1u);
   d.half_pot = P.half_pot;
   d.norm = P.norm;
 #define AT(T, o) ((const T*)(base + (o)))
-  d.kind = AT(uint8_t, o_kind); d.na = AT(uint8_t, o_na); d.tflags = AT(uint8_t, o_tfl);
-  d.first_child = AT(uint32_t, o_fc); d.vslot = AT(uint32_t, o_vs); d.tboard = AT(uint32_t, o_tb); d.rslot = AT(uint32_t, o_rs);
-  d.offset = AT(uint64_t, o_off); d.invest = AT(int32_t, o_inv); d.rake = AT(float, o_rake); d.factor = AT(float, o_fac);
-  d.card = AT(int8_t, o_card); d.local = AT(uint16_t, o_loc); d.alive_src = AT(uint16_t, o_src);
-  d.seg_root = AT(uint32_t, o_sr); d.seg_parent = AT(uint32_t, o_sp); d.seg_mem_off = AT(uint32_t, o_smo);
-  d.seg_mem_cnt = AT(uint32_t, o_smc); d.members = AT(uint32_t, o_mem); d.terminals = AT(uint32_t, o_term);
+  d.card = AT(int8_t, o_card);
+  d.seg_mem_off = AT(uint32_t, o_smo); d.seg_mem_cnt = AT(uint32_t, o_smc); d.seg_pslot = AT(uint32_t, o_sps);
+  d.seg_factor = AT(float, o_sf); d.mrec = AT(MemRec, o_mr); d.trec = AT(TermRec, o_tr);
+  d.crec = AT(MemRec, o_cr); d.chance_child_vslot = AT(uint32_t, o_ccv);
   for (int q = 0; q < 2; q++) {
     d.cards[q] = AT(uint8_t, o_cards[q]); d.same[q] = AT(uint32_t, o_same[q]);
     d.bl_off[q] = AT(uint32_t, o_bo[q]); d.bl_idx[q] = AT(uint16_t, o_bi[q]);
     d.ent_off[q] = AT(uint32_t, o_eo[q]); d.ent_cnt[q] = AT(uint32_t, o_ec[q]); d.cl_base[q] = AT(uint32_t, o_cb[q]);
-    d.ent_idx[q] = AT(uint16_t, o_ei[q]); d.cl_off[q] = AT(uint16_t, o_co[q]); d.cl_idx[q] = AT(uint16_t, o_ci[q]); d.my_cnt[q] = AT(uint16_t, o_mc[q]);
-    d.hand_ent[q] = AT(uint16_t, o_he[q]);
+    d.ent_idx[q] = AT(uint16_t, o_ei[q]); d.cl_off[q] = AT(uint16_t, o_co[q]); d.cl_idx[q] = AT(uint16_t, o_ci[q]);
+    d.my_cnt[q] = AT(uint16_t, o_mc[q]); d.hand_ent[q] = AT(uint16_t, o_he[q]);
+    d.hand_cnt[q] = AT(uint4, o_hc[q]);
     e->weights_[q] = (float*)AT(float, o_w[q]);
   }
 #undef AT
@@ -655,12 +991,13 @@ Status Engine::create(int device, Plan&& plan, const float* regrets, const float
   if ((st = e->upload(regrets, sums)) != kOk) return st;
 
   // LDS sizes
-  e->lds_down_ = std::max<size_t>(P.max_members, 1);
+  const size_t mm = std::max<size_t>(P.max_members, 1);
+  e->lds_down_ = mm * sizeof(MemRec) + mm;
   for (int f64 = 0; f64 < 2; f64++) {
     size_t vs = f64 ? 8 : 4;
-    size_t mm = std::max<size_t>(P.max_entries, 1);
-    e->lds_term_[f64] = (mm + std::max<size_t>(2 * mm, 53)) * 8 + (size_t)rowlen * vs;
-    e->lds_up_[f64] = (size_t)P.max_members * UP_T * vs;
+    e->lds_term_[f64] = precision == kExact ? (size_t)rowlen * 3 * 8 + (size_t)rowlen * vs
+                                            : (size_t)rowlen * 6 * 8 + (size_t)rowlen * vs;
+    e->lds_up_[f64] = mm * sizeof(MemRec) + ((mm + 15) & ~size_t(15)) + mm * UP_T * vs;
   }
   size_t lds_max = std::max({e->lds_term_[1], e->lds_up_[1], e->lds_down_});
   if (lds_max > 160 * 1024) {
@@ -670,6 +1007,8 @@ Status Engine::create(int device, Plan&& plan, const float* regrets, const float
   if (lds_max > 64 * 1024) {
     HIPCHK(hipFuncSetAttribute((const void*)terminal_kernel<double>, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
     HIPCHK(hipFuncSetAttribute((const void*)terminal_kernel<float>, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
+    HIPCHK(hipFuncSetAttribute((const void*)terminal_fast_kernel<double>, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
+    HIPCHK(hipFuncSetAttribute((const void*)terminal_fast_kernel<float>, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
     HIPCHK(hipFuncSetAttribute((const void*)up_kernel<double, kBest>, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
     HIPCHK(hipFuncSetAttribute((const void*)up_kernel<double, kAverage>, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
     HIPCHK(hipFuncSetAttribute((const void*)up_kernel<float, kAverage>, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
@@ -734,25 +1073,38 @@ struct ProfRec {
 template <typename V, int DMODE, int UMODE>
 static void launch_span(const DPlan& d, const Plan& P, hipStream_t st, int p, Span span, uint32_t first_depth, void* reach,
                         uint32_t* alive, float* regrets, float* sums, const IterParams* prm, void* vals, uint32_t rowlen,
-                        size_t lds_down, size_t lds_term, size_t lds_up, V* qout, ProfRec* prof = nullptr) {
+                        size_t lds_down, size_t lds_term, size_t lds_up, V* qout, bool fast, ProfRec* prof = nullptr) {
   const uint32_t D = P.depths();
   for (uint32_t dd = first_depth; dd < D; dd++) {
     uint32_t a = P.depth_off[dd], b = P.depth_off[dd + 1];
     if (b > a) {
+      // few segments (the top of the tree): one lane per opponent hand
+      const uint32_t threads = (b - a) < 64 ? 1024 : DOWN_T;
       PROF_START(kKDown);
-      hipLaunchKernelGGL((down_kernel<V, DMODE>), dim3(b - a), dim3(DOWN_T), lds_down, st, d, p, a, span, (V*)reach, alive,
+      hipLaunchKernelGGL((down_kernel<V, DMODE>), dim3(b - a), dim3(threads), lds_down, st, d, p, a, span, (V*)reach, alive,
                          regrets, sums, prm);
       PROF_STOP();
     }
   }
   if (!P.terminals.empty()) {
     PROF_START(kKTerminal);
-    hipLaunchKernelGGL((terminal_kernel<V>), dim3((uint32_t)P.terminals.size()), dim3(TERM_T), lds_term, st, d, p, span,
-                       (const V*)reach, (const uint32_t*)alive, (V*)vals, rowlen);
+    if (fast)
+      hipLaunchKernelGGL((terminal_fast_kernel<V>), dim3((uint32_t)P.terminals.size()), dim3(FT), lds_term, st, d, p, span,
+                         (const V*)reach, (const uint32_t*)alive, (V*)vals, rowlen);
+    else
+      hipLaunchKernelGGL((terminal_kernel<V>), dim3((uint32_t)P.terminals.size()), dim3(TERM_T), lds_term, st, d, p, span,
+                         (const V*)reach, (const uint32_t*)alive, (V*)vals, rowlen);
     PROF_STOP();
   }
   const uint32_t tiles = (P.n[p] + UP_T - 1) / UP_T;
   for (uint32_t dd = D; dd-- > first_depth;) {
+    uint32_t ca = P.chance_off[dd], cb = P.chance_off[dd + 1];
+    if (cb > ca) {
+      PROF_START(kKUp);
+      hipLaunchKernelGGL((chance_kernel<V>), dim3(cb - ca, tiles), dim3(UP_T), 0, st, d, p, ca, span, (V*)vals, rowlen,
+                         (const uint32_t*)alive);
+      PROF_STOP();
+    }
     uint32_t a = P.depth_off[dd], b = P.depth_off[dd + 1];
     if (b > a) {
       PROF_START(kKUp);
@@ -775,16 +1127,16 @@ Status Engine::launch_traversal(int p, Mode mode, bool f64, ProfRec* prof) {
   PROF_STOP();
   if (mode == kCfr)
     launch_span<float, kCfr, kCfr>(d, P, stream_, p, span, 0, reach_, alive_, regrets_, sums_, params_, vals_, rowlen_, lds_down_,
-                                   lds_term_[0], lds_up_[0], (float*)nullptr, prof);
+                                   lds_term_[0], lds_up_[0], (float*)nullptr, precision_ == kFast, prof);
   else if (mode == kBest)
     launch_span<double, kAverage, kBest>(d, P, stream_, p, span, 0, reach_, alive_, regrets_, sums_, params_, vals_, rowlen_,
-                                         lds_down_, lds_term_[1], lds_up_[1], (double*)nullptr);
+                                         lds_down_, lds_term_[1], lds_up_[1], (double*)nullptr, precision_ == kFast);
   else if (f64)
     launch_span<double, kAverage, kAverage>(d, P, stream_, p, span, 0, reach_, alive_, regrets_, sums_, params_, vals_, rowlen_,
-                                            lds_down_, lds_term_[1], lds_up_[1], (double*)nullptr);
+                                            lds_down_, lds_term_[1], lds_up_[1], (double*)nullptr, precision_ == kFast);
   else
     launch_span<float, kAverage, kAverage>(d, P, stream_, p, span, 0, reach_, alive_, regrets_, sums_, params_, vals_, rowlen_,
-                                           lds_down_, lds_term_[0], lds_up_[0], (float*)nullptr);
+                                           lds_down_, lds_term_[0], lds_up_[0], (float*)nullptr, precision_ == kFast);
   // keep the root row of p
   size_t vs = f64 ? 8 : 4;
   HIPCHK(hipMemcpyAsync((char*)params_ + result_offset(p, rowlen_), (char*)vals_ + (size_t)P.vslot[0] * rowlen_ * vs,
@@ -933,7 +1285,7 @@ Status Engine::subtree_values(uint32_t node, int q, const std::vector<float>& op
   uint32_t seg = P.seg_of[node], first_depth = 0;
   while (first_depth + 1 < P.depth_off.size() && P.depth_off[first_depth + 1] <= seg) first_depth++;
   launch_span<float, kAverage, kAverage>(dp_->d, P, stream_, q, span, first_depth, reach_, alive_, regrets_, sums_, params_, vals_,
-                                         rowlen_, lds_down_, lds_term_[0], lds_up_[0], dout);
+                                         rowlen_, lds_down_, lds_term_[0], lds_up_[0], dout, precision_ == kFast);
   HIPCHK(hipGetLastError());
   child_vals.assign((size_t)na * np, 0.0f);
   HIPCHK(hipMemcpyAsync(child_vals.data(), dout, (size_t)na * np * 4, hipMemcpyDeviceToHost, stream_));

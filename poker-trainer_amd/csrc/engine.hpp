@@ -29,6 +29,9 @@ const std::string& last_error();
 // Traversal modes (crates/solver/src/cfr.rs:248-256)
 enum Mode : int { kCfr = 0, kBest = 1, kAverage = 2 };
 
+// Terminal-sum precision (crates/solver/src/gpu/mod.rs:51-63)
+enum Precision : int { kFast = 0, kExact = 1 };
+
 struct IterParams {
   float pos, neg, strat;
   int reach_weighted;
@@ -51,7 +54,11 @@ class Engine {
  public:
   ~Engine();
   // Uploads the plan and the storage; fails with kOutOfMemory when HBM is short.
-  static Status create(int device, Plan&& plan, const float* regrets, const float* sums, std::unique_ptr<Engine>& out);
+  // precision: kFast (block-scan terminal sums, f64) or kExact (the
+  // reference's sequential order, bit-identical to the CPU solve).
+  static Status create(int device, Plan&& plan, const float* regrets, const float* sums, int precision,
+                       std::unique_ptr<Engine>& out);
+  int precision() const { return precision_; }
 
   // One DCFR iteration: player 0's update, then player 1's (crates/solver/src/lib.rs:577-585).
   Status cfr_iteration(const IterParams& d);
@@ -103,6 +110,7 @@ class Engine {
   size_t lds_down_ = 0, lds_term_[2] = {0, 0}, lds_up_[2] = {0, 0};
   hipGraphExec_t graph_[3] = {nullptr, nullptr, nullptr};  // cfr iteration, best response, average
   bool use_graph_ = true;
+  int precision_ = kFast;
   struct DPlanHolder;
   std::unique_ptr<DPlanHolder> dp_;
 };

@@ -183,6 +183,12 @@ std::string build_plan(const PlanInput& in, Plan& P) {
   P.tboard.assign(N, kNone);
   uint32_t nv = 0;
   for (uint32_t r : P.seg_root) P.vslot[r] = nv++;
+  std::vector<uint32_t> chance_vslot(N, kNone);
+  for (size_t g = 0; g < P.seg_root.size(); g++)
+    for (uint32_t m = 0; m < P.seg_mem_cnt[g]; m++) {
+      uint32_t id = P.members[P.seg_mem_off[g] + m];
+      if (P.kind[id] == kChance) chance_vslot[id] = nv++;
+    }
   for (size_t i = 0; i < N; i++) {
     if (!is_terminal(P.kind[i])) continue;
     P.terminals.push_back((uint32_t)i);
@@ -206,6 +212,91 @@ std::string build_plan(const PlanInput& in, Plan& P) {
     }
   }
   P.num_vslots = nv;
+
+  // ---- member and terminal records
+  const size_t S = P.seg_root.size();
+  P.seg_pslot.assign(2 * S, kNone);
+  P.seg_factor.assign(S, 0.0f);
+  for (size_t g = 0; g < S; g++) {
+    uint32_t par = P.seg_parent[g];
+    if (par == kNone) continue;
+    P.seg_pslot[g] = P.rslot[par];
+    P.seg_pslot[S + g] = P.rslot[N + par];
+    P.seg_factor[g] = P.factor[par];
+  }
+  P.mrec.assign(P.members.size(), MemRec{});
+  for (size_t m = 0; m < P.members.size(); m++) {
+    uint32_t id = P.members[m];
+    MemRec& r = P.mrec[m];
+    r.node = id;
+    r.kind = P.kind[id];
+    r.na = P.na[id];
+    r.card = P.card[id];
+    r.tflags = P.tflags[id];
+    r.vslot = P.vslot[id];
+    for (int p = 0; p < 2; p++) {
+      r.slot[p] = P.rslot[(size_t)p * N + id];
+      r.src[p] = P.alive_src[(size_t)p * N + id];
+      r.cslot[p] = kNone;
+    }
+    if (is_terminal(r.kind)) continue;
+    uint32_t fc = P.first_child[id];
+    for (int p = 0; p < 2; p++) {
+      const uint32_t* rs = P.rslot.data() + (size_t)p * N;
+      bool fresh = r.kind == kChance || (int)r.kind == (p ^ 1);
+      if (fresh) {
+        r.cslot[p] = rs[fc];
+        for (uint32_t a = 0; a < r.na; a++)
+          if (rs[fc + a] != rs[fc] + a) return fail("child reach slots not consecutive", id);
+      }
+    }
+    if (r.kind == kChance) {
+      r.offset = fc;
+      for (uint32_t a = 0; a < r.na; a++)
+        if (P.vslot[fc + a] != P.vslot[fc] + a) return fail("chance children value rows not consecutive", id);
+      // a chance node that is itself a segment root keeps that row; the
+      // chance kernel writes the node's value there
+      if (r.vslot == kNone) r.vslot = chance_vslot[id];
+    } else {
+      r.offset = P.offset[id];
+      r.clocal = P.local[fc];
+      for (uint32_t a = 0; a < r.na; a++)
+        if (P.local[fc + a] != P.local[fc] + a || P.seg_of[fc + a] != P.seg_of[id]) return fail("children not consecutive members", id);
+    }
+  }
+  // chance nodes by segment depth
+  {
+    uint32_t D = (uint32_t)P.depth_off.size() - 1;
+    P.chance_off.assign(D + 1, 0);
+    P.crec.clear();
+    P.chance_child_vslot.clear();
+    for (uint32_t d = 0; d < D; d++) {
+      P.chance_off[d] = (uint32_t)P.crec.size();
+      for (uint32_t g = P.depth_off[d]; g < P.depth_off[d + 1]; g++)
+        for (uint32_t m = 0; m < P.seg_mem_cnt[g]; m++) {
+          const MemRec& r = P.mrec[P.seg_mem_off[g] + m];
+          if (r.kind != kChance) continue;
+          P.crec.push_back(r);
+          P.chance_child_vslot.push_back(P.vslot[r.offset]);
+        }
+    }
+    P.chance_off[D] = (uint32_t)P.crec.size();
+  }
+  P.trec.assign(P.terminals.size(), TermRec{});
+  for (size_t t = 0; t < P.terminals.size(); t++) {
+    uint32_t id = P.terminals[t];
+    TermRec& r = P.trec[t];
+    r.node = id;
+    r.slot[0] = P.rslot[id];
+    r.slot[1] = P.rslot[N + id];
+    r.vslot = P.vslot[id];
+    r.board = P.tboard[id];
+    r.invest[0] = P.invest[2 * id];
+    r.invest[1] = P.invest[2 * id + 1];
+    r.rake = P.rake[id];
+    r.kind = P.kind[id];
+    r.tflags = P.tflags[id];
+  }
 
   // ---- strength tables (crates/solver/src/hands.rs:128-159)
   uint32_t B = in.num_boards;
@@ -275,6 +366,20 @@ std::string build_plan(const PlanInput& in, Plan& P) {
         }
       }
     }
+  }
+  for (int q = 0; q < 2; q++) {
+    P.hand_cnt[q].assign((size_t)B * P.n[q] * 8, 0);
+    for (uint32_t b = 0; b < B; b++)
+      for (uint32_t h = 0; h < P.n[q]; h++) {
+        uint16_t* d = P.hand_cnt[q].data() + ((size_t)b * P.n[q] + h) * 8;
+        uint16_t e = P.hand_ent[q][(size_t)b * P.n[q] + h];
+        if (e == 0xFFFF) {
+          d[0] = 0xFFFF;
+          continue;
+        }
+        const uint16_t* c = P.my_cnt[q].data() + (size_t)(P.ent_off[q][b] + e) * 6;
+        for (int k = 0; k < 6; k++) d[k] = c[k];
+      }
   }
   return "";
 }

@@ -33,6 +33,33 @@ constexpr uint8_t kTermBatched = 1;   // inside a river subtree (river.rs batche
 constexpr uint8_t kTermPaired0 = 2;   // fold read off the sibling showdown's sums (traverser 0)
 constexpr uint8_t kTermPaired1 = 4;   // (traverser 1)
 
+// One member of a segment, everything the down / up kernels read about it
+// (staged in LDS once per workgroup). Children of a decision node are
+// consecutive members; children of a chance node are consecutive segment
+// roots, with consecutive reach slots and value rows.
+struct MemRec {
+  uint64_t offset;     // decision: storage offset; chance: first child's node id
+  uint32_t node;       // arena id
+  uint32_t vslot;      // value row (terminal / segment root); chance: first child's row
+  uint32_t slot[2];    // reach slot, per traverser
+  uint32_t cslot[2];   // first child's reach slot, per traverser (fresh slots only)
+  uint16_t clocal;     // local index of the first child (decision nodes)
+  uint16_t src[2];     // member whose slot decides this node's alive flag, per traverser
+  uint8_t kind, na;
+  int8_t card;         // chance child: the dealt card
+  uint8_t tflags;
+  uint8_t pad[6];
+};
+static_assert(sizeof(MemRec) == 48, "MemRec layout");
+
+struct TermRec {
+  uint32_t node, slot[2], vslot, board;
+  int32_t invest[2];
+  float rake;
+  uint8_t kind, tflags, pad[6];
+};
+static_assert(sizeof(TermRec) == 40, "TermRec layout");
+
 struct PlanInput {
   size_t num_nodes = 0;
   const uint8_t* kind = nullptr;
@@ -75,6 +102,15 @@ struct Plan {
   std::vector<uint32_t> depth_off;   // segments of depth d: [depth_off[d], depth_off[d+1])
   std::vector<uint32_t> members;
   std::vector<uint32_t> terminals;
+  std::vector<MemRec> mrec;          // parallel to members
+  std::vector<TermRec> trec;         // parallel to terminals
+  std::vector<uint32_t> seg_pslot;   // [2][segments]: the parent chance node's reach slot
+  std::vector<float> seg_factor;     // the parent chance node's factor
+  // chance nodes grouped by the depth of their segment: their values are
+  // summed by a wide kernel before that depth's up pass
+  std::vector<uint32_t> chance_off;  // [depth + 1]
+  std::vector<MemRec> crec;          // one per chance node (vslot: its own row; offset: first child; cslot: children)
+  std::vector<uint32_t> chance_child_vslot;  // first child's value row, parallel to crec
   uint32_t num_slots[2] = {0, 0}, num_vslots = 0, max_members = 0;
   // hands
   std::vector<uint8_t> cards[2];
@@ -94,6 +130,10 @@ struct Plan {
   // lt/le counts and per-card lt/le counts (u16 x 6), and the entry of each hand
   std::vector<uint16_t> my_cnt[2];   // 6 per entry, at ent_off
   std::vector<uint16_t> hand_ent[2]; // [b * n_q + hand] -> entry or 0xFFFF
+  // the same counts keyed by hand: [(b * n_q + hand) * 8 + k], k = lt, le,
+  // lt(c1), le(c1), lt(c2), le(c2), then two zero pads; lt = 0xFFFF for
+  // hands blocked by the board
+  std::vector<uint16_t> hand_cnt[2];
   uint32_t max_entries = 0;
   double half_pot = 0, norm = 0;
   uint64_t storage_len = 0;

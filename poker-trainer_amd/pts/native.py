@@ -87,7 +87,7 @@ def _declare(L):
     L.pts_last_error.argtypes = [ctypes.c_char_p, ctypes.c_size_t]
     L.pts_device_count.argtypes = [ctypes.POINTER(ctypes.c_int)]
     L.pts_backend_wants_gpu.argtypes = [ctypes.c_int, ctypes.c_int]
-    L.pts_engine_create.argtypes = [ctypes.c_int, ctypes.POINTER(TreeDescC), f32p, f32p, ctypes.POINTER(P)]
+    L.pts_engine_create.argtypes = [ctypes.c_int, ctypes.POINTER(TreeDescC), f32p, f32p, ctypes.c_int, ctypes.POINTER(P)]
     L.pts_engine_iterate.argtypes = [P, ctypes.c_float, ctypes.c_float, ctypes.c_float, ctypes.c_int]
     L.pts_engine_traverse.argtypes = [P, ctypes.c_int, ctypes.c_int, ctypes.c_float, ctypes.c_float, ctypes.c_float,
                                       ctypes.c_int, f64p, ctypes.c_size_t]
@@ -100,7 +100,7 @@ def _declare(L):
     L.pts_engine_device_bytes.argtypes = [P]
     L.pts_engine_destroy.argtypes = [P]
     L.pts_solver_estimate.argtypes = [ctypes.POINTER(ConfigC), ctypes.POINTER(StatsC)]
-    L.pts_solver_new.argtypes = [ctypes.POINTER(ConfigC), ctypes.c_int, ctypes.POINTER(P)]
+    L.pts_solver_new.argtypes = [ctypes.POINTER(ConfigC), ctypes.c_int, ctypes.c_int, ctypes.POINTER(P)]
     L.pts_solver_free.argtypes = [P]
     L.pts_solver_stats.argtypes = [P, ctypes.POINTER(StatsC)]
     L.pts_solver_solve.argtypes = [P, ctypes.c_uint32, ctypes.c_double, ctypes.c_uint32, ctypes.POINTER(ProgressC),
@@ -126,6 +126,9 @@ def _declare(L):
     L.pts_solver_query.argtypes = [P, ctypes.POINTER(StepC), ctypes.c_int32] + [ctypes.c_void_p] * 12 + \
         [ctypes.c_int32, ctypes.c_int32]
     L.pts_solver_ranges_after.argtypes = [P, ctypes.POINTER(StepC), ctypes.c_int32, f32p, f32p]
+
+
+FAST, EXACT = 0, 1
 
 
 def last_error() -> str:

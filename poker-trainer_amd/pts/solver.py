@@ -61,11 +61,12 @@ class NodeInfo:
 class Solver:
     """A built game whose storage lives in HBM; solves run in HIP kernels."""
 
-    def __init__(self, cfg: SolveConfig, device: int = 0):
+    def __init__(self, cfg: SolveConfig, device: int = 0, precision: int = 0):
+        """precision: 0 fast (default, as the reference's GpuPrecision), 1 exact."""
         self.cfg = cfg
         self._c = cfg.to_c()
         h = ctypes.c_void_p()
-        check(lib().pts_solver_new(ctypes.byref(self._c), device, ctypes.byref(h)))
+        check(lib().pts_solver_new(ctypes.byref(self._c), device, precision, ctypes.byref(h)))
         self.h = h
         self.n = [lib().pts_solver_num_hands(self.h, p) for p in range(2)]
 
@@ -218,11 +219,11 @@ class TreeArrays:
 class Engine:
     """pts_engine_*: the GPU backend boundary (crates/solver/src/gpu/engine.rs)."""
 
-    def __init__(self, tree: TreeArrays, regrets, sums, device: int = 0):
+    def __init__(self, tree: TreeArrays, regrets, sums, device: int = 0, precision: int = 0):
         self.tree = tree
         h = ctypes.c_void_p()
         check(lib().pts_engine_create(device, ctypes.byref(tree.desc), np.ascontiguousarray(regrets, np.float32),
-                                      np.ascontiguousarray(sums, np.float32), ctypes.byref(h)))
+                                      np.ascontiguousarray(sums, np.float32), precision, ctypes.byref(h)))
         self.h = h
         self.storage_len = tree.desc.storage_len
         self.n = [tree.desc.num_hands[0], tree.desc.num_hands[1]]

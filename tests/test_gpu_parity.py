@@ -1,13 +1,16 @@
 """GPU parity: the HIP engine against the oracle (the reference CPU solve
-restated), on the same seeded inputs. The engine repeats the reference's
-operation order, so regrets, strategy sums, values and exploitability must be
-bit-identical (compared as floats, so +0 == -0)."""
+restated), on the same seeded inputs. In exact precision the engine repeats
+the reference's operation order, so regrets, strategy sums, values and
+exploitability must be bit-identical (compared as floats, so +0 == -0). Fast
+precision (block-scan terminal sums) is checked against the same oracle with
+tolerances."""
 import numpy as np
 import pytest
 
 import pyoracle as O
 from pts import spots, PostflopMenu, make_config, parse_cards, combo_index, DcfrParams
 from pts.solver import Engine, Solver, SolveOptions
+from pts.native import EXACT, FAST
 
 from test_host_cpu import oracle_tree
 
@@ -43,7 +46,7 @@ def test_traversals_from_random_storage(name, make):
     o, t = oracle_tree(cfg)
     r, s = random_storage(o.storage_len, 11)
     o.set_storage(r, s)
-    e = Engine(t, r, s)
+    e = Engine(t, r, s, precision=EXACT)
     for p in (0, 1):
         for mode in (1, 2):  # FP64 best response and average
             g = e.traverse(p, mode)
@@ -64,7 +67,7 @@ def test_traversals_from_random_storage(name, make):
 @pytest.mark.parametrize("seed", [0, 1, 2, 3])
 def test_river_solves_bit_exact(seed):
     cfg = spots.river_spot(seed)
-    g, c = Solver(cfg), O.OracleSolver(cfg)
+    g, c = Solver(cfg, precision=EXACT), O.OracleSolver(cfg)
     g.iterate(40)
     c.iterate(40)
     rg, sg = g.storage()
@@ -76,7 +79,7 @@ def test_river_solves_bit_exact(seed):
 
 def test_turn_solve_bit_exact():
     cfg = spots.turn_spot(0)
-    g, c = Solver(cfg), O.OracleSolver(cfg)
+    g, c = Solver(cfg, precision=EXACT), O.OracleSolver(cfg)
     g.iterate(6)
     c.iterate(6)
     rg, sg = g.storage()
@@ -91,7 +94,7 @@ def test_turn_solve_bit_exact():
 
 def test_solve_loop_matches_the_oracle():
     cfg = spots.river_spot(4)
-    g, c = Solver(cfg), O.OracleSolver(cfg)
+    g, c = Solver(cfg, precision=EXACT), O.OracleSolver(cfg)
     hg = []
     pg = g.solve(SolveOptions(200, 0.5, 10), hg)
     pc, hc = c.solve(200, 0.5, 10)
@@ -103,7 +106,7 @@ def test_solve_loop_matches_the_oracle():
 
 def test_queries_match_the_oracle():
     cfg = spots.turn_spot(2)
-    g, c = Solver(cfg), O.OracleSolver(cfg)
+    g, c = Solver(cfg, precision=EXACT), O.OracleSolver(cfg)
     g.iterate(8)
     c.iterate(8)
     qg, qc = g.query([]), c.query([])
@@ -134,7 +137,7 @@ def combo(s):
 def test_polarized_toy_game_on_the_gpu():
     # crates/solver/src/tests.rs:42-88
     cfg = make_config("Kc8d5h3s2c", "AA, JTs, J9s:0.5", "KQ", 1000, 5000, PostflopMenu.one(["100%"], []), dcfr=REF)
-    s = Solver(cfg)
+    s = Solver(cfg, precision=EXACT)
     p = s.solve(SolveOptions(2000, 0.05, 25))
     assert p.exploitability_pct < 0.2
     root = s.query([])
@@ -147,7 +150,7 @@ def test_polarized_toy_game_on_the_gpu():
 def test_all_in_equity_on_the_gpu():
     # crates/solver/src/tests.rs:92-104
     cfg = make_config("2c7d9hTs", "AhAd", "KhKd", 1000, 1000, PostflopMenu.one(["allin"], []), dcfr=REF)
-    s = Solver(cfg)
+    s = Solver(cfg, precision=EXACT)
     s.solve(SolveOptions(50, 0.0, 25))
     ev = s.query([("raise", 1000)]).hand_ev(0)
     assert abs(ev[1] - (2.0 / 44.0 * 3000.0 - 1000.0)) < 0.5 and abs(ev[0]) < 1e-3
@@ -160,7 +163,7 @@ def test_ties_and_rake_on_the_gpu():
         cfg = make_config("AsKsQsJsTs", "22", "33", 1500, 5000, PostflopMenu.one(["100%"], ["3x"]), dcfr=REF,
                           rake_rate=rake, rake_cap=300.0 if rake else 0.0)
         cfg.root = BettingState(3, 1500, [0, 500], [5000, 4500], 500, 500, 1, 0, [True, False])
-        s, c = Solver(cfg), O.OracleSolver(cfg)
+        s, c = Solver(cfg, precision=EXACT), O.OracleSolver(cfg)
         s.iterate(30)
         c.iterate(30)
         rg, sg = s.storage()
@@ -169,3 +172,53 @@ def test_ties_and_rake_on_the_gpu():
         s.iterate(170)
         ev = s.query([]).hand_ev(0)
         assert abs(ev[0]) < 1e-3 and abs(ev[1] - expect) < 1.0
+
+
+# ---------------------------------------------------------------- fast precision
+@pytest.mark.parametrize("name,make", SPOTS, ids=[s[0] for s in SPOTS])
+def test_fast_traversals_close_to_the_oracle(name, make):
+    # one traversal from random storage: terminal sums in block-scan order
+    # differ from the sequential ones by f64 rounding only
+    cfg = make()
+    o, t = oracle_tree(cfg)
+    r, s = random_storage(o.storage_len, 12)
+    o.set_storage(r, s)
+    e = Engine(t, r, s, precision=FAST)
+    for p in (0, 1):
+        for mode in (1, 2):
+            g = e.traverse(p, mode)
+            c = o.traverse_root_f64(p, mode)
+            assert np.allclose(g, c, rtol=1e-12, atol=1e-15), describe(g, c)
+    d = O.discount(3)
+    for p in (0, 1):
+        g = e.traverse(p, 0, (d[0], d[1], d[2], 1))
+        c = o.traverse_root(p, 0, 3)
+        assert np.allclose(g, c, rtol=1e-5, atol=1e-9), describe(g, c)
+        # almost every storage cell is bit-identical after one update
+        rg, _ = e.download()
+        rc, _ = o.storage()
+        assert np.mean(rg == rc) > 0.999, describe(rg, rc)
+
+
+def test_fast_solve_tracks_the_exact_solve():
+    # crates/solver/src/gpu/engine/tests.rs:493-522 (fast vs exact after a
+    # converged solve): the same convergence, close strategies
+    cfg = spots.river_spot(1)
+    f, x = Solver(cfg, precision=FAST), Solver(cfg, precision=EXACT)
+    f.iterate(200)
+    x.iterate(200)
+    ef, ex = f.exploitability(), x.exploitability()
+    assert abs(ef - ex) < 1e-4 * cfg.root.pot, (ef, ex)
+    a, b = f.query([]), x.query([])
+    ra, rb = a.range_strategy(), b.range_strategy()
+    assert np.max(np.abs(ra - rb)) < 0.03
+
+
+def test_fast_turn_converges_like_the_oracle():
+    cfg = spots.turn_spot(0)
+    f, c = Solver(cfg, precision=FAST), O.OracleSolver(cfg)
+    f.iterate(30)
+    c.iterate(30)
+    ef, ec = f.exploitability(), c.exploitability_f64()
+    # exploitability within 0.01% of the pot after the same iterations
+    assert abs(ef - ec) < 1e-4 * cfg.root.pot, (ef, ec)

@@ -42,7 +42,7 @@ def test_struct_layouts_match_the_header():
 def host_solver(cfg):
     c = cfg.to_c()
     h = ctypes.c_void_p()
-    check(lib().pts_solver_new(ctypes.byref(c), -1, ctypes.byref(h)))
+    check(lib().pts_solver_new(ctypes.byref(c), -1, 1, ctypes.byref(h)))
     return h
 
 
@@ -153,7 +153,7 @@ def oracle_tree(cfg):
 def create_status(t, n):
     h = ctypes.c_void_p()
     z = np.zeros(n, np.float32)
-    st = lib().pts_engine_create(0, ctypes.byref(t.desc), z, z, ctypes.byref(h))
+    st = lib().pts_engine_create(0, ctypes.byref(t.desc), z, z, 1, ctypes.byref(h))
     if st == 0:
         lib().pts_engine_destroy(h)
     return st
