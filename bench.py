@@ -77,6 +77,9 @@ def main():
     ap.add_argument("--check-every", type=int, default=10)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-probe-mib", type=int, default=0,
+                    help="only run a plain device copy of this many MiB (5 launches) and exit: the known-size "
+                         "kernel that scales the FETCH_SIZE / WRITE_SIZE counters")
     ap.add_argument("--precision", default="fast", choices=["fast", "exact"],
                     help="fast: block-scan terminal sums (default, as the reference's GPU default); exact: the "
                          "reference's summation order, bit-identical to the CPU solve")
@@ -85,6 +88,17 @@ def main():
         print(f"bench.py measures one GPU only (this build has no multi-GPU path); got --gpus {args.gpus}",
               file=sys.stderr)
         sys.exit(2)
+
+    if args.copy_probe_mib:
+        import torch
+        n = args.copy_probe_mib * (1 << 20) // 4
+        a = torch.ones(n, dtype=torch.float32, device="cuda:0")
+        b = torch.empty_like(a)
+        for _ in range(5):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        log(f"[bench] copy probe: 5 copies of {args.copy_probe_mib} MiB")
+        return
 
     from pts import spots
     from pts.solver import Solver, SolveOptions

@@ -89,9 +89,13 @@ typedef struct {
 } pts_tree_desc;
 
 /* Gpu::new + Plan::build_with + Engine::new (gpu/mod.rs:244, plan.rs:270,
- * engine.rs:292): validates and flattens the tree, uploads it and the storage. */
+ * engine.rs:292): validates and flattens the tree, uploads it and the storage
+ * (regrets / sums may be null: zeros). scratch_budget (Engine::scratch_budget
+ * and Scratch, engine.rs:286, plan.rs:270) caps the bytes of reach and value
+ * rows; 0 uses the HBM left after the storage. The subtrees below the root
+ * street's chance nodes run in groups that fit it. */
 pts_status pts_engine_create(int device, const pts_tree_desc* tree, const float* regrets, const float* sums, int precision,
-                             pts_engine** out);
+                             uint64_t scratch_budget, pts_engine** out);
 /* Engine::cfr_iteration (engine.rs:386-392): player 0's update, then player 1's,
  * with the discounts of cfr.rs:83-107 computed by the caller. */
 pts_status pts_engine_iterate(pts_engine* e, float pos, float neg, float strat, int reach_weighted);
@@ -120,6 +124,8 @@ pts_status pts_engine_profile_iteration(pts_engine* e, float pos, float neg, flo
                                         uint32_t launches[4], double* wall_ms);
 /* Bytes of HBM the engine holds. */
 uint64_t pts_engine_device_bytes(pts_engine* e);
+/* Chunk groups one traversal runs in (f64 = 0: DCFR rows, 1: FP64 best-response rows). */
+uint32_t pts_engine_num_groups(pts_engine* e, int f64);
 void pts_engine_destroy(pts_engine* e);
 
 /* ------------------------------------------------------------------ solver */
@@ -182,7 +188,15 @@ pts_status pts_solver_estimate(const pts_config_c* cfg, pts_stats_c* out);
 /* Solver::new (lib.rs:365-396) plus the engine (Solver::use_gpu,
  * lib.rs:656-660): the tree is built on the host and lives in HBM with its
  * storage. device -1 builds and validates on the host only. */
-pts_status pts_solver_new(const pts_config_c* cfg, int device, int precision, pts_solver** out);
+pts_status pts_solver_new(const pts_config_c* cfg, int device, int precision, uint64_t scratch_budget, pts_solver** out);
+/* Chunk groups of the solver's engine (as pts_engine_num_groups). */
+uint32_t pts_solver_num_groups(pts_solver* s, int f64);
+/* Host-only solvers (device -1): the chunk groups a scratch budget gives and
+ * the scratch bytes they need, after checking that every slot and row a
+ * grouped traversal touches maps inside that scratch (Plan::build_with's
+ * Scratch sizing, plan.rs:270). */
+pts_status pts_solver_plan_groups(pts_solver* s, uint64_t scratch_budget, int f64, uint32_t* num_groups,
+                                  uint64_t* scratch_bytes);
 void pts_solver_free(pts_solver* s);
 void pts_solver_stats(pts_solver* s, pts_stats_c* out);
 /* Solver::solve (lib.rs:475-642). `history` (may be null) receives the

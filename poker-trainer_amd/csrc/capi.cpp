@@ -24,6 +24,7 @@ struct pts_solver {
   Hands hands;
   TreeCounts counts;
   std::unique_ptr<Engine> engine;
+  Plan host_plan;  // kept by host-only solvers (device -1)
   uint32_t iterations = 0;
   double exploitability = std::numeric_limits<double>::infinity();
 };
@@ -210,7 +211,7 @@ int pts_backend_wants_gpu(int backend, int street) {
 }
 
 pts_status pts_engine_create(int device, const pts_tree_desc* t, const float* regrets, const float* sums, int precision,
-                             pts_engine** out) {
+                             uint64_t scratch_budget, pts_engine** out) {
   *out = nullptr;
   if (!t || !regrets || !sums) return bad("invalid input: null argument");
   PlanInput in;
@@ -240,7 +241,7 @@ pts_status pts_engine_create(int device, const pts_tree_desc* t, const float* re
   std::string err = build_plan(in, plan);
   if (!err.empty()) return bad(err);
   std::unique_ptr<Engine> e;
-  Status s = Engine::create(device, std::move(plan), regrets, sums, precision, e);
+  Status s = Engine::create(device, std::move(plan), regrets, sums, precision, scratch_budget, e);
   if (s != kOk) return st(s);
   *out = new pts_engine{std::move(e)};
   return PTS_OK;
@@ -290,6 +291,7 @@ pts_status pts_engine_subtree_values(pts_engine* e, uint32_t node, int player, c
 }
 
 uint64_t pts_engine_device_bytes(pts_engine* e) { return e->e->device_bytes(); }
+uint32_t pts_engine_num_groups(pts_engine* e, int f64) { return (uint32_t)e->e->groups(f64 != 0).size(); }
 
 pts_status pts_engine_profile_iteration(pts_engine* e, float pos, float neg, float strat, int rw, double ms[4],
                                         uint32_t launches[4], double* wall_ms) {
@@ -350,7 +352,8 @@ static PlanInput plan_input(pts_solver* s, std::vector<uint8_t>& kind, std::vect
   return in;
 }
 
-pts_status pts_solver_new(const pts_config_c* c, int device, int precision, pts_solver** out) {  // lib.rs:365-396
+pts_status pts_solver_new(const pts_config_c* c, int device, int precision, uint64_t scratch_budget,
+                          pts_solver** out) {  // lib.rs:365-396
   *out = nullptr;
   std::unique_ptr<pts_solver> s(new pts_solver());
   s->cfg = *c;
@@ -378,11 +381,11 @@ pts_status pts_solver_new(const pts_config_c* c, int device, int precision, pts_
   err = build_plan(in, plan);
   if (!err.empty()) return bad(err);
   if (device < 0) {  // host only: tree, hands and a validated plan, no engine
+    s->host_plan = std::move(plan);
     *out = s.release();
     return PTS_OK;
   }
-  std::vector<float> zeros(s->tree.storage_len, 0.0f);
-  Status e = Engine::create(device, std::move(plan), zeros.data(), zeros.data(), precision, s->engine);
+  Status e = Engine::create(device, std::move(plan), nullptr, nullptr, precision, scratch_budget, s->engine);
   if (e != kOk) return st(e);
   *out = s.release();
   return PTS_OK;
@@ -467,6 +470,21 @@ pts_status pts_solver_root_values(pts_solver* s, int mode, double out[2]) {
 uint32_t pts_solver_iterations(pts_solver* s) { return s->iterations; }
 void pts_solver_set_iterations(pts_solver* s, uint32_t t) { s->iterations = t; }
 uint64_t pts_solver_storage_len(pts_solver* s) { return s->tree.storage_len; }
+uint32_t pts_solver_num_groups(pts_solver* s, int f64) { return s->engine ? (uint32_t)s->engine->groups(f64 != 0).size() : 0u; }
+
+pts_status pts_solver_plan_groups(pts_solver* s, uint64_t scratch_budget, int f64, uint32_t* num_groups,
+                                  uint64_t* scratch_bytes) {
+  if (s->engine) return bad("invalid input: the solver has an engine (use pts_solver_num_groups)");
+  const Plan& P = s->host_plan;
+  const uint32_t rowlen = (std::max(P.n[0], P.n[1]) + 31u) & ~31u;
+  std::vector<ChunkGroup> g;
+  ScratchRows r = partition_chunks(P, f64 ? 8 : 4, rowlen, scratch_budget, g);
+  std::string err = check_groups(P, g, r);
+  if (!err.empty()) return bad("invalid input: " + err);
+  *num_groups = (uint32_t)g.size();
+  *scratch_bytes = r.reach_bytes + r.val_bytes + r.alive_rows * 4;
+  return PTS_OK;
+}
 int32_t pts_solver_num_hands(pts_solver* s, int p) { return (int32_t)s->hands.n(p); }
 pts_status pts_solver_get_storage(pts_solver* s, float* r, float* m) { NEED_ENGINE(s); return st(s->engine->download(r, m)); }
 pts_status pts_solver_set_storage(pts_solver* s, const float* r, const float* m) { NEED_ENGINE(s); return st(s->engine->upload(r, m)); }

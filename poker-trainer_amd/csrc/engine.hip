@@ -72,6 +72,15 @@ struct Span {
   __device__ bool has(uint32_t id) const { return id == root || (id >= lo && id < hi); }
 };
 
+// Reach slots and value rows of one chunk group: resident ones (below s0 /
+// v0) are used as numbered, the group's own are shifted onto the shared
+// scratch rows that follow the resident ones.
+struct Bias {
+  uint32_t s0, sb, v0, vb;
+  __device__ __forceinline__ uint32_t s(uint32_t x) const { return x < s0 ? x : x - sb; }
+  __device__ __forceinline__ uint32_t v(uint32_t x) const { return x < v0 ? x : x - vb; }
+};
+
 template <typename V> __device__ __forceinline__ V pos0(V x) { return x > V(0) ? x : V(0); }
 template <typename V> __device__ __forceinline__ V recip(V d) {  // cfr.rs:113-121
   V i = V(1) / d;
@@ -112,7 +121,7 @@ __global__ void begin_kernel(uint32_t nq, const float* __restrict__ w, V* __rest
 // the strategy-sum update (cfr.rs:156-207,389-419; river.rs:116-160).
 // Otherwise the normalized average strategy (cfr.rs:124-154).
 template <typename V, int MODE>
-__global__ void __launch_bounds__(1024) down_kernel(DPlan P, int p, uint32_t seg0, Span span, V* __restrict__ reach,
+__global__ void __launch_bounds__(1024) down_kernel(DPlan P, int p, uint32_t seg0, Span span, Bias B, V* __restrict__ reach,
                                                       uint32_t* __restrict__ alive, const float* __restrict__ regrets,
                                                       float* __restrict__ sums, const IterParams* __restrict__ prm) {
   extern __shared__ uint4 smem_down[];
@@ -135,12 +144,12 @@ __global__ void __launch_bounds__(1024) down_kernel(DPlan P, int p, uint32_t seg
       if (l != 0)
         for (uint32_t m = 0; m < mcnt; m++)
           if (rec[m].node == span.root) { rl = m; break; }
-      m_alive[rec[rl].src[p]] = (uint8_t)(alive[rec[rl].slot[p]] != 0);
+      m_alive[rec[rl].src[p]] = (uint8_t)(alive[B.s(rec[rl].slot[p])] != 0);
     }
   } else {
     // Chance child: the parent's reach times the chance factor, opponent
     // hands holding the dealt card zeroed (cfr.rs:449-456).
-    const uint32_t sp = P.seg_pslot[(size_t)p * P.num_segs + seg], sm = rec[0].slot[p];
+    const uint32_t sp = B.s(P.seg_pslot[(size_t)p * P.num_segs + seg]), sm = B.s(rec[0].slot[p]);
     uint32_t any = 0;
     if (alive[sp]) {
       const V f = (V)P.seg_factor[seg];
@@ -170,7 +179,7 @@ __global__ void __launch_bounds__(1024) down_kernel(DPlan P, int p, uint32_t seg
   for (uint32_t m = 0; m < mcnt; m++) {
     const MemRec& r = rec[m];
     if ((int)r.kind != o || !span.has(r.node)) continue;
-    const uint32_t na = r.na, cs = r.cslot[p], cl = r.clocal;
+    const uint32_t na = r.na, cs = B.s(r.cslot[p]), cl = r.clocal;
     if (!m_alive[r.src[p]]) {
       // A dead node's children are dead (cfr.rs:271-274, river.rs:103-107).
       if (tid < na) {
@@ -180,7 +189,7 @@ __global__ void __launch_bounds__(1024) down_kernel(DPlan P, int p, uint32_t seg
       __syncthreads();
       continue;
     }
-    const V* rrow = reach + (size_t)r.slot[p] * nq;
+    const V* rrow = reach + (size_t)B.s(r.slot[p]) * nq;
     V* crow = reach + (size_t)cs * nq;
     const uint64_t off = r.offset;
     uint32_t bits = 0;
@@ -312,14 +321,14 @@ __device__ __forceinline__ void gather(double* x, const V* rv, const uint16_t* _
 }
 
 template <typename V>
-__global__ void __launch_bounds__(TERM_T) terminal_kernel(DPlan P, int p, Span span, const V* __restrict__ reach,
+__global__ void __launch_bounds__(TERM_T) terminal_kernel(DPlan P, int p, uint32_t t0, Span span, Bias B, const V* __restrict__ reach,
                                                           const uint32_t* __restrict__ alive, V* __restrict__ vals,
                                                           uint32_t rowlen) {
   extern __shared__ double sh[];
   __shared__ double tot[53];
-  const TermRec tr = P.trec[blockIdx.x];
+  const TermRec tr = P.trec[t0 + blockIdx.x];
   if (!span.has(tr.node)) return;
-  const uint32_t s = tr.slot[p];
+  const uint32_t s = B.s(tr.slot[p]);
   if (!alive[s]) return;
   const int o = p ^ 1;
   const uint32_t nq = P.n[o], np = P.n[p], tid = threadIdx.x;
@@ -327,7 +336,7 @@ __global__ void __launch_bounds__(TERM_T) terminal_kernel(DPlan P, int p, Span s
   double* Ca = sh + rowlen;      // [2 * rowlen]
   V* rv = (V*)(sh + 3 * rowlen); // [rowlen]: the opponent's reach row, hand order
   const V* R = reach + (size_t)s * nq;
-  V* out = vals + (size_t)tr.vslot * rowlen;
+  V* out = vals + (size_t)B.v(tr.vslot) * rowlen;
   for (uint32_t j0 = 0; j0 < nq; j0 += TERM_T * 8) {
     V x[8];
 #pragma unroll
@@ -494,7 +503,14 @@ __global__ void __launch_bounds__(TERM_T) terminal_kernel(DPlan P, int p, Span s
 // rounding step, as the reference's own `Fast` precision does
 // (crates/solver/src/gpu/mod.rs:51-63). The per-card sums (at most 49
 // terms) stay sequential.
-constexpr int FT = 256;  // fast terminal workgroup
+//
+// Every global load a block needs is issued up front, in two dependent
+// rounds (the record's tables, then the strength-order index lists), into
+// registers: a block then waits for memory twice instead of once per loop
+// trip.
+constexpr int FT = 256;               // fast terminal workgroup
+constexpr int FK = (1326 + FT - 1) / FT;      // hands per lane
+constexpr int FC = (2 * 1326 + FT - 1) / FT;  // per-card entries per lane
 
 // Inclusive scan of x[0..n) in place (fwd) or suffix sums (bwd), block-wide.
 template <bool FWD>
@@ -543,123 +559,161 @@ __device__ __forceinline__ double block_sum(const double* x, uint32_t n, double*
 }
 
 template <typename V>
-__device__ __forceinline__ void gather_b(double* x, const V* rv, const uint16_t* __restrict__ idx, uint32_t n) {
-  for (uint32_t j = threadIdx.x; j < n; j += FT) x[j] = (double)rv[idx[j]];
-}
-
-template <typename V>
-__global__ void __launch_bounds__(FT) terminal_fast_kernel(DPlan P, int p, Span span, const V* __restrict__ reach,
+__global__ void __launch_bounds__(FT) terminal_fast_kernel(DPlan P, int p, uint32_t t0, Span span, Bias B, const V* __restrict__ reach,
                                                            const uint32_t* __restrict__ alive, V* __restrict__ vals,
                                                            uint32_t rowlen) {
   extern __shared__ double sh[];
-  __shared__ double tot[53];
+  __shared__ double tot[52];
   __shared__ double wsum[FT / 64];
-  const TermRec tr = P.trec[blockIdx.x];
+  __shared__ uint32_t offs[53];
+  const TermRec tr = P.trec[t0 + blockIdx.x];
   if (!span.has(tr.node)) return;
-  const uint32_t s = tr.slot[p];
-  if (!alive[s]) return;
   const int o = p ^ 1;
   const uint32_t nq = P.n[o], np = P.n[p], tid = threadIdx.x;
-  double* Pa = sh;                   // [rowlen] ascending prefix
-  double* Sa = sh + rowlen;          // [rowlen] descending suffix
-  double* Ca = sh + 2 * rowlen;      // [2 rowlen] per-card ascending
-  double* Cd = sh + 4 * rowlen;      // [2 rowlen] per-card descending
-  V* rv = (V*)(sh + 6 * rowlen);     // [rowlen]
+  const uint32_t s = B.s(tr.slot[p]), b = tr.board;
+  const bool showdown = tr.kind == kShowdown;
+  const bool paired = !showdown && (tr.tflags & (p == 0 ? kTermPaired0 : kTermPaired1)) != 0;
+  const bool sorted = showdown || paired;  // strength order of the board, else hand order
+
+  // round 1: everything that depends on the record only
+  const uint32_t live = alive[s];
   const V* R = reach + (size_t)s * nq;
-  V* out = vals + (size_t)tr.vslot * rowlen;
-  for (uint32_t j = tid; j < nq; j += FT) rv[j] = R[j];
+  V r[FK];
+#pragma unroll
+  for (int k = 0; k < FK; k++) {
+    const uint32_t j = tid + k * FT;
+    r[k] = j < nq ? R[j] : V(0);
+  }
+  uint32_t hcard[FK], hsame[FK];
+  uint4 hc[FK];
+  const uint16_t* cp2 = (const uint16_t*)P.cards[p];
+  const uint4* hcnt = P.hand_cnt[p] + (size_t)b * np;
+#pragma unroll
+  for (int k = 0; k < FK; k++) {
+    const uint32_t i = tid + k * FT;
+    hcard[k] = i < np ? cp2[i] : 0u;
+    hsame[k] = i < np ? P.same[p][i] : NONE;
+    hc[k] = (showdown && i < np) ? hcnt[i] : make_uint4(0, 0, 0, 0);
+  }
+  const uint32_t nA = sorted ? P.ent_cnt[o][b] : nq;
+  const uint32_t baseA = sorted ? P.ent_off[o][b] : 0u;
+  const uint16_t* idxC = sorted ? P.cl_idx[o] + P.cl_base[o][b] : P.bl_idx[o];
+  if (tid < 53) offs[tid] = sorted ? (uint32_t)P.cl_off[o][(size_t)b * 53 + tid] : P.bl_off[o][tid];
+  // round 2: the index lists
+  const uint32_t nC = sorted ? 2 * nA : P.bl_off[o][52];
+  const uint16_t* idxA = P.ent_idx[o] + baseA;
+  uint32_t ia[FK], ic[FC];
+#pragma unroll
+  for (int k = 0; k < FK; k++) {
+    const uint32_t j = tid + k * FT;
+    ia[k] = j < nA ? (sorted ? (uint32_t)idxA[j] : j) : 0u;
+  }
+#pragma unroll
+  for (int k = 0; k < FC; k++) {
+    const uint32_t j = tid + k * FT;
+    ic[k] = j < nC ? (uint32_t)idxC[j] : 0u;
+  }
+  if (!live) return;  // dead slot: the value row is never read
+
+  double* Pa = sh;                // [rowlen] ascending prefix (strength order) or reach (hand order)
+  double* Sa = sh + rowlen;       // [rowlen] descending suffix
+  double* Ca = sh + 2 * rowlen;   // [2 rowlen] per-card ascending
+  double* Cd = sh + 4 * rowlen;   // [2 rowlen] per-card descending
+  V* rv = (V*)(sh + 6 * rowlen);  // [rowlen] the opponent's reach row, hand order
+#pragma unroll
+  for (int k = 0; k < FK; k++) {
+    const uint32_t j = tid + k * FT;
+    if (j < nq) rv[j] = r[k];
+  }
   __syncthreads();
-  const bool batched = (tr.tflags & kTermBatched) != 0;
+#pragma unroll
+  for (int k = 0; k < FK; k++) {
+    const uint32_t j = tid + k * FT;
+    if (j < nA) {
+      const double x = (double)rv[ia[k]];
+      Pa[j] = x;
+      Sa[j] = x;
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < FC; k++) {
+    const uint32_t j = tid + k * FT;
+    if (j < nC) {
+      const double x = (double)rv[ic[k]];
+      Ca[j] = x;
+      Cd[j] = x;
+    }
+  }
+  __syncthreads();
+
   const double hp = P.half_pot, norm = P.norm, rake = (double)tr.rake;
   const int32_t inv_p = tr.invest[p], inv_o = tr.invest[o];
-  const uint8_t* cp = P.cards[p];
-  const uint8_t* co = P.cards[o];
-  const uint32_t* same_p = P.same[p];
+  const bool batched = (tr.tflags & kTermBatched) != 0;
+  V* out = vals + (size_t)B.v(tr.vslot) * rowlen;
 
-  if (tr.kind != kShowdown) {
+  if (!showdown) {  // cfr.rs:224-235, eval.rs:39-64,170-206
     const int f = tr.kind - kFold0;
     const double pay = (f == p ? -(hp + (double)inv_p) : hp + (double)tr.invest[f] - rake) * norm;
-    const bool paired = (tr.tflags & (p == 0 ? kTermPaired0 : kTermPaired1)) != 0;
-    uint32_t cnt;
-    if (paired) {
-      const uint32_t b = tr.board;
-      cnt = P.ent_cnt[o][b];
-      const uint16_t* clo = P.cl_off[o] + (size_t)b * 53;
-      gather_b(Pa, rv, P.ent_idx[o] + P.ent_off[o][b], cnt);
-      gather_b(Ca, rv, P.cl_idx[o] + P.cl_base[o][b], 2 * cnt);
-      __syncthreads();
-      if (tid < 52) tot[tid] = chain_sum(Ca, clo[tid], clo[tid + 1]);
-    } else {
-      cnt = nq;
-      const uint32_t* bo = P.bl_off[o];
-      for (uint32_t j = tid; j < nq; j += FT) Pa[j] = (double)rv[j];
-      gather_b(Ca, rv, P.bl_idx[o], bo[52]);
-      __syncthreads();
-      if (tid < 52) tot[tid] = chain_sum(Ca, bo[tid], bo[tid + 1]);
-    }
-    const double total = block_sum(Pa, cnt, wsum);
-    for (uint32_t i = tid; i < np; i += FT) {
-      const int a = cp[2 * i], b = cp[2 * i + 1];
-      const uint32_t sm = same_p[i];
-      const double sw = sm == NONE ? 0.0 : (double)rv[sm];
-      V v;
-      if (batched) v = (V)(pay * (total - tot[a] - tot[b] + sw));
-      else v = total == 0.0 ? V(0) : (V)(pay * (total - (tot[a] + tot[b]) + sw));
-      out[i] = v;
+    if (tid < 52) tot[tid] = chain_sum(Ca, offs[tid], offs[tid + 1]);
+    const double total = block_sum(Pa, nA, wsum);  // its barriers publish tot
+#pragma unroll
+    for (int k = 0; k < FK; k++) {
+      const uint32_t i = tid + k * FT;
+      if (i < np) {
+        const int a = hcard[k] & 0xFF, c = hcard[k] >> 8;
+        const double sw = hsame[k] == NONE ? 0.0 : (double)rv[hsame[k]];
+        V v;
+        if (batched) v = (V)(pay * (total - tot[a] - tot[c] + sw));
+        else v = total == 0.0 ? V(0) : (V)(pay * (total - (tot[a] + tot[c]) + sw));
+        out[i] = v;
+      }
     }
     return;
   }
 
+  // showdown (cfr.rs:238-246 payoffs; eval.rs:66-148,208-379 sweeps)
   const double win = (hp + (double)inv_o - rake) * norm;
   const double lose = -(hp + (double)inv_p) * norm;
   const double tie = 0.5 * ((double)(inv_o - inv_p) - rake) * norm;
-  const uint32_t b = tr.board;
-  const uint32_t mo = P.ent_cnt[o][b];
-  const uint16_t* eidx = P.ent_idx[o] + P.ent_off[o][b];
-  const uint16_t* clo = P.cl_off[o] + (size_t)b * 53;
-  const uint16_t* ci = P.cl_idx[o] + P.cl_base[o][b];
-  const uint4* hcnt = P.hand_cnt[p] + (size_t)b * np;
-  gather_b(Pa, rv, eidx, mo);
-  for (uint32_t j = tid; j < mo; j += FT) Sa[j] = Pa[j];
-  gather_b(Ca, rv, ci, 2 * mo);
-  for (uint32_t j = tid; j < 2 * mo; j += FT) Cd[j] = Ca[j];
-  __syncthreads();
-  if (tid < 52) chain_fwd(Ca, clo[tid], clo[tid + 1]);
-  else if (tid >= 64 && tid < 116) chain_bwd(Cd, clo[tid - 64], clo[tid - 63]);
+  const uint32_t mo = nA;
+  if (tid < 52) chain_fwd(Ca, offs[tid], offs[tid + 1]);
+  else if (tid >= 64 && tid < 116) chain_bwd(Cd, offs[tid - 64], offs[tid - 63]);
   block_scan<true>(Pa, mo, wsum);
-  block_scan<false>(Sa, mo, wsum);
-  for (uint32_t i = tid; i < np; i += FT) {
-    const uint4 hc = hcnt[i];
-    const uint32_t lt = hc.x & 0xFFFF, le = hc.x >> 16;
+  block_scan<false>(Sa, mo, wsum);  // its barriers also publish the chains
+  const uint8_t* co = P.cards[o];
+#pragma unroll
+  for (int k = 0; k < FK; k++) {
+    const uint32_t i = tid + k * FT;
+    if (i >= np) continue;
+    const uint32_t lt = hc[k].x & 0xFFFF, le = hc[k].x >> 16;
     V v = V(0);
     if (lt != 0xFFFF) {
-      const int c1 = cp[2 * i], c2 = cp[2 * i + 1];
-      const uint32_t l1 = hc.y & 0xFFFF, m1 = hc.y >> 16, l2 = hc.z & 0xFFFF, m2 = hc.z >> 16;
-      const uint32_t len1 = clo[c1 + 1] - clo[c1], len2 = clo[c2 + 1] - clo[c2];
+      const int c1 = hcard[k] & 0xFF, c2 = hcard[k] >> 8;
+      const uint32_t l1 = hc[k].y & 0xFFFF, m1 = hc[k].y >> 16, l2 = hc[k].z & 0xFFFF, m2 = hc[k].z >> 16;
+      const uint32_t o1 = offs[c1], o2 = offs[c2];
+      const uint32_t len1 = offs[c1 + 1] - o1, len2 = offs[c2 + 1] - o2;
       const double Pv = lt ? Pa[lt - 1] : 0.0;
-      const double P1 = l1 ? Ca[clo[c1] + l1 - 1] : 0.0;
-      const double P2 = l2 ? Ca[clo[c2] + l2 - 1] : 0.0;
+      const double P1 = l1 ? Ca[o1 + l1 - 1] : 0.0;
+      const double P2 = l2 ? Ca[o2 + l2 - 1] : 0.0;
       const double Sv = le < mo ? Sa[le] : 0.0;
-      const double S1 = m1 < len1 ? Cd[clo[c1] + m1] : 0.0;
-      const double S2 = m2 < len2 ? Cd[clo[c2] + m2] : 0.0;
+      const double S1 = m1 < len1 ? Cd[o1 + m1] : 0.0;
+      const double S2 = m2 < len2 ? Cd[o2 + m2] : 0.0;
       if (batched) {
-        const double r = win * (Pv - P1 - P2) + lose * (Sv - S1 - S2);
-        v = (V)r;
+        v = (V)(win * (Pv - P1 - P2) + lose * (Sv - S1 - S2));
       } else {
         v = (V)(win * (Pv - (P1 + P2)));
         v = v + (V)(lose * (Sv - (S1 + S2)));
       }
-      if (tie != 0.0) {
+      if (tie != 0.0) {  // eval.rs:347-379
         double es = 0.0, e1 = 0.0, e2 = 0.0;
         for (uint32_t j = lt; j < le; j++) {
-          const uint32_t idx = eidx[j];
+          const uint32_t idx = idxA[j];
           const double w = (double)rv[idx];
           es += w;
           if (holds(co, idx, c1)) e1 += w;
           if (holds(co, idx, c2)) e2 += w;
         }
-        const uint32_t sm = same_p[i];
-        const double sw = sm == NONE ? 0.0 : (double)rv[sm];
+        const double sw = hsame[k] == NONE ? 0.0 : (double)rv[hsame[k]];
         v = v + (V)(tie * (es - (e1 + e2) + sw));
       }
     }
@@ -673,20 +727,20 @@ __global__ void __launch_bounds__(FT) terminal_fast_kernel(DPlan P, int p, Span 
 // the card (cfr.rs:457-478); dead nodes get a zero row. One lane per hand,
 // 16 child rows in flight per lane.
 template <typename V>
-__global__ void __launch_bounds__(UP_T) chance_kernel(DPlan P, int p, uint32_t c0, Span span, V* __restrict__ vals,
+__global__ void __launch_bounds__(UP_T) chance_kernel(DPlan P, int p, uint32_t c0, Span span, Bias B, V* __restrict__ vals,
                                                       uint32_t rowlen, const uint32_t* __restrict__ alive) {
   const uint32_t ci = c0 + blockIdx.x;
   const MemRec& r = P.crec[ci];
   const uint32_t np = P.n[p];
   const uint32_t h = blockIdx.y * UP_T + threadIdx.x;
   if (!span.has(r.node) || h >= np) return;
-  V* out = vals + (size_t)r.vslot * rowlen + h;
-  if (!alive[r.slot[p]]) {
+  V* out = vals + (size_t)B.v(r.vslot) * rowlen + h;
+  if (!alive[B.s(r.slot[p])]) {
     *out = V(0);
     return;
   }
   const uint32_t na = r.na;
-  const V* row = vals + (size_t)P.chance_child_vslot[ci] * rowlen + h;
+  const V* row = vals + (size_t)B.v(P.chance_child_vslot[ci]) * rowlen + h;
   const int8_t* cards = P.card + r.offset;
   const int c1 = P.cards[p][2 * h], c2 = P.cards[p][2 * h + 1];
   double acc = 0.0;
@@ -719,7 +773,7 @@ __global__ void __launch_bounds__(UP_T) chance_kernel(DPlan P, int p, uint32_t c
 // Every segment root's row is written (zeros when dead), so a chance node
 // reads its children's rows unconditionally.
 template <typename V, int MODE>
-__global__ void __launch_bounds__(UP_T) up_kernel(DPlan P, int p, uint32_t seg0, Span span, V* __restrict__ vals,
+__global__ void __launch_bounds__(UP_T) up_kernel(DPlan P, int p, uint32_t seg0, Span span, Bias B, V* __restrict__ vals,
                                                   uint32_t rowlen, const uint32_t* __restrict__ alive,
                                                   float* __restrict__ regrets, const float* __restrict__ sums,
                                                   const IterParams* __restrict__ prm, V* __restrict__ qout) {
@@ -735,7 +789,7 @@ __global__ void __launch_bounds__(UP_T) up_kernel(DPlan P, int p, uint32_t seg0,
   V* lv = (V*)(live + ((P.max_members + 15) & ~15u));
   stage_records(P.mrec + moff, mcnt, rec);
   __syncthreads();
-  for (uint32_t m = tid; m < mcnt; m += UP_T) live[m] = alive[rec[m].slot[p]] != 0;
+  for (uint32_t m = tid; m < mcnt; m += UP_T) live[m] = alive[B.s(rec[m].slot[p])] != 0;
   __syncthreads();
   if (h >= np) return;
   // leaves first: terminal and chance values, 8 rows in flight per lane
@@ -746,7 +800,7 @@ __global__ void __launch_bounds__(UP_T) up_kernel(DPlan P, int p, uint32_t seg0,
       const uint32_t m = m0 + k;
       x[k] = V(0);
       if (m < mcnt && rec[m].kind >= kChance && live[m] && span.has(rec[m].node))
-        x[k] = vals[(size_t)rec[m].vslot * rowlen + h];
+        x[k] = vals[(size_t)B.v(rec[m].vslot) * rowlen + h];
     }
 #pragma unroll
     for (int k = 0; k < 8; k++) {
@@ -836,7 +890,7 @@ __global__ void __launch_bounds__(UP_T) up_kernel(DPlan P, int p, uint32_t seg0,
     }
     lv[(size_t)m * UP_T + tid] = v;
   }
-  vals[(size_t)rec[0].vslot * rowlen + h] = lv[tid];
+  vals[(size_t)B.v(rec[0].vslot) * rowlen + h] = lv[tid];
 }
 
 // ================================================================== engine
@@ -877,7 +931,7 @@ static size_t put(std::vector<unsigned char>& blob, const std::vector<T>& v) {
 }
 
 Status Engine::create(int device, Plan&& plan, const float* regrets, const float* sums, int precision,
-                      std::unique_ptr<Engine>& out) {
+                      uint64_t scratch_budget, std::unique_ptr<Engine>& out) {
   std::unique_ptr<Engine> e(new Engine());
   if (precision != kFast && precision != kExact) {
     set_error("invalid input: precision");
@@ -906,17 +960,17 @@ Status Engine::create(int device, Plan&& plan, const float* regrets, const float
   // the records are copied into LDS as 16-byte words
   static_assert(sizeof(MemRec) % 16 == 0, "MemRec staging");
 
-  // sizes and the HBM budget (crates/solver/src/gpu/engine.rs:261-315 declines with a 512 MiB margin)
-  const uint32_t rowlen = std::max(P.n[0], P.n[1]);
+  // value rows start on 128-byte boundaries
+  const uint32_t rowlen = (std::max(P.n[0], P.n[1]) + 31u) & ~31u;
   e->rowlen_ = rowlen;
-  size_t reach_elems = std::max((size_t)P.num_slots[0] * P.n[1], (size_t)P.num_slots[1] * P.n[0]);
-  size_t slots = std::max(P.num_slots[0], P.num_slots[1]);
-  size_t need = 2 * P.storage_len * sizeof(float) + reach_elems * 8 + (size_t)P.num_vslots * rowlen * 8 + slots * 4;
+  // the storage must fit with a 512 MiB margin (crates/solver/src/gpu/engine.rs:261-315)
   size_t fr = 0, tot = 0;
   HIPCHK(hipMemGetInfo(&fr, &tot));
-  if (need + (512ull << 20) > fr) {
+  const size_t margin = 512ull << 20;
+  const size_t base_need = 2 * P.storage_len * sizeof(float) + P.num_nodes * 16 + (size_t)P.num_boards * (P.n[0] + P.n[1]) * 24;
+  if (base_need + margin > fr) {
     char m[200];
-    std::snprintf(m, sizeof m, "out of GPU memory: the solve needs %zu bytes, %zu available", need, fr);
+    std::snprintf(m, sizeof m, "out of GPU memory: the storage needs %zu bytes, %zu available", base_need, fr);
     set_error(m);
     return kOutOfMemory;
   }
@@ -973,14 +1027,30 @@ Status Engine::create(int device, Plan&& plan, const float* regrets, const float
     e->weights_[q] = (float*)AT(float, o_w[q]);
   }
 #undef AT
-  // storage and scratch
+  // storage, then the scratch rows: chunk groups sized to what is left
   DeviceBuf br, bs, bre, bv, ba, bp;
   if ((st = e->alloc(br, P.storage_len * sizeof(float))) != kOk) return st;
   if ((st = e->alloc(bs, P.storage_len * sizeof(float))) != kOk) return st;
-  if ((st = e->alloc(bre, reach_elems * 8)) != kOk) return st;
-  if ((st = e->alloc(bv, (size_t)P.num_vslots * rowlen * 8)) != kOk) return st;
-  if ((st = e->alloc(ba, slots * 4)) != kOk) return st;
   if ((st = e->alloc(bp, sizeof(IterParams) + 2 * rowlen * 8)) != kOk) return st;
+  HIPCHK(hipMemGetInfo(&fr, &tot));
+  uint64_t budget = fr > margin ? fr - margin : 0;
+  if (scratch_budget != 0) budget = std::min<uint64_t>(budget, scratch_budget);
+  ScratchRows need{};
+  for (int f64 = 0; f64 < 2; f64++) {
+    const size_t es = f64 ? 8 : 4;
+    ScratchRows r = partition_chunks(P, es, rowlen, budget, e->groups_[f64]);
+    std::string err = check_groups(P, e->groups_[f64], r);
+    if (!err.empty()) {
+      set_error("invalid input: " + err);
+      return kInvalidInput;
+    }
+    need.reach_bytes = std::max(need.reach_bytes, r.reach_bytes);
+    need.val_bytes = std::max(need.val_bytes, r.val_bytes);
+    need.alive_rows = std::max(need.alive_rows, r.alive_rows);
+  }
+  if ((st = e->alloc(bre, need.reach_bytes)) != kOk) return st;
+  if ((st = e->alloc(bv, need.val_bytes)) != kOk) return st;
+  if ((st = e->alloc(ba, need.alive_rows * 4)) != kOk) return st;
   e->regrets_ = (float*)br.p;
   e->sums_ = (float*)bs.p;
   e->reach_ = bre.p;
@@ -1022,10 +1092,14 @@ Status Engine::create(int device, Plan&& plan, const float* regrets, const float
 }
 
 Status Engine::upload(const float* regrets, const float* sums) {
-  HIPCHK(hipMemcpy(regrets_, regrets, plan_.storage_len * sizeof(float), hipMemcpyHostToDevice));
-  HIPCHK(hipMemcpy(sums_, sums, plan_.storage_len * sizeof(float), hipMemcpyHostToDevice));
+  const size_t bytes = plan_.storage_len * sizeof(float);
+  if (regrets) HIPCHK(hipMemcpy(regrets_, regrets, bytes, hipMemcpyHostToDevice));
+  else HIPCHK(hipMemset(regrets_, 0, bytes));
+  if (sums) HIPCHK(hipMemcpy(sums_, sums, bytes, hipMemcpyHostToDevice));
+  else HIPCHK(hipMemset(sums_, 0, bytes));
   return kOk;
 }
+
 
 Status Engine::download(float* regrets, float* sums) const {
   HIPCHK(hipStreamSynchronize(stream_));
@@ -1071,48 +1145,67 @@ struct ProfRec {
   if (prof) prof->stop()
 
 template <typename V, int DMODE, int UMODE>
-static void launch_span(const DPlan& d, const Plan& P, hipStream_t st, int p, Span span, uint32_t first_depth, void* reach,
-                        uint32_t* alive, float* regrets, float* sums, const IterParams* prm, void* vals, uint32_t rowlen,
-                        size_t lds_down, size_t lds_term, size_t lds_up, V* qout, bool fast, ProfRec* prof = nullptr) {
-  const uint32_t D = P.depths();
-  for (uint32_t dd = first_depth; dd < D; dd++) {
-    uint32_t a = P.depth_off[dd], b = P.depth_off[dd + 1];
-    if (b > a) {
-      // few segments (the top of the tree): one lane per opponent hand
-      const uint32_t threads = (b - a) < 64 ? 1024 : DOWN_T;
-      PROF_START(kKDown);
-      hipLaunchKernelGGL((down_kernel<V, DMODE>), dim3(b - a), dim3(threads), lds_down, st, d, p, a, span, (V*)reach, alive,
-                         regrets, sums, prm);
-      PROF_STOP();
-    }
-  }
-  if (!P.terminals.empty()) {
+static void launch_span(const DPlan& d, const Plan& P, hipStream_t st, int p, Span span, uint32_t first_depth,
+                        const std::vector<ChunkGroup>& groups, void* reach, uint32_t* alive, float* regrets, float* sums,
+                        const IterParams* prm, void* vals, uint32_t rowlen, size_t lds_down, size_t lds_term, size_t lds_up,
+                        V* qout, bool fast, ProfRec* prof = nullptr) {
+  const uint32_t D = P.depths(), C = P.num_chunks;
+  const uint32_t tiles = (P.n[p] + UP_T - 1) / UP_T;
+  const Bias B0{P.slot_s0[p], 0, P.val_v0, 0};
+  auto down = [&](uint32_t a, uint32_t b, const Bias& B) {
+    if (b <= a) return;
+    // few segments (the top of the tree): one lane per opponent hand
+    const uint32_t threads = (b - a) < 64 ? 1024 : DOWN_T;
+    PROF_START(kKDown);
+    hipLaunchKernelGGL((down_kernel<V, DMODE>), dim3(b - a), dim3(threads), lds_down, st, d, p, a, span, B, (V*)reach, alive,
+                       regrets, sums, prm);
+    PROF_STOP();
+  };
+  auto terminals = [&](uint32_t a, uint32_t b, const Bias& B) {
+    if (b <= a) return;
     PROF_START(kKTerminal);
     if (fast)
-      hipLaunchKernelGGL((terminal_fast_kernel<V>), dim3((uint32_t)P.terminals.size()), dim3(FT), lds_term, st, d, p, span,
-                         (const V*)reach, (const uint32_t*)alive, (V*)vals, rowlen);
+      hipLaunchKernelGGL((terminal_fast_kernel<V>), dim3(b - a), dim3(FT), lds_term, st, d, p, a, span, B, (const V*)reach,
+                         (const uint32_t*)alive, (V*)vals, rowlen);
     else
-      hipLaunchKernelGGL((terminal_kernel<V>), dim3((uint32_t)P.terminals.size()), dim3(TERM_T), lds_term, st, d, p, span,
-                         (const V*)reach, (const uint32_t*)alive, (V*)vals, rowlen);
+      hipLaunchKernelGGL((terminal_kernel<V>), dim3(b - a), dim3(TERM_T), lds_term, st, d, p, a, span, B, (const V*)reach,
+                         (const uint32_t*)alive, (V*)vals, rowlen);
     PROF_STOP();
-  }
-  const uint32_t tiles = (P.n[p] + UP_T - 1) / UP_T;
-  for (uint32_t dd = D; dd-- > first_depth;) {
-    uint32_t ca = P.chance_off[dd], cb = P.chance_off[dd + 1];
+  };
+  auto up = [&](uint32_t ca, uint32_t cb, uint32_t a, uint32_t b, const Bias& B) {
     if (cb > ca) {
       PROF_START(kKUp);
-      hipLaunchKernelGGL((chance_kernel<V>), dim3(cb - ca, tiles), dim3(UP_T), 0, st, d, p, ca, span, (V*)vals, rowlen,
+      hipLaunchKernelGGL((chance_kernel<V>), dim3(cb - ca, tiles), dim3(UP_T), 0, st, d, p, ca, span, B, (V*)vals, rowlen,
                          (const uint32_t*)alive);
       PROF_STOP();
     }
-    uint32_t a = P.depth_off[dd], b = P.depth_off[dd + 1];
     if (b > a) {
       PROF_START(kKUp);
-      hipLaunchKernelGGL((up_kernel<V, UMODE>), dim3(b - a, tiles), dim3(UP_T), lds_up, st, d, p, a, span, (V*)vals, rowlen,
+      hipLaunchKernelGGL((up_kernel<V, UMODE>), dim3(b - a, tiles), dim3(UP_T), lds_up, st, d, p, a, span, B, (V*)vals, rowlen,
                          (const uint32_t*)alive, regrets, (const float*)sums, prm, qout);
       PROF_STOP();
     }
+  };
+  if (first_depth == 0) down(P.depth_off[0], P.depth_off[1], B0);
+  if (C == 0) {
+    terminals(0, (uint32_t)P.terminals.size(), B0);
+  } else {
+    const uint32_t qc = first_depth == 0 ? kNone : P.chunk_of[span.root];
+    bool first = true;
+    for (const ChunkGroup& g : groups) {
+      if (qc != kNone && (qc < g.c0 || qc >= g.c1)) continue;
+      const Bias B{P.slot_s0[p], P.slot_coff[p][g.c0] - P.slot_s0[p], P.val_v0, P.val_coff[g.c0] - P.val_v0};
+      const uint32_t* so = P.seg_coff.data();
+      const uint32_t* co = P.ch_coff.data();
+      for (uint32_t dd = std::max(1u, first_depth); dd < D; dd++) down(so[dd * (C + 1) + g.c0], so[dd * (C + 1) + g.c1], B);
+      // the top segment's terminals (resident rows) ride with the first group
+      terminals(first && first_depth == 0 ? 0u : P.term_coff[g.c0], P.term_coff[g.c1], B);
+      first = false;
+      for (uint32_t dd = D; dd-- > std::max(1u, first_depth);)
+        up(co[dd * (C + 1) + g.c0], co[dd * (C + 1) + g.c1], so[dd * (C + 1) + g.c0], so[dd * (C + 1) + g.c1], B);
+    }
   }
+  if (first_depth == 0) up(P.chance_off[0], P.chance_off[1], P.depth_off[0], P.depth_off[1], B0);
 }
 
 Status Engine::launch_traversal(int p, Mode mode, bool f64, ProfRec* prof) {
@@ -1126,16 +1219,16 @@ Status Engine::launch_traversal(int p, Mode mode, bool f64, ProfRec* prof) {
   else hipLaunchKernelGGL((begin_kernel<float>), dim3((nq + 255) / 256), dim3(256), 0, stream_, nq, weights_[o], (float*)reach_, 0u, alive_, 1u);
   PROF_STOP();
   if (mode == kCfr)
-    launch_span<float, kCfr, kCfr>(d, P, stream_, p, span, 0, reach_, alive_, regrets_, sums_, params_, vals_, rowlen_, lds_down_,
+    launch_span<float, kCfr, kCfr>(d, P, stream_, p, span, 0, groups_[0], reach_, alive_, regrets_, sums_, params_, vals_, rowlen_, lds_down_,
                                    lds_term_[0], lds_up_[0], (float*)nullptr, precision_ == kFast, prof);
   else if (mode == kBest)
-    launch_span<double, kAverage, kBest>(d, P, stream_, p, span, 0, reach_, alive_, regrets_, sums_, params_, vals_, rowlen_,
+    launch_span<double, kAverage, kBest>(d, P, stream_, p, span, 0, groups_[1], reach_, alive_, regrets_, sums_, params_, vals_, rowlen_,
                                          lds_down_, lds_term_[1], lds_up_[1], (double*)nullptr, precision_ == kFast);
   else if (f64)
-    launch_span<double, kAverage, kAverage>(d, P, stream_, p, span, 0, reach_, alive_, regrets_, sums_, params_, vals_, rowlen_,
+    launch_span<double, kAverage, kAverage>(d, P, stream_, p, span, 0, groups_[1], reach_, alive_, regrets_, sums_, params_, vals_, rowlen_,
                                             lds_down_, lds_term_[1], lds_up_[1], (double*)nullptr, precision_ == kFast);
   else
-    launch_span<float, kAverage, kAverage>(d, P, stream_, p, span, 0, reach_, alive_, regrets_, sums_, params_, vals_, rowlen_,
+    launch_span<float, kAverage, kAverage>(d, P, stream_, p, span, 0, groups_[0], reach_, alive_, regrets_, sums_, params_, vals_, rowlen_,
                                            lds_down_, lds_term_[0], lds_up_[0], (float*)nullptr, precision_ == kFast);
   // keep the root row of p
   size_t vs = f64 ? 8 : 4;
@@ -1277,14 +1370,18 @@ Status Engine::subtree_values(uint32_t node, int q, const std::vector<float>& op
   HIPCHK(hipMemcpyAsync(dreach, opp.data(), (size_t)nq * 4, hipMemcpyHostToDevice, stream_));
   bool any = false;
   for (float x : opp) any |= x != 0.0f;
-  const uint32_t slot = P.rslot[(size_t)q * P.num_nodes + node];
+  uint32_t slot = P.rslot[(size_t)q * P.num_nodes + node];
+  const uint32_t qc = P.chunk_of[node];
+  if (slot >= P.slot_s0[q] && qc != kNone)
+    for (const ChunkGroup& g : groups_[0])
+      if (qc >= g.c0 && qc < g.c1) slot -= P.slot_coff[q][g.c0] - P.slot_s0[q];
   hipLaunchKernelGGL((begin_kernel<float>), dim3((nq + 255) / 256), dim3(256), 0, stream_, nq, (const float*)dreach, (float*)reach_,
                      slot, alive_, any ? 1u : 0u);
   Span span{node, P.first_child[node], P.subtree_end[node], P.seg_of[node]};
   // depths from the node's segment on
   uint32_t seg = P.seg_of[node], first_depth = 0;
   while (first_depth + 1 < P.depth_off.size() && P.depth_off[first_depth + 1] <= seg) first_depth++;
-  launch_span<float, kAverage, kAverage>(dp_->d, P, stream_, q, span, first_depth, reach_, alive_, regrets_, sums_, params_, vals_,
+  launch_span<float, kAverage, kAverage>(dp_->d, P, stream_, q, span, first_depth, groups_[0], reach_, alive_, regrets_, sums_, params_, vals_,
                                          rowlen_, lds_down_, lds_term_[0], lds_up_[0], dout, precision_ == kFast);
   HIPCHK(hipGetLastError());
   child_vals.assign((size_t)na * np, 0.0f);

@@ -56,8 +56,11 @@ class Engine {
   // Uploads the plan and the storage; fails with kOutOfMemory when HBM is short.
   // precision: kFast (block-scan terminal sums, f64) or kExact (the
   // reference's sequential order, bit-identical to the CPU solve).
+  // regrets / sums may be null (zeros). scratch_budget caps the reach and
+  // value rows in bytes (0: all free HBM after the storage, less a margin);
+  // the chunks below the root street's chance nodes run in groups that fit.
   static Status create(int device, Plan&& plan, const float* regrets, const float* sums, int precision,
-                       std::unique_ptr<Engine>& out);
+                       uint64_t scratch_budget, std::unique_ptr<Engine>& out);
   int precision() const { return precision_; }
 
   // One DCFR iteration: player 0's update, then player 1's (crates/solver/src/lib.rs:577-585).
@@ -83,10 +86,13 @@ class Engine {
   // Event-timed kernel durations of the last graph-less CFR iteration (profiling aid)
   double last_ms = 0;
   size_t device_bytes() const { return bytes_; }
+  // Chunk groups of a traversal, for f32 (CFR) or f64 rows.
+  const std::vector<ChunkGroup>& groups(bool f64) const { return groups_[f64 ? 1 : 0]; }
 
  private:
   Engine() = default;
   Status alloc(DeviceBuf& b, size_t bytes);
+
   Status launch_traversal(int p, Mode mode, bool f64, struct ProfRec* prof = nullptr);
   Status build_graphs();
   Status run_graph(int which);
@@ -111,6 +117,7 @@ class Engine {
   hipGraphExec_t graph_[3] = {nullptr, nullptr, nullptr};  // cfr iteration, best response, average
   bool use_graph_ = true;
   int precision_ = kFast;
+  std::vector<ChunkGroup> groups_[2];
   struct DPlanHolder;
   std::unique_ptr<DPlanHolder> dp_;
 };

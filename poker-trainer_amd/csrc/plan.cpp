@@ -155,48 +155,129 @@ std::string build_plan(const PlanInput& in, Plan& P) {
   for (size_t i = 0; i < segs.size(); i++)
     for (uint32_t m = 0; m < P.seg_mem_cnt[i]; m++) P.seg_of[P.members[P.seg_mem_off[i] + m]] = (uint32_t)i;
 
-  // ---- reach slots and alive sources, per traverser (parents precede children in the arena)
+  // ---- chunks: the subtrees rooted at depth-1 segments (the root street's
+  // chance children). A traversal runs them in groups sized to the scratch
+  // budget, so every reach slot and value row used only inside one chunk is
+  // numbered contiguously per chunk; rows read across chunks (the top
+  // segment's, the chunk roots') come first and stay resident.
+  const uint32_t D = P.depths();
+  P.num_chunks = D >= 2 ? P.depth_off[2] - P.depth_off[1] : 0;
+  const uint32_t C = P.num_chunks;
+  std::vector<uint32_t> seg_chunk(P.seg_root.size(), kNone);
+  for (uint32_t g = 0; g < (uint32_t)P.seg_root.size(); g++) {
+    if (g < P.depth_off[1]) continue;
+    if (g < P.depth_off[std::min(2u, D)]) seg_chunk[g] = g - P.depth_off[1];
+    else seg_chunk[g] = seg_chunk[P.seg_of[P.seg_parent[g]]];
+  }
+  P.chunk_of.assign(N, kNone);
+  for (size_t i = 0; i < N; i++) P.chunk_of[i] = seg_chunk[P.seg_of[i]];
+  // per depth d >= 1 the segments of chunk c are [seg_coff[d][c], seg_coff[d][c+1])
+  P.seg_coff.assign((size_t)D * (C + 1), 0);
+  for (uint32_t d = 1; d < D; d++) {
+    uint32_t* o = P.seg_coff.data() + (size_t)d * (C + 1);
+    uint32_t g = P.depth_off[d];
+    for (uint32_t c = 0; c <= C; c++) {
+      while (g < P.depth_off[d + 1] && seg_chunk[g] < c) g++;
+      o[c] = g;
+    }
+    for (uint32_t h = P.depth_off[d]; h + 1 < P.depth_off[d + 1]; h++)
+      if (seg_chunk[h] > seg_chunk[h + 1]) return fail("segments not ordered by chunk", h);
+  }
+  // nodes of each chunk in arena order (parents first)
+  std::vector<uint32_t> chunk_nodes_off(C + 1, 0), chunk_nodes;
+  {
+    std::vector<uint32_t> cnt(C + 1, 0);
+    for (size_t i = 0; i < N; i++)
+      if (P.chunk_of[i] != kNone) cnt[P.chunk_of[i] + 1]++;
+    for (uint32_t c = 0; c < C; c++) cnt[c + 1] += cnt[c];
+    chunk_nodes_off = cnt;
+    chunk_nodes.resize(cnt[C]);
+    for (size_t i = 0; i < N; i++)
+      if (P.chunk_of[i] != kNone) chunk_nodes[cnt[P.chunk_of[i]]++] = (uint32_t)i;
+  }
+
+  // ---- reach slots and alive sources, per traverser: the top segment's
+  // children first (chunk roots included), then chunk by chunk
   P.rslot.assign(2 * N, kNone);
   P.alive_src.assign(2 * N, 0);
   for (int p = 0; p < 2; p++) {
     uint32_t* rs = P.rslot.data() + p * N;
     uint16_t* src = P.alive_src.data() + p * N;
-    uint32_t next = 1;
+    uint64_t next = 1;
     rs[0] = 0;
     src[0] = 0;
-    for (size_t i = 0; i < N; i++) {
+    auto give = [&](size_t i) {
       uint8_t k = P.kind[i];
-      if (is_terminal(k)) continue;
+      if (is_terminal(k)) return;
       for (uint32_t a = 0; a < P.na[i]; a++) {
         uint32_t c = P.first_child[i] + a;
         bool fresh = k == kChance || (int)k == (p ^ 1);
-        rs[c] = fresh ? next++ : rs[i];
+        rs[c] = fresh ? (uint32_t)next++ : rs[i];
         src[c] = fresh ? P.local[c] : src[i];
       }
+    };
+    for (size_t i = 0; i < N; i++)
+      if (P.chunk_of[i] == kNone) give(i);
+    P.slot_s0[p] = (uint32_t)next;
+    P.slot_coff[p].assign(C + 1, 0);
+    for (uint32_t c = 0; c < C; c++) {
+      P.slot_coff[p][c] = (uint32_t)next;
+      for (uint32_t j = chunk_nodes_off[c]; j < chunk_nodes_off[c + 1]; j++) give(chunk_nodes[j]);
     }
-    P.num_slots[p] = next;
+    P.slot_coff[p][C] = (uint32_t)next;
+    if (next >= 0xFFFFFFFFull) return fail("too many reach slots", next);
+    P.num_slots[p] = (uint32_t)next;
   }
 
-  // ---- value rows, terminals and their flags
+  // ---- value rows: the root and chunk roots, the top segment's chance
+  // nodes and terminals; then per chunk its deeper segment roots, chance
+  // nodes and terminals
   P.vslot.assign(N, kNone);
   P.tflags.assign(N, 0);
   P.tboard.assign(N, kNone);
-  uint32_t nv = 0;
-  for (uint32_t r : P.seg_root) P.vslot[r] = nv++;
+  uint64_t nv = 0;
   std::vector<uint32_t> chance_vslot(N, kNone);
-  for (size_t g = 0; g < P.seg_root.size(); g++)
+  for (uint32_t g = 0; g < P.depth_off[std::min(2u, D)]; g++) P.vslot[P.seg_root[g]] = (uint32_t)nv++;
+  for (uint32_t g = 0; g < P.depth_off[1]; g++)
     for (uint32_t m = 0; m < P.seg_mem_cnt[g]; m++) {
       uint32_t id = P.members[P.seg_mem_off[g] + m];
-      if (P.kind[id] == kChance) chance_vslot[id] = nv++;
+      if (P.kind[id] == kChance && P.vslot[id] == kNone) chance_vslot[id] = (uint32_t)nv++;
     }
-  for (size_t i = 0; i < N; i++) {
-    if (!is_terminal(P.kind[i])) continue;
-    P.terminals.push_back((uint32_t)i);
-    if (P.vslot[i] == kNone) P.vslot[i] = nv++;
-    uint32_t par = parent[i];
+  P.terminals.clear();
+  for (size_t i = 0; i < N; i++)
+    if (P.chunk_of[i] == kNone && is_terminal(P.kind[i])) {
+      P.terminals.push_back((uint32_t)i);
+      if (P.vslot[i] == kNone) P.vslot[i] = (uint32_t)nv++;  // a chance child keeps its segment-root row
+    }
+  P.val_v0 = (uint32_t)nv;
+  P.val_coff.assign(C + 1, 0);
+  P.term_coff.assign(C + 1, 0);
+  for (uint32_t c = 0; c < C; c++) {
+    P.val_coff[c] = (uint32_t)nv;
+    P.term_coff[c] = (uint32_t)P.terminals.size();
+    for (uint32_t d = 2; d < D; d++) {
+      const uint32_t* o = P.seg_coff.data() + (size_t)d * (C + 1);
+      for (uint32_t g = o[c]; g < o[c + 1]; g++) P.vslot[P.seg_root[g]] = (uint32_t)nv++;
+    }
+    for (uint32_t j = chunk_nodes_off[c]; j < chunk_nodes_off[c + 1]; j++) {
+      uint32_t id = chunk_nodes[j];
+      if (P.kind[id] == kChance && P.vslot[id] == kNone) chance_vslot[id] = (uint32_t)nv++;
+    }
+    for (uint32_t j = chunk_nodes_off[c]; j < chunk_nodes_off[c + 1]; j++) {
+      uint32_t id = chunk_nodes[j];
+      if (!is_terminal(P.kind[id])) continue;
+      P.terminals.push_back(id);
+      if (P.vslot[id] == kNone) P.vslot[id] = (uint32_t)nv++;
+    }
+  }
+  P.val_coff[C] = (uint32_t)nv;
+  P.term_coff[C] = (uint32_t)P.terminals.size();
+  if (nv >= 0xFFFFFFFFull) return fail("too many value rows", nv);
+  for (uint32_t id : P.terminals) {
+    uint32_t par = parent[id];
     bool batched = in.batch_river && is_player(P.kind[par]) && in.street[par] == kRiver;
-    if (batched) P.tflags[i] |= kTermBatched;
-    if (P.kind[i] == kShowdown) P.tboard[i] = (uint32_t)P.offset[i];
+    if (batched) P.tflags[id] |= kTermBatched;
+    if (P.kind[id] == kShowdown) P.tboard[id] = (uint32_t)P.offset[id];
   }
   // A fold and a showdown under one river node of the traverser share its
   // reach and one strength sweep (crates/solver/src/river.rs:163-213).
@@ -211,7 +292,7 @@ std::string build_plan(const PlanInput& in, Plan& P) {
       break;
     }
   }
-  P.num_vslots = nv;
+  P.num_vslots = (uint32_t)nv;
 
   // ---- member and terminal records
   const size_t S = P.seg_root.size();
@@ -266,19 +347,26 @@ std::string build_plan(const PlanInput& in, Plan& P) {
   }
   // chance nodes by segment depth
   {
-    uint32_t D = (uint32_t)P.depth_off.size() - 1;
     P.chance_off.assign(D + 1, 0);
     P.crec.clear();
     P.chance_child_vslot.clear();
+    P.ch_coff.assign((size_t)D * (C + 1), 0);
     for (uint32_t d = 0; d < D; d++) {
       P.chance_off[d] = (uint32_t)P.crec.size();
-      for (uint32_t g = P.depth_off[d]; g < P.depth_off[d + 1]; g++)
+      uint32_t* co = P.ch_coff.data() + (size_t)d * (C + 1);
+      uint32_t c = 0;
+      for (uint32_t g = P.depth_off[d]; g < P.depth_off[d + 1]; g++) {
+        if (d >= 1)
+          while (c <= seg_chunk[g]) co[c++] = (uint32_t)P.crec.size();
         for (uint32_t m = 0; m < P.seg_mem_cnt[g]; m++) {
           const MemRec& r = P.mrec[P.seg_mem_off[g] + m];
           if (r.kind != kChance) continue;
           P.crec.push_back(r);
           P.chance_child_vslot.push_back(P.vslot[r.offset]);
         }
+      }
+      if (d >= 1)
+        while (c <= C) co[c++] = (uint32_t)P.crec.size();
     }
     P.chance_off[D] = (uint32_t)P.crec.size();
   }
@@ -380,6 +468,133 @@ std::string build_plan(const PlanInput& in, Plan& P) {
         const uint16_t* c = P.my_cnt[q].data() + (size_t)(P.ent_off[q][b] + e) * 6;
         for (int k = 0; k < 6; k++) d[k] = c[k];
       }
+  }
+  return "";
+}
+
+ScratchRows partition_chunks(const Plan& P, size_t es, uint32_t rowlen, uint64_t budget, std::vector<ChunkGroup>& G) {
+  const uint32_t C = P.num_chunks;
+  auto rows = [&](uint32_t c0, uint32_t c1, ScratchRows& r) {
+    r.reach_bytes = 0;
+    r.alive_rows = 0;
+    for (int p = 0; p < 2; p++) {
+      const uint64_t n = (uint64_t)P.slot_s0[p] + (C ? P.slot_coff[p][c1] - P.slot_coff[p][c0] : 0);
+      r.slot_rows[p] = n;
+      r.reach_bytes = std::max<uint64_t>(r.reach_bytes, n * P.n[p ^ 1] * es);
+      r.alive_rows = std::max<uint64_t>(r.alive_rows, n);
+    }
+    r.val_rows = (uint64_t)P.val_v0 + (C ? P.val_coff[c1] - P.val_coff[c0] : 0);
+    r.val_bytes = r.val_rows * rowlen * es;
+    return r.reach_bytes + r.val_bytes + r.alive_rows * 4;
+  };
+  ScratchRows best, cur;
+  auto take = [&](const ScratchRows& r) {
+    best.reach_bytes = std::max(best.reach_bytes, r.reach_bytes);
+    best.val_bytes = std::max(best.val_bytes, r.val_bytes);
+    best.alive_rows = std::max(best.alive_rows, r.alive_rows);
+    for (int p = 0; p < 2; p++) best.slot_rows[p] = std::max(best.slot_rows[p], r.slot_rows[p]);
+    best.val_rows = std::max(best.val_rows, r.val_rows);
+  };
+  G.clear();
+  if (C == 0) {
+    rows(0, 0, cur);
+    take(cur);
+    G.push_back({0, 0});
+    return best;
+  }
+  // the allocation is the largest reach, value and alive rows over all
+  // groups, so a group grows only while that combination fits
+  auto total = [&](const ScratchRows& r) {
+    return std::max(best.reach_bytes, r.reach_bytes) + std::max(best.val_bytes, r.val_bytes) +
+           std::max(best.alive_rows, r.alive_rows) * 4;
+  };
+  uint32_t c0 = 0;
+  while (c0 < C) {
+    uint32_t c1 = c0 + 1;
+    rows(c0, c1, cur);
+    ScratchRows nx;
+    while (c1 < C) {
+      rows(c0, c1 + 1, nx);
+      if (total(nx) > budget) break;
+      cur = nx;
+      c1++;
+    }
+    take(cur);
+    G.push_back({c0, c1});
+    c0 = c1;
+  }
+  return best;
+}
+
+std::string check_groups(const Plan& P, const std::vector<ChunkGroup>& G, const ScratchRows& R) {
+  const uint32_t C = P.num_chunks, D = P.depths(), N = (uint32_t)P.num_nodes;
+  char b[200];
+  for (int p = 0; p < 2; p++) {
+    const uint32_t s0 = P.slot_s0[p], v0 = P.val_v0;
+    auto check = [&](uint32_t s, uint32_t n, bool val, uint32_t c0, uint32_t c1, const char* what) -> bool {
+      if (s == kNone) return true;
+      const uint32_t r0 = val ? v0 : s0;
+      const uint32_t lo = C ? (val ? P.val_coff[c0] : P.slot_coff[p][c0]) : r0;
+      const uint32_t hi = C ? (val ? P.val_coff[c1] : P.slot_coff[p][c1]) : r0;
+      const uint64_t rows = val ? R.val_rows : R.slot_rows[p];
+      bool ok;
+      if ((uint64_t)s + n <= r0) ok = true;  // resident
+      else ok = s >= r0 && s >= lo && (uint64_t)s + n <= hi && (uint64_t)(s - lo) + r0 + n <= rows;
+      if (!ok) std::snprintf(b, sizeof b, "group scratch: %s %u (+%u) of traverser %d outside chunks [%u, %u)", what, s, n, p, c0, c1);
+      return ok;
+    };
+    auto seg_ok = [&](uint32_t g, uint32_t c0, uint32_t c1, bool top) -> bool {
+      if (!check(P.seg_pslot[(size_t)p * P.seg_root.size() + g], 1, false, c0, c1, "parent slot")) return false;
+      for (uint32_t m = 0; m < P.seg_mem_cnt[g]; m++) {
+        const MemRec& r = P.mrec[P.seg_mem_off[g] + m];
+        if (top && P.chunk_of[r.node] != kNone) {
+          std::snprintf(b, sizeof b, "group scratch: node %u of the top segment is in a chunk", r.node);
+          return false;
+        }
+        if (!check(r.slot[p], 1, false, c0, c1, "slot")) return false;
+        if (r.cslot[p] != kNone && !check(r.cslot[p], r.na, false, c0, c1, "child slots")) return false;
+        if (r.vslot != kNone && !check(r.vslot, 1, true, c0, c1, "value row")) return false;
+      }
+      return true;
+    };
+    auto term_ok = [&](uint32_t t, uint32_t c0, uint32_t c1) {
+      const TermRec& r = P.trec[t];
+      return check(r.slot[p], 1, false, c0, c1, "terminal slot") && check(r.vslot, 1, true, c0, c1, "terminal row");
+    };
+    auto chance_ok = [&](uint32_t i, uint32_t c0, uint32_t c1) {
+      const MemRec& r = P.crec[i];
+      return check(r.slot[p], 1, false, c0, c1, "chance slot") && check(r.vslot, 1, true, c0, c1, "chance row") &&
+             check(P.chance_child_vslot[i], r.na, true, c0, c1, "chance child rows");
+    };
+    // the top segment and its chance records use resident rows only
+    for (uint32_t g = P.depth_off[0]; g < P.depth_off[1]; g++)
+      if (!seg_ok(g, 0, 0, true)) return std::string(b);
+    for (uint32_t i = P.chance_off[0]; i < P.chance_off[1]; i++)
+      if (!chance_ok(i, 0, 0)) return std::string(b);
+    if (C == 0) {
+      for (uint32_t t = 0; t < (uint32_t)P.trec.size(); t++)
+        if (!term_ok(t, 0, 0)) return std::string(b);
+      continue;
+    }
+    std::vector<uint8_t> seen_t(P.trec.size(), 0);
+    for (const ChunkGroup& g : G) {
+      for (uint32_t d = 1; d < D; d++) {
+        const uint32_t* so = P.seg_coff.data() + (size_t)d * (C + 1);
+        const uint32_t* co = P.ch_coff.data() + (size_t)d * (C + 1);
+        for (uint32_t s = so[g.c0]; s < so[g.c1]; s++)
+          if (!seg_ok(s, g.c0, g.c1, false)) return std::string(b);
+        for (uint32_t i = co[g.c0]; i < co[g.c1]; i++)
+          if (!chance_ok(i, g.c0, g.c1)) return std::string(b);
+      }
+      const uint32_t t0 = g.c0 == 0 ? 0u : P.term_coff[g.c0];
+      for (uint32_t t = t0; t < P.term_coff[g.c1]; t++) {
+        if (!term_ok(t, g.c0, g.c1)) return std::string(b);
+        seen_t[t] = 1;
+      }
+    }
+    for (size_t t = 0; t < seen_t.size(); t++)
+      if (!seen_t[t]) return fail("terminal outside every group", t);
+    (void)N;
   }
   return "";
 }

@@ -112,6 +112,19 @@ struct Plan {
   std::vector<MemRec> crec;          // one per chance node (vslot: its own row; offset: first child; cslot: children)
   std::vector<uint32_t> chance_child_vslot;  // first child's value row, parallel to crec
   uint32_t num_slots[2] = {0, 0}, num_vslots = 0, max_members = 0;
+  // chunks: the subtrees below the root street's chance nodes (the depth-1
+  // segments and their descendants), run in groups that share scratch.
+  // Reach slots below slot_s0[p] and value rows below val_v0 are resident;
+  // chunk c owns slots [slot_coff[p][c], slot_coff[p][c+1]) and rows
+  // [val_coff[c], val_coff[c+1]).
+  uint32_t num_chunks = 0;
+  std::vector<uint32_t> chunk_of;      // per node: chunk, or kNone in the top segment
+  std::vector<uint32_t> seg_coff;      // [d * (C+1) + c], d >= 1: first segment of chunk c at depth d
+  std::vector<uint32_t> ch_coff;       // [d * (C+1) + c], d >= 1: first chance record of chunk c at depth d
+  std::vector<uint32_t> term_coff;     // [c]: first terminal of chunk c ([0]: after the top segment's)
+  std::vector<uint32_t> slot_coff[2];  // [c], c <= C
+  std::vector<uint32_t> val_coff;      // [c], c <= C
+  uint32_t slot_s0[2] = {0, 0}, val_v0 = 0;
   // hands
   std::vector<uint8_t> cards[2];
   std::vector<float> weights[2];
@@ -143,5 +156,21 @@ struct Plan {
 
 // Builds and validates the plan; returns "" or the reason it is invalid.
 std::string build_plan(const PlanInput& in, Plan& out);
+
+// Chunk groups [c0, c1) and the scratch rows they need.
+struct ChunkGroup {
+  uint32_t c0, c1;
+};
+struct ScratchRows {
+  uint64_t reach_bytes = 0, val_bytes = 0, alive_rows = 0;
+  uint64_t slot_rows[2] = {0, 0}, val_rows = 0;  // rows per traverser / value rows
+};
+// Consecutive chunks whose own reach slots and value rows (of `es`-byte
+// elements, rows of `rowlen`) fit `budget` beside the resident ones; a chunk
+// that alone exceeds it gets a group of its own.
+ScratchRows partition_chunks(const Plan& P, size_t es, uint32_t rowlen, uint64_t budget, std::vector<ChunkGroup>& groups);
+// Checks that every slot and row a grouped traversal touches maps inside the
+// scratch rows (the shift the kernels apply); returns "" or the first failure.
+std::string check_groups(const Plan& P, const std::vector<ChunkGroup>& groups, const ScratchRows& rows);
 
 }  // namespace pts

@@ -61,7 +61,7 @@ EXPORTS = [
     "pts_solver_iterate", "pts_solver_exploitability", "pts_solver_root_values", "pts_solver_iterations",
     "pts_solver_set_iterations", "pts_solver_storage_len", "pts_solver_num_hands", "pts_solver_get_storage",
     "pts_solver_set_storage", "pts_solver_profile_iteration", "pts_engine_profile_iteration", "pts_solver_num_nodes", "pts_solver_nodes", "pts_solver_query",
-    "pts_solver_ranges_after",
+    "pts_solver_ranges_after", "pts_engine_num_groups", "pts_solver_num_groups", "pts_solver_plan_groups",
 ]
 
 
@@ -87,7 +87,8 @@ def _declare(L):
     L.pts_last_error.argtypes = [ctypes.c_char_p, ctypes.c_size_t]
     L.pts_device_count.argtypes = [ctypes.POINTER(ctypes.c_int)]
     L.pts_backend_wants_gpu.argtypes = [ctypes.c_int, ctypes.c_int]
-    L.pts_engine_create.argtypes = [ctypes.c_int, ctypes.POINTER(TreeDescC), f32p, f32p, ctypes.c_int, ctypes.POINTER(P)]
+    L.pts_engine_create.argtypes = [ctypes.c_int, ctypes.POINTER(TreeDescC), f32p, f32p, ctypes.c_int, ctypes.c_uint64,
+                                    ctypes.POINTER(P)]
     L.pts_engine_iterate.argtypes = [P, ctypes.c_float, ctypes.c_float, ctypes.c_float, ctypes.c_int]
     L.pts_engine_traverse.argtypes = [P, ctypes.c_int, ctypes.c_int, ctypes.c_float, ctypes.c_float, ctypes.c_float,
                                       ctypes.c_int, f64p, ctypes.c_size_t]
@@ -100,7 +101,13 @@ def _declare(L):
     L.pts_engine_device_bytes.argtypes = [P]
     L.pts_engine_destroy.argtypes = [P]
     L.pts_solver_estimate.argtypes = [ctypes.POINTER(ConfigC), ctypes.POINTER(StatsC)]
-    L.pts_solver_new.argtypes = [ctypes.POINTER(ConfigC), ctypes.c_int, ctypes.c_int, ctypes.POINTER(P)]
+    L.pts_solver_new.argtypes = [ctypes.POINTER(ConfigC), ctypes.c_int, ctypes.c_int, ctypes.c_uint64, ctypes.POINTER(P)]
+    L.pts_solver_num_groups.restype = ctypes.c_uint32
+    L.pts_solver_num_groups.argtypes = [P, ctypes.c_int]
+    L.pts_solver_plan_groups.argtypes = [P, ctypes.c_uint64, ctypes.c_int, ctypes.POINTER(ctypes.c_uint32),
+                                         ctypes.POINTER(ctypes.c_uint64)]
+    L.pts_engine_num_groups.restype = ctypes.c_uint32
+    L.pts_engine_num_groups.argtypes = [P, ctypes.c_int]
     L.pts_solver_free.argtypes = [P]
     L.pts_solver_stats.argtypes = [P, ctypes.POINTER(StatsC)]
     L.pts_solver_solve.argtypes = [P, ctypes.c_uint32, ctypes.c_double, ctypes.c_uint32, ctypes.POINTER(ProgressC),

@@ -61,12 +61,14 @@ class NodeInfo:
 class Solver:
     """A built game whose storage lives in HBM; solves run in HIP kernels."""
 
-    def __init__(self, cfg: SolveConfig, device: int = 0, precision: int = 0):
-        """precision: 0 fast (default, as the reference's GpuPrecision), 1 exact."""
+    def __init__(self, cfg: SolveConfig, device: int = 0, precision: int = 0, scratch_budget: int = 0):
+        """precision: 0 fast (default, as the reference's GpuPrecision), 1 exact.
+        scratch_budget: bytes of reach / value rows (0: the HBM left after the
+        storage); the turn or river subtrees run in groups that fit."""
         self.cfg = cfg
         self._c = cfg.to_c()
         h = ctypes.c_void_p()
-        check(lib().pts_solver_new(ctypes.byref(self._c), device, precision, ctypes.byref(h)))
+        check(lib().pts_solver_new(ctypes.byref(self._c), device, precision, scratch_budget, ctypes.byref(h)))
         self.h = h
         self.n = [lib().pts_solver_num_hands(self.h, p) for p in range(2)]
 
@@ -93,6 +95,10 @@ class Solver:
     @property
     def storage_len(self):
         return lib().pts_solver_storage_len(self.h)
+
+    def num_groups(self, f64: bool = False) -> int:
+        """Chunk groups a traversal runs in (DCFR rows, or FP64 best-response rows)."""
+        return lib().pts_solver_num_groups(self.h, int(f64))
 
     @property
     def iterations(self):
@@ -219,11 +225,11 @@ class TreeArrays:
 class Engine:
     """pts_engine_*: the GPU backend boundary (crates/solver/src/gpu/engine.rs)."""
 
-    def __init__(self, tree: TreeArrays, regrets, sums, device: int = 0, precision: int = 0):
+    def __init__(self, tree: TreeArrays, regrets, sums, device: int = 0, precision: int = 0, scratch_budget: int = 0):
         self.tree = tree
         h = ctypes.c_void_p()
         check(lib().pts_engine_create(device, ctypes.byref(tree.desc), np.ascontiguousarray(regrets, np.float32),
-                                      np.ascontiguousarray(sums, np.float32), precision, ctypes.byref(h)))
+                                      np.ascontiguousarray(sums, np.float32), precision, scratch_budget, ctypes.byref(h)))
         self.h = h
         self.storage_len = tree.desc.storage_len
         self.n = [tree.desc.num_hands[0], tree.desc.num_hands[1]]
@@ -269,3 +275,6 @@ class Engine:
 
     def device_bytes(self):
         return lib().pts_engine_device_bytes(self.h)
+
+    def num_groups(self, f64: bool = False) -> int:
+        return lib().pts_engine_num_groups(self.h, int(f64))

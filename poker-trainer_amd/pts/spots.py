@@ -145,3 +145,29 @@ def flop_spot(seed: int = 0) -> SolveConfig:
 
 
 ITERATIONS = {"river": 1000, "turn": 500, "flop": 300}
+
+
+def small_flop_spot(seed: int = 0, classes: int = 14) -> SolveConfig:
+    """A flop spot small enough for the oracle within seconds: the flop
+    board of `flop_spot(seed)`, pot 6bb, stacks 97bb, bet 75% only, no raises
+    beyond one, and ranges of the `classes` best-scoring 169-grid classes of
+    each player (weights as in `flop_spot`)."""
+    b = board(stream(2, seed, "board"), 3)
+    sr = stream(2, seed, "score")
+    score = [sr.normal() for _ in range(169)]
+    keep = sorted(range(169), key=lambda c: -score[c])[:classes]
+    r = []
+    for p in range(2):
+        w = class_range(score, stream(2, seed, f"range{p}"))
+        mask = [0.0] * 1326
+        for c in keep:
+            for k in HandClass.from_grid_index(c).combos():
+                mask[k] = 1.0
+        r.append([x * m if not _blocked(k, b) else 0.0 for k, (x, m) in enumerate(zip(w, mask))])
+    return SolveConfig(b, r, BettingState.street_start(FLOP, 600, 9700, 9700),
+                       PostflopMenu.one(["75%"], [], max_raises=1), max_memory_bytes=1 << 40, dcfr=DcfrParams())
+
+
+def _blocked(k: int, b: List[int]) -> bool:
+    a, c = COMBOS[k]
+    return a in b or c in b

@@ -222,3 +222,52 @@ def test_fast_turn_converges_like_the_oracle():
     ef, ec = f.exploitability(), c.exploitability_f64()
     # exploitability within 0.01% of the pot after the same iterations
     assert abs(ef - ec) < 1e-4 * cfg.root.pot, (ef, ec)
+
+
+# ---------------------------------------------------------------- chunk groups
+def test_grouped_turn_solve_bit_exact():
+    # A 16 MiB scratch budget: the 720 river subtrees run in 5 (DCFR) and 17
+    # (FP64 best response) groups that share the scratch rows.
+    cfg = spots.turn_spot(0)
+    g, c = Solver(cfg, precision=EXACT, scratch_budget=16 << 20), O.OracleSolver(cfg)
+    assert g.num_groups() == 5 and g.num_groups(True) == 17
+    g.iterate(6)
+    c.iterate(6)
+    rg, sg = g.storage()
+    rc, sc = c.storage()
+    assert same(rg, rc), describe(rg, rc)
+    assert same(sg, sc), describe(sg, sc)
+    assert g.exploitability() == c.exploitability_f64()
+    path = [c.query([]).actions[0]]
+    path = path + [c.query(path).actions[0]]
+    card = next(x for x in range(52) if x not in cfg.board)
+    for pth in ([], path + [("card", card)]):
+        a, b = g.query(pth), c.query(pth)
+        assert same(a.strategy, b.strategy) and same(a.ev, b.ev), describe(a.ev, b.ev)
+
+
+@pytest.mark.parametrize("seed", [0, 1])
+def test_small_flop_solve_bit_exact(seed):
+    # flop -> 49 turns -> 48 rivers, in groups of turn subtrees
+    cfg = spots.small_flop_spot(seed)
+    g, c = Solver(cfg, precision=EXACT, scratch_budget=4 << 20), O.OracleSolver(cfg)
+    assert g.num_groups() > 10
+    g.iterate(4)
+    c.iterate(4)
+    rg, sg = g.storage()
+    rc, sc = c.storage()
+    assert same(rg, rc), describe(rg, rc)
+    assert same(sg, sc), describe(sg, sc)
+    assert g.exploitability() == c.exploitability_f64()
+
+
+def test_grouping_does_not_change_the_fast_solve():
+    cfg = spots.small_flop_spot(1)
+    a, b = Solver(cfg, precision=FAST), Solver(cfg, precision=FAST, scratch_budget=4 << 20)
+    assert a.num_groups() == 1 and b.num_groups() > 10
+    a.iterate(12)
+    b.iterate(12)
+    ra, sa = a.storage()
+    rb, sb = b.storage()
+    assert same(ra, rb) and same(sa, sb)
+    assert a.exploitability() == b.exploitability()

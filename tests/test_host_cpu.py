@@ -42,7 +42,7 @@ def test_struct_layouts_match_the_header():
 def host_solver(cfg):
     c = cfg.to_c()
     h = ctypes.c_void_p()
-    check(lib().pts_solver_new(ctypes.byref(c), -1, 1, ctypes.byref(h)))
+    check(lib().pts_solver_new(ctypes.byref(c), -1, 1, 0, ctypes.byref(h)))
     return h
 
 
@@ -153,7 +153,7 @@ def oracle_tree(cfg):
 def create_status(t, n):
     h = ctypes.c_void_p()
     z = np.zeros(n, np.float32)
-    st = lib().pts_engine_create(0, ctypes.byref(t.desc), z, z, 1, ctypes.byref(h))
+    st = lib().pts_engine_create(0, ctypes.byref(t.desc), z, z, 1, 0, ctypes.byref(h))
     if st == 0:
         lib().pts_engine_destroy(h)
     return st
@@ -227,3 +227,50 @@ def test_spot_generators_are_deterministic():
     r = spots.river_spot(0)
     nz = sum(1 for x in r.ranges[0] if x > 0)
     assert 500 < nz < 800  # 1081 unblocked combos, 40% zeroed
+
+
+# ---------------------------------------------------------------- chunk groups
+def plan_groups(h, budget, f64):
+    n, b = ctypes.c_uint32(), ctypes.c_uint64()
+    check(lib().pts_solver_plan_groups(h, budget, f64, ctypes.byref(n), ctypes.byref(b)))
+    return n.value, b.value
+
+
+@pytest.mark.parametrize("name,make", [("river", lambda: spots.river_spot(0)), ("turn", lambda: spots.turn_spot(0)),
+                                       ("flop", lambda: spots.small_flop_spot(0))])
+def test_chunk_groups_map_every_row_into_the_scratch(name, make):
+    # Every reach slot and value row a grouped traversal touches maps inside
+    # the scratch the groups were sized for, at every budget (the check runs
+    # on the host before an engine would launch anything).
+    h = host_solver(make())
+    try:
+        for f64 in (0, 1):
+            one, full = plan_groups(h, 1 << 62, f64)
+            assert one == 1
+            most, least = plan_groups(h, 0, f64)
+            assert least <= full
+            if name == "river":
+                assert most == 1 and least == full
+                continue
+            assert most > 1
+            for budget in (full // 7, full // 3, full // 2):
+                n, b = plan_groups(h, budget, f64)
+                assert 1 <= n <= most
+                assert b <= max(budget, least)
+    finally:
+        lib().pts_solver_free(h)
+
+
+def test_flop_plan_has_one_chunk_per_turn_subtree():
+    cfg = spots.small_flop_spot(0)
+    h = host_solver(cfg)
+    try:
+        most, _ = plan_groups(h, 0, 0)
+        nodes = product_nodes(h)
+        # chunks are the chance children of the flop's chance nodes
+        kind, fc, nc = nodes["kind"], nodes["first_child"], nodes["num_children"]
+        street = nodes["street"]
+        flop_chance = [i for i in range(len(kind)) if kind[i] == 2 and street[i] == 1]  # street 1: flop
+        assert most == sum(int(nc[i]) for i in flop_chance)
+    finally:
+        lib().pts_solver_free(h)
