@@ -1035,18 +1035,31 @@ Status Engine::create(int device, Plan&& plan, const float* regrets, const float
   HIPCHK(hipMemGetInfo(&fr, &tot));
   uint64_t budget = fr > margin ? fr - margin : 0;
   if (scratch_budget != 0) budget = std::min<uint64_t>(budget, scratch_budget);
+  // The buffers hold the largest reach, value and alive rows of any group of
+  // either row type; groups are sized per component, so shrink the budget
+  // until the combination fits it (or every group is a single chunk).
   ScratchRows need{};
+  for (int attempt = 0;; attempt++) {
+    need = ScratchRows{};
+    bool single = true;
+    for (int f64 = 0; f64 < 2; f64++) {
+      ScratchRows r = partition_chunks(P, f64 ? 8 : 4, rowlen, budget, e->groups_[f64]);
+      need.reach_bytes = std::max(need.reach_bytes, r.reach_bytes);
+      need.val_bytes = std::max(need.val_bytes, r.val_bytes);
+      need.alive_rows = std::max(need.alive_rows, r.alive_rows);
+      single = single && e->groups_[f64].size() == std::max<size_t>(P.num_chunks, 1);
+    }
+    const uint64_t total = need.reach_bytes + need.val_bytes + need.alive_rows * 4;
+    if (total <= budget || single || attempt >= 40) break;
+    budget = budget / 8 * 7;
+  }
   for (int f64 = 0; f64 < 2; f64++) {
-    const size_t es = f64 ? 8 : 4;
-    ScratchRows r = partition_chunks(P, es, rowlen, budget, e->groups_[f64]);
+    ScratchRows r = partition_chunks(P, f64 ? 8 : 4, rowlen, budget, e->groups_[f64]);
     std::string err = check_groups(P, e->groups_[f64], r);
     if (!err.empty()) {
       set_error("invalid input: " + err);
       return kInvalidInput;
     }
-    need.reach_bytes = std::max(need.reach_bytes, r.reach_bytes);
-    need.val_bytes = std::max(need.val_bytes, r.val_bytes);
-    need.alive_rows = std::max(need.alive_rows, r.alive_rows);
   }
   if ((st = e->alloc(bre, need.reach_bytes)) != kOk) return st;
   if ((st = e->alloc(bv, need.val_bytes)) != kOk) return st;
@@ -1070,20 +1083,29 @@ Status Engine::create(int device, Plan&& plan, const float* regrets, const float
     e->lds_up_[f64] = mm * sizeof(MemRec) + ((mm + 15) & ~size_t(15)) + mm * UP_T * vs;
   }
   size_t lds_max = std::max({e->lds_term_[1], e->lds_up_[1], e->lds_down_});
-  if (lds_max > 160 * 1024) {
+  if (lds_max > 159 * 1024) {  // 160 KiB per workgroup, less the kernels' static arrays
     set_error("invalid input: a segment or strength table is too large for the LDS");
     return kInvalidInput;
   }
-  if (lds_max > 64 * 1024) {
-    HIPCHK(hipFuncSetAttribute((const void*)terminal_kernel<double>, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
-    HIPCHK(hipFuncSetAttribute((const void*)terminal_kernel<float>, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
-    HIPCHK(hipFuncSetAttribute((const void*)terminal_fast_kernel<double>, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
-    HIPCHK(hipFuncSetAttribute((const void*)terminal_fast_kernel<float>, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
-    HIPCHK(hipFuncSetAttribute((const void*)up_kernel<double, kBest>, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
-    HIPCHK(hipFuncSetAttribute((const void*)up_kernel<double, kAverage>, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
-    HIPCHK(hipFuncSetAttribute((const void*)up_kernel<float, kAverage>, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
-    HIPCHK(hipFuncSetAttribute((const void*)up_kernel<float, kCfr>, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
-  }
+  // dynamic LDS beyond the default 64 KiB is requested per kernel, for
+  // exactly the bytes it is launched with
+  auto lds_attr = [&](const void* f, size_t bytes) -> Status {
+    if (bytes > 64 * 1024) HIPCHK(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
+    return kOk;
+  };
+  const bool fastp = precision == kFast;
+  if ((st = lds_attr(fastp ? (const void*)terminal_fast_kernel<double> : (const void*)terminal_kernel<double>,
+                     e->lds_term_[1])) != kOk ||
+      (st = lds_attr(fastp ? (const void*)terminal_fast_kernel<float> : (const void*)terminal_kernel<float>,
+                     e->lds_term_[0])) != kOk ||
+      (st = lds_attr((const void*)up_kernel<double, kBest>, e->lds_up_[1])) != kOk ||
+      (st = lds_attr((const void*)up_kernel<double, kAverage>, e->lds_up_[1])) != kOk ||
+      (st = lds_attr((const void*)up_kernel<float, kAverage>, e->lds_up_[0])) != kOk ||
+      (st = lds_attr((const void*)up_kernel<float, kCfr>, e->lds_up_[0])) != kOk ||
+      (st = lds_attr((const void*)down_kernel<float, kCfr>, e->lds_down_)) != kOk ||
+      (st = lds_attr((const void*)down_kernel<double, kAverage>, e->lds_down_)) != kOk ||
+      (st = lds_attr((const void*)down_kernel<float, kAverage>, e->lds_down_)) != kOk)
+    return st;
   const char* ng = std::getenv("PTS_NO_GRAPH");
   e->use_graph_ = !(ng && ng[0] == '1');
   if (e->use_graph_ && (st = e->build_graphs()) != kOk) return st;
