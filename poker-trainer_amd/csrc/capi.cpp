@@ -104,8 +104,28 @@ static Menu menu_of(const pts_menu_c& m) {
 
 static const int kPairs[6][2] = {{0, 1}, {0, 2}, {0, 3}, {1, 2}, {1, 3}, {2, 3}};
 
-// Whether the ranges are symmetric under some suit swap (crates/solver/src/iso.rs:245-273).
-static bool any_symmetric_pair(const Hands& h) {
+// Whether suit isomorphism would merge any chance outcomes: some suit pair
+// leaves both ranges unchanged (crates/solver/src/iso.rs:245-273) and has
+// equal rank sets on the board of some chance node (iso.rs:277-300): the
+// root board, and for a flop root also every flop + turn board.
+static bool isomorphism_applies(const Hands& h, const int* board, int n) {
+  auto masks = [](const std::vector<int>& b, uint16_t m[4]) {
+    m[0] = m[1] = m[2] = m[3] = 0;
+    for (int c : b) m[c & 3] |= (uint16_t)(1u << (c >> 2));
+  };
+  std::vector<std::vector<int>> boards;
+  if (n == 5) return false;  // a river root has no chance nodes
+  std::vector<int> root(board, board + n);
+  boards.push_back(root);
+  if (n == 3) {
+    uint64_t used = 0;
+    for (int c : root) used |= 1ull << c;
+    for (int t = 0; t < 52; t++)
+      if (!(used >> t & 1)) {
+        boards.push_back(root);
+        boards.back().push_back(t);
+      }
+  }
   for (auto& pr : kPairs) {
     bool ok = true;
     for (int p = 0; p < 2 && ok; p++) {
@@ -123,7 +143,12 @@ static bool any_symmetric_pair(const Hands& h) {
         if (j == kNone || std::fabs(h.weights[p][i] - h.weights[p][j]) > 1e-4f * mx) ok = false;
       }
     }
-    if (ok) return true;
+    if (!ok) continue;
+    for (auto& b : boards) {
+      uint16_t m[4];
+      masks(b, m);
+      if (m[pr[0]] == m[pr[1]]) return true;
+    }
   }
   return false;
 }
@@ -147,8 +172,9 @@ static std::string validate(const pts_config_c& c, TreeSpec& spec, Hands& hands)
   for (int p = 0; p < 2; p++)
     if (hands.n(p) == 0) return "player " + std::to_string(p) + " has an empty range";
   if (hands.num_combinations <= 0.0) return "player 0 has an empty range";
-  if (c.isomorphism && any_symmetric_pair(hands))
-    return "invalid input: suit isomorphism is not supported by the GPU plan (set isomorphism = false)";
+  if (c.isomorphism && isomorphism_applies(hands, c.board, n))
+    return "invalid input: suit isomorphism would merge chance outcomes of this spot, which the GPU plan does not "
+           "support (set isomorphism = false)";
   spec.root_street = (Street)street;
   spec.board.assign(c.board, c.board + n);
   Betting& b = spec.root;
