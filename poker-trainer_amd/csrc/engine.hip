@@ -1028,31 +1028,17 @@ Status Engine::create(int device, Plan&& plan, const float* regrets, const float
   }
 #undef AT
   // storage, then the scratch rows: chunk groups sized to what is left
-  DeviceBuf br, bs, bre, bv, ba, bp;
+  DeviceBuf br, bs, bre, bp;
   if ((st = e->alloc(br, P.storage_len * sizeof(float))) != kOk) return st;
   if ((st = e->alloc(bs, P.storage_len * sizeof(float))) != kOk) return st;
   if ((st = e->alloc(bp, sizeof(IterParams) + 2 * rowlen * 8)) != kOk) return st;
   HIPCHK(hipMemGetInfo(&fr, &tot));
   uint64_t budget = fr > margin ? fr - margin : 0;
   if (scratch_budget != 0) budget = std::min<uint64_t>(budget, scratch_budget);
-  // The buffers hold the largest reach, value and alive rows of any group of
-  // either row type; groups are sized per component, so shrink the budget
-  // until the combination fits it (or every group is a single chunk).
-  ScratchRows need{};
-  for (int attempt = 0;; attempt++) {
-    need = ScratchRows{};
-    bool single = true;
-    for (int f64 = 0; f64 < 2; f64++) {
-      ScratchRows r = partition_chunks(P, f64 ? 8 : 4, rowlen, budget, e->groups_[f64]);
-      need.reach_bytes = std::max(need.reach_bytes, r.reach_bytes);
-      need.val_bytes = std::max(need.val_bytes, r.val_bytes);
-      need.alive_rows = std::max(need.alive_rows, r.alive_rows);
-      single = single && e->groups_[f64].size() == std::max<size_t>(P.num_chunks, 1);
-    }
-    const uint64_t total = need.reach_bytes + need.val_bytes + need.alive_rows * 4;
-    if (total <= budget || single || attempt >= 40) break;
-    budget = budget / 8 * 7;
-  }
+  // One scratch pool; the f32 (DCFR) and f64 (best response, average) row
+  // types each lay out their reach rows, value rows and alive flags in it
+  // (they never run at the same time), grouped to fit the budget.
+  size_t lay[2][3];
   for (int f64 = 0; f64 < 2; f64++) {
     ScratchRows r = partition_chunks(P, f64 ? 8 : 4, rowlen, budget, e->groups_[f64]);
     std::string err = check_groups(P, e->groups_[f64], r);
@@ -1060,15 +1046,20 @@ Status Engine::create(int device, Plan&& plan, const float* regrets, const float
       set_error("invalid input: " + err);
       return kInvalidInput;
     }
+    lay[f64][0] = 0;
+    lay[f64][1] = (r.reach_bytes + 255) & ~size_t(255);
+    lay[f64][2] = lay[f64][1] + ((r.val_bytes + 255) & ~size_t(255));
+    e->pool_bytes_[f64] = lay[f64][2] + r.alive_rows * 4;
   }
-  if ((st = e->alloc(bre, need.reach_bytes)) != kOk) return st;
-  if ((st = e->alloc(bv, need.val_bytes)) != kOk) return st;
-  if ((st = e->alloc(ba, need.alive_rows * 4)) != kOk) return st;
+  const size_t pool = std::max(e->pool_bytes_[0], e->pool_bytes_[1]);
+  if ((st = e->alloc(bre, pool)) != kOk) return st;
   e->regrets_ = (float*)br.p;
   e->sums_ = (float*)bs.p;
-  e->reach_ = bre.p;
-  e->vals_ = bv.p;
-  e->alive_ = (uint32_t*)ba.p;
+  for (int f64 = 0; f64 < 2; f64++) {
+    e->reach_[f64] = (char*)bre.p + lay[f64][0];
+    e->vals_[f64] = (char*)bre.p + lay[f64][1];
+    e->alive_[f64] = (uint32_t*)((char*)bre.p + lay[f64][2]);
+  }
   e->params_ = (IterParams*)bp.p;
   HIPCHK(hipHostMalloc((void**)&e->params_host_, sizeof(IterParams)));
   if ((st = e->upload(regrets, sums)) != kOk) return st;
@@ -1237,24 +1228,28 @@ Status Engine::launch_traversal(int p, Mode mode, bool f64, ProfRec* prof) {
   Span span{0, 1, (uint32_t)P.num_nodes, 0};
   const uint32_t nq = P.n[o];
   PROF_START(kKBegin);
-  if (f64) hipLaunchKernelGGL((begin_kernel<double>), dim3((nq + 255) / 256), dim3(256), 0, stream_, nq, weights_[o], (double*)reach_, 0u, alive_, 1u);
-  else hipLaunchKernelGGL((begin_kernel<float>), dim3((nq + 255) / 256), dim3(256), 0, stream_, nq, weights_[o], (float*)reach_, 0u, alive_, 1u);
+  const int x = f64 ? 1 : 0;
+  void* reach = reach_[x];
+  void* vals = vals_[x];
+  uint32_t* alive = alive_[x];
+  if (f64) hipLaunchKernelGGL((begin_kernel<double>), dim3((nq + 255) / 256), dim3(256), 0, stream_, nq, weights_[o], (double*)reach, 0u, alive, 1u);
+  else hipLaunchKernelGGL((begin_kernel<float>), dim3((nq + 255) / 256), dim3(256), 0, stream_, nq, weights_[o], (float*)reach, 0u, alive, 1u);
   PROF_STOP();
   if (mode == kCfr)
-    launch_span<float, kCfr, kCfr>(d, P, stream_, p, span, 0, groups_[0], reach_, alive_, regrets_, sums_, params_, vals_, rowlen_, lds_down_,
+    launch_span<float, kCfr, kCfr>(d, P, stream_, p, span, 0, groups_[0], reach, alive, regrets_, sums_, params_, vals, rowlen_, lds_down_,
                                    lds_term_[0], lds_up_[0], (float*)nullptr, precision_ == kFast, prof);
   else if (mode == kBest)
-    launch_span<double, kAverage, kBest>(d, P, stream_, p, span, 0, groups_[1], reach_, alive_, regrets_, sums_, params_, vals_, rowlen_,
+    launch_span<double, kAverage, kBest>(d, P, stream_, p, span, 0, groups_[1], reach, alive, regrets_, sums_, params_, vals, rowlen_,
                                          lds_down_, lds_term_[1], lds_up_[1], (double*)nullptr, precision_ == kFast);
   else if (f64)
-    launch_span<double, kAverage, kAverage>(d, P, stream_, p, span, 0, groups_[1], reach_, alive_, regrets_, sums_, params_, vals_, rowlen_,
+    launch_span<double, kAverage, kAverage>(d, P, stream_, p, span, 0, groups_[1], reach, alive, regrets_, sums_, params_, vals, rowlen_,
                                             lds_down_, lds_term_[1], lds_up_[1], (double*)nullptr, precision_ == kFast);
   else
-    launch_span<float, kAverage, kAverage>(d, P, stream_, p, span, 0, groups_[0], reach_, alive_, regrets_, sums_, params_, vals_, rowlen_,
+    launch_span<float, kAverage, kAverage>(d, P, stream_, p, span, 0, groups_[0], reach, alive, regrets_, sums_, params_, vals, rowlen_,
                                            lds_down_, lds_term_[0], lds_up_[0], (float*)nullptr, precision_ == kFast);
   // keep the root row of p
   size_t vs = f64 ? 8 : 4;
-  HIPCHK(hipMemcpyAsync((char*)params_ + result_offset(p, rowlen_), (char*)vals_ + (size_t)P.vslot[0] * rowlen_ * vs,
+  HIPCHK(hipMemcpyAsync((char*)params_ + result_offset(p, rowlen_), (char*)vals + (size_t)P.vslot[0] * rowlen_ * vs,
                         (size_t)P.n[p] * vs, hipMemcpyDeviceToDevice, stream_));
   HIPCHK(hipGetLastError());
   return kOk;
@@ -1397,13 +1392,13 @@ Status Engine::subtree_values(uint32_t node, int q, const std::vector<float>& op
   if (slot >= P.slot_s0[q] && qc != kNone)
     for (const ChunkGroup& g : groups_[0])
       if (qc >= g.c0 && qc < g.c1) slot -= P.slot_coff[q][g.c0] - P.slot_s0[q];
-  hipLaunchKernelGGL((begin_kernel<float>), dim3((nq + 255) / 256), dim3(256), 0, stream_, nq, (const float*)dreach, (float*)reach_,
-                     slot, alive_, any ? 1u : 0u);
+  hipLaunchKernelGGL((begin_kernel<float>), dim3((nq + 255) / 256), dim3(256), 0, stream_, nq, (const float*)dreach, (float*)reach_[0],
+                     slot, alive_[0], any ? 1u : 0u);
   Span span{node, P.first_child[node], P.subtree_end[node], P.seg_of[node]};
   // depths from the node's segment on
   uint32_t seg = P.seg_of[node], first_depth = 0;
   while (first_depth + 1 < P.depth_off.size() && P.depth_off[first_depth + 1] <= seg) first_depth++;
-  launch_span<float, kAverage, kAverage>(dp_->d, P, stream_, q, span, first_depth, groups_[0], reach_, alive_, regrets_, sums_, params_, vals_,
+  launch_span<float, kAverage, kAverage>(dp_->d, P, stream_, q, span, first_depth, groups_[0], reach_[0], alive_[0], regrets_, sums_, params_, vals_[0],
                                          rowlen_, lds_down_, lds_term_[0], lds_up_[0], dout, precision_ == kFast);
   HIPCHK(hipGetLastError());
   child_vals.assign((size_t)na * np, 0.0f);

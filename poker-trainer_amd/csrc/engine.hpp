@@ -106,9 +106,11 @@ class Engine {
   void* dplan_blob_ = nullptr;
   float* regrets_ = nullptr;
   float* sums_ = nullptr;
-  void* reach_ = nullptr;
-  void* vals_ = nullptr;
-  uint32_t* alive_ = nullptr;
+  // scratch pool: reach rows, value rows and alive flags for f32 [0] and f64 [1] rows
+  void* reach_[2] = {nullptr, nullptr};
+  void* vals_[2] = {nullptr, nullptr};
+  uint32_t* alive_[2] = {nullptr, nullptr};
+  size_t pool_bytes_[2] = {0, 0};
   float* weights_[2] = {nullptr, nullptr};
   IterParams* params_ = nullptr;
   IterParams* params_host_ = nullptr;
