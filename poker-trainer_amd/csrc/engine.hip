@@ -54,6 +54,7 @@ struct DPlan {
   const float* seg_factor;
   const MemRec* mrec;
   const TermRec* trec;
+  const TermBatch* tbatch;
   const MemRec* crec;
   const uint32_t* chance_child_vslot;
   const uint8_t* cards[2];
@@ -80,6 +81,11 @@ struct Bias {
   __device__ __forceinline__ uint32_t s(uint32_t x) const { return x < s0 ? x : x - sb; }
   __device__ __forceinline__ uint32_t v(uint32_t x) const { return x < v0 ? x : x - vb; }
 };
+
+// Per-player fields of a record held in registers: selects, not a runtime
+// index (which would put the record in scratch memory).
+__device__ __forceinline__ uint32_t pick(const uint32_t (&a)[2], int p) { return p ? a[1] : a[0]; }
+__device__ __forceinline__ int32_t pick(const int32_t (&a)[2], int p) { return p ? a[1] : a[0]; }
 
 template <typename V> __device__ __forceinline__ V pos0(V x) { return x > V(0) ? x : V(0); }
 template <typename V> __device__ __forceinline__ V recip(V d) {  // cfr.rs:113-121
@@ -328,7 +334,7 @@ __global__ void __launch_bounds__(TERM_T) terminal_kernel(DPlan P, int p, uint32
   __shared__ double tot[53];
   const TermRec tr = P.trec[t0 + blockIdx.x];
   if (!span.has(tr.node)) return;
-  const uint32_t s = B.s(tr.slot[p]);
+  const uint32_t s = B.s(pick(tr.slot, p));
   if (!alive[s]) return;
   const int o = p ^ 1;
   const uint32_t nq = P.n[o], np = P.n[p], tid = threadIdx.x;
@@ -353,7 +359,7 @@ __global__ void __launch_bounds__(TERM_T) terminal_kernel(DPlan P, int p, uint32
   __syncthreads();
   const bool batched = (tr.tflags & kTermBatched) != 0;
   const double hp = P.half_pot, norm = P.norm, rake = (double)tr.rake;
-  const int32_t inv_p = tr.invest[p], inv_o = tr.invest[o];
+  const int32_t inv_p = pick(tr.invest, p), inv_o = pick(tr.invest, o);
   const uint8_t* cp = P.cards[p];
   const uint8_t* co = P.cards[o];
   const uint32_t* same_p = P.same[p];
@@ -361,7 +367,7 @@ __global__ void __launch_bounds__(TERM_T) terminal_kernel(DPlan P, int p, uint32
   if (tr.kind != kShowdown) {
     const int f = tr.kind - kFold0;
     // cfr.rs:224-235
-    const double pay = (f == p ? -(hp + (double)inv_p) : hp + (double)tr.invest[f] - rake) * norm;
+    const double pay = (f == p ? -(hp + (double)inv_p) : hp + (double)pick(tr.invest, f) - rake) * norm;
     const bool paired = (tr.tflags & (p == 0 ? kTermPaired0 : kTermPaired1)) != 0;
     if (paired) {  // strength order of the sibling showdown's board
       const uint32_t b = tr.board;
@@ -496,228 +502,307 @@ __global__ void __launch_bounds__(TERM_T) terminal_kernel(DPlan P, int p, uint32
 }
 
 // ------------------------------------------------------------------ fast terminals
-// The same terminal values with the two long strength sweeps (and the fold
-// totals) computed as 256-thread block scans in f64 instead of one lane's
-// sequential chain: the sums differ from the reference's only in summation
-// order (a few f64 ulps), so an f32 value occasionally differs by one
-// rounding step, as the reference's own `Fast` precision does
-// (crates/solver/src/gpu/mod.rs:51-63). The per-card sums (at most 49
-// terms) stay sequential.
+// The same terminal values with every f64 sum computed as a parallel scan
+// instead of one lane's sequential chain: the sums differ from the
+// reference's only in summation order (a few f64 ulps), so an f32 value
+// occasionally differs by one rounding step, as the reference's own `Fast`
+// precision does (crates/solver/src/gpu/mod.rs:51-63). Prefix and suffix
+// sums are scanned separately, so a sum over terms that are all zero is an
+// exact zero, as in the sequential order.
 //
-// Every global load a block needs is issued up front, in two dependent
-// rounds (the record's tables, then the strength-order index lists), into
-// registers: a block then waits for memory twice instead of once per loop
-// trip.
-constexpr int FT = 256;               // fast terminal workgroup
-constexpr int FK = (1326 + FT - 1) / FT;      // hands per lane
-constexpr int FC = (2 * 1326 + FT - 1) / FT;  // per-card entries per lane
+// One workgroup per run of terminals that share a river board (TermBatch):
+// the board's index lists and per-hand counts are loaded into registers
+// once, and each terminal's reach row is fetched while the previous
+// terminal is summed. Per terminal:
+//   the reach row to LDS; barrier;
+//   per-card prefix and suffix scans (8 lanes per card, shuffles) and the
+//   strength-order prefix and suffix scans (contiguous slices per lane,
+//   wave shuffles, wave totals); barrier; the cross-wave offsets; barrier;
+//   one value per hand; barrier.
+constexpr int FT = 512;                       // fast terminal workgroup
+constexpr int FK = (1326 + FT - 1) / FT;      // hands per lane (strided)
+constexpr int FE = (1326 + FT - 1) / FT;      // strength-order entries per lane (contiguous)
+constexpr int CG = 8;                         // lanes per card list
+constexpr int CL = (51 + CG - 1) / CG;        // card-list entries per lane (a list holds at most 51 hands)
+static_assert(52 * CG <= FT, "card groups fit the workgroup");
 
-// Inclusive scan of x[0..n) in place (fwd) or suffix sums (bwd), block-wide.
-template <bool FWD>
-__device__ __forceinline__ void block_scan(double* x, uint32_t n, double* wsum) {
-  const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const uint32_t E = (n + FT - 1) / FT;
-  const uint32_t b = tid * E;
-  auto at = [&](uint32_t k) -> double& { return FWD ? x[k] : x[n - 1 - k]; };
-  double loc = 0.0;
-  for (uint32_t k = 0; k < E; k++)
-    if (b + k < n) loc += at(b + k);
-  // wave-inclusive scan of thread totals
-  double inc = loc;
-#pragma unroll
-  for (int d = 1; d < 64; d <<= 1) {
-    const double y = __shfl_up(inc, d, 64);
-    if ((int)lane >= d) inc += y;
-  }
-  if (lane == 63) wsum[wave] = inc;
-  __syncthreads();
-  double off = 0.0;
-  for (uint32_t w = 0; w < wave; w++) off += wsum[w];
-  double ex = __shfl_up(inc, 1, 64);
-  if (lane == 0) ex = 0.0;
-  double acc = off + ex;
-  for (uint32_t k = 0; k < E; k++)
-    if (b + k < n) {
-      acc += at(b + k);
-      at(b + k) = acc;
-    }
-  __syncthreads();
+// The fields of a terminal record one traversal of player p reads, as
+// wave-uniform scalars.
+struct TermFields {
+  uint32_t node, slot, vslot, kf;  // kf: kind | tflags << 8
+  int32_t inv0, inv1;
+  float rake;
+};
+template <int p>
+__device__ __forceinline__ TermFields term_fields(const TermRec* __restrict__ r) {
+  const uint32_t* w = (const uint32_t*)r;  // node, slot[2], vslot, board, invest[2], rake, kind | tflags << 8
+  TermFields f;
+  f.node = __builtin_amdgcn_readfirstlane(w[0]);
+  f.slot = __builtin_amdgcn_readfirstlane(w[1 + p]);
+  f.vslot = __builtin_amdgcn_readfirstlane(w[3]);
+  f.inv0 = (int32_t)__builtin_amdgcn_readfirstlane(w[5]);
+  f.inv1 = (int32_t)__builtin_amdgcn_readfirstlane(w[6]);
+  f.rake = __uint_as_float(__builtin_amdgcn_readfirstlane(w[7]));
+  f.kf = __builtin_amdgcn_readfirstlane(w[8]) & 0xFFFFu;
+  return f;
 }
+static_assert(offsetof(TermRec, invest) == 20 && offsetof(TermRec, rake) == 28 && offsetof(TermRec, kind) == 32 &&
+                  offsetof(TermRec, tflags) == 33,
+              "TermRec word layout");
 
-__device__ __forceinline__ double block_sum(const double* x, uint32_t n, double* wsum) {
-  const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  double s = 0.0;
-  for (uint32_t k = tid; k < n; k += FT) s += x[k];
-#pragma unroll
-  for (int d = 32; d > 0; d >>= 1) s += __shfl_xor(s, d, 64);
-  if (lane == 0) wsum[wave] = s;
-  __syncthreads();
-  double t = 0.0;
-  for (uint32_t w = 0; w < FT / 64; w++) t += wsum[w];
-  __syncthreads();
-  return t;
-}
-
-template <typename V>
-__global__ void __launch_bounds__(FT) terminal_fast_kernel(DPlan P, int p, uint32_t t0, Span span, Bias B, const V* __restrict__ reach,
-                                                           const uint32_t* __restrict__ alive, V* __restrict__ vals,
-                                                           uint32_t rowlen) {
+template <typename V, int p>  // p: the traverser (a constant, so per-player tables are static fields)
+__global__ void __launch_bounds__(FT) terminal_fast_kernel(DPlan P, uint32_t b0, Span span, Bias B,
+                                                              const V* __restrict__ reach, const uint32_t* __restrict__ alive,
+                                                              V* __restrict__ vals, uint32_t rowlen) {
   extern __shared__ double sh[];
   __shared__ double tot[52];
-  __shared__ double wsum[FT / 64];
+  __shared__ double wtot[FT / 64];
   __shared__ uint32_t offs[53];
-  const TermRec tr = P.trec[t0 + blockIdx.x];
-  if (!span.has(tr.node)) return;
-  const int o = p ^ 1;
-  const uint32_t nq = P.n[o], np = P.n[p], tid = threadIdx.x;
-  const uint32_t s = B.s(tr.slot[p]), b = tr.board;
-  const bool showdown = tr.kind == kShowdown;
-  const bool paired = !showdown && (tr.tflags & (p == 0 ? kTermPaired0 : kTermPaired1)) != 0;
-  const bool sorted = showdown || paired;  // strength order of the board, else hand order
+  const TermBatch tb = P.tbatch[b0 + blockIdx.x];
+  constexpr int o = p ^ 1;
+  const uint32_t nq = P.n[o], np = P.n[p], tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const uint32_t b = tb.board;
+  const bool sorted = b != NONE;  // strength order of the board, else hand order
 
-  // round 1: everything that depends on the record only
-  const uint32_t live = alive[s];
-  const V* R = reach + (size_t)s * nq;
-  V r[FK];
-#pragma unroll
-  for (int k = 0; k < FK; k++) {
-    const uint32_t j = tid + k * FT;
-    r[k] = j < nq ? R[j] : V(0);
-  }
-  uint32_t hcard[FK], hsame[FK];
+  // per hand (strided): cards (c1 | c2 << 8) | same-combo index << 16
+  // (0xFFFF: none), and the board's lt / le counts
+  uint32_t hcs[FK];
   uint4 hc[FK];
   const uint16_t* cp2 = (const uint16_t*)P.cards[p];
-  const uint4* hcnt = P.hand_cnt[p] + (size_t)b * np;
+  const uint4* hcnt = P.hand_cnt[p] + (size_t)(sorted ? b : 0) * np;
 #pragma unroll
   for (int k = 0; k < FK; k++) {
     const uint32_t i = tid + k * FT;
-    hcard[k] = i < np ? cp2[i] : 0u;
-    hsame[k] = i < np ? P.same[p][i] : NONE;
-    hc[k] = (showdown && i < np) ? hcnt[i] : make_uint4(0, 0, 0, 0);
+    const uint32_t sm = i < np ? P.same[p][i] : NONE;
+    hcs[k] = (i < np ? (uint32_t)cp2[i] : 0u) | ((sm == NONE ? 0xFFFFu : sm) << 16);
+    hc[k] = (sorted && i < np) ? hcnt[i] : make_uint4(0, 0, 0, 0);
   }
+  // the opponent's entries in strength order (or hand order): a contiguous
+  // slice per lane
   const uint32_t nA = sorted ? P.ent_cnt[o][b] : nq;
-  const uint32_t baseA = sorted ? P.ent_off[o][b] : 0u;
-  const uint16_t* idxC = sorted ? P.cl_idx[o] + P.cl_base[o][b] : P.bl_idx[o];
+  const uint16_t* idxA = P.ent_idx[o] + (sorted ? P.ent_off[o][b] : 0u);
+  uint32_t ja[FE];
+#pragma unroll
+  for (int k = 0; k < FE; k++) {
+    const uint32_t j = tid * FE + k;
+    ja[k] = j < nA ? (sorted ? (uint32_t)idxA[j] : j) : NONE;
+  }
+  // the opponent's hands holding card tid / CG, a slice per lane of the group
   if (tid < 53) offs[tid] = sorted ? (uint32_t)P.cl_off[o][(size_t)b * 53 + tid] : P.bl_off[o][tid];
-  // round 2: the index lists
-  const uint32_t nC = sorted ? 2 * nA : P.bl_off[o][52];
-  const uint16_t* idxA = P.ent_idx[o] + baseA;
-  uint32_t ia[FK], ic[FC];
-#pragma unroll
-  for (int k = 0; k < FK; k++) {
-    const uint32_t j = tid + k * FT;
-    ia[k] = j < nA ? (sorted ? (uint32_t)idxA[j] : j) : 0u;
+  const uint16_t* idxC = sorted ? P.cl_idx[o] + P.cl_base[o][b] : P.bl_idx[o];
+  const uint32_t cq = tid & (CG - 1), card = tid / CG;
+  uint32_t cs = 0, ce = 0;
+  if (tid < 52 * CG) {
+    const uint32_t a = sorted ? (uint32_t)P.cl_off[o][(size_t)b * 53 + card] : P.bl_off[o][card];
+    const uint32_t e = sorted ? (uint32_t)P.cl_off[o][(size_t)b * 53 + card + 1] : P.bl_off[o][card + 1];
+    const uint32_t L = (e - a + CG - 1) / CG;
+    cs = min(a + cq * L, e);
+    ce = min(cs + L, e);
   }
+  uint32_t jc[CL];
 #pragma unroll
-  for (int k = 0; k < FC; k++) {
-    const uint32_t j = tid + k * FT;
-    ic[k] = j < nC ? (uint32_t)idxC[j] : 0u;
-  }
-  if (!live) return;  // dead slot: the value row is never read
+  for (int k = 0; k < CL; k++) jc[k] = cs + k < ce ? (uint32_t)idxC[cs + k] : NONE;
 
-  double* Pa = sh;                // [rowlen] ascending prefix (strength order) or reach (hand order)
+  double* Pa = sh;                // [rowlen] ascending prefix over entries
   double* Sa = sh + rowlen;       // [rowlen] descending suffix
-  double* Ca = sh + 2 * rowlen;   // [2 rowlen] per-card ascending
-  double* Cd = sh + 4 * rowlen;   // [2 rowlen] per-card descending
+  double* Ca = sh + 2 * rowlen;   // [2 rowlen] per-card ascending prefix
+  double* Cd = sh + 4 * rowlen;   // [2 rowlen] per-card descending suffix
   V* rv = (V*)(sh + 6 * rowlen);  // [rowlen] the opponent's reach row, hand order
-#pragma unroll
-  for (int k = 0; k < FK; k++) {
-    const uint32_t j = tid + k * FT;
-    if (j < nq) rv[j] = r[k];
-  }
-  __syncthreads();
-#pragma unroll
-  for (int k = 0; k < FK; k++) {
-    const uint32_t j = tid + k * FT;
-    if (j < nA) {
-      const double x = (double)rv[ia[k]];
-      Pa[j] = x;
-      Sa[j] = x;
-    }
-  }
-#pragma unroll
-  for (int k = 0; k < FC; k++) {
-    const uint32_t j = tid + k * FT;
-    if (j < nC) {
-      const double x = (double)rv[ic[k]];
-      Ca[j] = x;
-      Cd[j] = x;
-    }
-  }
-  __syncthreads();
+  const double hp = P.half_pot, norm = P.norm;
+  const uint8_t* co = P.cards[o];
 
-  const double hp = P.half_pot, norm = P.norm, rake = (double)tr.rake;
-  const int32_t inv_p = tr.invest[p], inv_o = tr.invest[o];
-  const bool batched = (tr.tflags & kTermBatched) != 0;
-  V* out = vals + (size_t)B.v(tr.vslot) * rowlen;
-
-  if (!showdown) {  // cfr.rs:224-235, eval.rs:39-64,170-206
-    const int f = tr.kind - kFold0;
-    const double pay = (f == p ? -(hp + (double)inv_p) : hp + (double)tr.invest[f] - rake) * norm;
-    if (tid < 52) tot[tid] = chain_sum(Ca, offs[tid], offs[tid + 1]);
-    const double total = block_sum(Pa, nA, wsum);  // its barriers publish tot
+  // the first terminal's record, flag and reach row
+  uint32_t t = tb.t0;
+  const uint32_t tend = tb.t0 + tb.cnt;
+  TermFields tr = term_fields<p>(P.trec + t);
+  uint32_t live = span.has(tr.node) ? alive[B.s(tr.slot)] : 0u;
+  V r[FK];
+  {
+    const V* R = reach + (size_t)B.s(tr.slot) * nq;
 #pragma unroll
     for (int k = 0; k < FK; k++) {
-      const uint32_t i = tid + k * FT;
-      if (i < np) {
-        const int a = hcard[k] & 0xFF, c = hcard[k] >> 8;
-        const double sw = hsame[k] == NONE ? 0.0 : (double)rv[hsame[k]];
-        V v;
-        if (batched) v = (V)(pay * (total - tot[a] - tot[c] + sw));
-        else v = total == 0.0 ? V(0) : (V)(pay * (total - (tot[a] + tot[c]) + sw));
-        out[i] = v;
-      }
+      const uint32_t j = tid + k * FT;
+      r[k] = j < nq ? R[j] : V(0);
     }
-    return;
   }
-
-  // showdown (cfr.rs:238-246 payoffs; eval.rs:66-148,208-379 sweeps)
-  const double win = (hp + (double)inv_o - rake) * norm;
-  const double lose = -(hp + (double)inv_p) * norm;
-  const double tie = 0.5 * ((double)(inv_o - inv_p) - rake) * norm;
-  const uint32_t mo = nA;
-  if (tid < 52) chain_fwd(Ca, offs[tid], offs[tid + 1]);
-  else if (tid >= 64 && tid < 116) chain_bwd(Cd, offs[tid - 64], offs[tid - 63]);
-  block_scan<true>(Pa, mo, wsum);
-  block_scan<false>(Sa, mo, wsum);  // its barriers also publish the chains
-  const uint8_t* co = P.cards[o];
+  for (;;) {
+    // fetch the next terminal while this one is summed (its slot is in
+    // range whether or not it is alive)
+    const bool more = t + 1 < tend;
+    TermFields trn = tr;
+    uint32_t liven = 0;
+    V rn[FK];
+    if (more) {
+      trn = term_fields<p>(P.trec + t + 1);
+      const uint32_t sn = B.s(trn.slot);
+      liven = span.has(trn.node) ? alive[sn] : 0u;
+      const V* R = reach + (size_t)sn * nq;
 #pragma unroll
-  for (int k = 0; k < FK; k++) {
-    const uint32_t i = tid + k * FT;
-    if (i >= np) continue;
-    const uint32_t lt = hc[k].x & 0xFFFF, le = hc[k].x >> 16;
-    V v = V(0);
-    if (lt != 0xFFFF) {
-      const int c1 = hcard[k] & 0xFF, c2 = hcard[k] >> 8;
-      const uint32_t l1 = hc[k].y & 0xFFFF, m1 = hc[k].y >> 16, l2 = hc[k].z & 0xFFFF, m2 = hc[k].z >> 16;
-      const uint32_t o1 = offs[c1], o2 = offs[c2];
-      const uint32_t len1 = offs[c1 + 1] - o1, len2 = offs[c2 + 1] - o2;
-      const double Pv = lt ? Pa[lt - 1] : 0.0;
-      const double P1 = l1 ? Ca[o1 + l1 - 1] : 0.0;
-      const double P2 = l2 ? Ca[o2 + l2 - 1] : 0.0;
-      const double Sv = le < mo ? Sa[le] : 0.0;
-      const double S1 = m1 < len1 ? Cd[o1 + m1] : 0.0;
-      const double S2 = m2 < len2 ? Cd[o2 + m2] : 0.0;
-      if (batched) {
-        v = (V)(win * (Pv - P1 - P2) + lose * (Sv - S1 - S2));
-      } else {
-        v = (V)(win * (Pv - (P1 + P2)));
-        v = v + (V)(lose * (Sv - (S1 + S2)));
-      }
-      if (tie != 0.0) {  // eval.rs:347-379
-        double es = 0.0, e1 = 0.0, e2 = 0.0;
-        for (uint32_t j = lt; j < le; j++) {
-          const uint32_t idx = idxA[j];
-          const double w = (double)rv[idx];
-          es += w;
-          if (holds(co, idx, c1)) e1 += w;
-          if (holds(co, idx, c2)) e2 += w;
-        }
-        const double sw = hsame[k] == NONE ? 0.0 : (double)rv[hsame[k]];
-        v = v + (V)(tie * (es - (e1 + e2) + sw));
+      for (int k = 0; k < FK; k++) {
+        const uint32_t j = tid + k * FT;
+        rn[k] = j < nq ? R[j] : V(0);
       }
     }
-    out[i] = v;
+    if (live) {  // a dead slot's value row is never read
+#pragma unroll
+      for (int k = 0; k < FK; k++) {
+        const uint32_t j = tid + k * FT;
+        if (j < nq) rv[j] = r[k];
+      }
+      __syncthreads();
+      const double rake = (double)tr.rake;
+      const int32_t inv_p = p ? tr.inv1 : tr.inv0, inv_o = p ? tr.inv0 : tr.inv1;
+      const uint32_t kind = tr.kf & 0xFF;
+      const bool batched = ((tr.kf >> 8) & kTermBatched) != 0;
+      const bool showdown = kind == kShowdown;
+      V* out = vals + (size_t)B.v(tr.vslot) * rowlen;
+
+      // per-card scans: 8 lanes per card, prefix and suffix
+      if (tid < 52 * CG) {
+        double v[CL];
+        double tsum = 0.0;
+#pragma unroll
+        for (int k = 0; k < CL; k++) {
+          v[k] = jc[k] != NONE ? (double)rv[jc[k]] : 0.0;
+          tsum += v[k];
+        }
+        double inc = tsum, incs = tsum;
+#pragma unroll
+        for (int d = 1; d < CG; d <<= 1) {
+          const double y = __shfl_up(inc, d, CG);
+          const double z = __shfl_down(incs, d, CG);
+          if ((int)cq >= d) inc += y;
+          if ((int)cq + d < CG) incs += z;
+        }
+        if (cq == 0) tot[card] = incs;  // the card's total
+        if (showdown) {
+          double ex = __shfl_up(inc, 1, CG), exs = __shfl_down(incs, 1, CG);
+          if (cq == 0) ex = 0.0;
+          if (cq == CG - 1) exs = 0.0;
+#pragma unroll
+          for (int k = 0; k < CL; k++) {
+            ex += v[k];
+            if (cs + k < ce) Ca[cs + k] = ex;
+          }
+#pragma unroll
+          for (int k = CL - 1; k >= 0; k--) {
+            exs += v[k];
+            if (cs + k < ce) Cd[cs + k] = exs;
+          }
+        }
+      }
+      // strength-order scans (showdown) or the total (fold)
+      double v[FE];
+      double tsum = 0.0;
+#pragma unroll
+      for (int k = 0; k < FE; k++) {
+        v[k] = ja[k] != NONE ? (double)rv[ja[k]] : 0.0;
+        tsum += v[k];
+      }
+      double inc = tsum, incs = tsum;
+#pragma unroll
+      for (int d = 1; d < 64; d <<= 1) {
+        const double y = __shfl_up(inc, d, 64);
+        const double z = __shfl_down(incs, d, 64);
+        if ((int)lane >= d) inc += y;
+        if ((int)lane + d < 64) incs += z;
+      }
+      if (lane == 63) wtot[wave] = inc;
+      __syncthreads();
+      if (!showdown) {  // cfr.rs:224-235, eval.rs:39-64,170-206
+        double total = 0.0;
+#pragma unroll
+        for (int w = 0; w < FT / 64; w++) total += wtot[w];
+        const int f = (int)kind - kFold0;
+        const double pay = (f == p ? -(hp + (double)inv_p) : hp + (double)(f ? tr.inv1 : tr.inv0) - rake) * norm;
+#pragma unroll
+        for (int k = 0; k < FK; k++) {
+          const uint32_t i = tid + k * FT;
+          if (i < np) {
+            const int a = hcs[k] & 0xFF, c = (hcs[k] >> 8) & 0xFF;
+            const uint32_t sm = hcs[k] >> 16;
+            const double sw = sm == 0xFFFF ? 0.0 : (double)rv[sm];
+            V x;
+            if (batched) x = (V)(pay * (total - tot[a] - tot[c] + sw));
+            else x = total == 0.0 ? V(0) : (V)(pay * (total - (tot[a] + tot[c]) + sw));
+            out[i] = x;
+          }
+        }
+      } else {  // showdown: cfr.rs:238-246 payoffs, eval.rs:66-148,208-379 sweeps
+        double offp = 0.0, offs_ = 0.0;
+#pragma unroll
+        for (int w = 0; w < FT / 64; w++) {
+          const double x = wtot[w];
+          if (w < (int)wave) offp += x;
+          if (w > (int)wave) offs_ += x;
+        }
+        double ex = __shfl_up(inc, 1, 64), exs = __shfl_down(incs, 1, 64);
+        if (lane == 0) ex = 0.0;
+        if (lane == 63) exs = 0.0;
+        double bp = offp + ex, bs = offs_ + exs;
+#pragma unroll
+        for (int k = 0; k < FE; k++) {
+          bp += v[k];
+          if (tid * FE + k < nA) Pa[tid * FE + k] = bp;
+        }
+#pragma unroll
+        for (int k = FE - 1; k >= 0; k--) {
+          bs += v[k];
+          if (tid * FE + k < nA) Sa[tid * FE + k] = bs;
+        }
+        __syncthreads();
+        const double win = (hp + (double)inv_o - rake) * norm;
+        const double lose = -(hp + (double)inv_p) * norm;
+        const double tie = 0.5 * ((double)(inv_o - inv_p) - rake) * norm;
+        const uint32_t mo = nA;
+#pragma unroll
+        for (int k = 0; k < FK; k++) {
+          const uint32_t i = tid + k * FT;
+          if (i >= np) continue;
+          const uint32_t lt = hc[k].x & 0xFFFF, le = hc[k].x >> 16;
+          V x = V(0);
+          if (lt != 0xFFFF) {
+            const int c1 = hcs[k] & 0xFF, c2 = (hcs[k] >> 8) & 0xFF;
+            const uint32_t l1 = hc[k].y & 0xFFFF, m1 = hc[k].y >> 16, l2 = hc[k].z & 0xFFFF, m2 = hc[k].z >> 16;
+            const uint32_t o1 = offs[c1], o2 = offs[c2];
+            const uint32_t len1 = offs[c1 + 1] - o1, len2 = offs[c2 + 1] - o2;
+            const double Pv = lt ? Pa[lt - 1] : 0.0;
+            const double P1 = l1 ? Ca[o1 + l1 - 1] : 0.0;
+            const double P2 = l2 ? Ca[o2 + l2 - 1] : 0.0;
+            const double Sv = le < mo ? Sa[le] : 0.0;
+            const double S1 = m1 < len1 ? Cd[o1 + m1] : 0.0;
+            const double S2 = m2 < len2 ? Cd[o2 + m2] : 0.0;
+            if (batched) {
+              x = (V)(win * (Pv - P1 - P2) + lose * (Sv - S1 - S2));
+            } else {
+              x = (V)(win * (Pv - (P1 + P2)));
+              x = x + (V)(lose * (Sv - (S1 + S2)));
+            }
+            if (tie != 0.0) {  // eval.rs:347-379
+              double es = 0.0, e1 = 0.0, e2 = 0.0;
+              for (uint32_t j = lt; j < le; j++) {
+                const uint32_t idx = idxA[j];
+                const double w = (double)rv[idx];
+                es += w;
+                if (holds(co, idx, c1)) e1 += w;
+                if (holds(co, idx, c2)) e2 += w;
+              }
+              const uint32_t sm = hcs[k] >> 16;
+              const double sw = sm == 0xFFFF ? 0.0 : (double)rv[sm];
+              x = x + (V)(tie * (es - (e1 + e2) + sw));
+            }
+          }
+          out[i] = x;
+        }
+      }
+      __syncthreads();  // the next terminal overwrites rv, the scans, tot and wtot
+    }
+    if (!more) break;
+    t++;
+    tr = trn;
+    live = liven;
+#pragma unroll
+    for (int k = 0; k < FK; k++) r[k] = rn[k];
   }
 }
 
@@ -979,7 +1064,7 @@ Status Engine::create(int device, Plan&& plan, const float* regrets, const float
   std::vector<unsigned char> blob;
   size_t o_card = put(blob, P.card), o_smo = put(blob, P.seg_mem_off), o_smc = put(blob, P.seg_mem_cnt);
   size_t o_sps = put(blob, P.seg_pslot), o_sf = put(blob, P.seg_factor), o_mr = put(blob, P.mrec), o_tr = put(blob, P.trec);
-  size_t o_cr = put(blob, P.crec), o_ccv = put(blob, P.chance_child_vslot);
+  size_t o_cr = put(blob, P.crec), o_ccv = put(blob, P.chance_child_vslot), o_tb = put(blob, P.tbatch);
   size_t o_cards[2], o_same[2], o_bo[2], o_bi[2], o_eo[2], o_ec[2], o_cb[2], o_ei[2], o_co[2], o_ci[2], o_mc[2], o_he[2], o_w[2], o_hc[2];
   for (int q = 0; q < 2; q++) {
     o_cards[q] = put(blob, P.cards[q]);
@@ -1016,7 +1101,7 @@ Status Engine::create(int device, Plan&& plan, const float* regrets, const float
   d.card = AT(int8_t, o_card);
   d.seg_mem_off = AT(uint32_t, o_smo); d.seg_mem_cnt = AT(uint32_t, o_smc); d.seg_pslot = AT(uint32_t, o_sps);
   d.seg_factor = AT(float, o_sf); d.mrec = AT(MemRec, o_mr); d.trec = AT(TermRec, o_tr);
-  d.crec = AT(MemRec, o_cr); d.chance_child_vslot = AT(uint32_t, o_ccv);
+  d.crec = AT(MemRec, o_cr); d.chance_child_vslot = AT(uint32_t, o_ccv); d.tbatch = AT(TermBatch, o_tb);
   for (int q = 0; q < 2; q++) {
     d.cards[q] = AT(uint8_t, o_cards[q]); d.same[q] = AT(uint32_t, o_same[q]);
     d.bl_off[q] = AT(uint32_t, o_bo[q]); d.bl_idx[q] = AT(uint16_t, o_bi[q]);
@@ -1085,9 +1170,13 @@ Status Engine::create(int device, Plan&& plan, const float* regrets, const float
     return kOk;
   };
   const bool fastp = precision == kFast;
-  if ((st = lds_attr(fastp ? (const void*)terminal_fast_kernel<double> : (const void*)terminal_kernel<double>,
+  if ((st = lds_attr(fastp ? (const void*)terminal_fast_kernel<double, 0> : (const void*)terminal_kernel<double>,
                      e->lds_term_[1])) != kOk ||
-      (st = lds_attr(fastp ? (const void*)terminal_fast_kernel<float> : (const void*)terminal_kernel<float>,
+      (st = lds_attr(fastp ? (const void*)terminal_fast_kernel<double, 1> : (const void*)terminal_kernel<double>,
+                     e->lds_term_[1])) != kOk ||
+      (st = lds_attr(fastp ? (const void*)terminal_fast_kernel<float, 0> : (const void*)terminal_kernel<float>,
+                     e->lds_term_[0])) != kOk ||
+      (st = lds_attr(fastp ? (const void*)terminal_fast_kernel<float, 1> : (const void*)terminal_kernel<float>,
                      e->lds_term_[0])) != kOk ||
       (st = lds_attr((const void*)up_kernel<double, kBest>, e->lds_up_[1])) != kOk ||
       (st = lds_attr((const void*)up_kernel<double, kAverage>, e->lds_up_[1])) != kOk ||
@@ -1177,8 +1266,11 @@ static void launch_span(const DPlan& d, const Plan& P, hipStream_t st, int p, Sp
   auto terminals = [&](uint32_t a, uint32_t b, const Bias& B) {
     if (b <= a) return;
     PROF_START(kKTerminal);
-    if (fast)
-      hipLaunchKernelGGL((terminal_fast_kernel<V>), dim3(b - a), dim3(FT), lds_term, st, d, p, a, span, B, (const V*)reach,
+    if (fast && p == 0)
+      hipLaunchKernelGGL((terminal_fast_kernel<V, 0>), dim3(b - a), dim3(FT), lds_term, st, d, a, span, B, (const V*)reach,
+                         (const uint32_t*)alive, (V*)vals, rowlen);
+    else if (fast)
+      hipLaunchKernelGGL((terminal_fast_kernel<V, 1>), dim3(b - a), dim3(FT), lds_term, st, d, a, span, B, (const V*)reach,
                          (const uint32_t*)alive, (V*)vals, rowlen);
     else
       hipLaunchKernelGGL((terminal_kernel<V>), dim3(b - a), dim3(TERM_T), lds_term, st, d, p, a, span, B, (const V*)reach,
@@ -1200,8 +1292,10 @@ static void launch_span(const DPlan& d, const Plan& P, hipStream_t st, int p, Sp
     }
   };
   if (first_depth == 0) down(P.depth_off[0], P.depth_off[1], B0);
+  // fast terminals launch over runs (TermBatch), exact ones over records
+  auto tlo = [&](uint32_t c) { return fast ? P.tb_coff[c] : P.term_coff[c]; };
   if (C == 0) {
-    terminals(0, (uint32_t)P.terminals.size(), B0);
+    terminals(0, fast ? (uint32_t)P.tbatch.size() : (uint32_t)P.terminals.size(), B0);
   } else {
     const uint32_t qc = first_depth == 0 ? kNone : P.chunk_of[span.root];
     bool first = true;
@@ -1212,7 +1306,7 @@ static void launch_span(const DPlan& d, const Plan& P, hipStream_t st, int p, Sp
       const uint32_t* co = P.ch_coff.data();
       for (uint32_t dd = std::max(1u, first_depth); dd < D; dd++) down(so[dd * (C + 1) + g.c0], so[dd * (C + 1) + g.c1], B);
       // the top segment's terminals (resident rows) ride with the first group
-      terminals(first && first_depth == 0 ? 0u : P.term_coff[g.c0], P.term_coff[g.c1], B);
+      terminals(first && first_depth == 0 ? 0u : tlo(g.c0), tlo(g.c1), B);
       first = false;
       for (uint32_t dd = D; dd-- > std::max(1u, first_depth);)
         up(co[dd * (C + 1) + g.c0], co[dd * (C + 1) + g.c1], so[dd * (C + 1) + g.c0], so[dd * (C + 1) + g.c1], B);

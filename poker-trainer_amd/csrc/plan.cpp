@@ -370,6 +370,17 @@ std::string build_plan(const PlanInput& in, Plan& P) {
     }
     P.chance_off[D] = (uint32_t)P.crec.size();
   }
+  // Folds inside a river subtree get the subtree's board, so the fast
+  // terminal kernel can sum them in that board's strength order (the exact
+  // kernel reads the board of paired folds only).
+  {
+    std::vector<uint32_t> seg_board(P.seg_root.size(), kNone);
+    for (uint32_t id : P.terminals)
+      if (P.kind[id] == kShowdown && seg_board[P.seg_of[id]] == kNone) seg_board[P.seg_of[id]] = (uint32_t)P.offset[id];
+    for (uint32_t id : P.terminals)
+      if (P.kind[id] != kShowdown && P.tboard[id] == kNone && in.street[parent[id]] == kRiver)
+        P.tboard[id] = seg_board[P.seg_of[id]];
+  }
   P.trec.assign(P.terminals.size(), TermRec{});
   for (size_t t = 0; t < P.terminals.size(); t++) {
     uint32_t id = P.terminals[t];
@@ -384,6 +395,31 @@ std::string build_plan(const PlanInput& in, Plan& P) {
     r.rake = P.rake[id];
     r.kind = P.kind[id];
     r.tflags = P.tflags[id];
+  }
+
+  // ---- terminal runs (per chunk, consecutive terminals with one board)
+  {
+    const uint32_t C = P.num_chunks;
+    P.tbatch.clear();
+    P.tb_coff.assign(C + 1, 0);
+    // long runs amortize the tables; short ones keep a small tree's
+    // workgroups spread over the chip (about 4096 runs at least)
+    const uint32_t bmax = std::min<uint32_t>(kBatchMax, std::max<uint32_t>(1, (uint32_t)(P.trec.size() / 4096)));
+    auto runs = [&](uint32_t a, uint32_t b) {
+      uint32_t t = a;
+      while (t < b) {
+        TermBatch r{t, 1, P.trec[t].board, 0};
+        while (t + r.cnt < b && r.cnt < bmax && P.trec[t + r.cnt].board == r.board) r.cnt++;
+        P.tbatch.push_back(r);
+        t += r.cnt;
+      }
+    };
+    runs(0, C ? P.term_coff[0] : (uint32_t)P.trec.size());
+    for (uint32_t c = 0; c < C; c++) {
+      P.tb_coff[c] = (uint32_t)P.tbatch.size();
+      runs(P.term_coff[c], P.term_coff[c + 1]);
+    }
+    P.tb_coff[C] = (uint32_t)P.tbatch.size();
   }
 
   // ---- strength tables (crates/solver/src/hands.rs:128-159)
@@ -574,6 +610,12 @@ std::string check_groups(const Plan& P, const std::vector<ChunkGroup>& G, const 
     if (C == 0) {
       for (uint32_t t = 0; t < (uint32_t)P.trec.size(); t++)
         if (!term_ok(t, 0, 0)) return std::string(b);
+      uint32_t t = 0;
+      for (const TermBatch& r : P.tbatch) {
+        if (r.t0 != t || r.cnt == 0 || r.cnt > kBatchMax) return fail("terminal runs do not tile the tree", t);
+        t += r.cnt;
+      }
+      if (t != (uint32_t)P.trec.size()) return fail("terminal runs do not tile the tree", t);
       continue;
     }
     std::vector<uint8_t> seen_t(P.trec.size(), 0);
@@ -594,6 +636,19 @@ std::string check_groups(const Plan& P, const std::vector<ChunkGroup>& G, const 
     }
     for (size_t t = 0; t < seen_t.size(); t++)
       if (!seen_t[t]) return fail("terminal outside every group", t);
+    // the terminal runs of a group cover exactly its terminals
+    for (const ChunkGroup& g : G) {
+      const uint32_t b0 = g.c0 == 0 ? 0u : P.tb_coff[g.c0], b1 = P.tb_coff[g.c1];
+      uint32_t t = g.c0 == 0 ? 0u : P.term_coff[g.c0];
+      for (uint32_t k = b0; k < b1; k++) {
+        const TermBatch& r = P.tbatch[k];
+        if (r.t0 != t || r.cnt == 0 || r.cnt > kBatchMax) return fail("terminal runs do not tile the group", k);
+        for (uint32_t j = r.t0; j < r.t0 + r.cnt; j++)
+          if (P.trec[j].board != r.board) return fail("terminal run with two boards", j);
+        t += r.cnt;
+      }
+      if (t != P.term_coff[g.c1]) return fail("terminal runs do not tile the group", b1);
+    }
     (void)N;
   }
   return "";

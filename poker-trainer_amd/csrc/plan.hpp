@@ -60,6 +60,18 @@ struct TermRec {
 };
 static_assert(sizeof(TermRec) == 40, "TermRec layout");
 
+// A run of consecutive terminals that read the same strength tables (one
+// river board) or none (folds before the river, summed in hand order): the
+// fast terminal kernel loads the tables once per run.
+struct TermBatch {
+  uint32_t t0;     // first terminal record
+  uint32_t cnt;    // terminals in the run
+  uint32_t board;  // strength-table board, or kNone for hand order
+  uint32_t pad;
+};
+static_assert(sizeof(TermBatch) == 16, "TermBatch layout");
+constexpr uint32_t kBatchMax = 16;  // terminals per run
+
 struct PlanInput {
   size_t num_nodes = 0;
   const uint8_t* kind = nullptr;
@@ -125,6 +137,10 @@ struct Plan {
   std::vector<uint32_t> slot_coff[2];  // [c], c <= C
   std::vector<uint32_t> val_coff;      // [c], c <= C
   uint32_t slot_s0[2] = {0, 0}, val_v0 = 0;
+  // terminal runs: the top segment's first, then chunk by chunk
+  // ([tb_coff[c], tb_coff[c+1]) for chunk c; tb_coff[0] = the top's count)
+  std::vector<TermBatch> tbatch;
+  std::vector<uint32_t> tb_coff;
   // hands
   std::vector<uint8_t> cards[2];
   std::vector<float> weights[2];
