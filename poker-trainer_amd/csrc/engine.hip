@@ -43,12 +43,27 @@ constexpr int UP_T = 64;      // up workgroup: one wave, one hand per lane
 constexpr int DOWN_T = 512;   // down workgroup
 constexpr int MAX_K = 21;     // ceil(1326 / 64): hands per lane in the terminal kernel
 
+// One spot's hands and strength tables on the device (a batch holds many
+// spots; every record names its spot).
+struct SpotTab {
+  uint32_t n[2];
+  uint32_t root_slot[2];  // the root's reach slot, per traverser
+  uint32_t root_vslot, pad;
+  double half_pot, norm;
+  const float* weights[2];
+  const uint8_t* cards[2];
+  const uint32_t* same[2];
+  const uint32_t* bl_off[2];
+  const uint16_t* bl_idx[2];
+  const uint32_t *ent_off[2], *ent_cnt[2], *cl_base[2];
+  const uint16_t *ent_idx[2], *cl_off[2], *cl_idx[2];
+  const uint4* hand_cnt[2];
+};
+
 // Device view of the plan (all pointers into one HBM blob).
 struct DPlan {
-  uint32_t num_nodes, num_segs;
-  uint32_t n[2];
+  uint32_t num_nodes, num_segs, num_spots;
   uint32_t max_entries, max_members;
-  double half_pot, norm;
   const int8_t* card;
   const uint32_t *seg_mem_off, *seg_mem_cnt, *seg_pslot;
   const float* seg_factor;
@@ -57,13 +72,8 @@ struct DPlan {
   const TermBatch* tbatch;
   const MemRec* crec;
   const uint32_t* chance_child_vslot;
-  const uint8_t* cards[2];
-  const uint32_t* same[2];
-  const uint32_t* bl_off[2];
-  const uint16_t* bl_idx[2];
-  const uint32_t *ent_off[2], *ent_cnt[2], *cl_base[2];
-  const uint16_t *ent_idx[2], *cl_off[2], *cl_idx[2], *my_cnt[2], *hand_ent[2];
-  const uint4* hand_cnt[2];
+  const SpotTab* spots;
+  const uint32_t* active;  // per spot: 0 freezes the spot (its root is dead)
 };
 
 // The nodes a traversal covers: {root} and [lo, hi) (the whole tree, or the
@@ -113,12 +123,31 @@ __device__ __forceinline__ void stage_records(const MemRec* __restrict__ g, uint
 }
 
 // ------------------------------------------------------------------ begin
+// Every spot's root reach row: the opponent's range weights (blockIdx.y: spot).
 template <typename V>
-__global__ void begin_kernel(uint32_t nq, const float* __restrict__ w, V* __restrict__ reach, uint32_t slot,
-                             uint32_t* __restrict__ alive, uint32_t is_alive) {
-  uint32_t h = blockIdx.x * blockDim.x + threadIdx.x;
-  if (h < nq) reach[(size_t)slot * nq + h] = (V)w[h];
+__global__ void begin_kernel(DPlan P, int p, V* __restrict__ reach, uint32_t* __restrict__ alive, uint32_t rowlen) {
+  const SpotTab& S = P.spots[blockIdx.y];
+  const uint32_t nq = S.n[p ^ 1], slot = S.root_slot[p];
+  const uint32_t h = blockIdx.x * blockDim.x + threadIdx.x;
+  if (h < nq) reach[(size_t)slot * rowlen + h] = (V)S.weights[p ^ 1][h];
+  if (h == 0) alive[slot] = P.active[blockIdx.y];
+}
+
+// One reach row for a query: the caller's opponent reach at `slot`.
+template <typename V>
+__global__ void begin_row_kernel(uint32_t nq, const float* __restrict__ w, V* __restrict__ reach, uint32_t slot,
+                                 uint32_t* __restrict__ alive, uint32_t is_alive, uint32_t rowlen) {
+  const uint32_t h = blockIdx.x * blockDim.x + threadIdx.x;
+  if (h < nq) reach[(size_t)slot * rowlen + h] = (V)w[h];
   if (h == 0) alive[slot] = is_alive;
+}
+
+// Every spot's root value row of traverser p into out[(spot * 2 + p) * rowlen].
+template <typename V>
+__global__ void collect_kernel(DPlan P, int p, const V* __restrict__ vals, V* __restrict__ out, uint32_t rowlen) {
+  const SpotTab& S = P.spots[blockIdx.y];
+  const uint32_t h = blockIdx.x * blockDim.x + threadIdx.x;
+  if (h < S.n[p]) out[((size_t)blockIdx.y * 2 + p) * rowlen + h] = vals[(size_t)S.root_vslot * rowlen + h];
 }
 
 // ------------------------------------------------------------------ down
@@ -129,22 +158,25 @@ __global__ void begin_kernel(uint32_t nq, const float* __restrict__ w, V* __rest
 template <typename V, int MODE>
 __global__ void __launch_bounds__(1024) down_kernel(DPlan P, int p, uint32_t seg0, Span span, Bias B, V* __restrict__ reach,
                                                       uint32_t* __restrict__ alive, const float* __restrict__ regrets,
-                                                      float* __restrict__ sums, const IterParams* __restrict__ prm) {
+                                                      float* __restrict__ sums, const IterParams* __restrict__ prm,
+                                                      uint32_t rowlen) {
   extern __shared__ uint4 smem_down[];
   __shared__ uint32_t red;
   const uint32_t seg = seg0 + blockIdx.x;
   const int o = p ^ 1;
-  const uint32_t nq = P.n[o], tid = threadIdx.x;
   const uint32_t moff = P.seg_mem_off[seg], mcnt = P.seg_mem_cnt[seg];
   const uint32_t root = P.mrec[moff].node;
+  const SpotTab& S = P.spots[P.mrec[moff].spot];
+  const uint32_t nq = S.n[o], tid = threadIdx.x;
   if (!span.has(root) && seg != span.seg) return;
   MemRec* rec = (MemRec*)smem_down;
   uint8_t* m_alive = (uint8_t*)(rec + P.max_members);
   stage_records(P.mrec + moff, mcnt, rec);
   __syncthreads();
-  if (seg == span.seg) {
+  const uint32_t sp_raw = P.seg_pslot[(size_t)p * P.num_segs + seg];
+  if (seg == span.seg || sp_raw == NONE) {
     if (tid == 0) {
-      // the span root's row was written by begin_kernel
+      // the span root's (or a spot root's) row was written by a begin kernel
       const uint32_t l = span.root == root ? 0u : 0xFFFFu;
       uint32_t rl = l;
       if (l != 0)
@@ -155,16 +187,16 @@ __global__ void __launch_bounds__(1024) down_kernel(DPlan P, int p, uint32_t seg
   } else {
     // Chance child: the parent's reach times the chance factor, opponent
     // hands holding the dealt card zeroed (cfr.rs:449-456).
-    const uint32_t sp = B.s(P.seg_pslot[(size_t)p * P.num_segs + seg]), sm = B.s(rec[0].slot[p]);
+    const uint32_t sp = B.s(sp_raw), sm = B.s(rec[0].slot[p]);
     uint32_t any = 0;
     if (alive[sp]) {
       const V f = (V)P.seg_factor[seg];
       const int c = rec[0].card;
-      const V* pr = reach + (size_t)sp * nq;
-      V* mr = reach + (size_t)sm * nq;
+      const V* pr = reach + (size_t)sp * rowlen;
+      V* mr = reach + (size_t)sm * rowlen;
       for (uint32_t h = tid; h < nq; h += blockDim.x) {
         V v = pr[h] * f;
-        if (holds(P.cards[o], h, c)) v = V(0);
+        if (holds(S.cards[o], h, c)) v = V(0);
         mr[h] = v;
         any |= v != V(0);
       }
@@ -195,8 +227,8 @@ __global__ void __launch_bounds__(1024) down_kernel(DPlan P, int p, uint32_t seg
       __syncthreads();
       continue;
     }
-    const V* rrow = reach + (size_t)B.s(r.slot[p]) * nq;
-    V* crow = reach + (size_t)cs * nq;
+    const V* rrow = reach + (size_t)B.s(r.slot[p]) * rowlen;
+    V* crow = reach + (size_t)cs * rowlen;
     const uint64_t off = r.offset;
     uint32_t bits = 0;
     for (uint32_t h = tid; h < nq; h += blockDim.x) {
@@ -238,7 +270,7 @@ __global__ void __launch_bounds__(1024) down_kernel(DPlan P, int p, uint32_t seg
       for (int a = 0; a < MAXA; a++)
         if (a < (int)na) {
           const V v = sa[a] * rr;
-          crow[(size_t)a * nq + h] = v;
+          crow[(size_t)a * rowlen + h] = v;
           if (v != V(0)) bits |= 1u << a;
         }
     }
@@ -337,11 +369,12 @@ __global__ void __launch_bounds__(TERM_T) terminal_kernel(DPlan P, int p, uint32
   const uint32_t s = B.s(pick(tr.slot, p));
   if (!alive[s]) return;
   const int o = p ^ 1;
-  const uint32_t nq = P.n[o], np = P.n[p], tid = threadIdx.x;
+  const SpotTab& S = P.spots[tr.spot];
+  const uint32_t nq = S.n[o], np = S.n[p], tid = threadIdx.x;
   double* Pa = sh;               // [rowlen]
   double* Ca = sh + rowlen;      // [2 * rowlen]
   V* rv = (V*)(sh + 3 * rowlen); // [rowlen]: the opponent's reach row, hand order
-  const V* R = reach + (size_t)s * nq;
+  const V* R = reach + (size_t)s * rowlen;
   V* out = vals + (size_t)B.v(tr.vslot) * rowlen;
   for (uint32_t j0 = 0; j0 < nq; j0 += TERM_T * 8) {
     V x[8];
@@ -358,11 +391,11 @@ __global__ void __launch_bounds__(TERM_T) terminal_kernel(DPlan P, int p, uint32
   }
   __syncthreads();
   const bool batched = (tr.tflags & kTermBatched) != 0;
-  const double hp = P.half_pot, norm = P.norm, rake = (double)tr.rake;
+  const double hp = S.half_pot, norm = S.norm, rake = (double)tr.rake;
   const int32_t inv_p = pick(tr.invest, p), inv_o = pick(tr.invest, o);
-  const uint8_t* cp = P.cards[p];
-  const uint8_t* co = P.cards[o];
-  const uint32_t* same_p = P.same[p];
+  const uint8_t* cp = S.cards[p];
+  const uint8_t* co = S.cards[o];
+  const uint32_t* same_p = S.same[p];
 
   if (tr.kind != kShowdown) {
     const int f = tr.kind - kFold0;
@@ -371,16 +404,16 @@ __global__ void __launch_bounds__(TERM_T) terminal_kernel(DPlan P, int p, uint32
     const bool paired = (tr.tflags & (p == 0 ? kTermPaired0 : kTermPaired1)) != 0;
     if (paired) {  // strength order of the sibling showdown's board
       const uint32_t b = tr.board;
-      const uint32_t cnt = P.ent_cnt[o][b];
-      const uint16_t* clo = P.cl_off[o] + (size_t)b * 53;
-      gather(Pa, rv, P.ent_idx[o] + P.ent_off[o][b], cnt);
-      gather(Ca, rv, P.cl_idx[o] + P.cl_base[o][b], 2 * cnt);
+      const uint32_t cnt = S.ent_cnt[o][b];
+      const uint16_t* clo = S.cl_off[o] + (size_t)b * 53;
+      gather(Pa, rv, S.ent_idx[o] + S.ent_off[o][b], cnt);
+      gather(Ca, rv, S.cl_idx[o] + S.cl_base[o][b], 2 * cnt);
       __syncthreads();
       if (tid == 52) tot[52] = chain_sum(Pa, 0, cnt);
       else if (tid < 52) tot[tid] = chain_sum(Ca, clo[tid], clo[tid + 1]);
     } else {  // hand order
-      const uint32_t* bo = P.bl_off[o];
-      gather(Ca, rv, P.bl_idx[o], bo[52]);
+      const uint32_t* bo = S.bl_off[o];
+      gather(Ca, rv, S.bl_idx[o], bo[52]);
       __syncthreads();
       for (uint32_t j = tid; j < nq; j += TERM_T) Pa[j] = (double)rv[j];
       __syncthreads();
@@ -420,11 +453,11 @@ __global__ void __launch_bounds__(TERM_T) terminal_kernel(DPlan P, int p, uint32
   const double lose = -(hp + (double)inv_p) * norm;
   const double tie = 0.5 * ((double)(inv_o - inv_p) - rake) * norm;
   const uint32_t b = tr.board;
-  const uint32_t mo = P.ent_cnt[o][b];
-  const uint16_t* eidx = P.ent_idx[o] + P.ent_off[o][b];
-  const uint16_t* clo = P.cl_off[o] + (size_t)b * 53;
-  const uint16_t* ci = P.cl_idx[o] + P.cl_base[o][b];
-  const uint4* hcnt = P.hand_cnt[p] + (size_t)b * np;
+  const uint32_t mo = S.ent_cnt[o][b];
+  const uint16_t* eidx = S.ent_idx[o] + S.ent_off[o][b];
+  const uint16_t* clo = S.cl_off[o] + (size_t)b * 53;
+  const uint16_t* ci = S.cl_idx[o] + S.cl_base[o][b];
+  const uint4* hcnt = S.hand_cnt[p] + (size_t)b * np;
   const uint16_t* cp2 = (const uint16_t*)cp;
 
   // ascending sweep: weaker opponent hands
@@ -560,7 +593,8 @@ __global__ void __launch_bounds__(FT) terminal_fast_kernel(DPlan P, uint32_t b0,
   __shared__ uint32_t offs[53];
   const TermBatch tb = P.tbatch[b0 + blockIdx.x];
   constexpr int o = p ^ 1;
-  const uint32_t nq = P.n[o], np = P.n[p], tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const SpotTab& S = P.spots[tb.spot];
+  const uint32_t nq = S.n[o], np = S.n[p], tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const uint32_t b = tb.board;
   const bool sorted = b != NONE;  // strength order of the board, else hand order
 
@@ -568,19 +602,19 @@ __global__ void __launch_bounds__(FT) terminal_fast_kernel(DPlan P, uint32_t b0,
   // (0xFFFF: none), and the board's lt / le counts
   uint32_t hcs[FK];
   uint4 hc[FK];
-  const uint16_t* cp2 = (const uint16_t*)P.cards[p];
-  const uint4* hcnt = P.hand_cnt[p] + (size_t)(sorted ? b : 0) * np;
+  const uint16_t* cp2 = (const uint16_t*)S.cards[p];
+  const uint4* hcnt = S.hand_cnt[p] + (size_t)(sorted ? b : 0) * np;
 #pragma unroll
   for (int k = 0; k < FK; k++) {
     const uint32_t i = tid + k * FT;
-    const uint32_t sm = i < np ? P.same[p][i] : NONE;
+    const uint32_t sm = i < np ? S.same[p][i] : NONE;
     hcs[k] = (i < np ? (uint32_t)cp2[i] : 0u) | ((sm == NONE ? 0xFFFFu : sm) << 16);
     hc[k] = (sorted && i < np) ? hcnt[i] : make_uint4(0, 0, 0, 0);
   }
   // the opponent's entries in strength order (or hand order): a contiguous
   // slice per lane
-  const uint32_t nA = sorted ? P.ent_cnt[o][b] : nq;
-  const uint16_t* idxA = P.ent_idx[o] + (sorted ? P.ent_off[o][b] : 0u);
+  const uint32_t nA = sorted ? S.ent_cnt[o][b] : nq;
+  const uint16_t* idxA = S.ent_idx[o] + (sorted ? S.ent_off[o][b] : 0u);
   uint32_t ja[FE];
 #pragma unroll
   for (int k = 0; k < FE; k++) {
@@ -588,13 +622,13 @@ __global__ void __launch_bounds__(FT) terminal_fast_kernel(DPlan P, uint32_t b0,
     ja[k] = j < nA ? (sorted ? (uint32_t)idxA[j] : j) : NONE;
   }
   // the opponent's hands holding card tid / CG, a slice per lane of the group
-  if (tid < 53) offs[tid] = sorted ? (uint32_t)P.cl_off[o][(size_t)b * 53 + tid] : P.bl_off[o][tid];
-  const uint16_t* idxC = sorted ? P.cl_idx[o] + P.cl_base[o][b] : P.bl_idx[o];
+  if (tid < 53) offs[tid] = sorted ? (uint32_t)S.cl_off[o][(size_t)b * 53 + tid] : S.bl_off[o][tid];
+  const uint16_t* idxC = sorted ? S.cl_idx[o] + S.cl_base[o][b] : S.bl_idx[o];
   const uint32_t cq = tid & (CG - 1), card = tid / CG;
   uint32_t cs = 0, ce = 0;
   if (tid < 52 * CG) {
-    const uint32_t a = sorted ? (uint32_t)P.cl_off[o][(size_t)b * 53 + card] : P.bl_off[o][card];
-    const uint32_t e = sorted ? (uint32_t)P.cl_off[o][(size_t)b * 53 + card + 1] : P.bl_off[o][card + 1];
+    const uint32_t a = sorted ? (uint32_t)S.cl_off[o][(size_t)b * 53 + card] : S.bl_off[o][card];
+    const uint32_t e = sorted ? (uint32_t)S.cl_off[o][(size_t)b * 53 + card + 1] : S.bl_off[o][card + 1];
     const uint32_t L = (e - a + CG - 1) / CG;
     cs = min(a + cq * L, e);
     ce = min(cs + L, e);
@@ -608,8 +642,8 @@ __global__ void __launch_bounds__(FT) terminal_fast_kernel(DPlan P, uint32_t b0,
   double* Ca = sh + 2 * rowlen;   // [2 rowlen] per-card ascending prefix
   double* Cd = sh + 4 * rowlen;   // [2 rowlen] per-card descending suffix
   V* rv = (V*)(sh + 6 * rowlen);  // [rowlen] the opponent's reach row, hand order
-  const double hp = P.half_pot, norm = P.norm;
-  const uint8_t* co = P.cards[o];
+  const double hp = S.half_pot, norm = S.norm;
+  const uint8_t* co = S.cards[o];
 
   // the first terminal's record, flag and reach row
   uint32_t t = tb.t0;
@@ -618,7 +652,7 @@ __global__ void __launch_bounds__(FT) terminal_fast_kernel(DPlan P, uint32_t b0,
   uint32_t live = span.has(tr.node) ? alive[B.s(tr.slot)] : 0u;
   V r[FK];
   {
-    const V* R = reach + (size_t)B.s(tr.slot) * nq;
+    const V* R = reach + (size_t)B.s(tr.slot) * rowlen;
 #pragma unroll
     for (int k = 0; k < FK; k++) {
       const uint32_t j = tid + k * FT;
@@ -636,7 +670,7 @@ __global__ void __launch_bounds__(FT) terminal_fast_kernel(DPlan P, uint32_t b0,
       trn = term_fields<p>(P.trec + t + 1);
       const uint32_t sn = B.s(trn.slot);
       liven = span.has(trn.node) ? alive[sn] : 0u;
-      const V* R = reach + (size_t)sn * nq;
+      const V* R = reach + (size_t)sn * rowlen;
 #pragma unroll
       for (int k = 0; k < FK; k++) {
         const uint32_t j = tid + k * FT;
@@ -816,7 +850,8 @@ __global__ void __launch_bounds__(UP_T) chance_kernel(DPlan P, int p, uint32_t c
                                                       uint32_t rowlen, const uint32_t* __restrict__ alive) {
   const uint32_t ci = c0 + blockIdx.x;
   const MemRec& r = P.crec[ci];
-  const uint32_t np = P.n[p];
+  const SpotTab& S = P.spots[r.spot];
+  const uint32_t np = S.n[p];
   const uint32_t h = blockIdx.y * UP_T + threadIdx.x;
   if (!span.has(r.node) || h >= np) return;
   V* out = vals + (size_t)B.v(r.vslot) * rowlen + h;
@@ -827,7 +862,7 @@ __global__ void __launch_bounds__(UP_T) chance_kernel(DPlan P, int p, uint32_t c
   const uint32_t na = r.na;
   const V* row = vals + (size_t)B.v(P.chance_child_vslot[ci]) * rowlen + h;
   const int8_t* cards = P.card + r.offset;
-  const int c1 = P.cards[p][2 * h], c2 = P.cards[p][2 * h + 1];
+  const int c1 = S.cards[p][2 * h], c2 = S.cards[p][2 * h + 1];
   double acc = 0.0;
   uint32_t a = 0;
   for (; a + 16 <= na; a += 16) {
@@ -864,9 +899,10 @@ __global__ void __launch_bounds__(UP_T) up_kernel(DPlan P, int p, uint32_t seg0,
                                                   const IterParams* __restrict__ prm, V* __restrict__ qout) {
   extern __shared__ uint4 smem_up[];
   const uint32_t seg = seg0 + blockIdx.x;
-  const uint32_t np = P.n[p], tid = threadIdx.x;
-  const uint32_t h = blockIdx.y * UP_T + tid;
   const uint32_t moff = P.seg_mem_off[seg], mcnt = P.seg_mem_cnt[seg];
+  const SpotTab& S = P.spots[P.mrec[moff].spot];
+  const uint32_t np = S.n[p], tid = threadIdx.x;
+  const uint32_t h = blockIdx.y * UP_T + tid;
   const uint32_t root = P.mrec[moff].node;
   if (!span.has(root) && seg != span.seg) return;
   MemRec* rec = (MemRec*)smem_up;
@@ -1060,63 +1096,86 @@ Status Engine::create(int device, Plan&& plan, const float* regrets, const float
     return kOutOfMemory;
   }
 
-  // plan blob
+  // plan blob: the records, then every spot's hands and strength tables
   std::vector<unsigned char> blob;
   size_t o_card = put(blob, P.card), o_smo = put(blob, P.seg_mem_off), o_smc = put(blob, P.seg_mem_cnt);
   size_t o_sps = put(blob, P.seg_pslot), o_sf = put(blob, P.seg_factor), o_mr = put(blob, P.mrec), o_tr = put(blob, P.trec);
   size_t o_cr = put(blob, P.crec), o_ccv = put(blob, P.chance_child_vslot), o_tb = put(blob, P.tbatch);
-  size_t o_cards[2], o_same[2], o_bo[2], o_bi[2], o_eo[2], o_ec[2], o_cb[2], o_ei[2], o_co[2], o_ci[2], o_mc[2], o_he[2], o_w[2], o_hc[2];
-  for (int q = 0; q < 2; q++) {
-    o_cards[q] = put(blob, P.cards[q]);
-    o_same[q] = put(blob, P.same[q]);
-    o_bo[q] = put(blob, P.bl_off[q]);
-    o_bi[q] = put(blob, P.bl_idx[q]);
-    o_eo[q] = put(blob, P.ent_off[q]);
-    o_ec[q] = put(blob, P.ent_cnt[q]);
-    o_cb[q] = put(blob, P.cl_base[q]);
-    o_ei[q] = put(blob, P.ent_idx[q]);
-    o_co[q] = put(blob, P.cl_off[q]);
-    o_ci[q] = put(blob, P.cl_idx[q]);
-    o_mc[q] = put(blob, P.my_cnt[q]);
-    o_he[q] = put(blob, P.hand_ent[q]);
-    o_hc[q] = put(blob, P.hand_cnt[q]);
-    o_w[q] = put(blob, P.weights[q]);
+  const uint32_t K = P.num_spots;
+  struct SpotOff {
+    size_t cards[2], same[2], bo[2], bi[2], eo[2], ec[2], cb[2], ei[2], co[2], ci[2], hc[2], w[2];
+  };
+  std::vector<SpotOff> so(K);
+  for (uint32_t k = 0; k < K; k++) {
+    const Plan& T = P.spot(k);
+    for (int q = 0; q < 2; q++) {
+      so[k].cards[q] = put(blob, T.cards[q]);
+      so[k].same[q] = put(blob, T.same[q]);
+      so[k].bo[q] = put(blob, T.bl_off[q]);
+      so[k].bi[q] = put(blob, T.bl_idx[q]);
+      so[k].eo[q] = put(blob, T.ent_off[q]);
+      so[k].ec[q] = put(blob, T.ent_cnt[q]);
+      so[k].cb[q] = put(blob, T.cl_base[q]);
+      so[k].ei[q] = put(blob, T.ent_idx[q]);
+      so[k].co[q] = put(blob, T.cl_off[q]);
+      so[k].ci[q] = put(blob, T.cl_idx[q]);
+      so[k].hc[q] = put(blob, T.hand_cnt[q]);
+      so[k].w[q] = put(blob, T.weights[q]);
+    }
   }
+  std::vector<uint32_t> ones(K, 1u);
+  const size_t o_act = put(blob, ones);
+  const size_t o_spots = (blob.size() + 255) & ~size_t(255);
+  blob.resize(o_spots + K * sizeof(SpotTab));
   DeviceBuf b;
   Status st;
   if ((st = e->alloc(b, blob.size())) != kOk) return st;
-  HIPCHK(hipMemcpy(b.p, blob.data(), blob.size(), hipMemcpyHostToDevice));
   unsigned char* base = (unsigned char*)b.p;
+#define AT(T, o) ((const T*)(base + (o)))
+  std::vector<SpotTab> tabs(K);
+  for (uint32_t k = 0; k < K; k++) {
+    const Plan& T = P.spot(k);
+    SpotTab& t = tabs[k];
+    t.n[0] = T.n[0];
+    t.n[1] = T.n[1];
+    t.root_slot[0] = P.spot_root_slot[0][k];
+    t.root_slot[1] = P.spot_root_slot[1][k];
+    t.root_vslot = P.spot_root_vslot[k];
+    t.pad = 0;
+    t.half_pot = T.half_pot;
+    t.norm = T.norm;
+    for (int q = 0; q < 2; q++) {
+      t.weights[q] = AT(float, so[k].w[q]);
+      t.cards[q] = AT(uint8_t, so[k].cards[q]); t.same[q] = AT(uint32_t, so[k].same[q]);
+      t.bl_off[q] = AT(uint32_t, so[k].bo[q]); t.bl_idx[q] = AT(uint16_t, so[k].bi[q]);
+      t.ent_off[q] = AT(uint32_t, so[k].eo[q]); t.ent_cnt[q] = AT(uint32_t, so[k].ec[q]);
+      t.cl_base[q] = AT(uint32_t, so[k].cb[q]); t.ent_idx[q] = AT(uint16_t, so[k].ei[q]);
+      t.cl_off[q] = AT(uint16_t, so[k].co[q]); t.cl_idx[q] = AT(uint16_t, so[k].ci[q]);
+      t.hand_cnt[q] = AT(uint4, so[k].hc[q]);
+    }
+  }
+  std::memcpy(blob.data() + o_spots, tabs.data(), K * sizeof(SpotTab));
+  HIPCHK(hipMemcpy(b.p, blob.data(), blob.size(), hipMemcpyHostToDevice));
   e->dp_.reset(new DPlanHolder());
   DPlan& d = e->dp_->d;
   d.num_nodes = (uint32_t)P.num_nodes;
   d.num_segs = (uint32_t)P.seg_root.size();
-  d.n[0] = P.n[0];
-  d.n[1] = P.n[1];
+  d.num_spots = K;
   d.max_entries = std::max(P.max_entries, 1u);
   d.max_members = std::max(P.max_members, 1u);
-  d.half_pot = P.half_pot;
-  d.norm = P.norm;
-#define AT(T, o) ((const T*)(base + (o)))
   d.card = AT(int8_t, o_card);
   d.seg_mem_off = AT(uint32_t, o_smo); d.seg_mem_cnt = AT(uint32_t, o_smc); d.seg_pslot = AT(uint32_t, o_sps);
   d.seg_factor = AT(float, o_sf); d.mrec = AT(MemRec, o_mr); d.trec = AT(TermRec, o_tr);
   d.crec = AT(MemRec, o_cr); d.chance_child_vslot = AT(uint32_t, o_ccv); d.tbatch = AT(TermBatch, o_tb);
-  for (int q = 0; q < 2; q++) {
-    d.cards[q] = AT(uint8_t, o_cards[q]); d.same[q] = AT(uint32_t, o_same[q]);
-    d.bl_off[q] = AT(uint32_t, o_bo[q]); d.bl_idx[q] = AT(uint16_t, o_bi[q]);
-    d.ent_off[q] = AT(uint32_t, o_eo[q]); d.ent_cnt[q] = AT(uint32_t, o_ec[q]); d.cl_base[q] = AT(uint32_t, o_cb[q]);
-    d.ent_idx[q] = AT(uint16_t, o_ei[q]); d.cl_off[q] = AT(uint16_t, o_co[q]); d.cl_idx[q] = AT(uint16_t, o_ci[q]);
-    d.my_cnt[q] = AT(uint16_t, o_mc[q]); d.hand_ent[q] = AT(uint16_t, o_he[q]);
-    d.hand_cnt[q] = AT(uint4, o_hc[q]);
-    e->weights_[q] = (float*)AT(float, o_w[q]);
-  }
+  d.spots = AT(SpotTab, o_spots);
+  d.active = AT(uint32_t, o_act);
+  e->active_ = (uint32_t*)(base + o_act);
 #undef AT
   // storage, then the scratch rows: chunk groups sized to what is left
   DeviceBuf br, bs, bre, bp;
   if ((st = e->alloc(br, P.storage_len * sizeof(float))) != kOk) return st;
   if ((st = e->alloc(bs, P.storage_len * sizeof(float))) != kOk) return st;
-  if ((st = e->alloc(bp, sizeof(IterParams) + 2 * rowlen * 8)) != kOk) return st;
+  if ((st = e->alloc(bp, sizeof(IterParams) + (size_t)K * 2 * rowlen * 8)) != kOk) return st;
   HIPCHK(hipMemGetInfo(&fr, &tot));
   uint64_t budget = fr > margin ? fr - margin : 0;
   if (scratch_budget != 0) budget = std::min<uint64_t>(budget, scratch_budget);
@@ -1221,7 +1280,11 @@ Status Engine::read_sums(uint64_t offset, size_t len, float* out) const {
 }
 
 // The root-value row of traverser p is kept past the other traversal.
-static inline size_t result_offset(int p, uint32_t rowlen) { return sizeof(IterParams) + (size_t)p * rowlen * 8; }
+// Root value rows of every spot, kept past the traversals: (spot, p) at
+// byte offset sizeof(IterParams) + (spot * 2 + p) * rowlen * element size.
+static inline size_t result_offset(uint32_t spot, int p, uint32_t rowlen, size_t vs) {
+  return sizeof(IterParams) + ((size_t)spot * 2 + p) * rowlen * vs;
+}
 
 struct ProfRec {
   hipStream_t st;
@@ -1260,7 +1323,7 @@ static void launch_span(const DPlan& d, const Plan& P, hipStream_t st, int p, Sp
     const uint32_t threads = (b - a) < 64 ? 1024 : DOWN_T;
     PROF_START(kKDown);
     hipLaunchKernelGGL((down_kernel<V, DMODE>), dim3(b - a), dim3(threads), lds_down, st, d, p, a, span, B, (V*)reach, alive,
-                       regrets, sums, prm);
+                       regrets, sums, prm, rowlen);
     PROF_STOP();
   };
   auto terminals = [&](uint32_t a, uint32_t b, const Bias& B) {
@@ -1320,14 +1383,15 @@ Status Engine::launch_traversal(int p, Mode mode, bool f64, ProfRec* prof) {
   const DPlan& d = dp_->d;
   const int o = p ^ 1;
   Span span{0, 1, (uint32_t)P.num_nodes, 0};
-  const uint32_t nq = P.n[o];
+  const uint32_t tiles = (std::max(P.n[0], P.n[1]) + 255) / 256;
+  (void)o;
   PROF_START(kKBegin);
   const int x = f64 ? 1 : 0;
   void* reach = reach_[x];
   void* vals = vals_[x];
   uint32_t* alive = alive_[x];
-  if (f64) hipLaunchKernelGGL((begin_kernel<double>), dim3((nq + 255) / 256), dim3(256), 0, stream_, nq, weights_[o], (double*)reach, 0u, alive, 1u);
-  else hipLaunchKernelGGL((begin_kernel<float>), dim3((nq + 255) / 256), dim3(256), 0, stream_, nq, weights_[o], (float*)reach, 0u, alive, 1u);
+  if (f64) hipLaunchKernelGGL((begin_kernel<double>), dim3(tiles, P.num_spots), dim3(256), 0, stream_, d, p, (double*)reach, alive, rowlen_);
+  else hipLaunchKernelGGL((begin_kernel<float>), dim3(tiles, P.num_spots), dim3(256), 0, stream_, d, p, (float*)reach, alive, rowlen_);
   PROF_STOP();
   if (mode == kCfr)
     launch_span<float, kCfr, kCfr>(d, P, stream_, p, span, 0, groups_[0], reach, alive, regrets_, sums_, params_, vals, rowlen_, lds_down_,
@@ -1341,10 +1405,10 @@ Status Engine::launch_traversal(int p, Mode mode, bool f64, ProfRec* prof) {
   else
     launch_span<float, kAverage, kAverage>(d, P, stream_, p, span, 0, groups_[0], reach, alive, regrets_, sums_, params_, vals, rowlen_,
                                            lds_down_, lds_term_[0], lds_up_[0], (float*)nullptr, precision_ == kFast);
-  // keep the root row of p
-  size_t vs = f64 ? 8 : 4;
-  HIPCHK(hipMemcpyAsync((char*)params_ + result_offset(p, rowlen_), (char*)vals + (size_t)P.vslot[0] * rowlen_ * vs,
-                        (size_t)P.n[p] * vs, hipMemcpyDeviceToDevice, stream_));
+  // keep every spot's root row of p
+  void* res = (char*)params_ + sizeof(IterParams);
+  if (f64) hipLaunchKernelGGL((collect_kernel<double>), dim3(tiles, P.num_spots), dim3(256), 0, stream_, d, p, (const double*)vals, (double*)res, rowlen_);
+  else hipLaunchKernelGGL((collect_kernel<float>), dim3(tiles, P.num_spots), dim3(256), 0, stream_, d, p, (const float*)vals, (float*)res, rowlen_);
   HIPCHK(hipGetLastError());
   return kOk;
 }
@@ -1417,14 +1481,14 @@ Status Engine::traverse(int p, Mode mode, const IterParams& dsc, std::vector<dou
   bool f64 = mode != kCfr;
   Status st = launch_traversal(p, mode, f64);
   if (st != kOk) return st;
-  size_t n = plan_.n[p];
+  size_t n = plan_.spot(0).n[p];
   out.assign(n, 0.0);
   if (f64) {
-    HIPCHK(hipMemcpyAsync(out.data(), (char*)params_ + result_offset(p, rowlen_), n * 8, hipMemcpyDeviceToHost, stream_));
+    HIPCHK(hipMemcpyAsync(out.data(), (char*)params_ + result_offset(0, p, rowlen_, 8), n * 8, hipMemcpyDeviceToHost, stream_));
     HIPCHK(hipStreamSynchronize(stream_));
   } else {
     std::vector<float> f(n);
-    HIPCHK(hipMemcpyAsync(f.data(), (char*)params_ + result_offset(p, rowlen_), n * 4, hipMemcpyDeviceToHost, stream_));
+    HIPCHK(hipMemcpyAsync(f.data(), (char*)params_ + result_offset(0, p, rowlen_, 4), n * 4, hipMemcpyDeviceToHost, stream_));
     HIPCHK(hipStreamSynchronize(stream_));
     for (size_t i = 0; i < n; i++) out[i] = f[i];
   }
@@ -1432,17 +1496,41 @@ Status Engine::traverse(int p, Mode mode, const IterParams& dsc, std::vector<dou
 }
 
 Status Engine::root_values(Mode mode, double res[2]) {  // cfr.rs:482-490, in FP64
+  std::vector<double> all;
+  Status st = root_values_all(mode, all);
+  if (st != kOk) return st;
+  res[0] = all[0];
+  res[1] = all[1];
+  return kOk;
+}
+
+Status Engine::root_values_all(Mode mode, std::vector<double>& res) {
   Status st = run_graph(mode == kBest ? 1 : 2);
   if (st != kOk) return st;
-  std::vector<double> rows(2 * (size_t)rowlen_);
-  HIPCHK(hipMemcpyAsync(rows.data(), (char*)params_ + result_offset(0, rowlen_), 2 * (size_t)rowlen_ * 8, hipMemcpyDeviceToHost,
+  const uint32_t K = plan_.num_spots;
+  std::vector<double> rows((size_t)K * 2 * rowlen_);
+  HIPCHK(hipMemcpyAsync(rows.data(), (char*)params_ + result_offset(0, 0, rowlen_, 8), rows.size() * 8, hipMemcpyDeviceToHost,
                         stream_));
   HIPCHK(hipStreamSynchronize(stream_));
-  for (int p = 0; p < 2; p++) {
-    double s = 0.0;
-    for (uint32_t i = 0; i < plan_.n[p]; i++) s += rows[(size_t)p * rowlen_ + i] * (double)plan_.weights[p][i];
-    res[p] = s;
+  res.assign((size_t)K * 2, 0.0);
+  for (uint32_t k = 0; k < K; k++) {
+    const Plan& T = plan_.spot(k);
+    for (int p = 0; p < 2; p++) {
+      double s = 0.0;
+      const double* row = rows.data() + ((size_t)k * 2 + p) * rowlen_;
+      for (uint32_t i = 0; i < T.n[p]; i++) s += row[i] * (double)T.weights[p][i];
+      res[(size_t)k * 2 + p] = s;
+    }
   }
+  return kOk;
+}
+
+Status Engine::set_active(const std::vector<uint32_t>& active) {
+  if (active.size() != plan_.num_spots) {
+    set_error("invalid input: one flag per spot");
+    return kInvalidInput;
+  }
+  HIPCHK(hipMemcpy(active_, active.data(), active.size() * 4, hipMemcpyHostToDevice));
   return kOk;
 }
 
@@ -1486,8 +1574,8 @@ Status Engine::subtree_values(uint32_t node, int q, const std::vector<float>& op
   if (slot >= P.slot_s0[q] && qc != kNone)
     for (const ChunkGroup& g : groups_[0])
       if (qc >= g.c0 && qc < g.c1) slot -= P.slot_coff[q][g.c0] - P.slot_s0[q];
-  hipLaunchKernelGGL((begin_kernel<float>), dim3((nq + 255) / 256), dim3(256), 0, stream_, nq, (const float*)dreach, (float*)reach_[0],
-                     slot, alive_[0], any ? 1u : 0u);
+  hipLaunchKernelGGL((begin_row_kernel<float>), dim3((nq + 255) / 256), dim3(256), 0, stream_, nq, (const float*)dreach,
+                     (float*)reach_[0], slot, alive_[0], any ? 1u : 0u, rowlen_);
   Span span{node, P.first_child[node], P.subtree_end[node], P.seg_of[node]};
   // depths from the node's segment on
   uint32_t seg = P.seg_of[node], first_depth = 0;

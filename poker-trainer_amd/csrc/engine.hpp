@@ -72,6 +72,10 @@ class Engine {
   Status traverse(int p, Mode mode, const IterParams& d, std::vector<double>& out);
   // Expected payoffs (chips) at the root under `mode` (kBest / kAverage), traversals in FP64.
   Status root_values(Mode mode, double out[2]);
+  // Every spot's root values (a batch): out[spot * 2 + player].
+  Status root_values_all(Mode mode, std::vector<double>& out);
+  // Spots with active = 0 are skipped by every traversal (their roots are dead).
+  Status set_active(const std::vector<uint32_t>& active);
   Status exploitability(bool raked, double* out);
   Status download(float* regrets, float* sums) const;
   Status upload(const float* regrets, const float* sums);
@@ -111,7 +115,7 @@ class Engine {
   void* vals_[2] = {nullptr, nullptr};
   uint32_t* alive_[2] = {nullptr, nullptr};
   size_t pool_bytes_[2] = {0, 0};
-  float* weights_[2] = {nullptr, nullptr};
+  uint32_t* active_ = nullptr;  // device, one flag per spot
   IterParams* params_ = nullptr;
   IterParams* params_host_ = nullptr;
   uint32_t rowlen_ = 0;

@@ -408,8 +408,8 @@ std::string build_plan(const PlanInput& in, Plan& P) {
     auto runs = [&](uint32_t a, uint32_t b) {
       uint32_t t = a;
       while (t < b) {
-        TermBatch r{t, 1, P.trec[t].board, 0};
-        while (t + r.cnt < b && r.cnt < bmax && P.trec[t + r.cnt].board == r.board) r.cnt++;
+        TermBatch r{t, 1, P.trec[t].board, P.trec[t].spot};
+        while (t + r.cnt < b && r.cnt < bmax && P.trec[t + r.cnt].board == r.board && P.trec[t + r.cnt].spot == r.spot) r.cnt++;
         P.tbatch.push_back(r);
         t += r.cnt;
       }
@@ -421,6 +421,14 @@ std::string build_plan(const PlanInput& in, Plan& P) {
     }
     P.tb_coff[C] = (uint32_t)P.tbatch.size();
   }
+
+  // ---- the one spot's root
+  P.num_spots = 1;
+  P.parts.clear();
+  P.spot_root_node.assign(1, 0);
+  P.spot_root_vslot.assign(1, P.vslot[0]);
+  P.spot_root_slot[0].assign(1, P.rslot[0]);
+  P.spot_root_slot[1].assign(1, P.rslot[N]);
 
   // ---- strength tables (crates/solver/src/hands.rs:128-159)
   uint32_t B = in.num_boards;
@@ -516,7 +524,7 @@ ScratchRows partition_chunks(const Plan& P, size_t es, uint32_t rowlen, uint64_t
     for (int p = 0; p < 2; p++) {
       const uint64_t n = (uint64_t)P.slot_s0[p] + (C ? P.slot_coff[p][c1] - P.slot_coff[p][c0] : 0);
       r.slot_rows[p] = n;
-      r.reach_bytes = std::max<uint64_t>(r.reach_bytes, n * P.n[p ^ 1] * es);
+      r.reach_bytes = std::max<uint64_t>(r.reach_bytes, n * rowlen * es);
       r.alive_rows = std::max<uint64_t>(r.alive_rows, n);
     }
     r.val_rows = (uint64_t)P.val_v0 + (C ? P.val_coff[c1] - P.val_coff[c0] : 0);

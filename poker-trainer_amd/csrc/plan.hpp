@@ -48,7 +48,8 @@ struct MemRec {
   uint8_t kind, na;
   int8_t card;         // chance child: the dealt card
   uint8_t tflags;
-  uint8_t pad[6];
+  uint16_t spot;       // the spot (tree) of a batch it belongs to
+  uint8_t pad[4];
 };
 static_assert(sizeof(MemRec) == 48, "MemRec layout");
 
@@ -56,7 +57,9 @@ struct TermRec {
   uint32_t node, slot[2], vslot, board;
   int32_t invest[2];
   float rake;
-  uint8_t kind, tflags, pad[6];
+  uint8_t kind, tflags;
+  uint16_t spot;
+  uint8_t pad[4];
 };
 static_assert(sizeof(TermRec) == 40, "TermRec layout");
 
@@ -67,7 +70,7 @@ struct TermBatch {
   uint32_t t0;     // first terminal record
   uint32_t cnt;    // terminals in the run
   uint32_t board;  // strength-table board, or kNone for hand order
-  uint32_t pad;
+  uint32_t spot;
 };
 static_assert(sizeof(TermBatch) == 16, "TermBatch layout");
 constexpr uint32_t kBatchMax = 16;  // terminals per run
@@ -141,6 +144,12 @@ struct Plan {
   // ([tb_coff[c], tb_coff[c+1]) for chunk c; tb_coff[0] = the top's count)
   std::vector<TermBatch> tbatch;
   std::vector<uint32_t> tb_coff;
+  // spots: one for a single tree; a batch (merge_plans) keeps every spot's
+  // own plan in `parts` for its hands and strength tables
+  uint32_t num_spots = 1;
+  std::vector<Plan> parts;
+  std::vector<uint32_t> spot_root_node, spot_root_vslot, spot_root_slot[2];
+  const Plan& spot(uint32_t k) const { return parts.empty() ? *this : parts[k]; }
   // hands
   std::vector<uint8_t> cards[2];
   std::vector<float> weights[2];
