@@ -229,6 +229,41 @@ pts_status pts_solver_query(pts_solver* s, const pts_step_c* path, int32_t npath
 /* Solver::ranges_after (lib.rs:903-912) */
 pts_status pts_solver_ranges_after(pts_solver* s, const pts_step_c* path, int32_t npath, float* out0, float* out1);
 
+/* ------------------------------------------------------------------ batch */
+/* Many independent spots solved together (no counterpart in the reference,
+ * which solves one spot per Solver): every spot's tree is built as
+ * pts_solver_new builds it, the trees are merged into one plan (segments of
+ * all spots by depth), and each DCFR iteration of every active spot is one
+ * hipGraph launch. The spots share one set of DCFR parameters and one
+ * iteration count; a spot's storage evolves exactly as it would alone. */
+typedef struct pts_batch pts_batch;
+pts_status pts_batch_new(const pts_config_c* cfgs, uint32_t num_spots, int device, int precision, pts_batch** out);
+void pts_batch_free(pts_batch* b);
+uint32_t pts_batch_num_spots(pts_batch* b);
+uint32_t pts_batch_iterations(pts_batch* b);
+pts_status pts_batch_stats(pts_batch* b, uint32_t spot, pts_stats_c* out);
+uint64_t pts_batch_storage_len(pts_batch* b, uint32_t spot);
+pts_status pts_batch_get_storage(pts_batch* b, uint32_t spot, float* regrets, float* sums);
+/* `count` DCFR iterations of every active spot. */
+pts_status pts_batch_iterate(pts_batch* b, uint32_t count);
+/* Exploitability of every spot in chips (FP64 best responses, raked as Solver). */
+pts_status pts_batch_exploitability(pts_batch* b, double* out);
+/* Iterates until every spot's exploitability is at most target_pct of its
+ * pot (checked every check_every iterations; a spot that reaches it is
+ * frozen) or max_iterations; out[spot] is each spot's progress. */
+pts_status pts_batch_solve(pts_batch* b, uint32_t max_iterations, double target_pct, uint32_t check_every,
+                           pts_progress_c* out);
+/* Autotuner: times `iterations` iterations for every tile choice (terminals
+ * per terminal workgroup 1..16, opponent hands per down workgroup
+ * 256/512/1024), keeps the fastest (best[0], best[1], best_ms per
+ * iteration) and restarts the solve from zero storage. */
+pts_status pts_batch_autotune(pts_batch* b, uint32_t iterations, uint32_t best[2], double* best_ms);
+pts_status pts_batch_set_tiles(pts_batch* b, uint32_t run_length, uint32_t down_threads);
+pts_status pts_batch_profile_iteration(pts_batch* b, double ms[4], uint32_t launches[4], double* wall_ms);
+/* Sizes of the merged plan; for a host-only batch (device -1, built and
+ * checked without a GPU) also the scratch bytes of its DCFR and FP64 rows. */
+pts_status pts_batch_host_info(pts_batch* b, uint64_t* num_nodes, uint64_t* storage_len, uint64_t scratch[2]);
+
 #ifdef __cplusplus
 }
 #endif

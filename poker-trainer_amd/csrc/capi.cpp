@@ -2,6 +2,7 @@
 // crates/solver/src/lib.rs on top of it.
 #include <cmath>
 #include <cstdio>
+#include <chrono>
 #include <cstring>
 #include <limits>
 #include <memory>
@@ -378,20 +379,21 @@ static PlanInput plan_input(pts_solver* s, std::vector<uint8_t>& kind, std::vect
   return in;
 }
 
-pts_status pts_solver_new(const pts_config_c* c, int device, int precision, uint64_t scratch_budget,
-                          pts_solver** out) {  // lib.rs:365-396
-  *out = nullptr;
-  std::unique_ptr<pts_solver> s(new pts_solver());
+}  // extern "C"
+
+// Solver::new on the host (lib.rs:365-396): validation, the tree, the hands
+// and the validated GPU plan.
+static std::string host_build(const pts_config_c* c, pts_solver* s, Plan& plan) {
   s->cfg = *c;
   std::string err = validate(*c, s->spec, s->hands);
-  if (!err.empty()) return bad(err);
+  if (!err.empty()) return err;
   s->counts = count_tree(s->spec);
   uint64_t need = memory_bytes(s->counts);
   if (need > c->max_memory_bytes) {
     char m[160];
     std::snprintf(m, sizeof m, "game tree needs ~%.2f GB but the limit is %.2f GB", (double)need / 1e9,
                   (double)c->max_memory_bytes / 1e9);
-    return bad(m);
+    return m;
   }
   s->tree = build_tree(s->spec);
   std::vector<uint8_t> kind, street;
@@ -402,9 +404,18 @@ pts_status pts_solver_new(const pts_config_c* c, int device, int precision, uint
   std::vector<float> rake, fac;
   std::vector<int8_t> card;
   std::vector<uint32_t> same[2];
-  PlanInput in = plan_input(s.get(), kind, street, nc, fc, inv, off, rake, fac, card, boards, same);
+  PlanInput in = plan_input(s, kind, street, nc, fc, inv, off, rake, fac, card, boards, same);
+  return build_plan(in, plan);
+}
+
+extern "C" {
+
+pts_status pts_solver_new(const pts_config_c* c, int device, int precision, uint64_t scratch_budget,
+                          pts_solver** out) {  // lib.rs:365-396
+  *out = nullptr;
+  std::unique_ptr<pts_solver> s(new pts_solver());
   Plan plan;
-  err = build_plan(in, plan);
+  std::string err = host_build(c, s.get(), plan);
   if (!err.empty()) return bad(err);
   if (device < 0) {  // host only: tree, hands and a validated plan, no engine
     s->host_plan = std::move(plan);
@@ -643,6 +654,224 @@ pts_status pts_solver_ranges_after(pts_solver* s, const pts_step_c* path, int32_
     std::memset(outs[p], 0, 1326 * sizeof(float));
     for (size_t h = 0; h < s->hands.n(p); h++) outs[p][s->hands.combos[p][h]] = reach[p][h];
   }
+  return PTS_OK;
+}
+
+// ------------------------------------------------------------------ batch
+}  // extern "C"
+
+struct pts_batch {
+  std::vector<std::unique_ptr<pts_solver>> spots;  // host side of every spot (no engines)
+  std::vector<uint64_t> store_off;
+  std::unique_ptr<Engine> engine;
+  uint32_t iterations = 0;
+  std::vector<uint32_t> active;
+  Plan host_plan;                  // host-only batches (device -1)
+  uint64_t host_scratch[2] = {0, 0};
+};
+
+#define NEED_BATCH_ENGINE(b) \
+  if (!(b)->engine) return bad("no GPU engine (the batch was built host-only)")
+
+static bool same_dcfr(const pts_config_c& a, const pts_config_c& b) {
+  return a.alpha == b.alpha && a.beta_some == b.beta_some && a.beta == b.beta && a.gamma == b.gamma &&
+         a.restart_average == b.restart_average && a.unweighted_average == b.unweighted_average;
+}
+
+// Exploitability of every spot (chips), all spots active for the traversals.
+static pts_status batch_expl(pts_batch* b, std::vector<double>& out) {
+  const uint32_t K = (uint32_t)b->spots.size();
+  std::vector<uint32_t> all(K, 1u);
+  Status e;
+  if ((e = b->engine->set_active(all)) != kOk) return st(e);
+  std::vector<double> br, ev;
+  if ((e = b->engine->root_values_all(kBest, br)) != kOk) return st(e);
+  bool any_raked = false;
+  for (auto& s : b->spots) any_raked |= raked(s.get());
+  if (any_raked && (e = b->engine->root_values_all(kAverage, ev)) != kOk) return st(e);
+  out.assign(K, 0.0);
+  for (uint32_t k = 0; k < K; k++)  // cfr.rs:494-502
+    out[k] = raked(b->spots[k].get()) ? ((br[2 * k] - ev[2 * k]) + (br[2 * k + 1] - ev[2 * k + 1])) / 2.0
+                                      : (br[2 * k] + br[2 * k + 1]) / 2.0;
+  if ((e = b->engine->set_active(b->active)) != kOk) return st(e);
+  return PTS_OK;
+}
+
+extern "C" {
+
+pts_status pts_batch_new(const pts_config_c* cfgs, uint32_t num_spots, int device, int precision, pts_batch** out) {
+  *out = nullptr;
+  if (num_spots == 0) return bad("invalid input: empty batch");
+  std::unique_ptr<pts_batch> b(new pts_batch());
+  std::vector<Plan> plans(num_spots);
+  uint64_t off = 0;
+  for (uint32_t k = 0; k < num_spots; k++) {
+    if (!same_dcfr(cfgs[k], cfgs[0])) return bad("invalid input: the spots of a batch share one set of DCFR parameters");
+    std::unique_ptr<pts_solver> s(new pts_solver());
+    std::string err = host_build(&cfgs[k], s.get(), plans[k]);
+    if (!err.empty()) return bad("spot " + std::to_string(k) + ": " + err);
+    b->store_off.push_back(off);
+    off += s->tree.storage_len;
+    b->spots.push_back(std::move(s));
+  }
+  Plan merged;
+  std::string err = merge_plans(std::move(plans), merged);
+  if (!err.empty()) return bad(err);
+  b->active.assign(num_spots, 1u);
+  if (device < 0) {  // host only: the merged plan, checked as an engine would check it
+    const uint32_t rowlen = (std::max(merged.n[0], merged.n[1]) + 31u) & ~31u;
+    for (int f64 = 0; f64 < 2; f64++) {
+      std::vector<ChunkGroup> g;
+      ScratchRows r = partition_chunks(merged, f64 ? 8 : 4, rowlen, ~0ull, g);
+      err = check_groups(merged, g, r);
+      if (!err.empty()) return bad("invalid input: " + err);
+      b->host_scratch[f64] = r.reach_bytes + r.val_bytes + r.alive_rows * 4;
+    }
+    b->host_plan = std::move(merged);
+    *out = b.release();
+    return PTS_OK;
+  }
+  Status e = Engine::create(device, std::move(merged), nullptr, nullptr, precision, 0, b->engine);
+  if (e != kOk) return st(e);
+  *out = b.release();
+  return PTS_OK;
+}
+
+void pts_batch_free(pts_batch* b) { delete b; }
+uint32_t pts_batch_num_spots(pts_batch* b) { return (uint32_t)b->spots.size(); }
+uint32_t pts_batch_iterations(pts_batch* b) { return b->iterations; }
+
+pts_status pts_batch_stats(pts_batch* b, uint32_t spot, pts_stats_c* out) {
+  if (spot >= b->spots.size()) return bad("invalid input: spot");
+  const TreeCounts& t = b->spots[spot]->counts;
+  *out = {t.player, t.chance, t.terminal, t.cells, memory_bytes(t)};
+  return PTS_OK;
+}
+
+uint64_t pts_batch_storage_len(pts_batch* b, uint32_t spot) {
+  return spot < b->spots.size() ? b->spots[spot]->tree.storage_len : 0;
+}
+
+pts_status pts_batch_get_storage(pts_batch* b, uint32_t spot, float* regrets, float* sums) {
+  NEED_BATCH_ENGINE(b);
+  if (spot >= b->spots.size()) return bad("invalid input: spot");
+  return st(b->engine->read_storage(b->store_off[spot], b->spots[spot]->tree.storage_len, regrets, sums));
+}
+
+pts_status pts_batch_iterate(pts_batch* b, uint32_t count) {
+  NEED_BATCH_ENGINE(b);
+  for (uint32_t i = 0; i < count; i++) {
+    b->iterations += 1;
+    Status e = b->engine->cfr_iteration(discount(b->iterations, b->spots[0]->cfg));
+    if (e != kOk) return st(e);
+  }
+  return PTS_OK;
+}
+
+pts_status pts_batch_exploitability(pts_batch* b, double* out) {
+  NEED_BATCH_ENGINE(b);
+  std::vector<double> x;
+  pts_status r = batch_expl(b, x);
+  if (r != PTS_OK) return r;
+  std::memcpy(out, x.data(), x.size() * sizeof(double));
+  return PTS_OK;
+}
+
+pts_status pts_batch_solve(pts_batch* b, uint32_t max_iterations, double target_pct, uint32_t check_every,
+                           pts_progress_c* out) {
+  NEED_BATCH_ENGINE(b);
+  const uint32_t K = (uint32_t)b->spots.size(), ce = std::max(check_every, 1u);
+  for (uint32_t k = 0; k < K; k++)
+    out[k] = {b->iterations, max_iterations, std::numeric_limits<double>::infinity(), std::numeric_limits<double>::infinity(),
+              0, 0};
+  uint32_t left = 0;
+  for (uint32_t k = 0; k < K; k++) left += b->active[k];
+  while (b->iterations < max_iterations && left > 0) {
+    pts_status r = pts_batch_iterate(b, 1);
+    if (r != PTS_OK) return r;
+    const bool last = b->iterations >= max_iterations;
+    if (b->iterations % ce != 0 && !last) continue;
+    std::vector<double> x;
+    if ((r = batch_expl(b, x)) != PTS_OK) return r;
+    for (uint32_t k = 0; k < K; k++) {
+      if (!b->active[k]) continue;
+      const double pot = (double)b->spots[k]->cfg.pot, pct = 100.0 * x[k] / pot;
+      out[k].iteration = b->iterations;
+      out[k].exploitability = x[k];
+      out[k].exploitability_pct = pct;
+      if (pct <= target_pct || last) {
+        out[k].finished = 1;
+        b->active[k] = 0;  // a solved spot is frozen: its traversals are skipped
+        left--;
+      }
+    }
+    Status e = b->engine->set_active(b->active);
+    if (e != kOk) return st(e);
+  }
+  return PTS_OK;
+}
+
+// Times `iterations` DCFR iterations for every run length x down-kernel
+// workgroup size, keeps the fastest, then restarts the solve from zero.
+pts_status pts_batch_autotune(pts_batch* b, uint32_t iterations, uint32_t best[2], double* best_ms) {
+  NEED_BATCH_ENGINE(b);
+  static const uint32_t runs[] = {1, 2, 4, 8, 16};
+  static const uint32_t downs[] = {256, 512, 1024};
+  double bm = std::numeric_limits<double>::infinity();
+  uint32_t br = b->engine->run_length(), bd = b->engine->down_threads();
+  const uint32_t it = std::max(iterations, 1u);
+  for (uint32_t rl : runs)
+    for (uint32_t dt : downs) {
+      Status e = b->engine->retune(rl, dt);
+      if (e != kOk) return st(e);
+      if ((e = b->engine->cfr_iteration(discount(1, b->spots[0]->cfg))) != kOk) return st(e);
+      auto t0 = std::chrono::steady_clock::now();
+      for (uint32_t i = 0; i < it; i++)
+        if ((e = b->engine->cfr_iteration(discount(2 + i, b->spots[0]->cfg))) != kOk) return st(e);
+      double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count() / it;
+      if (ms < bm) {
+        bm = ms;
+        br = rl;
+        bd = dt;
+      }
+    }
+  Status e = b->engine->retune(br, bd);
+  if (e != kOk) return st(e);
+  if ((e = b->engine->zero_storage()) != kOk) return st(e);
+  b->iterations = 0;
+  b->active.assign(b->spots.size(), 1u);
+  if ((e = b->engine->set_active(b->active)) != kOk) return st(e);
+  best[0] = br;
+  best[1] = bd;
+  *best_ms = bm;
+  return PTS_OK;
+}
+
+pts_status pts_batch_set_tiles(pts_batch* b, uint32_t run_length, uint32_t down_threads) {
+  NEED_BATCH_ENGINE(b);
+  return st(b->engine->retune(run_length, down_threads));
+}
+
+pts_status pts_batch_profile_iteration(pts_batch* b, double ms[4], uint32_t launches[4], double* wall_ms) {
+  NEED_BATCH_ENGINE(b);
+  b->iterations += 1;
+  Profile pr;
+  Status e = b->engine->profile_iteration(discount(b->iterations, b->spots[0]->cfg), pr);
+  if (e != kOk) return st(e);
+  for (int i = 0; i < 4; i++) {
+    ms[i] = pr.ms[i];
+    launches[i] = pr.launches[i];
+  }
+  *wall_ms = pr.wall_ms;
+  return PTS_OK;
+}
+
+pts_status pts_batch_host_info(pts_batch* b, uint64_t* num_nodes, uint64_t* storage_len, uint64_t scratch[2]) {
+  const Plan& P = b->engine ? b->engine->plan() : b->host_plan;
+  *num_nodes = P.num_nodes;
+  *storage_len = P.storage_len;
+  scratch[0] = b->host_scratch[0];
+  scratch[1] = b->host_scratch[1];
   return PTS_OK;
 }
 

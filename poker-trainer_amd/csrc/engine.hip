@@ -176,12 +176,16 @@ __global__ void __launch_bounds__(1024) down_kernel(DPlan P, int p, uint32_t seg
   const uint32_t sp_raw = P.seg_pslot[(size_t)p * P.num_segs + seg];
   if (seg == span.seg || sp_raw == NONE) {
     if (tid == 0) {
-      // the span root's (or a spot root's) row was written by a begin kernel
-      const uint32_t l = span.root == root ? 0u : 0xFFFFu;
-      uint32_t rl = l;
-      if (l != 0)
-        for (uint32_t m = 0; m < mcnt; m++)
-          if (rec[m].node == span.root) { rl = m; break; }
+      // the span root's (or a spot root's) row was written by a begin kernel:
+      // a spot's root segment starts at member 0; the span's segment at the
+      // member holding span.root (the host checks that it is one)
+      uint32_t rl = 0;
+      if (seg == span.seg && span.root != root)
+        for (uint32_t m = 1; m < mcnt; m++)
+          if (rec[m].node == span.root) {
+            rl = m;
+            break;
+          }
       m_alive[rec[rl].src[p]] = (uint8_t)(alive[B.s(rec[rl].slot[p])] != 0);
     }
   } else {
@@ -1100,7 +1104,11 @@ Status Engine::create(int device, Plan&& plan, const float* regrets, const float
   std::vector<unsigned char> blob;
   size_t o_card = put(blob, P.card), o_smo = put(blob, P.seg_mem_off), o_smc = put(blob, P.seg_mem_cnt);
   size_t o_sps = put(blob, P.seg_pslot), o_sf = put(blob, P.seg_factor), o_mr = put(blob, P.mrec), o_tr = put(blob, P.trec);
-  size_t o_cr = put(blob, P.crec), o_ccv = put(blob, P.chance_child_vslot), o_tb = put(blob, P.tbatch);
+  size_t o_cr = put(blob, P.crec), o_ccv = put(blob, P.chance_child_vslot);
+  // room for the most runs (runs of one terminal), so the run length can be retuned
+  std::vector<TermBatch> tb_room(std::max<size_t>(P.trec.size(), P.tbatch.size()) + 1);
+  std::copy(P.tbatch.begin(), P.tbatch.end(), tb_room.begin());
+  size_t o_tb = put(blob, tb_room);
   const uint32_t K = P.num_spots;
   struct SpotOff {
     size_t cards[2], same[2], bo[2], bi[2], eo[2], ec[2], cb[2], ei[2], co[2], ci[2], hc[2], w[2];
@@ -1168,6 +1176,8 @@ Status Engine::create(int device, Plan&& plan, const float* regrets, const float
   d.seg_factor = AT(float, o_sf); d.mrec = AT(MemRec, o_mr); d.trec = AT(TermRec, o_tr);
   d.crec = AT(MemRec, o_cr); d.chance_child_vslot = AT(uint32_t, o_ccv); d.tbatch = AT(TermBatch, o_tb);
   d.spots = AT(SpotTab, o_spots);
+  e->tbatch_dev_ = (TermBatch*)(base + o_tb);
+  e->run_length_ = default_run_length(P);
   d.active = AT(uint32_t, o_act);
   e->active_ = (uint32_t*)(base + o_act);
 #undef AT
@@ -1179,6 +1189,7 @@ Status Engine::create(int device, Plan&& plan, const float* regrets, const float
   HIPCHK(hipMemGetInfo(&fr, &tot));
   uint64_t budget = fr > margin ? fr - margin : 0;
   if (scratch_budget != 0) budget = std::min<uint64_t>(budget, scratch_budget);
+  e->budget_ = budget;
   // One scratch pool; the f32 (DCFR) and f64 (best response, average) row
   // types each lay out their reach rows, value rows and alive flags in it
   // (they never run at the same time), grouped to fit the budget.
@@ -1313,14 +1324,14 @@ template <typename V, int DMODE, int UMODE>
 static void launch_span(const DPlan& d, const Plan& P, hipStream_t st, int p, Span span, uint32_t first_depth,
                         const std::vector<ChunkGroup>& groups, void* reach, uint32_t* alive, float* regrets, float* sums,
                         const IterParams* prm, void* vals, uint32_t rowlen, size_t lds_down, size_t lds_term, size_t lds_up,
-                        V* qout, bool fast, ProfRec* prof = nullptr) {
+                        V* qout, bool fast, uint32_t down_t, ProfRec* prof = nullptr) {
   const uint32_t D = P.depths(), C = P.num_chunks;
   const uint32_t tiles = (P.n[p] + UP_T - 1) / UP_T;
   const Bias B0{P.slot_s0[p], 0, P.val_v0, 0};
   auto down = [&](uint32_t a, uint32_t b, const Bias& B) {
     if (b <= a) return;
     // few segments (the top of the tree): one lane per opponent hand
-    const uint32_t threads = (b - a) < 64 ? 1024 : DOWN_T;
+    const uint32_t threads = (b - a) < 64 ? 1024 : down_t;
     PROF_START(kKDown);
     hipLaunchKernelGGL((down_kernel<V, DMODE>), dim3(b - a), dim3(threads), lds_down, st, d, p, a, span, B, (V*)reach, alive,
                        regrets, sums, prm, rowlen);
@@ -1357,8 +1368,11 @@ static void launch_span(const DPlan& d, const Plan& P, hipStream_t st, int p, Sp
   if (first_depth == 0) down(P.depth_off[0], P.depth_off[1], B0);
   // fast terminals launch over runs (TermBatch), exact ones over records
   auto tlo = [&](uint32_t c) { return fast ? P.tb_coff[c] : P.term_coff[c]; };
-  if (C == 0) {
+  if (C == 0) {  // every row resident (a river tree, or a batch of spots): all depths at once
+    for (uint32_t dd = std::max(1u, first_depth); dd < D; dd++) down(P.depth_off[dd], P.depth_off[dd + 1], B0);
     terminals(0, fast ? (uint32_t)P.tbatch.size() : (uint32_t)P.terminals.size(), B0);
+    for (uint32_t dd = D; dd-- > std::max(1u, first_depth);)
+      up(P.chance_off[dd], P.chance_off[dd + 1], P.depth_off[dd], P.depth_off[dd + 1], B0);
   } else {
     const uint32_t qc = first_depth == 0 ? kNone : P.chunk_of[span.root];
     bool first = true;
@@ -1395,16 +1409,16 @@ Status Engine::launch_traversal(int p, Mode mode, bool f64, ProfRec* prof) {
   PROF_STOP();
   if (mode == kCfr)
     launch_span<float, kCfr, kCfr>(d, P, stream_, p, span, 0, groups_[0], reach, alive, regrets_, sums_, params_, vals, rowlen_, lds_down_,
-                                   lds_term_[0], lds_up_[0], (float*)nullptr, precision_ == kFast, prof);
+                                   lds_term_[0], lds_up_[0], (float*)nullptr, precision_ == kFast, down_threads_, prof);
   else if (mode == kBest)
     launch_span<double, kAverage, kBest>(d, P, stream_, p, span, 0, groups_[1], reach, alive, regrets_, sums_, params_, vals, rowlen_,
-                                         lds_down_, lds_term_[1], lds_up_[1], (double*)nullptr, precision_ == kFast);
+                                         lds_down_, lds_term_[1], lds_up_[1], (double*)nullptr, precision_ == kFast, down_threads_);
   else if (f64)
     launch_span<double, kAverage, kAverage>(d, P, stream_, p, span, 0, groups_[1], reach, alive, regrets_, sums_, params_, vals, rowlen_,
-                                            lds_down_, lds_term_[1], lds_up_[1], (double*)nullptr, precision_ == kFast);
+                                            lds_down_, lds_term_[1], lds_up_[1], (double*)nullptr, precision_ == kFast, down_threads_);
   else
     launch_span<float, kAverage, kAverage>(d, P, stream_, p, span, 0, groups_[0], reach, alive, regrets_, sums_, params_, vals, rowlen_,
-                                           lds_down_, lds_term_[0], lds_up_[0], (float*)nullptr, precision_ == kFast);
+                                           lds_down_, lds_term_[0], lds_up_[0], (float*)nullptr, precision_ == kFast, down_threads_);
   // keep every spot's root row of p
   void* res = (char*)params_ + sizeof(IterParams);
   if (f64) hipLaunchKernelGGL((collect_kernel<double>), dim3(tiles, P.num_spots), dim3(256), 0, stream_, d, p, (const double*)vals, (double*)res, rowlen_);
@@ -1525,6 +1539,53 @@ Status Engine::root_values_all(Mode mode, std::vector<double>& res) {
   return kOk;
 }
 
+Status Engine::retune(uint32_t run_length, uint32_t down_threads) {
+  if (down_threads != 256 && down_threads != 512 && down_threads != 1024) {
+    set_error("invalid input: down-kernel workgroup size");
+    return kInvalidInput;
+  }
+  HIPCHK(hipStreamSynchronize(stream_));
+  build_runs(plan_, run_length);
+  std::string err;
+  for (int f64 = 0; f64 < 2 && err.empty(); f64++) {
+    ScratchRows r = partition_chunks(plan_, f64 ? 8 : 4, rowlen_, budget_, groups_[f64]);
+    err = check_groups(plan_, groups_[f64], r);
+  }
+  if (!err.empty()) {
+    set_error("invalid input: " + err);
+    return kInvalidInput;
+  }
+  HIPCHK(hipMemcpy(tbatch_dev_, plan_.tbatch.data(), plan_.tbatch.size() * sizeof(TermBatch), hipMemcpyHostToDevice));
+  down_threads_ = down_threads;
+  run_length_ = std::min(std::max(run_length, 1u), kBatchMax);
+  if (use_graph_) {
+    for (int i = 0; i < 3; i++)
+      if (graph_[i]) {
+        (void)hipGraphExecDestroy(graph_[i]);
+        graph_[i] = nullptr;
+      }
+    Status st = build_graphs();
+    if (st != kOk) return st;
+  }
+  return kOk;
+}
+
+Status Engine::read_storage(uint64_t offset, size_t len, float* regrets, float* sums) const {
+  if (offset + len > plan_.storage_len) {
+    set_error("invalid input: storage range");
+    return kInvalidInput;
+  }
+  HIPCHK(hipStreamSynchronize(stream_));
+  if (regrets) HIPCHK(hipMemcpy(regrets, regrets_ + offset, len * sizeof(float), hipMemcpyDeviceToHost));
+  if (sums) HIPCHK(hipMemcpy(sums, sums_ + offset, len * sizeof(float), hipMemcpyDeviceToHost));
+  return kOk;
+}
+
+Status Engine::zero_storage() {
+  HIPCHK(hipStreamSynchronize(stream_));
+  return upload(nullptr, nullptr);
+}
+
 Status Engine::set_active(const std::vector<uint32_t>& active) {
   if (active.size() != plan_.num_spots) {
     set_error("invalid input: one flag per spot");
@@ -1550,6 +1611,10 @@ Status Engine::exploitability(bool raked, double* out) {  // cfr.rs:494-502
 
 Status Engine::subtree_values(uint32_t node, int q, const std::vector<float>& opp, std::vector<float>& child_vals) {
   const Plan& P = plan_;
+  if (P.num_spots != 1) {
+    set_error("invalid input: queries run on single-spot engines");
+    return kInvalidInput;
+  }
   if (node >= P.num_nodes || (int)P.kind[node] != q) {
     set_error("invalid input: query node is not a decision node of that player");
     return kInvalidInput;
@@ -1581,7 +1646,7 @@ Status Engine::subtree_values(uint32_t node, int q, const std::vector<float>& op
   uint32_t seg = P.seg_of[node], first_depth = 0;
   while (first_depth + 1 < P.depth_off.size() && P.depth_off[first_depth + 1] <= seg) first_depth++;
   launch_span<float, kAverage, kAverage>(dp_->d, P, stream_, q, span, first_depth, groups_[0], reach_[0], alive_[0], regrets_, sums_, params_, vals_[0],
-                                         rowlen_, lds_down_, lds_term_[0], lds_up_[0], dout, precision_ == kFast);
+                                         rowlen_, lds_down_, lds_term_[0], lds_up_[0], dout, precision_ == kFast, down_threads_);
   HIPCHK(hipGetLastError());
   child_vals.assign((size_t)na * np, 0.0f);
   HIPCHK(hipMemcpyAsync(child_vals.data(), dout, (size_t)na * np * 4, hipMemcpyDeviceToHost, stream_));

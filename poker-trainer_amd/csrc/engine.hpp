@@ -76,6 +76,15 @@ class Engine {
   Status root_values_all(Mode mode, std::vector<double>& out);
   // Spots with active = 0 are skipped by every traversal (their roots are dead).
   Status set_active(const std::vector<uint32_t>& active);
+  // Tiles: terminals per fast-terminal workgroup (run length, 1..16) and
+  // opponent hands per down-kernel workgroup (256, 512 or 1024); rebuilds
+  // the graphs. Same scratch groups (their rows do not depend on the runs).
+  Status retune(uint32_t run_length, uint32_t down_threads);
+  uint32_t run_length() const { return run_length_; }
+  uint32_t down_threads() const { return down_threads_; }
+  // A storage range to the host (either pointer may be null).
+  Status read_storage(uint64_t offset, size_t len, float* regrets, float* sums) const;
+  Status zero_storage();
   Status exploitability(bool raked, double* out);
   Status download(float* regrets, float* sums) const;
   Status upload(const float* regrets, const float* sums);
@@ -116,6 +125,9 @@ class Engine {
   uint32_t* alive_[2] = {nullptr, nullptr};
   size_t pool_bytes_[2] = {0, 0};
   uint32_t* active_ = nullptr;  // device, one flag per spot
+  TermBatch* tbatch_dev_ = nullptr;
+  uint32_t down_threads_ = 512, run_length_ = 1;
+  uint64_t budget_ = 0;
   IterParams* params_ = nullptr;
   IterParams* params_host_ = nullptr;
   uint32_t rowlen_ = 0;

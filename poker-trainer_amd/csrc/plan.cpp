@@ -398,29 +398,7 @@ std::string build_plan(const PlanInput& in, Plan& P) {
   }
 
   // ---- terminal runs (per chunk, consecutive terminals with one board)
-  {
-    const uint32_t C = P.num_chunks;
-    P.tbatch.clear();
-    P.tb_coff.assign(C + 1, 0);
-    // long runs amortize the tables; short ones keep a small tree's
-    // workgroups spread over the chip (about 4096 runs at least)
-    const uint32_t bmax = std::min<uint32_t>(kBatchMax, std::max<uint32_t>(1, (uint32_t)(P.trec.size() / 4096)));
-    auto runs = [&](uint32_t a, uint32_t b) {
-      uint32_t t = a;
-      while (t < b) {
-        TermBatch r{t, 1, P.trec[t].board, P.trec[t].spot};
-        while (t + r.cnt < b && r.cnt < bmax && P.trec[t + r.cnt].board == r.board && P.trec[t + r.cnt].spot == r.spot) r.cnt++;
-        P.tbatch.push_back(r);
-        t += r.cnt;
-      }
-    };
-    runs(0, C ? P.term_coff[0] : (uint32_t)P.trec.size());
-    for (uint32_t c = 0; c < C; c++) {
-      P.tb_coff[c] = (uint32_t)P.tbatch.size();
-      runs(P.term_coff[c], P.term_coff[c + 1]);
-    }
-    P.tb_coff[C] = (uint32_t)P.tbatch.size();
-  }
+  build_runs(P, default_run_length(P));
 
   // ---- the one spot's root
   P.num_spots = 1;
@@ -513,6 +491,179 @@ std::string build_plan(const PlanInput& in, Plan& P) {
         for (int k = 0; k < 6; k++) d[k] = c[k];
       }
   }
+  return "";
+}
+
+uint32_t default_run_length(const Plan& P) {
+  // long runs amortize the tables; short ones keep a small tree's
+  // workgroups spread over the chip (about 4096 runs at least)
+  return std::min<uint32_t>(kBatchMax, std::max<uint32_t>(1, (uint32_t)(P.trec.size() / 4096)));
+}
+
+void build_runs(Plan& P, uint32_t bmax) {
+  bmax = std::min(std::max(bmax, 1u), kBatchMax);
+  const uint32_t C = P.num_chunks;
+  P.tbatch.clear();
+  P.tb_coff.assign(C + 1, 0);
+  auto runs = [&](uint32_t a, uint32_t b) {
+    uint32_t t = a;
+    while (t < b) {
+      TermBatch r{t, 1, P.trec[t].board, P.trec[t].spot};
+      while (t + r.cnt < b && r.cnt < bmax && P.trec[t + r.cnt].board == r.board && P.trec[t + r.cnt].spot == r.spot) r.cnt++;
+      P.tbatch.push_back(r);
+      t += r.cnt;
+    }
+  };
+  runs(0, C ? P.term_coff[0] : (uint32_t)P.trec.size());
+  for (uint32_t c = 0; c < C; c++) {
+    P.tb_coff[c] = (uint32_t)P.tbatch.size();
+    runs(P.term_coff[c], P.term_coff[c + 1]);
+  }
+  P.tb_coff[C] = (uint32_t)P.tbatch.size();
+}
+
+std::string merge_plans(std::vector<Plan>&& parts, Plan& M) {
+  const uint32_t K = (uint32_t)parts.size();
+  if (K == 0) return fail("empty batch", 0);
+  if (K > 0xFFFF) return fail("too many spots in a batch", K);
+  M = Plan();
+  // offsets of every spot's ids in the merged numbering
+  std::vector<uint64_t> node_off(K), vslot_off(K), store_off(K), term_off(K), slot_off[2];
+  slot_off[0].resize(K);
+  slot_off[1].resize(K);
+  uint64_t nn = 0, nv = 0, ns = 0, nt = 0, sl[2] = {0, 0}, nb = 0;
+  uint32_t D = 0;
+  for (uint32_t k = 0; k < K; k++) {
+    const Plan& T = parts[k];
+    node_off[k] = nn;
+    vslot_off[k] = nv;
+    store_off[k] = ns;
+    term_off[k] = nt;
+    for (int p = 0; p < 2; p++) {
+      slot_off[p][k] = sl[p];
+      sl[p] += T.num_slots[p];
+    }
+    nn += T.num_nodes;
+    nv += T.num_vslots;
+    ns += T.storage_len;
+    nt += T.trec.size();
+    nb += T.num_boards;
+    D = std::max(D, T.depths());
+    M.n[0] = std::max(M.n[0], T.n[0]);
+    M.n[1] = std::max(M.n[1], T.n[1]);
+    M.max_members = std::max(M.max_members, T.max_members);
+    M.max_entries = std::max(M.max_entries, T.max_entries);
+  }
+  if (nn >= 0xFFFFFFFFull || nv >= 0xFFFFFFFFull || sl[0] >= 0xFFFFFFFFull || sl[1] >= 0xFFFFFFFFull || nt >= 0xFFFFFFFFull)
+    return fail("batch too large for 32-bit ids", nn);
+  M.num_nodes = nn;
+  M.num_vslots = (uint32_t)nv;
+  M.storage_len = ns;
+  M.num_slots[0] = (uint32_t)sl[0];
+  M.num_slots[1] = (uint32_t)sl[1];
+  M.num_boards = (uint32_t)nb;
+  M.kind.reserve(nn);
+  M.na.reserve(nn);
+  M.card.reserve(nn);
+  for (uint32_t k = 0; k < K; k++) {
+    const Plan& T = parts[k];
+    M.kind.insert(M.kind.end(), T.kind.begin(), T.kind.end());
+    M.na.insert(M.na.end(), T.na.begin(), T.na.end());
+    M.card.insert(M.card.end(), T.card.begin(), T.card.end());
+  }
+  auto shift = [&](MemRec r, uint32_t k) {
+    r.node += (uint32_t)node_off[k];
+    if (r.vslot != kNone) r.vslot += (uint32_t)vslot_off[k];
+    for (int p = 0; p < 2; p++) {
+      if (r.slot[p] != kNone) r.slot[p] += (uint32_t)slot_off[p][k];
+      if (r.cslot[p] != kNone) r.cslot[p] += (uint32_t)slot_off[p][k];
+    }
+    if (r.kind == kChance) r.offset += node_off[k];  // first child's node id
+    else if (is_player(r.kind)) r.offset += store_off[k];
+    r.spot = (uint16_t)k;
+    return r;
+  };
+  // segments by depth, spot by spot within a depth
+  M.depth_off.assign(D + 1, 0);
+  std::vector<uint32_t> pslot[2];
+  for (uint32_t d = 0; d < D; d++) {
+    M.depth_off[d] = (uint32_t)M.seg_root.size();
+    for (uint32_t k = 0; k < K; k++) {
+      const Plan& T = parts[k];
+      if (d >= T.depths()) continue;
+      const size_t S = T.seg_root.size();
+      for (uint32_t g = T.depth_off[d]; g < T.depth_off[d + 1]; g++) {
+        M.seg_root.push_back(T.seg_root[g] + (uint32_t)node_off[k]);
+        M.seg_mem_off.push_back((uint32_t)M.mrec.size());
+        M.seg_mem_cnt.push_back(T.seg_mem_cnt[g]);
+        for (uint32_t m = 0; m < T.seg_mem_cnt[g]; m++) M.mrec.push_back(shift(T.mrec[T.seg_mem_off[g] + m], k));
+        for (int p = 0; p < 2; p++) {
+          uint32_t x = T.seg_pslot[(size_t)p * S + g];
+          pslot[p].push_back(x == kNone ? kNone : x + (uint32_t)slot_off[p][k]);
+        }
+        M.seg_factor.push_back(T.seg_factor[g]);
+      }
+    }
+  }
+  M.depth_off[D] = (uint32_t)M.seg_root.size();
+  M.seg_pslot = pslot[0];
+  M.seg_pslot.insert(M.seg_pslot.end(), pslot[1].begin(), pslot[1].end());
+  // chance records by depth
+  M.chance_off.assign(D + 1, 0);
+  for (uint32_t d = 0; d < D; d++) {
+    M.chance_off[d] = (uint32_t)M.crec.size();
+    for (uint32_t k = 0; k < K; k++) {
+      const Plan& T = parts[k];
+      if (d >= T.depths()) continue;
+      for (uint32_t i = T.chance_off[d]; i < T.chance_off[d + 1]; i++) {
+        M.crec.push_back(shift(T.crec[i], k));
+        M.chance_child_vslot.push_back(T.chance_child_vslot[i] + (uint32_t)vslot_off[k]);
+      }
+    }
+  }
+  M.chance_off[D] = (uint32_t)M.crec.size();
+  // terminals, spot by spot
+  for (uint32_t k = 0; k < K; k++) {
+    const Plan& T = parts[k];
+    for (size_t t = 0; t < T.trec.size(); t++) {
+      TermRec r = T.trec[t];
+      r.node += (uint32_t)node_off[k];
+      r.slot[0] += (uint32_t)slot_off[0][k];
+      r.slot[1] += (uint32_t)slot_off[1][k];
+      r.vslot += (uint32_t)vslot_off[k];
+      r.spot = (uint16_t)k;
+      M.trec.push_back(r);
+      M.terminals.push_back(T.terminals[t] + (uint32_t)node_off[k]);
+    }
+  }
+  // one resident group: no chunks
+  M.num_chunks = 0;
+  M.chunk_of.assign(nn, kNone);
+  M.seg_coff.assign(D, 0);
+  M.ch_coff.assign(D, 0);
+  M.term_coff.assign(1, (uint32_t)M.trec.size());
+  for (int p = 0; p < 2; p++) {
+    M.slot_s0[p] = M.num_slots[p];
+    M.slot_coff[p].assign(1, M.num_slots[p]);
+  }
+  M.val_v0 = M.num_vslots;
+  M.val_coff.assign(1, M.num_vslots);
+  build_runs(M, default_run_length(M));
+  // spot roots
+  M.num_spots = K;
+  M.spot_root_node.resize(K);
+  M.spot_root_vslot.resize(K);
+  M.spot_root_slot[0].resize(K);
+  M.spot_root_slot[1].resize(K);
+  for (uint32_t k = 0; k < K; k++) {
+    M.spot_root_node[k] = (uint32_t)node_off[k];
+    M.spot_root_vslot[k] = parts[k].spot_root_vslot[0] + (uint32_t)vslot_off[k];
+    for (int p = 0; p < 2; p++) M.spot_root_slot[p][k] = parts[k].spot_root_slot[p][0] + (uint32_t)slot_off[p][k];
+  }
+  M.half_pot = parts[0].half_pot;
+  M.norm = parts[0].norm;
+  M.batch_river = parts[0].batch_river;
+  M.parts = std::move(parts);
   return "";
 }
 
@@ -615,7 +766,11 @@ std::string check_groups(const Plan& P, const std::vector<ChunkGroup>& G, const 
       if (!seg_ok(g, 0, 0, true)) return std::string(b);
     for (uint32_t i = P.chance_off[0]; i < P.chance_off[1]; i++)
       if (!chance_ok(i, 0, 0)) return std::string(b);
-    if (C == 0) {
+    if (C == 0) {  // every row resident: every segment and chance record at every depth
+      for (uint32_t g = P.depth_off[1]; g < P.depth_off[D]; g++)
+        if (!seg_ok(g, 0, 0, false)) return std::string(b);
+      for (uint32_t i = P.chance_off[1]; i < P.chance_off[D]; i++)
+        if (!chance_ok(i, 0, 0)) return std::string(b);
       for (uint32_t t = 0; t < (uint32_t)P.trec.size(); t++)
         if (!term_ok(t, 0, 0)) return std::string(b);
       uint32_t t = 0;

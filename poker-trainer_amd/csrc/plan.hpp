@@ -182,6 +182,16 @@ struct Plan {
 // Builds and validates the plan; returns "" or the reason it is invalid.
 std::string build_plan(const PlanInput& in, Plan& out);
 
+// Terminal runs of at most `bmax` terminals (rebuilds tbatch / tb_coff), and
+// the default length for a tree of this size.
+void build_runs(Plan& P, uint32_t bmax);
+uint32_t default_run_length(const Plan& P);
+
+// A batch of independent spots as one plan: segments of every spot merged by
+// depth, ids shifted, storage concatenated in spot order, every row
+// resident (no chunk groups). The spots' own plans move into `parts`.
+std::string merge_plans(std::vector<Plan>&& parts, Plan& out);
+
 // Chunk groups [c0, c1) and the scratch rows they need.
 struct ChunkGroup {
   uint32_t c0, c1;

@@ -62,6 +62,9 @@ EXPORTS = [
     "pts_solver_set_iterations", "pts_solver_storage_len", "pts_solver_num_hands", "pts_solver_get_storage",
     "pts_solver_set_storage", "pts_solver_profile_iteration", "pts_engine_profile_iteration", "pts_solver_num_nodes", "pts_solver_nodes", "pts_solver_query",
     "pts_solver_ranges_after", "pts_engine_num_groups", "pts_solver_num_groups", "pts_solver_plan_groups",
+    "pts_batch_new", "pts_batch_free", "pts_batch_num_spots", "pts_batch_iterations", "pts_batch_stats",
+    "pts_batch_storage_len", "pts_batch_get_storage", "pts_batch_iterate", "pts_batch_exploitability", "pts_batch_solve",
+    "pts_batch_autotune", "pts_batch_set_tiles", "pts_batch_profile_iteration", "pts_batch_host_info",
 ]
 
 
@@ -106,6 +109,24 @@ def _declare(L):
     L.pts_solver_num_groups.argtypes = [P, ctypes.c_int]
     L.pts_solver_plan_groups.argtypes = [P, ctypes.c_uint64, ctypes.c_int, ctypes.POINTER(ctypes.c_uint32),
                                          ctypes.POINTER(ctypes.c_uint64)]
+    L.pts_batch_new.argtypes = [ctypes.POINTER(ConfigC), ctypes.c_uint32, ctypes.c_int, ctypes.c_int, ctypes.POINTER(P)]
+    L.pts_batch_free.argtypes = [P]
+    L.pts_batch_num_spots.restype = ctypes.c_uint32
+    L.pts_batch_num_spots.argtypes = [P]
+    L.pts_batch_iterations.restype = ctypes.c_uint32
+    L.pts_batch_iterations.argtypes = [P]
+    L.pts_batch_stats.argtypes = [P, ctypes.c_uint32, ctypes.POINTER(StatsC)]
+    L.pts_batch_storage_len.restype = ctypes.c_uint64
+    L.pts_batch_storage_len.argtypes = [P, ctypes.c_uint32]
+    L.pts_batch_get_storage.argtypes = [P, ctypes.c_uint32, f32p, f32p]
+    L.pts_batch_iterate.argtypes = [P, ctypes.c_uint32]
+    L.pts_batch_exploitability.argtypes = [P, f64p]
+    L.pts_batch_solve.argtypes = [P, ctypes.c_uint32, ctypes.c_double, ctypes.c_uint32, ctypes.POINTER(ProgressC)]
+    L.pts_batch_autotune.argtypes = [P, ctypes.c_uint32, ctypes.POINTER(ctypes.c_uint32), ctypes.POINTER(ctypes.c_double)]
+    L.pts_batch_set_tiles.argtypes = [P, ctypes.c_uint32, ctypes.c_uint32]
+    L.pts_batch_profile_iteration.argtypes = [P, f64p, ctypes.POINTER(ctypes.c_uint32), ctypes.POINTER(ctypes.c_double)]
+    L.pts_batch_host_info.argtypes = [P, ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_uint64),
+                                      ctypes.POINTER(ctypes.c_uint64)]
     L.pts_engine_num_groups.restype = ctypes.c_uint32
     L.pts_engine_num_groups.argtypes = [P, ctypes.c_int]
     L.pts_solver_free.argtypes = [P]

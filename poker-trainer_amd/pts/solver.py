@@ -14,7 +14,7 @@ from typing import List, Optional
 
 import numpy as np
 
-from .config import SolveConfig
+from .config import ConfigC, SolveConfig
 from .native import ProgressC, StatsC, TreeDescC, check, lib, steps_c
 
 ACT_NAMES = {0: "fold", 1: "check", 2: "call"}
@@ -278,3 +278,79 @@ class Engine:
 
     def num_groups(self, f64: bool = False) -> int:
         return lib().pts_engine_num_groups(self.h, int(f64))
+
+
+class BatchSolver:
+    """pts_batch_*: many independent spots solved together on one GPU (one
+    hipGraph launch per DCFR iteration for all of them)."""
+
+    def __init__(self, cfgs: List[SolveConfig], device: int = 0, precision: int = 0):
+        self.cfgs = list(cfgs)
+        arr = (ConfigC * len(self.cfgs))()
+        for i, c in enumerate(self.cfgs):
+            arr[i] = c.to_c()
+        self._arr = arr
+        h = ctypes.c_void_p()
+        check(lib().pts_batch_new(arr, len(self.cfgs), device, precision, ctypes.byref(h)))
+        self.h = h
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().pts_batch_free(self.h)
+            self.h = None
+
+    def __len__(self):
+        return len(self.cfgs)
+
+    def host_info(self):
+        """(merged nodes, storage cells, scratch bytes of DCFR rows, of FP64 rows)."""
+        nn, sl = ctypes.c_uint64(), ctypes.c_uint64()
+        sc = (ctypes.c_uint64 * 2)()
+        check(lib().pts_batch_host_info(self.h, ctypes.byref(nn), ctypes.byref(sl), sc))
+        return nn.value, sl.value, sc[0], sc[1]
+
+    @property
+    def iterations(self):
+        return lib().pts_batch_iterations(self.h)
+
+    def stats(self, k) -> StatsC:
+        st = StatsC()
+        check(lib().pts_batch_stats(self.h, k, ctypes.byref(st)))
+        return st
+
+    def iterate(self, count: int = 1):
+        check(lib().pts_batch_iterate(self.h, count))
+
+    def exploitability(self):
+        out = np.zeros(len(self.cfgs), np.float64)
+        check(lib().pts_batch_exploitability(self.h, out))
+        return out
+
+    def solve(self, max_iterations: int, target_pct: float, check_every: int = 10):
+        out = (ProgressC * len(self.cfgs))()
+        check(lib().pts_batch_solve(self.h, max_iterations, target_pct, check_every, out))
+        return [out[i] for i in range(len(self.cfgs))]
+
+    def storage(self, k):
+        n = lib().pts_batch_storage_len(self.h, k)
+        r = np.zeros(n, np.float32)
+        s = np.zeros(n, np.float32)
+        check(lib().pts_batch_get_storage(self.h, k, r, s))
+        return r, s
+
+    def autotune(self, iterations: int = 3):
+        best = (ctypes.c_uint32 * 2)()
+        ms = ctypes.c_double()
+        check(lib().pts_batch_autotune(self.h, iterations, best, ctypes.byref(ms)))
+        return (best[0], best[1]), ms.value
+
+    def set_tiles(self, run_length: int, down_threads: int):
+        check(lib().pts_batch_set_tiles(self.h, run_length, down_threads))
+
+    def profile_iteration(self):
+        ms = np.zeros(4, np.float64)
+        ln = (ctypes.c_uint32 * 4)()
+        wall = ctypes.c_double()
+        check(lib().pts_batch_profile_iteration(self.h, ms, ln, ctypes.byref(wall)))
+        names = ("begin", "down", "terminal", "up")
+        return dict(zip(names, ms.tolist())), dict(zip(names, list(ln))), wall.value

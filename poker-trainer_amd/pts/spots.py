@@ -171,3 +171,52 @@ def small_flop_spot(seed: int = 0, classes: int = 14) -> SolveConfig:
 def _blocked(k: int, b: List[int]) -> bool:
     a, c = COMBOS[k]
     return a in b or c in b
+
+
+BATCH_SIZES = ["25%", "33%", "50%", "75%", "100%", "150%"]
+
+
+def batch_spot(kind: str, index: int) -> SolveConfig:
+    """One spot of the batched workload (workload id 3): a river or turn spot
+    with a seeded board; per player a range of n ~ U[80, 1326] nonzero combos
+    (n capped at the unblocked combos; weights U(0, 1]); stack = SPR x pot
+    with SPR log-uniform in [0.5, 10] (pot 10bb on the river, 20bb on the
+    turn); 1-3 bet sizes drawn from {25,33,50,75,100,150}% plus all-in,
+    raises 2.5x / all-in, 3 bets per street."""
+    river = kind == "river"
+    spot = index if river else 10000 + index
+    b = board(stream(3, spot, "board"), 5 if river else 4)
+    used = 0
+    for c in b:
+        used |= 1 << c
+    free = [i for i, (x, y) in enumerate(COMBOS) if not (used >> x & 1 or used >> y & 1)]
+    ranges = []
+    for p in range(2):
+        rng = stream(3, spot, f"range{p}")
+        n = min(80 + rng.below(1326 - 80 + 1), len(free))
+        pick = list(free)
+        for i in range(len(pick) - 1, 0, -1):  # Fisher-Yates, the first n kept
+            j = rng.below(i + 1)
+            pick[i], pick[j] = pick[j], pick[i]
+        w = [0.0] * 1326
+        for k in pick[:n]:
+            w[k] = 1.0 - rng.uniform()  # (0, 1]
+        ranges.append(w)
+    sh = stream(3, spot, "shape")
+    pot = 1000 if river else 2000
+    spr = math.exp(math.log(0.5) + sh.uniform() * (math.log(10.0) - math.log(0.5)))
+    stack = max(100, int(round(spr * pot)))
+    nsz = 1 + sh.below(3)
+    sizes = list(BATCH_SIZES)
+    for i in range(len(sizes) - 1, 0, -1):
+        j = sh.below(i + 1)
+        sizes[i], sizes[j] = sizes[j], sizes[i]
+    bets = sorted(sizes[:nsz], key=lambda x: float(x[:-1])) + ["allin"]
+    menu = PostflopMenu.one(bets, ["2.5x", "allin"], max_raises=3)
+    return SolveConfig(b, ranges, BettingState.street_start(RIVER if river else TURN, pot, stack, stack), menu,
+                       max_memory_bytes=1 << 40, dcfr=DcfrParams())
+
+
+def batch_spots(rivers: int = 256, turns: int = 64) -> List[SolveConfig]:
+    """The batched workload: `rivers` river spots then `turns` turn spots."""
+    return [batch_spot("river", i) for i in range(rivers)] + [batch_spot("turn", i) for i in range(turns)]

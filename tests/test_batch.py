@@ -1,0 +1,92 @@
+"""Batched spots (pts_batch_*): many independent trees merged into one plan
+and iterated by one hipGraph launch. A spot's arithmetic is the same as
+alone, so in exact precision every spot's storage after N batched
+iterations equals the single-spot GPU solve's and the oracle's bit for bit."""
+import numpy as np
+import pytest
+
+import pyoracle as O
+from pts import spots
+from pts.native import EXACT, FAST
+from pts.solver import BatchSolver, Solver
+
+
+def small_batch():
+    # mixed streets and hand counts, one set of DCFR parameters
+    return [spots.river_spot(0), spots.turn_spot(1), spots.river_spot(2), spots.batch_spot("river", 3),
+            spots.batch_spot("turn", 0), spots.river_spot(5)]
+
+
+def test_batch_plan_merges_and_checks_on_the_host():
+    cfgs = small_batch()
+    b = BatchSolver(cfgs, device=-1)
+    nodes, cells, s32, s64 = b.host_info()
+    single = [Solver(c, device=-1) for c in cfgs]
+    assert cells == sum(s.storage_len for s in single)
+    assert nodes == sum(s.stats().player_nodes + s.stats().chance_nodes + s.stats().terminal_nodes for s in single)
+    assert 0 < s32 < s64
+
+
+def test_batch_workload_spots_are_seeded_and_varied():
+    a, b = spots.batch_spot("river", 7), spots.batch_spot("river", 7)
+    assert a.board == b.board and a.ranges == b.ranges and a.root.stack == b.root.stack
+    cs = [spots.batch_spot("river", i) for i in range(40)]
+    n = [sum(1 for x in c.ranges[0] if x > 0) for c in cs]
+    assert min(n) >= 80 and max(n) <= 1326 and len(set(n)) > 30
+    spr = [c.root.stack[0] / c.root.pot for c in cs]
+    assert min(spr) >= 0.5 - 1e-9 and max(spr) <= 10 + 1e-9
+    sizes = {len(c.menu.river.bet) - 1 for c in cs}
+    assert sizes == {1, 2, 3}
+
+
+@pytest.mark.gpu
+def test_batch_iterations_bit_exact_per_spot():
+    cfgs = small_batch()
+    b = BatchSolver(cfgs, precision=EXACT)
+    b.iterate(8)
+    xb = b.exploitability()
+    for k, c in enumerate(cfgs):
+        s = Solver(c, precision=EXACT)
+        s.iterate(8)
+        rb, sb = b.storage(k)
+        rs, ss = s.storage()
+        assert np.array_equal(rb, rs) and np.array_equal(sb, ss), k
+        assert xb[k] == s.exploitability(), k
+    # and the oracle for one river and one turn spot
+    for k in (0, 1):
+        o = O.OracleSolver(cfgs[k])
+        o.iterate(8)
+        rb, sb = b.storage(k)
+        rc, sc = o.storage()
+        assert np.array_equal(rb, rc) and np.array_equal(sb, sc), k
+
+
+@pytest.mark.gpu
+def test_batch_solve_freezes_each_spot_at_its_target():
+    cfgs = small_batch()
+    b = BatchSolver(cfgs, precision=FAST)
+    pr = b.solve(400, 1.0, 10)
+    for k, p in enumerate(pr):
+        assert p.finished
+        assert p.exploitability_pct <= 1.0 or p.iteration == 400
+        # a frozen spot's storage is that of the same spot alone after the same iterations
+        s = Solver(cfgs[k], precision=FAST)
+        s.iterate(p.iteration)
+        rb, sb = b.storage(k)
+        rs, ss = s.storage()
+        assert np.array_equal(rb, rs) and np.array_equal(sb, ss), k
+
+
+@pytest.mark.gpu
+def test_autotune_keeps_the_arithmetic():
+    cfgs = small_batch()
+    a, b = BatchSolver(cfgs, precision=FAST), BatchSolver(cfgs, precision=FAST)
+    (rl, dt), ms = b.autotune(2)
+    assert rl in (1, 2, 4, 8, 16) and dt in (256, 512, 1024) and ms > 0
+    assert b.iterations == 0
+    a.iterate(6)
+    b.iterate(6)
+    for k in range(len(cfgs)):
+        ra, sa = a.storage(k)
+        rb, sb = b.storage(k)
+        assert np.array_equal(ra, rb) and np.array_equal(sa, sb), k
