@@ -32,7 +32,13 @@ WORKLOADS = {
              "and sums, alpha 1.5 beta 0 gamma 2", "turn_spot", 500),
     "river": ("river HU exact DCFR: seeded 5-card board, pot 10bb, stacks 40bb, bets 50%/100%/all-in, raise 2.5x, "
               "3 bets; ranges U(0,1) per combo with 40% zeroed; f32, alpha 1.5 beta 0 gamma 2", "river_spot", 1000),
+    "flop": ("flop HU exact DCFR: seeded 3-card board, pot 6bb, stacks 97bb, bets 33%/75%/all-in, raises 2.5x/all-in, "
+             "3 bets per street, 49 turn x 48 river cards; ranges from 169-class weights sigmoid(score + N(0,0.5)); f32 "
+             "regrets and sums (16.19 G cells), alpha 1.5 beta 0 gamma 2", "flop_spot", 300),
 }
+BATCH_DESC = ("batched HU exact DCFR: 256 river + 64 turn spots, seeded boards, per-range nonzero combos U[80,1326], "
+              "SPR log-uniform 0.5-10, 1-3 bet sizes from {25,33,50,75,100,150}% + all-in, raises 2.5x/all-in, 3 bets; "
+              "one hipGraph launch per iteration for all spots, autotuned tiles; f32")
 
 
 def log(*a):
@@ -71,7 +77,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--workload", default="turn", choices=sorted(WORKLOADS))
+    ap.add_argument("--workload", default="turn", choices=sorted(WORKLOADS) + ["batch"])
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--target", type=float, default=0.5)
     ap.add_argument("--check-every", type=int, default=10)
@@ -103,6 +109,8 @@ def main():
     from pts import spots
     from pts.solver import Solver, SolveOptions
 
+    if args.workload == "batch":
+        return bench_batch(args)
     desc, maker, max_iters = WORKLOADS[args.workload]
     cfg = getattr(spots, maker)(args.seed)
 
@@ -162,6 +170,101 @@ def main():
         "roofline": roofline, "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
+
+
+def bench_batch(args):
+    """spots/s to the target for the batched workload: autotuned tiles, then
+    one batched solve (checks every 10 iterations, a spot frozen at its
+    target) timed from the first iteration to the last spot's check."""
+    from pts import spots
+    from pts.solver import BatchSolver
+    prec = 1 if args.precision == "exact" else 0
+    cfgs = spots.batch_spots()
+    t0 = time.perf_counter()
+    b = BatchSolver(cfgs, precision=prec)
+    build_s = time.perf_counter() - t0
+    sts = [b.stats(k) for k in range(len(cfgs))]
+    cells = sum(st.storage_cells for st in sts)
+    hands = [[sum(1 for x in c.ranges[p] if x > 0) for p in range(2)] for c in cfgs]
+    log(f"[bench] batch: {len(cfgs)} spots, {cells} cells, {sum(st.terminal_nodes for st in sts)} terminals; "
+        f"build+upload {build_s:.2f}s")
+    (rl, dt), tune_ms = b.autotune(3)
+    log(f"[bench] autotuned tiles: {rl} terminals per terminal workgroup, {dt} hands per down workgroup "
+        f"({tune_ms:.3f} ms/iteration)")
+    b.iterate(args.warmup)
+    t0 = time.perf_counter()
+    b.iterate(args.steps)
+    step_s = (time.perf_counter() - t0) / max(args.steps, 1)
+    ms, launches, wall = b.profile_iteration()
+    log(f"[bench] {args.steps} timed iterations: {step_s * 1e3:.3f} ms each; profile {ms} {launches}")
+    del b
+    b = BatchSolver(cfgs, precision=prec)
+    b.set_tiles(rl, dt)
+    t0 = time.perf_counter()
+    pr = b.solve(1000, args.target, args.check_every)
+    solve_s = time.perf_counter() - t0
+    reached = sum(1 for p in pr if p.exploitability_pct <= args.target)
+    iters = [p.iteration for p in pr]
+    log(f"[bench] batched solve: {reached}/{len(cfgs)} spots at <= {args.target}% pot, {solve_s:.3f}s, "
+        f"iterations {min(iters)}..{max(iters)}")
+    # roofline of the dominant kernel class over one profiled iteration
+    terms = sum(st.terminal_nodes * (h[0] + h[1]) for st, h in zip(sts, hands))
+    bytes_cls = {"down": 12.0 * cells, "up": 8.0 * cells, "terminal": 4.0 * terms}
+    dom = max(("down", "terminal", "up"), key=lambda k: ms[k])
+    achieved = bytes_cls[dom] / (ms[dom] * 1e-3) / 1e9 if ms[dom] > 0 else 0.0
+    roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None, "kernel": dom,
+                "launches_per_iteration": launches[dom], "avg_launch_ms": round(ms[dom] / max(launches[dom], 1), 5),
+                "algorithmic_bytes_per_launch": round(bytes_cls[dom] / max(launches[dom], 1)),
+                "iteration_ms_by_class": {k: round(v, 4) for k, v in ms.items()}}
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline_batch(cfgs, iters, args.cpu_seconds)
+        log(f"[bench] cpu baseline: {cpu}")
+    line = {
+        "metric": "spots/s to 0.5%-pot exploitability, batched HU DCFR (256 river + 64 turn spots), 1 MI355X",
+        "value": round(len(cfgs) / solve_s, 3), "unit": "spots/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(step_s * 1e3, 4), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": BATCH_DESC, "spots": len(cfgs), "storage_cells": cells, "target_pct": args.target,
+                   "check_every": args.check_every, "spots_reaching_target": reached, "solve_s": round(solve_s, 4),
+                   "iterations_min_max": [min(iters), max(iters)], "tiles": {"terminal_run": rl, "down_hands": dt},
+                   "precision": args.precision, "build_s": round(build_s, 3), "parallelism": "1 GPU"},
+        "roofline": roofline, "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+def cpu_baseline_batch(cfgs, iters, seconds):
+    """The oracle on a bounded sample of the batch (every 16th river spot and
+    every 8th turn spot), each run for the iterations the GPU solve gave it
+    plus one check per 10 iterations, scaled to the whole batch."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or os.cpu_count()
+    sample = [k for k in range(len(cfgs)) if (k < 256 and k % 16 == 0) or (k >= 256 and (k - 256) % 8 == 0)]
+    spent, covered, done = 0.0, 0, []
+    for k in sample:
+        t0 = time.perf_counter()
+        o = pyoracle.OracleSolver(cfgs[k], parallel=True, threads=threads)
+        n = 0
+        while n < iters[k]:
+            o.iterate(min(10, iters[k] - n))
+            n += min(10, iters[k] - n)
+            o.exploitability()
+        spent += time.perf_counter() - t0
+        covered += 1
+        done.append(k)
+        if spent > seconds:
+            break
+    # scale: river and turn samples separately
+    riv = [k for k in done if k < 256]
+    trn = [k for k in done if k >= 256]
+    per = spent / max(covered, 1)
+    est = per * len(cfgs)
+    return {"value": round(len(cfgs) / est, 4), "unit": "spots/s", "cores": threads, "kind": "port",
+            "sample": f"{len(riv)} river + {len(trn)} turn spots of the batch solved by the oracle for the GPU run's "
+                      f"iteration counts with a check every 10 iterations ({spent:.1f}s), scaled to all {len(cfgs)} spots"}
 
 
 if __name__ == "__main__":
