@@ -264,6 +264,32 @@ pts_status pts_batch_profile_iteration(pts_batch* b, double ms[4], uint32_t laun
  * checked without a GPU) also the scratch bytes of its DCFR and FP64 rows. */
 pts_status pts_batch_host_info(pts_batch* b, uint64_t* num_nodes, uint64_t* storage_len, uint64_t scratch[2]);
 
+/* ------------------------------------------------------------------ value network */
+/* The counterfactual-value MLP of the depth-limited flop solve (no
+ * counterpart in the reference): 2706 inputs (52-card board mask, pot and
+ * stack as fractions of pot + stack, both ranges normalized to sum 1) ->
+ * 4 x 1024 ReLU -> 2652 outputs (both players' expected values per combo, in
+ * pot fractions). BF16 weights and activations, FP32 accumulation, hand-
+ * written MFMA kernels; masked Huber loss, Adam. Rows per call: a positive
+ * multiple of 64 up to the capacity. */
+typedef struct pts_net pts_net;
+uint64_t pts_net_num_params(void);
+double pts_net_forward_flops_per_row(void);
+/* He-normal weights from `seed`, zero biases. */
+pts_status pts_net_create(int device, uint32_t batch_cap, uint64_t seed, pts_net** out);
+void pts_net_destroy(pts_net* n);
+/* Packed f32 parameters: W0 (1024 x 2706, row-major), b0, W1, b1, W2, b2, W3,
+ * b3, W4 (2652 x 1024), b4. Setting them resets the Adam state. */
+pts_status pts_net_set_params(pts_net* n, const float* params);
+pts_status pts_net_get_params(pts_net* n, float* params);
+/* Gradients of the last pts_net_gradients / pts_net_train_step, packed alike. */
+pts_status pts_net_get_grads(pts_net* n, float* grads);
+pts_status pts_net_forward(pts_net* n, const float* x, uint32_t rows, float* y);
+pts_status pts_net_gradients(pts_net* n, const float* x, const float* target, const float* mask, uint32_t rows, double* loss);
+pts_status pts_net_train_step(pts_net* n, const float* x, const float* target, const float* mask, uint32_t rows, float lr,
+                              double* loss);
+pts_status pts_net_profile(pts_net* n, uint32_t rows, double* fwd_ms, double* train_ms);
+
 #ifdef __cplusplus
 }
 #endif

@@ -10,6 +10,7 @@
 #include "../../include/pts.h"
 #include "engine.hpp"
 #include "host.hpp"
+#include "net.hpp"
 #include "plan.hpp"
 
 using namespace pts;
@@ -873,6 +874,46 @@ pts_status pts_batch_host_info(pts_batch* b, uint64_t* num_nodes, uint64_t* stor
   scratch[0] = b->host_scratch[0];
   scratch[1] = b->host_scratch[1];
   return PTS_OK;
+}
+
+// ------------------------------------------------------------------ value network
+}  // extern "C"
+
+struct pts_net {
+  std::unique_ptr<net::Net> n;
+};
+
+extern "C" {
+
+uint64_t pts_net_num_params(void) { return net::num_params(); }
+double pts_net_forward_flops_per_row(void) { return net::forward_flops_per_row(); }
+
+pts_status pts_net_create(int device, uint32_t batch_cap, uint64_t seed, pts_net** out) {
+  *out = nullptr;
+  std::unique_ptr<pts_net> h(new pts_net());
+  Status e = net::Net::create(device, batch_cap, h->n);
+  if (e != kOk) return st(e);
+  std::vector<float> p;
+  net::he_init(seed, p);
+  if ((e = h->n->set_params(p.data())) != kOk) return st(e);
+  *out = h.release();
+  return PTS_OK;
+}
+
+void pts_net_destroy(pts_net* n) { delete n; }
+pts_status pts_net_set_params(pts_net* n, const float* params) { return st(n->n->set_params(params)); }
+pts_status pts_net_get_params(pts_net* n, float* params) { return st(n->n->get_params(params)); }
+pts_status pts_net_get_grads(pts_net* n, float* grads) { return st(n->n->get_grads(grads)); }
+pts_status pts_net_forward(pts_net* n, const float* x, uint32_t rows, float* y) { return st(n->n->forward(x, rows, y)); }
+pts_status pts_net_gradients(pts_net* n, const float* x, const float* target, const float* mask, uint32_t rows, double* loss) {
+  return st(n->n->gradients(x, target, mask, rows, loss));
+}
+pts_status pts_net_train_step(pts_net* n, const float* x, const float* target, const float* mask, uint32_t rows, float lr,
+                              double* loss) {
+  return st(n->n->train_step(x, target, mask, rows, lr, loss));
+}
+pts_status pts_net_profile(pts_net* n, uint32_t rows, double* fwd_ms, double* train_ms) {
+  return st(n->n->profile(rows, fwd_ms, train_ms));
 }
 
 }  // extern "C"

@@ -65,6 +65,8 @@ EXPORTS = [
     "pts_batch_new", "pts_batch_free", "pts_batch_num_spots", "pts_batch_iterations", "pts_batch_stats",
     "pts_batch_storage_len", "pts_batch_get_storage", "pts_batch_iterate", "pts_batch_exploitability", "pts_batch_solve",
     "pts_batch_autotune", "pts_batch_set_tiles", "pts_batch_profile_iteration", "pts_batch_host_info",
+    "pts_net_num_params", "pts_net_forward_flops_per_row", "pts_net_create", "pts_net_destroy", "pts_net_set_params",
+    "pts_net_get_params", "pts_net_get_grads", "pts_net_forward", "pts_net_gradients", "pts_net_train_step", "pts_net_profile",
 ]
 
 
@@ -127,6 +129,19 @@ def _declare(L):
     L.pts_batch_profile_iteration.argtypes = [P, f64p, ctypes.POINTER(ctypes.c_uint32), ctypes.POINTER(ctypes.c_double)]
     L.pts_batch_host_info.argtypes = [P, ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_uint64),
                                       ctypes.POINTER(ctypes.c_uint64)]
+    L.pts_net_num_params.restype = ctypes.c_uint64
+    L.pts_net_num_params.argtypes = []
+    L.pts_net_forward_flops_per_row.restype = ctypes.c_double
+    L.pts_net_forward_flops_per_row.argtypes = []
+    L.pts_net_create.argtypes = [ctypes.c_int, ctypes.c_uint32, ctypes.c_uint64, ctypes.POINTER(P)]
+    L.pts_net_destroy.argtypes = [P]
+    L.pts_net_set_params.argtypes = [P, f32p]
+    L.pts_net_get_params.argtypes = [P, f32p]
+    L.pts_net_get_grads.argtypes = [P, f32p]
+    L.pts_net_forward.argtypes = [P, f32p, ctypes.c_uint32, f32p]
+    L.pts_net_gradients.argtypes = [P, f32p, f32p, f32p, ctypes.c_uint32, ctypes.POINTER(ctypes.c_double)]
+    L.pts_net_train_step.argtypes = [P, f32p, f32p, f32p, ctypes.c_uint32, ctypes.c_float, ctypes.POINTER(ctypes.c_double)]
+    L.pts_net_profile.argtypes = [P, ctypes.c_uint32, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]
     L.pts_engine_num_groups.restype = ctypes.c_uint32
     L.pts_engine_num_groups.argtypes = [P, ctypes.c_int]
     L.pts_solver_free.argtypes = [P]

@@ -1,0 +1,44 @@
+"""The value network's CPU reference (oracle/net_ref.py) pinned by
+mathematics: its float64 gradients equal central finite differences of its
+loss on a small network of the same structure, and its bf16 rounding is
+round-to-nearest-even."""
+import numpy as np
+
+import net_ref as R
+
+
+def test_bf16_rounding_is_nearest_even():
+    assert R.bf16(1.0) == 1.0
+    assert R.bf16(1.0 + 2.0 ** -8) == 1.0          # a tie rounds to the even mantissa
+    assert R.bf16(1.0 + 3 * 2.0 ** -8) == 1.0 + 2.0 ** -6
+    assert R.bf16(1.0 + 2.0 ** -7) == 1.0 + 2.0 ** -7
+    assert R.bf16(-3.14159) == -3.140625
+    x = np.random.default_rng(0).standard_normal(1000)
+    assert np.all(np.abs(R.bf16(x) - x) <= np.abs(x) * 2.0 ** -8)
+
+
+def test_reference_gradients_match_finite_differences():
+    rng = np.random.default_rng(1)
+    shp = R.shapes(din=7, hid=6, dout=5, layers=5)
+    n = sum(o * i + o for o, i in shp)
+    params = rng.standard_normal(n) * 0.7
+    x = rng.standard_normal((4, 7))
+    target = rng.standard_normal((4, 5)) * 2.0   # some entries beyond the Huber knee
+    mask = (rng.random((4, 5)) < 0.8).astype(float)
+    loss, g = R.gradients(params, x, target, mask, shp, quantize=False)
+    eps = 1e-6
+    num = np.zeros(n)
+    for i in range(n):
+        p1, p2 = params.copy(), params.copy()
+        p1[i] += eps
+        p2[i] -= eps
+        num[i] = (R.gradients(p1, x, target, mask, shp, quantize=False)[0]
+                  - R.gradients(p2, x, target, mask, shp, quantize=False)[0]) / (2 * eps)
+    assert np.allclose(g, num, rtol=1e-5, atol=1e-7), np.max(np.abs(g - num))
+    assert loss > 0
+
+
+def test_reference_shapes_match_the_network():
+    shp = R.shapes()
+    assert shp[0] == (1024, 2706) and shp[-1] == (2652, 1024) and len(shp) == 5
+    assert sum(o * i + o for o, i in shp) == 8639068
