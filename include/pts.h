@@ -290,6 +290,21 @@ pts_status pts_net_train_step(pts_net* n, const float* x, const float* target, c
                               double* loss);
 pts_status pts_net_profile(pts_net* n, uint32_t rows, double* fwd_ms, double* train_ms);
 
+/* ------------------------------------------------------------------ depth-limited solve */
+/* A solver over the tree cut where the root street's betting ends with
+ * chips behind (all-in lines keep their exact runouts); every cut point is
+ * valued per dealt card by `net` (null: zero), on the GPU inside each
+ * iteration. Returns a pts_solver; its exploitability is the depth-limited
+ * game's. (New here: the reference has no value network.) */
+pts_status pts_dl_new(const pts_config_c* cfg, int device, int precision, pts_net* net, pts_solver** out);
+uint32_t pts_solver_num_leaves(pts_solver* s);
+/* Copies src's root-street decision nodes (a depth-limited solver) into the
+ * same action paths of dst (the full tree), scaling regrets and strategy
+ * sums: the warm start of the exact solve that judges a value-network
+ * solve. */
+pts_status pts_solver_copy_root_street(pts_solver* dst, pts_solver* src, float regret_scale, float sum_scale,
+                                       uint32_t* copied);
+
 #ifdef __cplusplus
 }
 #endif

@@ -377,6 +377,8 @@ static PlanInput plan_input(pts_solver* s, std::vector<uint8_t>& kind, std::vect
   in.num_combinations = s->hands.num_combinations;
   in.storage_len = s->tree.storage_len;
   in.batch_river = true;
+  in.root_stack[0] = s->cfg.stack[0];
+  in.root_stack[1] = s->cfg.stack[1];
   return in;
 }
 
@@ -384,10 +386,11 @@ static PlanInput plan_input(pts_solver* s, std::vector<uint8_t>& kind, std::vect
 
 // Solver::new on the host (lib.rs:365-396): validation, the tree, the hands
 // and the validated GPU plan.
-static std::string host_build(const pts_config_c* c, pts_solver* s, Plan& plan) {
+static std::string host_build(const pts_config_c* c, pts_solver* s, Plan& plan, bool depth_limit = false) {
   s->cfg = *c;
   std::string err = validate(*c, s->spec, s->hands);
   if (!err.empty()) return err;
+  s->spec.depth_limit = depth_limit;
   s->counts = count_tree(s->spec);
   uint64_t need = memory_bytes(s->counts);
   if (need > c->max_memory_bytes) {
@@ -914,6 +917,64 @@ pts_status pts_net_train_step(pts_net* n, const float* x, const float* target, c
 }
 pts_status pts_net_profile(pts_net* n, uint32_t rows, double* fwd_ms, double* train_ms) {
   return st(n->n->profile(rows, fwd_ms, train_ms));
+}
+
+// ------------------------------------------------------------------ depth-limited solve
+// A flop (or turn) solver whose tree stops where the root street's betting
+// ends with chips behind; the value network values those leaves per dealt
+// card. The result is a pts_solver: iterate / solve / storage / nodes work
+// as for a full tree (exploitability is that of the depth-limited game).
+pts_status pts_dl_new(const pts_config_c* cfg, int device, int precision, pts_net* n, pts_solver** out) {
+  *out = nullptr;
+  std::unique_ptr<pts_solver> s(new pts_solver());
+  Plan plan;
+  std::string err = host_build(cfg, s.get(), plan, true);
+  if (!err.empty()) return bad(err);
+  if (plan.leaves.empty()) return bad("invalid input: the depth-limited tree has no leaves");
+  Status e = Engine::create(device, std::move(plan), nullptr, nullptr, precision, 0, s->engine);
+  if (e != kOk) return st(e);
+  if ((e = s->engine->attach_net(n ? n->n.get() : nullptr, s->spec.board)) != kOk) return st(e);
+  *out = s.release();
+  return PTS_OK;
+}
+
+uint32_t pts_solver_num_leaves(pts_solver* s) {
+  const Plan& P = s->engine ? s->engine->plan() : s->host_plan;
+  return (uint32_t)P.leaves.size();
+}
+
+// The root street's decision nodes of `src` (a depth-limited tree) copied
+// into the same action paths of `dst` (the full tree): regrets times
+// regret_scale and strategy sums times sum_scale. Returns the nodes copied.
+pts_status pts_solver_copy_root_street(pts_solver* dst, pts_solver* src, float regret_scale, float sum_scale,
+                                       uint32_t* copied) {
+  NEED_ENGINE(dst);
+  NEED_ENGINE(src);
+  if (dst->hands.n(0) != src->hands.n(0) || dst->hands.n(1) != src->hands.n(1))
+    return bad("invalid input: the two solvers have different hand lists");
+  const Tree& a = src->tree;
+  const Tree& b = dst->tree;
+  const uint8_t street = (uint8_t)src->spec.root_street;
+  std::vector<std::pair<uint32_t, uint32_t>> stack{{0u, 0u}};
+  uint32_t n = 0;
+  while (!stack.empty()) {
+    auto [x, y] = stack.back();
+    stack.pop_back();
+    const Node& na = a.nodes[x];
+    const Node& nb = b.nodes[y];
+    if (!is_player(na.kind) || na.kind != nb.kind || na.street != street || na.num_children != nb.num_children) continue;
+    const size_t len = (size_t)na.num_children * src->hands.n(na.kind);
+    std::vector<float> r(len), m(len);
+    Status e = src->engine->read_storage(na.offset, len, r.data(), m.data());
+    if (e != kOk) return st(e);
+    for (auto& v : r) v *= regret_scale;
+    for (auto& v : m) v *= sum_scale;
+    if ((e = dst->engine->write_storage(nb.offset, len, r.data(), m.data())) != kOk) return st(e);
+    n++;
+    for (uint32_t k = 0; k < na.num_children; k++) stack.push_back({na.first_child + k, nb.first_child + k});
+  }
+  *copied = n;
+  return PTS_OK;
 }
 
 }  // extern "C"

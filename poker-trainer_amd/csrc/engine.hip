@@ -17,8 +17,10 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <functional>
 
 #include "engine.hpp"
+#include "net.hpp"
 
 namespace pts {
 
@@ -58,6 +60,7 @@ struct SpotTab {
   const uint32_t *ent_off[2], *ent_cnt[2], *cl_base[2];
   const uint16_t *ent_idx[2], *cl_off[2], *cl_idx[2];
   const uint4* hand_cnt[2];
+  const uint32_t* lookup[2];  // combo -> hand, or NONE
 };
 
 // Device view of the plan (all pointers into one HBM blob).
@@ -1018,6 +1021,182 @@ __global__ void __launch_bounds__(UP_T) up_kernel(DPlan P, int p, uint32_t seg0,
   vals[(size_t)B.v(rec[0].vslot) * rowlen + h] = lv[tid];
 }
 
+// ------------------------------------------------------------------ leaves
+// Depth-limited trees end the root street's betting in leaves whose value,
+// per dealt card, comes from the value network. Per traversal of player p:
+//   leaf_own_kernel    p's reach at every leaf (p's current strategy in CFR
+//                      mode, the average strategy otherwise): one lane per
+//                      hand walks the leaves' segment top-down;
+//   leaf_input_kernel  one network input row per (leaf, dealt card): board
+//                      mask, pot and stack fractions, both ranges at the leaf
+//                      without the hands holding the card, each summing to 1;
+//   (the network)      expected values per combo in pot fractions;
+//   leaf_value_kernel  v(h) = norm * factor * pot * sum over cards c not in h
+//                      of EV_p(c, h) * compat_c(h), compat_c(h) the
+//                      opponent's reach over hands avoiding h and c
+//                      (inclusion-exclusion over h's two cards and c), as the
+//                      chance node the leaf replaces would sum its children.
+template <typename V, bool AVG>
+__global__ void __launch_bounds__(1024) leaf_own_kernel(DPlan P, int p, uint32_t seg, const float* __restrict__ regrets,
+                                                        const float* __restrict__ sums, V* __restrict__ mreach,
+                                                        const LeafRec* __restrict__ leaves, uint32_t nleaves,
+                                                        V* __restrict__ own, uint32_t rowlen) {
+  const uint32_t moff = P.seg_mem_off[seg], mcnt = P.seg_mem_cnt[seg];
+  const SpotTab& S = P.spots[P.mrec[moff].spot];
+  const uint32_t np = S.n[p];
+  for (uint32_t h = threadIdx.x; h < np; h += blockDim.x) {
+    mreach[h] = (V)S.weights[p][h];
+    for (uint32_t m = 0; m < mcnt; m++) {
+      const MemRec& r = P.mrec[moff + m];
+      if (r.kind > kP1) continue;  // decision nodes only
+      const V rr = mreach[(size_t)m * rowlen + h];
+      const uint32_t na = r.na;
+      V sg[MAXA];
+      if ((int)r.kind == p) {
+        if (AVG) {
+          V sv[MAXA];
+          V d = V(0);
+#pragma unroll
+          for (int a = 0; a < MAXA; a++)
+            if (a < (int)na) {
+              sv[a] = (V)sums[r.offset + (size_t)a * np + h];
+              d += sv[a];
+            }
+          const V inv = recip(d), uni = V(1) / (V)na;
+#pragma unroll
+          for (int a = 0; a < MAXA; a++)
+            if (a < (int)na) sg[a] = inv == V(0) ? uni : sv[a] * inv;
+        } else {
+          float rg[MAXA];
+          float d = 0.0f;
+#pragma unroll
+          for (int a = 0; a < MAXA; a++)
+            if (a < (int)na) {
+              rg[a] = regrets[r.offset + (size_t)a * np + h];
+              d += pos0(rg[a]);
+            }
+          const float inv = recip(d), uni = 1.0f / (float)na;
+#pragma unroll
+          for (int a = 0; a < MAXA; a++)
+            if (a < (int)na) sg[a] = (V)(inv == 0.0f ? uni : pos0(rg[a]) * inv);
+        }
+      } else {
+#pragma unroll
+        for (int a = 0; a < MAXA; a++) sg[a] = V(1);
+      }
+#pragma unroll
+      for (int a = 0; a < MAXA; a++)
+        if (a < (int)na) mreach[(size_t)(r.clocal + a) * rowlen + h] = rr * sg[a];
+    }
+    for (uint32_t l = 0; l < nleaves; l++) own[(size_t)l * rowlen + h] = mreach[(size_t)leaves[l].local * rowlen + h];
+  }
+}
+
+__device__ __forceinline__ uint32_t combo_of(uint32_t a, uint32_t b) {
+  const uint32_t hi = a > b ? a : b, lo = a > b ? b : a;
+  return hi * (hi - 1) / 2 + lo;
+}
+
+template <typename V>
+__global__ void __launch_bounds__(256) leaf_input_kernel(DPlan P, int p, Bias B, const LeafRec* __restrict__ leaves,
+                                                         uint32_t nleaves, const int8_t* __restrict__ deal, uint32_t ncards,
+                                                         const int8_t* __restrict__ board, uint32_t board_len,
+                                                         const V* __restrict__ own, const V* __restrict__ reach,
+                                                         uint32_t rowlen, __bf16* __restrict__ rows) {
+  __shared__ double red[256];
+  const uint32_t r = blockIdx.x, tid = threadIdx.x;
+  __bf16* row = rows + (size_t)r * 2752;
+  for (uint32_t i = tid; i < 2752; i += 256) row[i] = (__bf16)0.0f;
+  if (r >= nleaves * ncards) return;  // padding rows stay zero
+  __syncthreads();
+  const LeafRec L = leaves[r / ncards];
+  const int c = deal[r % ncards];
+  const SpotTab& S = P.spots[L.spot];
+  if (tid < board_len) row[board[tid]] = (__bf16)1.0f;
+  if (tid == 0) {
+    row[c] = (__bf16)1.0f;
+    const float tot = (float)L.pot + (float)L.stack;
+    row[52] = (__bf16)((float)L.pot / tot);
+    row[53] = (__bf16)((float)L.stack / tot);
+  }
+  for (int q = 0; q < 2; q++) {
+    const V* rq = q == p ? own + (size_t)(r / ncards) * rowlen : reach + (size_t)B.s(L.slot[p]) * rowlen;
+    const uint32_t nq = S.n[q];
+    double s = 0.0;
+    for (uint32_t h = tid; h < nq; h += 256)
+      if (S.cards[q][2 * h] != c && S.cards[q][2 * h + 1] != c) s += (double)rq[h];
+    red[tid] = s;
+    __syncthreads();
+    for (int k = 128; k > 0; k >>= 1) {
+      if ((int)tid < k) red[tid] += red[tid + k];
+      __syncthreads();
+    }
+    const double tot = red[0];
+    __syncthreads();
+    if (!(tot > 0.0)) continue;
+    for (uint32_t h = tid; h < nq; h += 256) {
+      const uint32_t a = S.cards[q][2 * h], b = S.cards[q][2 * h + 1];
+      if ((int)a == c || (int)b == c) continue;
+      row[54 + q * 1326 + combo_of(a, b)] = (__bf16)((double)rq[h] / tot);
+    }
+  }
+}
+
+template <typename V>
+__global__ void __launch_bounds__(256) leaf_value_kernel(DPlan P, int p, Bias B, const LeafRec* __restrict__ leaves,
+                                                         const int8_t* __restrict__ deal, uint32_t ncards,
+                                                         const V* __restrict__ reach, const uint32_t* __restrict__ alive,
+                                                         const float* __restrict__ ev, V* __restrict__ vals, uint32_t rowlen) {
+  __shared__ double tot[52];
+  __shared__ double red[256];
+  const uint32_t li = blockIdx.x, tid = threadIdx.x;
+  const LeafRec L = leaves[li];
+  const SpotTab& S = P.spots[L.spot];
+  const int o = p ^ 1;
+  const uint32_t so = B.s(L.slot[p]);
+  V* out = vals + (size_t)B.v(L.vslot) * rowlen;
+  if (!alive[so]) return;  // dead: the row is never read
+  const V* ro = reach + (size_t)so * rowlen;
+  const uint32_t nq = S.n[o], np = S.n[p];
+  double s = 0.0;
+  for (uint32_t h = tid; h < nq; h += 256) s += (double)ro[h];
+  red[tid] = s;
+  if (tid < 52) {
+    double t = 0.0;
+    for (uint32_t k = S.bl_off[o][tid]; k < S.bl_off[o][tid + 1]; k++) t += (double)ro[S.bl_idx[o][k]];
+    tot[tid] = t;
+  }
+  __syncthreads();
+  for (int k = 128; k > 0; k >>= 1) {
+    if ((int)tid < k) red[tid] += red[tid + k];
+    __syncthreads();
+  }
+  const double total = red[0];
+  const double scale = S.norm * (double)L.factor * (double)L.pot;
+  for (uint32_t h = tid; h < np; h += 256) {
+    const uint32_t c1 = S.cards[p][2 * h], c2 = S.cards[p][2 * h + 1];
+    const uint32_t sm = S.same[p][h];
+    const double same = sm == NONE ? 0.0 : (double)ro[sm];
+    const uint32_t combo = combo_of(c1, c2);
+    double v = 0.0;
+    for (uint32_t ci = 0; ci < ncards; ci++) {
+      const uint32_t c = (uint32_t)deal[ci];
+      if (c == c1 || c == c2) continue;
+      const uint32_t h1 = S.lookup[o][combo_of(c1, c)], h2 = S.lookup[o][combo_of(c2, c)];
+      const double pair = (h1 == NONE ? 0.0 : (double)ro[h1]) + (h2 == NONE ? 0.0 : (double)ro[h2]);
+      const double compat = total - tot[c1] - tot[c2] - tot[c] + same + pair;
+      v += (double)ev[((size_t)li * ncards + ci) * 2652 + (size_t)p * 1326 + combo] * compat;
+    }
+    out[h] = (V)(v * scale);
+  }
+}
+
+template <typename V>
+__global__ void leaf_zero_kernel(Bias B, const LeafRec* __restrict__ leaves, V* __restrict__ vals, uint32_t rowlen) {
+  V* out = vals + (size_t)B.v(leaves[blockIdx.x].vslot) * rowlen;
+  for (uint32_t h = threadIdx.x; h < rowlen; h += blockDim.x) out[h] = V(0);
+}
+
 // ================================================================== engine
 struct Engine::DPlanHolder {
   DPlan d;
@@ -1111,7 +1290,7 @@ Status Engine::create(int device, Plan&& plan, const float* regrets, const float
   size_t o_tb = put(blob, tb_room);
   const uint32_t K = P.num_spots;
   struct SpotOff {
-    size_t cards[2], same[2], bo[2], bi[2], eo[2], ec[2], cb[2], ei[2], co[2], ci[2], hc[2], w[2];
+    size_t cards[2], same[2], bo[2], bi[2], eo[2], ec[2], cb[2], ei[2], co[2], ci[2], hc[2], w[2], lk[2];
   };
   std::vector<SpotOff> so(K);
   for (uint32_t k = 0; k < K; k++) {
@@ -1129,6 +1308,7 @@ Status Engine::create(int device, Plan&& plan, const float* regrets, const float
       so[k].ci[q] = put(blob, T.cl_idx[q]);
       so[k].hc[q] = put(blob, T.hand_cnt[q]);
       so[k].w[q] = put(blob, T.weights[q]);
+      so[k].lk[q] = put(blob, T.lookup[q]);
     }
   }
   std::vector<uint32_t> ones(K, 1u);
@@ -1160,6 +1340,7 @@ Status Engine::create(int device, Plan&& plan, const float* regrets, const float
       t.cl_base[q] = AT(uint32_t, so[k].cb[q]); t.ent_idx[q] = AT(uint16_t, so[k].ei[q]);
       t.cl_off[q] = AT(uint16_t, so[k].co[q]); t.cl_idx[q] = AT(uint16_t, so[k].ci[q]);
       t.hand_cnt[q] = AT(uint4, so[k].hc[q]);
+      t.lookup[q] = AT(uint32_t, so[k].lk[q]);
     }
   }
   std::memcpy(blob.data() + o_spots, tabs.data(), K * sizeof(SpotTab));
@@ -1324,7 +1505,8 @@ template <typename V, int DMODE, int UMODE>
 static void launch_span(const DPlan& d, const Plan& P, hipStream_t st, int p, Span span, uint32_t first_depth,
                         const std::vector<ChunkGroup>& groups, void* reach, uint32_t* alive, float* regrets, float* sums,
                         const IterParams* prm, void* vals, uint32_t rowlen, size_t lds_down, size_t lds_term, size_t lds_up,
-                        V* qout, bool fast, uint32_t down_t, ProfRec* prof = nullptr) {
+                        V* qout, bool fast, uint32_t down_t, ProfRec* prof = nullptr,
+                        const std::function<void()>* leaf_hook = nullptr) {
   const uint32_t D = P.depths(), C = P.num_chunks;
   const uint32_t tiles = (P.n[p] + UP_T - 1) / UP_T;
   const Bias B0{P.slot_s0[p], 0, P.val_v0, 0};
@@ -1366,6 +1548,7 @@ static void launch_span(const DPlan& d, const Plan& P, hipStream_t st, int p, Sp
     }
   };
   if (first_depth == 0) down(P.depth_off[0], P.depth_off[1], B0);
+  if (leaf_hook && first_depth == 0) (*leaf_hook)();  // leaves read the top segment's reach
   // fast terminals launch over runs (TermBatch), exact ones over records
   auto tlo = [&](uint32_t c) { return fast ? P.tb_coff[c] : P.term_coff[c]; };
   if (C == 0) {  // every row resident (a river tree, or a batch of spots): all depths at once
@@ -1407,23 +1590,143 @@ Status Engine::launch_traversal(int p, Mode mode, bool f64, ProfRec* prof) {
   if (f64) hipLaunchKernelGGL((begin_kernel<double>), dim3(tiles, P.num_spots), dim3(256), 0, stream_, d, p, (double*)reach, alive, rowlen_);
   else hipLaunchKernelGGL((begin_kernel<float>), dim3(tiles, P.num_spots), dim3(256), 0, stream_, d, p, (float*)reach, alive, rowlen_);
   PROF_STOP();
+  std::function<void()> hook;
+  if (!P.leaves.empty()) hook = [&, p, mode, f64]() { launch_leaves(p, mode, f64, reach, vals, alive); };
+  const std::function<void()>* hk = P.leaves.empty() ? nullptr : &hook;
   if (mode == kCfr)
     launch_span<float, kCfr, kCfr>(d, P, stream_, p, span, 0, groups_[0], reach, alive, regrets_, sums_, params_, vals, rowlen_, lds_down_,
-                                   lds_term_[0], lds_up_[0], (float*)nullptr, precision_ == kFast, down_threads_, prof);
+                                   lds_term_[0], lds_up_[0], (float*)nullptr, precision_ == kFast, down_threads_, prof, hk);
   else if (mode == kBest)
     launch_span<double, kAverage, kBest>(d, P, stream_, p, span, 0, groups_[1], reach, alive, regrets_, sums_, params_, vals, rowlen_,
-                                         lds_down_, lds_term_[1], lds_up_[1], (double*)nullptr, precision_ == kFast, down_threads_);
+                                         lds_down_, lds_term_[1], lds_up_[1], (double*)nullptr, precision_ == kFast, down_threads_,
+                                         nullptr, hk);
   else if (f64)
     launch_span<double, kAverage, kAverage>(d, P, stream_, p, span, 0, groups_[1], reach, alive, regrets_, sums_, params_, vals, rowlen_,
-                                            lds_down_, lds_term_[1], lds_up_[1], (double*)nullptr, precision_ == kFast, down_threads_);
+                                            lds_down_, lds_term_[1], lds_up_[1], (double*)nullptr, precision_ == kFast, down_threads_,
+                                            nullptr, hk);
   else
     launch_span<float, kAverage, kAverage>(d, P, stream_, p, span, 0, groups_[0], reach, alive, regrets_, sums_, params_, vals, rowlen_,
-                                           lds_down_, lds_term_[0], lds_up_[0], (float*)nullptr, precision_ == kFast, down_threads_);
+                                           lds_down_, lds_term_[0], lds_up_[0], (float*)nullptr, precision_ == kFast, down_threads_,
+                                           nullptr, hk);
   // keep every spot's root row of p
   void* res = (char*)params_ + sizeof(IterParams);
   if (f64) hipLaunchKernelGGL((collect_kernel<double>), dim3(tiles, P.num_spots), dim3(256), 0, stream_, d, p, (const double*)vals, (double*)res, rowlen_);
   else hipLaunchKernelGGL((collect_kernel<float>), dim3(tiles, P.num_spots), dim3(256), 0, stream_, d, p, (const float*)vals, (float*)res, rowlen_);
   HIPCHK(hipGetLastError());
+  return kOk;
+}
+
+// The leaves of a depth-limited tree for traverser p (after the top
+// segment's down pass): zeros without a network.
+void Engine::launch_leaves(int p, Mode mode, bool f64, void* reach, void* vals, uint32_t* alive) {
+  const Plan& P = plan_;
+  const DPlan& d = dp_->d;
+  const uint32_t nl = (uint32_t)P.leaves.size();
+  const Bias B0{P.slot_s0[p], 0, P.val_v0, 0};
+  if (!net_) {
+    if (f64) hipLaunchKernelGGL((leaf_zero_kernel<double>), dim3(nl), dim3(256), 0, stream_, B0, leaves_, (double*)vals, rowlen_);
+    else hipLaunchKernelGGL((leaf_zero_kernel<float>), dim3(nl), dim3(256), 0, stream_, B0, leaves_, (float*)vals, rowlen_);
+    return;
+  }
+  const uint32_t seg = P.seg_of[P.leaves[0].node];
+  if (f64) {
+    if (mode == kCfr)
+      hipLaunchKernelGGL((leaf_own_kernel<double, false>), dim3(1), dim3(1024), 0, stream_, d, p, seg, (const float*)regrets_,
+                         (const float*)sums_, (double*)mreach_, leaves_, nl, (double*)own_, rowlen_);
+    else
+      hipLaunchKernelGGL((leaf_own_kernel<double, true>), dim3(1), dim3(1024), 0, stream_, d, p, seg, (const float*)regrets_,
+                         (const float*)sums_, (double*)mreach_, leaves_, nl, (double*)own_, rowlen_);
+    hipLaunchKernelGGL((leaf_input_kernel<double>), dim3(net_rows_), dim3(256), 0, stream_, d, p, B0, leaves_, nl, deal_, ncards_,
+                       board_, board_len_, (const double*)own_, (const double*)reach, rowlen_, net_in_);
+  } else {
+    if (mode == kCfr)
+      hipLaunchKernelGGL((leaf_own_kernel<float, false>), dim3(1), dim3(1024), 0, stream_, d, p, seg, (const float*)regrets_,
+                         (const float*)sums_, (float*)mreach_, leaves_, nl, (float*)own_, rowlen_);
+    else
+      hipLaunchKernelGGL((leaf_own_kernel<float, true>), dim3(1), dim3(1024), 0, stream_, d, p, seg, (const float*)regrets_,
+                         (const float*)sums_, (float*)mreach_, leaves_, nl, (float*)own_, rowlen_);
+    hipLaunchKernelGGL((leaf_input_kernel<float>), dim3(net_rows_), dim3(256), 0, stream_, d, p, B0, leaves_, nl, deal_, ncards_,
+                       board_, board_len_, (const float*)own_, (const float*)reach, rowlen_, net_in_);
+  }
+  (void)net_->forward_device(net_in_, net_rows_, net_out_, stream_);
+  if (f64)
+    hipLaunchKernelGGL((leaf_value_kernel<double>), dim3(nl), dim3(256), 0, stream_, d, p, B0, leaves_, deal_, ncards_,
+                       (const double*)reach, (const uint32_t*)alive, (const float*)net_out_, (double*)vals, rowlen_);
+  else
+    hipLaunchKernelGGL((leaf_value_kernel<float>), dim3(nl), dim3(256), 0, stream_, d, p, B0, leaves_, deal_, ncards_,
+                       (const float*)reach, (const uint32_t*)alive, (const float*)net_out_, (float*)vals, rowlen_);
+}
+
+Status Engine::attach_net(net::Net* n, const std::vector<int>& board) {
+  const Plan& P = plan_;
+  if (P.leaves.empty()) {
+    set_error("invalid input: the tree has no leaves for a value network");
+    return kInvalidInput;
+  }
+  if (P.num_spots != 1) {
+    set_error("invalid input: value-network leaves run on single-spot engines");
+    return kInvalidInput;
+  }
+  std::vector<int8_t> deal, bd;
+  uint64_t used = 0;
+  for (int c : board) {
+    used |= 1ull << c;
+    bd.push_back((int8_t)c);
+  }
+  for (int c = 0; c < 52; c++)
+    if (!(used >> c & 1)) deal.push_back((int8_t)c);
+  ncards_ = (uint32_t)deal.size();
+  board_len_ = (uint32_t)bd.size();
+  const uint32_t rows = (uint32_t)P.leaves.size() * ncards_;
+  net_rows_ = (rows + 63) / 64 * 64;
+  if (n && net_rows_ > n->capacity()) {
+    set_error("invalid input: the network's batch capacity is below the leaves' rows");
+    return kInvalidInput;
+  }
+  if (!leaves_) {
+    DeviceBuf b;
+    Status st;
+    const size_t nl = P.leaves.size();
+    if ((st = alloc(b, nl * sizeof(LeafRec) + 64 + 64 + (size_t)nl * rowlen_ * 8 + (size_t)P.max_members * rowlen_ * 8 +
+                           (size_t)net_rows_ * 2752 * 2 + (size_t)net_rows_ * 2652 * 4 + 1024)) != kOk)
+      return st;
+    char* q = (char*)b.p;
+    auto take = [&](size_t bytes) {
+      char* r = q;
+      q += (bytes + 255) & ~size_t(255);
+      return r;
+    };
+    leaves_ = (const LeafRec*)take(nl * sizeof(LeafRec));
+    deal_ = (int8_t*)take(64);
+    board_ = (int8_t*)take(64);
+    own_ = take((size_t)nl * rowlen_ * 8);
+    mreach_ = take((size_t)P.max_members * rowlen_ * 8);
+    net_in_ = (__bf16*)take((size_t)net_rows_ * 2752 * 2);
+    net_out_ = (float*)take((size_t)net_rows_ * 2652 * 4);
+    HIPCHK(hipMemcpy((void*)leaves_, P.leaves.data(), nl * sizeof(LeafRec), hipMemcpyHostToDevice));
+  }
+  HIPCHK(hipMemcpy(deal_, deal.data(), deal.size(), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(board_, bd.data(), bd.size(), hipMemcpyHostToDevice));
+  net_ = n;
+  if (use_graph_) {
+    for (int i = 0; i < 3; i++)
+      if (graph_[i]) {
+        (void)hipGraphExecDestroy(graph_[i]);
+        graph_[i] = nullptr;
+      }
+    return build_graphs();
+  }
+  return kOk;
+}
+
+Status Engine::write_storage(uint64_t offset, size_t len, const float* regrets, const float* sums) {
+  if (offset + len > plan_.storage_len) {
+    set_error("invalid input: storage range");
+    return kInvalidInput;
+  }
+  HIPCHK(hipStreamSynchronize(stream_));
+  if (regrets) HIPCHK(hipMemcpy(regrets_ + offset, regrets, len * sizeof(float), hipMemcpyHostToDevice));
+  if (sums) HIPCHK(hipMemcpy(sums_ + offset, sums, len * sizeof(float), hipMemcpyHostToDevice));
   return kOk;
 }
 

@@ -12,6 +12,9 @@
 #include "plan.hpp"
 
 namespace pts {
+namespace net {
+class Net;
+}
 
 enum Status : int {
   kOk = 0,
@@ -85,6 +88,11 @@ class Engine {
   // A storage range to the host (either pointer may be null).
   Status read_storage(uint64_t offset, size_t len, float* regrets, float* sums) const;
   Status zero_storage();
+  Status write_storage(uint64_t offset, size_t len, const float* regrets, const float* sums);
+  // Depth-limited trees: the value network that values the leaves (null:
+  // zeros), with the root board; rebuilds the graphs. The network must stay
+  // alive while attached.
+  Status attach_net(net::Net* n, const std::vector<int>& board);
   Status exploitability(bool raked, double* out);
   Status download(float* regrets, float* sums) const;
   Status upload(const float* regrets, const float* sums);
@@ -107,6 +115,7 @@ class Engine {
   Status alloc(DeviceBuf& b, size_t bytes);
 
   Status launch_traversal(int p, Mode mode, bool f64, struct ProfRec* prof = nullptr);
+  void launch_leaves(int p, Mode mode, bool f64, void* reach, void* vals, uint32_t* alive);
   Status build_graphs();
   Status run_graph(int which);
 
@@ -128,6 +137,14 @@ class Engine {
   TermBatch* tbatch_dev_ = nullptr;
   uint32_t down_threads_ = 512, run_length_ = 1;
   uint64_t budget_ = 0;
+  // depth-limited leaves
+  net::Net* net_ = nullptr;
+  const LeafRec* leaves_ = nullptr;
+  int8_t *deal_ = nullptr, *board_ = nullptr;
+  void *own_ = nullptr, *mreach_ = nullptr;
+  __bf16* net_in_ = nullptr;
+  float* net_out_ = nullptr;
+  uint32_t ncards_ = 0, board_len_ = 0, net_rows_ = 0;
   IterParams* params_ = nullptr;
   IterParams* params_host_ = nullptr;
   uint32_t rowlen_ = 0;

@@ -213,8 +213,19 @@ struct TreeWalker {
       n.offset = b; n.rake = rake(st.pot);
       return;
     }
-    counts.chance++;
     bool all_in = st.stack[0] == 0 || st.stack[1] == 0;
+    if (spec.depth_limit && !all_in && st.street == spec.root_street) {
+      counts.terminal++;
+      if (!tree) return;
+      size_t board_len = spec.board.size() + dealt.size();
+      float cards = (float)(52 - board_len), compatible = (float)(52 - board_len - 4);
+      Node& n = tree->nodes[id];
+      n.kind = kLeaf; n.street = st.street; n.invest[0] = inv[0]; n.invest[1] = inv[1];
+      n.offset = (uint64_t)st.pot;
+      n.chance_factor = 1.0f / (cards * compatible / cards);
+      return;
+    }
+    counts.chance++;
     Street next = (Street)(st.street + 1);
     uint64_t used = 0;
     for (int c : spec.board) used |= 1ull << c;

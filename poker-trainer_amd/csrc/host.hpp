@@ -48,7 +48,9 @@ std::vector<Action> menu_actions(const Betting& b, const Menu& m, Chips big_blin
 int betting_apply(const Betting& b, const Action& a, Betting& next, Chips& put);
 
 // ----------------------------------------------------------------- tree
-enum Kind : uint8_t { kP0 = 0, kP1 = 1, kChance = 2, kFold0 = 3, kFold1 = 4, kShowdown = 5 };
+// kLeaf: the end of the root street's betting in a depth-limited tree (the
+// next street's cards and play are valued by the value network)
+enum Kind : uint8_t { kP0 = 0, kP1 = 1, kChance = 2, kFold0 = 3, kFold1 = 4, kShowdown = 5, kLeaf = 6 };
 inline bool is_player(uint8_t k) { return k <= kP1; }
 inline bool is_terminal(uint8_t k) { return k >= kFold0; }
 
@@ -57,7 +59,7 @@ struct Node {
   uint16_t num_children;
   uint32_t first_child;
   int32_t invest[2];
-  uint64_t offset;  // player: storage offset; showdown: board index
+  uint64_t offset;  // player: storage offset; showdown: board index; leaf: pot (chips)
   float rake, chance_factor;
   int8_t card;      // chance child: the dealt card, else -1
   Action action;    // action child: the action taken
@@ -71,6 +73,9 @@ struct TreeSpec {
   Chips big_blind = 100;
   double rake_rate = 0, rake_cap = 0;
   size_t hands[2] = {0, 0};
+  // cut the tree where the root street's betting ends with chips behind:
+  // such a node becomes a leaf (its chance factor as the chance node's)
+  bool depth_limit = false;
 };
 
 struct TreeCounts { uint64_t player = 0, chance = 0, terminal = 0, cells = 0; };

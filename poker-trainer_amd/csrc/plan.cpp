@@ -43,6 +43,12 @@ std::string build_plan(const PlanInput& in, Plan& P) {
       if (P.same[q][i] != kNone && P.same[q][i] >= in.num_hands[q ^ 1]) return fail("bad same-combo index", i);
       if (!(P.weights[q][i] > 0.0f)) return fail("non-positive hand weight", i);
     }
+    P.lookup[q].assign(1326, kNone);
+    for (uint32_t i = 0; i < n; i++) {
+      const uint32_t a = P.cards[q][2 * i], b = P.cards[q][2 * i + 1];
+      const uint32_t hi = std::max(a, b), lo = std::min(a, b);
+      P.lookup[q][hi * (hi - 1) / 2 + lo] = i;
+    }
     P.bl_off[q].assign(53, 0);
     std::vector<std::vector<uint16_t>> lists(52);
     for (uint32_t i = 0; i < n; i++) {
@@ -70,7 +76,8 @@ std::string build_plan(const PlanInput& in, Plan& P) {
   if (!is_player(P.kind[0])) return fail("root is not a decision node", 0);
   for (size_t i = 0; i < N; i++) {
     uint8_t k = P.kind[i];
-    if (k > kShowdown) return fail("bad node kind", i);
+    if (k > kLeaf) return fail("bad node kind", i);
+    if (k == kLeaf && !(P.factor[i] > 0.0f)) return fail("bad leaf factor", i);
     uint32_t nc = in.num_children[i];
     if (is_terminal(k)) {
       if (nc != 0) return fail("terminal with children", i);
@@ -244,8 +251,26 @@ std::string build_plan(const PlanInput& in, Plan& P) {
       if (P.kind[id] == kChance && P.vslot[id] == kNone) chance_vslot[id] = (uint32_t)nv++;
     }
   P.terminals.clear();
+  P.leaves.clear();
   for (size_t i = 0; i < N; i++)
-    if (P.chunk_of[i] == kNone && is_terminal(P.kind[i])) {
+    if (P.kind[i] == kLeaf) {
+      if (P.chunk_of[i] != kNone) return fail("leaf below a chance node", i);
+      LeafRec l{};
+      l.node = (uint32_t)i;
+      l.vslot = (uint32_t)nv++;
+      l.slot[0] = P.rslot[i];
+      l.slot[1] = P.rslot[N + i];
+      l.pot = (uint32_t)P.offset[i];
+      const int64_t s0 = in.root_stack[0] - P.invest[2 * i], s1 = in.root_stack[1] - P.invest[2 * i + 1];
+      l.stack = (uint32_t)std::max<int64_t>(0, std::min(s0, s1));
+      l.factor = P.factor[i];
+      l.local = P.local[i];
+      l.spot = 0;
+      P.vslot[i] = l.vslot;
+      P.leaves.push_back(l);
+    }
+  for (size_t i = 0; i < N; i++)
+    if (P.chunk_of[i] == kNone && is_terminal(P.kind[i]) && P.kind[i] != kLeaf) {
       P.terminals.push_back((uint32_t)i);
       if (P.vslot[i] == kNone) P.vslot[i] = (uint32_t)nv++;  // a chance child keeps its segment-root row
     }

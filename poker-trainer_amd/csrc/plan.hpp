@@ -75,6 +75,15 @@ struct TermBatch {
 static_assert(sizeof(TermBatch) == 16, "TermBatch layout");
 constexpr uint32_t kBatchMax = 16;  // terminals per run
 
+// A leaf of a depth-limited tree: valued by the value network per dealt card.
+struct LeafRec {
+  uint32_t node, vslot, slot[2];
+  uint32_t pot, stack;  // chips in the pot, the smaller stack behind
+  float factor;         // the cut chance node's reach factor
+  uint16_t local, spot; // member index in its segment; spot
+};
+static_assert(sizeof(LeafRec) == 32, "LeafRec layout");
+
 struct PlanInput {
   size_t num_nodes = 0;
   const uint8_t* kind = nullptr;
@@ -95,6 +104,7 @@ struct PlanInput {
   double half_pot = 0, num_combinations = 0;
   uint64_t storage_len = 0;
   bool batch_river = true;
+  int64_t root_stack[2] = {0, 0};  // chips behind at the root (for leaves)
 };
 
 struct Plan {
@@ -144,6 +154,7 @@ struct Plan {
   // ([tb_coff[c], tb_coff[c+1]) for chunk c; tb_coff[0] = the top's count)
   std::vector<TermBatch> tbatch;
   std::vector<uint32_t> tb_coff;
+  std::vector<LeafRec> leaves;        // depth-limited trees: the root street's cut points
   // spots: one for a single tree; a batch (merge_plans) keeps every spot's
   // own plan in `parts` for its hands and strength tables
   uint32_t num_spots = 1;
@@ -154,6 +165,7 @@ struct Plan {
   std::vector<uint8_t> cards[2];
   std::vector<float> weights[2];
   std::vector<uint32_t> same[2];
+  std::vector<uint32_t> lookup[2];   // combo (hi * (hi - 1) / 2 + lo) -> hand, or kNone
   std::vector<uint32_t> bl_off[2];   // 53 per player: hands holding each card (ascending)
   std::vector<uint16_t> bl_idx[2];
   // strength tables, per board b and player q (the player whose hands are listed)

@@ -61,14 +61,17 @@ class NodeInfo:
 class Solver:
     """A built game whose storage lives in HBM; solves run in HIP kernels."""
 
-    def __init__(self, cfg: SolveConfig, device: int = 0, precision: int = 0, scratch_budget: int = 0):
+    def __init__(self, cfg: SolveConfig, device: int = 0, precision: int = 0, scratch_budget: int = 0, _handle=None):
         """precision: 0 fast (default, as the reference's GpuPrecision), 1 exact.
         scratch_budget: bytes of reach / value rows (0: the HBM left after the
         storage); the turn or river subtrees run in groups that fit."""
         self.cfg = cfg
         self._c = cfg.to_c()
         h = ctypes.c_void_p()
-        check(lib().pts_solver_new(ctypes.byref(self._c), device, precision, scratch_budget, ctypes.byref(h)))
+        if _handle is not None:
+            h = _handle
+        else:
+            check(lib().pts_solver_new(ctypes.byref(self._c), device, precision, scratch_budget, ctypes.byref(h)))
         self.h = h
         self.n = [lib().pts_solver_num_hands(self.h, p) for p in range(2)]
 
@@ -188,6 +191,24 @@ class Solver:
         o1 = np.zeros(1326, np.float32)
         check(lib().pts_solver_ranges_after(self.h, steps_c(path), len(path), o0, o1))
         return o0, o1
+
+
+def depth_limited(cfg: SolveConfig, net=None, device: int = 0, precision: int = 0) -> Solver:
+    """A Solver over the depth-limited tree of `cfg` (the root street's betting,
+    cut points valued by the value network `net`, a pts.net.ValueNet; None:
+    zeros)."""
+    c = cfg.to_c()
+    h = ctypes.c_void_p()
+    check(lib().pts_dl_new(ctypes.byref(c), device, precision, net.h if net is not None else None, ctypes.byref(h)))
+    s = Solver(cfg, _handle=h)
+    s._net = net  # keep the network alive while the solver uses it
+    return s
+
+
+def copy_root_street(dst: Solver, src: Solver, regret_scale: float = 1.0, sum_scale: float = 1.0) -> int:
+    n = ctypes.c_uint32()
+    check(lib().pts_solver_copy_root_street(dst.h, src.h, regret_scale, sum_scale, ctypes.byref(n)))
+    return n.value
 
 
 class TreeArrays:
