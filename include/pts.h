@@ -259,6 +259,13 @@ pts_status pts_batch_solve(pts_batch* b, uint32_t max_iterations, double target_
  * iteration) and restarts the solve from zero storage. */
 pts_status pts_batch_autotune(pts_batch* b, uint32_t iterations, uint32_t best[2], double* best_ms);
 pts_status pts_batch_set_tiles(pts_batch* b, uint32_t run_length, uint32_t down_threads);
+/* Every spot's per-hand root values (FP64 traversal under `mode`: best
+ * response or average) in combo order: out[(spot * 2 + player) * 1326 +
+ * combo], 0 for combos outside the range. Values are normalized as the
+ * solver's (chips x reach / compatible weight). */
+pts_status pts_batch_root_rows(pts_batch* b, int mode, double* out);
+/* The spot's total weight of compatible hand pairs (the normalizer). */
+double pts_batch_num_combinations(pts_batch* b, uint32_t spot);
 pts_status pts_batch_profile_iteration(pts_batch* b, double ms[4], uint32_t launches[4], double* wall_ms);
 /* Sizes of the merged plan; for a host-only batch (device -1, built and
  * checked without a GPU) also the scratch bytes of its DCFR and FP64 rows. */
@@ -296,7 +303,7 @@ pts_status pts_net_profile(pts_net* n, uint32_t rows, double* fwd_ms, double* tr
  * valued per dealt card by `net` (null: zero), on the GPU inside each
  * iteration. Returns a pts_solver; its exploitability is the depth-limited
  * game's. (New here: the reference has no value network.) */
-pts_status pts_dl_new(const pts_config_c* cfg, int device, int precision, pts_net* net, pts_solver** out);
+pts_status pts_dl_new(const pts_config_c* cfg, int device, int precision, pts_net* net, pts_solver** out);  /* device -1: host only */
 uint32_t pts_solver_num_leaves(pts_solver* s);
 /* Copies src's root-street decision nodes (a depth-limited solver) into the
  * same action paths of dst (the full tree), scaling regrets and strategy

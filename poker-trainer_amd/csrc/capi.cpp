@@ -851,6 +851,31 @@ pts_status pts_batch_autotune(pts_batch* b, uint32_t iterations, uint32_t best[2
   return PTS_OK;
 }
 
+pts_status pts_batch_root_rows(pts_batch* b, int mode, double* out) {
+  NEED_BATCH_ENGINE(b);
+  if (mode != PTS_MODE_BEST_RESPONSE && mode != PTS_MODE_AVERAGE) return bad("invalid input: mode");
+  const uint32_t K = (uint32_t)b->spots.size();
+  std::vector<uint32_t> all(K, 1u);
+  Status e;
+  if ((e = b->engine->set_active(all)) != kOk) return st(e);
+  std::vector<double> rows;
+  if ((e = b->engine->root_rows((Mode)mode, rows)) != kOk) return st(e);
+  if ((e = b->engine->set_active(b->active)) != kOk) return st(e);
+  const uint32_t rl = b->engine->rowlen();
+  for (uint32_t k = 0; k < K; k++)
+    for (int p = 0; p < 2; p++) {
+      double* o = out + ((size_t)k * 2 + p) * 1326;
+      std::fill(o, o + 1326, 0.0);
+      const Hands& h = b->spots[k]->hands;
+      for (size_t i = 0; i < h.n(p); i++) o[h.combos[p][i]] = rows[((size_t)k * 2 + p) * rl + i];
+    }
+  return PTS_OK;
+}
+
+double pts_batch_num_combinations(pts_batch* b, uint32_t spot) {
+  return spot < b->spots.size() ? b->spots[spot]->hands.num_combinations : 0.0;
+}
+
 pts_status pts_batch_set_tiles(pts_batch* b, uint32_t run_length, uint32_t down_threads) {
   NEED_BATCH_ENGINE(b);
   return st(b->engine->retune(run_length, down_threads));
@@ -931,6 +956,11 @@ pts_status pts_dl_new(const pts_config_c* cfg, int device, int precision, pts_ne
   std::string err = host_build(cfg, s.get(), plan, true);
   if (!err.empty()) return bad(err);
   if (plan.leaves.empty()) return bad("invalid input: the depth-limited tree has no leaves");
+  if (device < 0) {  // host only: the cut tree and its plan
+    s->host_plan = std::move(plan);
+    *out = s.release();
+    return PTS_OK;
+  }
   Status e = Engine::create(device, std::move(plan), nullptr, nullptr, precision, 0, s->engine);
   if (e != kOk) return st(e);
   if ((e = s->engine->attach_net(n ? n->n.get() : nullptr, s->spec.board)) != kOk) return st(e);

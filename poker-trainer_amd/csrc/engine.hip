@@ -1821,14 +1821,22 @@ Status Engine::root_values(Mode mode, double res[2]) {  // cfr.rs:482-490, in FP
   return kOk;
 }
 
-Status Engine::root_values_all(Mode mode, std::vector<double>& res) {
+Status Engine::root_rows(Mode mode, std::vector<double>& rows) {
   Status st = run_graph(mode == kBest ? 1 : 2);
   if (st != kOk) return st;
   const uint32_t K = plan_.num_spots;
-  std::vector<double> rows((size_t)K * 2 * rowlen_);
+  rows.assign((size_t)K * 2 * rowlen_, 0.0);
   HIPCHK(hipMemcpyAsync(rows.data(), (char*)params_ + result_offset(0, 0, rowlen_, 8), rows.size() * 8, hipMemcpyDeviceToHost,
                         stream_));
   HIPCHK(hipStreamSynchronize(stream_));
+  return kOk;
+}
+
+Status Engine::root_values_all(Mode mode, std::vector<double>& res) {
+  std::vector<double> rows;
+  Status st = root_rows(mode, rows);
+  if (st != kOk) return st;
+  const uint32_t K = plan_.num_spots;
   res.assign((size_t)K * 2, 0.0);
   for (uint32_t k = 0; k < K; k++) {
     const Plan& T = plan_.spot(k);

@@ -75,6 +75,9 @@ class Engine {
   Status traverse(int p, Mode mode, const IterParams& d, std::vector<double>& out);
   // Expected payoffs (chips) at the root under `mode` (kBest / kAverage), traversals in FP64.
   Status root_values(Mode mode, double out[2]);
+  // Every spot's per-hand root values (FP64 traversals): rows[(spot * 2 + p) * rowlen + hand].
+  Status root_rows(Mode mode, std::vector<double>& rows);
+  uint32_t rowlen() const { return rowlen_; }
   // Every spot's root values (a batch): out[spot * 2 + player].
   Status root_values_all(Mode mode, std::vector<double>& out);
   // Spots with active = 0 are skipped by every traversal (their roots are dead).

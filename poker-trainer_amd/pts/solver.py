@@ -365,6 +365,15 @@ class BatchSolver:
         check(lib().pts_batch_autotune(self.h, iterations, best, ctypes.byref(ms)))
         return (best[0], best[1]), ms.value
 
+    def root_rows(self, mode: int = 2):
+        """Per-hand root values of every spot, combo order: (spots, 2, 1326)."""
+        out = np.zeros(len(self.cfgs) * 2 * 1326, np.float64)
+        check(lib().pts_batch_root_rows(self.h, mode, out))
+        return out.reshape(len(self.cfgs), 2, 1326)
+
+    def num_combinations(self, k):
+        return lib().pts_batch_num_combinations(self.h, k)
+
     def set_tiles(self, run_length: int, down_threads: int):
         check(lib().pts_batch_set_tiles(self.h, run_length, down_threads))
 
