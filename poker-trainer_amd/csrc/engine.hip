@@ -77,6 +77,17 @@ struct DPlan {
   const uint32_t* chance_child_vslot;
   const SpotTab* spots;
   const uint32_t* active;  // per spot: 0 freezes the spot (its root is dead)
+  // per segment: {first member, member count, root node, hands of player
+  // 0 | hands of player 1 << 16}
+  const uint4* seg_head;
+  // per member: its alive flag for the current traverser (written by the
+  // down pass, read by the up pass)
+  uint8_t* malive;
+  // segments of more than REG_M members, and of REG_S + 1 to REG_M members
+  // (ascending): device copies, and the host copies launches are cut from
+  const uint32_t *big, *mid;
+  const uint32_t *big_host, *mid_host;
+  uint32_t num_big, num_mid;
 };
 
 // The nodes a traversal covers: {root} and [lo, hi) (the whole tree, or the
@@ -289,6 +300,8 @@ __global__ void __launch_bounds__(1024) down_kernel(DPlan P, int p, uint32_t seg
     }
     __syncthreads();
   }
+  // every member's alive flag, for the up pass
+  for (uint32_t m = tid; m < mcnt; m += blockDim.x) P.malive[moff + m] = m_alive[rec[m].src[p]];
 }
 
 // ------------------------------------------------------------------ terminals
@@ -891,6 +904,176 @@ __global__ void __launch_bounds__(UP_T) chance_kernel(DPlan P, int p, uint32_t c
 }
 
 // ------------------------------------------------------------------ up
+// f32 walks keep a segment's member values in VGPRs (up_reg_kernel<CAP>:
+// up to 48 members at four waves per SIMD, up to 64 at two); larger
+// segments, and the f64 walks, keep them in LDS (up_kernel).
+constexpr uint32_t REG_S = 48, REG_M = 64;
+constexpr uint64_t kNoRow = ~0ull;
+
+// CAP f32 values per lane in two vector registers (32 + CAP - 32 lanes of
+// the member index), read and written at a wave-uniform index (indexed
+// register moves: no LDS, no scratch). The vectors are separate local
+// variables of the kernel (each small enough for the compiler to keep in
+// registers under a variable index); indices are masked into range and the
+// selects take values, not addresses.
+template <int CAP>
+struct MemberVec {
+  static_assert(CAP == 48 || CAP == 64, "48 or 64 members");
+  static constexpr int W1 = CAP - 32;
+  typedef float lo __attribute__((ext_vector_type(32)));
+  typedef float hi __attribute__((ext_vector_type(W1)));
+  __device__ static __forceinline__ float get(const lo& r0, const hi& r1, uint32_t i) {
+    const float x0 = r0[i & 31], x1 = r1[i & (W1 - 1)];
+    return i < 32 ? x0 : x1;
+  }
+  __device__ static __forceinline__ void set(lo& r0, hi& r1, uint32_t i, float v) {
+    if (i < 32) r0[i & 31] = v;
+    else r1[i & (W1 - 1)] = v;
+  }
+};
+
+__device__ __forceinline__ uint32_t lane_u32(uint32_t x, uint32_t m) { return (uint32_t)__builtin_amdgcn_readlane((int)x, (int)m); }
+__device__ __forceinline__ uint64_t lane_u64(uint64_t x, uint32_t m) {
+  return (uint64_t)lane_u32((uint32_t)x, m) | (uint64_t)lane_u32((uint32_t)(x >> 32), m) << 32;
+}
+
+// Record flags of a member held by its lane
+constexpr uint32_t kFLive = 8, kFSpan = 16, kFRoot = 32;
+
+// One wave per (segment, 64-hand tile), one hand per lane, the segment's
+// members in registers: lane m holds member m's record, read by every lane
+// with readlane; every value row the walk needs (terminal and chance rows,
+// and in CFR / average modes the regret or strategy-sum row of each live own
+// node's action, kept at the index of the child it leads to) is loaded
+// before the walk, all loads in flight together; the walk then runs on
+// registers, in reverse breadth-first order, with the arithmetic of
+// up_kernel below. CAP = 48 runs over a range of segments and leaves those
+// above 48 members; CAP = 64 runs over the plan's list of segments of 49 to
+// 64 members (list[seg0 + blockIdx.x]).
+template <int MODE, int CAP>
+__global__ void __launch_bounds__(UP_T, CAP == 48 ? 4 : 2) up_reg_kernel(DPlan P, int p, uint32_t seg0, Span span, Bias B, float* __restrict__ vals,
+                                                      uint32_t rowlen, const uint32_t* __restrict__ alive,
+                                                      float* __restrict__ regrets, const float* __restrict__ sums,
+                                                      const IterParams* __restrict__ prm, float* __restrict__ qout,
+                                                      const uint32_t* __restrict__ list) {
+  typedef float V;
+  constexpr bool ROWS = MODE != kBest;
+  __shared__ uint64_t src_lds[CAP];
+  const uint32_t seg = list != nullptr ? list[seg0 + blockIdx.x] : seg0 + blockIdx.x;
+  const uint4 head = P.seg_head[seg];
+  const uint32_t moff = head.x, mcnt = head.y, root = head.z;
+  if (mcnt > (uint32_t)CAP) return;
+  if (!span.has(root) && seg != span.seg) return;
+  const uint32_t np = p ? head.w >> 16 : head.w & 0xFFFF, lane = threadIdx.x;
+  const uint32_t h = blockIdx.y * UP_T + lane;
+  const bool act = h < np;
+  const uint32_t hh = act ? h : 0;  // idle lanes load row element 0 and store nothing
+  uint32_t f = 0, vslot = 0;
+  uint64_t off = 0;
+  if (lane < mcnt) {
+    const MemRec& r = P.mrec[moff + lane];
+    const bool lm = P.malive[moff + lane] != 0;
+    f = (uint32_t)r.kind | (lm ? kFLive : 0u) | (span.has(r.node) ? kFSpan : 0u) | (r.node == span.root ? kFRoot : 0u) |
+        (uint32_t)r.na << 8 | (uint32_t)r.clocal << 16;
+    vslot = r.vslot;
+    off = r.offset;
+    if (ROWS) src_lds[lane] = kNoRow;
+  }
+  uint64_t src = kNoRow;
+  if (ROWS) {
+    __syncthreads();
+    if (lane < mcnt && (f & (kFLive | kFSpan)) == (kFLive | kFSpan) && (int)(f & 7) == p)
+      for (uint32_t a = 0; a < ((f >> 8) & 0xFF); a++) src_lds[(f >> 16) + a] = off + (uint64_t)a * np;
+    __syncthreads();
+    if (lane < mcnt) src = src_lds[lane];
+  }
+  typedef MemberVec<CAP> MV;
+  typename MV::lo l0 = typename MV::lo(0.0f), s0 = typename MV::lo(0.0f);  // member values; storage rows, by child
+  typename MV::hi l1 = typename MV::hi(0.0f), s1 = typename MV::hi(0.0f);
+#define LV_GET(i) MV::get(l0, l1, (i))
+#define LV_SET(i, x) MV::set(l0, l1, (i), (x))
+#define SL_GET(i) MV::get(s0, s1, (i))
+#define SL_SET(i, x) MV::set(s0, s1, (i), (x))
+  const float* store = MODE == kCfr ? regrets : sums;
+#pragma unroll
+  for (uint32_t m = 0; m < (uint32_t)CAP; m++) {  // unrolled: constant indices, all loads in flight
+    if (m < mcnt) {
+      const uint32_t fm = lane_u32(f, m);
+      if ((fm & 7) >= kChance && (fm & (kFLive | kFSpan)) == (kFLive | kFSpan))
+        LV_SET(m, vals[(size_t)B.v(lane_u32(vslot, m)) * rowlen + hh]);
+      if (ROWS) {
+        const uint64_t sm = lane_u64(src, m);
+        if (sm != kNoRow) SL_SET(m, store[sm + hh]);
+      }
+    }
+  }
+  float pos = 0.0f, neg = 0.0f;
+  if (MODE == kCfr) {
+    pos = prm->pos;
+    neg = prm->neg;
+  }
+  for (uint32_t m = mcnt; m-- > 0;) {
+    const uint32_t fm = lane_u32(f, m);
+    if (!(fm & kFSpan)) continue;
+    const uint32_t k = fm & 7, na = (fm >> 8) & 0xFF, cl = fm >> 16;
+    if (qout != nullptr && (fm & kFRoot)) {
+      // query: the values of the decision node's children (lib.rs:854-863)
+      for (uint32_t a = 0; a < na; a++) {
+        const V x = LV_GET(cl + a);
+        if (act) qout[(size_t)a * np + h] = x;
+      }
+      return;
+    }
+    if (k >= kChance) continue;  // prefetched
+    V v = V(0);
+    if (fm & kFLive) {
+      if ((int)k == p) {
+        // passes over the actions re-read the registers (few values live at
+        // once); the operations and their order are up_kernel's
+        if (MODE == kCfr) {
+          float d = pos0(SL_GET(cl));
+          for (uint32_t a = 1; a < na; a++) d += pos0(SL_GET(cl + a));
+          float outv = 0.0f;
+          for (uint32_t a = 0; a < na; a++) outv += pos0(SL_GET(cl + a)) * LV_GET(cl + a);
+          const float inv = recip(d);
+          outv *= inv;
+          if (inv == 0.0f) {
+            float sm = 0.0f;
+            for (uint32_t a = 0; a < na; a++) sm += LV_GET(cl + a);
+            outv = sm * (1.0f / (float)na);
+          }
+          if (act)
+            for (uint32_t a = 0; a < na; a++) {
+              const float x = SL_GET(cl + a);
+              const float disc = x > 0.0f ? pos : neg;
+              // the regret row of action a is the storage row of child cl + a
+              regrets[lane_u64(src, cl + a) + h] = x * disc + LV_GET(cl + a) - outv;
+            }
+          v = outv;
+        } else {
+          float d = SL_GET(cl);
+          for (uint32_t a = 1; a < na; a++) d += SL_GET(cl + a);
+          const float inv = recip(d), uni = 1.0f / (float)na;
+          float acc = 0.0f;
+          for (uint32_t a = 0; a < na; a++) acc += (inv == 0.0f ? uni : SL_GET(cl + a) * inv) * LV_GET(cl + a);
+          v = acc;
+        }
+      } else {
+        for (uint32_t a = 0; a < na; a++) v += LV_GET(cl + a);
+      }
+    }
+    LV_SET(m, v);
+  }
+  const V top = LV_GET(0);
+  if (act) vals[(size_t)B.v(lane_u32(vslot, 0)) * rowlen + h] = top;
+#undef LV_GET
+#undef LV_SET
+#undef SL_GET
+#undef SL_SET
+}
+
+// Values in LDS: the f64 walks, and f32 segments above REG_M members
+// (list[seg0 + blockIdx.x] when a list is given, else seg0 + blockIdx.x).
 // One wave per (segment, 64-hand tile), one hand per lane: values pulled up
 // the segment's members in reverse breadth-first order (no barriers after
 // staging: every lane owns its hand). Own nodes: cfr.rs:315-382 (regret
@@ -903,9 +1086,10 @@ template <typename V, int MODE>
 __global__ void __launch_bounds__(UP_T) up_kernel(DPlan P, int p, uint32_t seg0, Span span, Bias B, V* __restrict__ vals,
                                                   uint32_t rowlen, const uint32_t* __restrict__ alive,
                                                   float* __restrict__ regrets, const float* __restrict__ sums,
-                                                  const IterParams* __restrict__ prm, V* __restrict__ qout) {
+                                                  const IterParams* __restrict__ prm, V* __restrict__ qout,
+                                                  const uint32_t* __restrict__ list) {
   extern __shared__ uint4 smem_up[];
-  const uint32_t seg = seg0 + blockIdx.x;
+  const uint32_t seg = list != nullptr ? list[seg0 + blockIdx.x] : seg0 + blockIdx.x;
   const uint32_t moff = P.seg_mem_off[seg], mcnt = P.seg_mem_cnt[seg];
   const SpotTab& S = P.spots[P.mrec[moff].spot];
   const uint32_t np = S.n[p], tid = threadIdx.x;
@@ -1200,6 +1384,8 @@ __global__ void leaf_zero_kernel(Bias B, const LeafRec* __restrict__ leaves, V* 
 // ================================================================== engine
 struct Engine::DPlanHolder {
   DPlan d;
+  std::vector<uint32_t> big;  // segments above REG_M members, ascending
+  std::vector<uint32_t> mid;  // segments of REG_S + 1 to REG_M members, ascending
 };
 
 Engine::~Engine() {
@@ -1313,6 +1499,16 @@ Status Engine::create(int device, Plan&& plan, const float* regrets, const float
   }
   std::vector<uint32_t> ones(K, 1u);
   const size_t o_act = put(blob, ones);
+  std::vector<uint32_t> big, mid;
+  std::vector<uint4> head(P.seg_mem_cnt.size());
+  for (uint32_t sg = 0; sg < (uint32_t)P.seg_mem_cnt.size(); sg++) {
+    if (P.seg_mem_cnt[sg] > REG_M) big.push_back(sg);
+    else if (P.seg_mem_cnt[sg] > REG_S) mid.push_back(sg);
+    const MemRec& r0 = P.mrec[P.seg_mem_off[sg]];
+    const Plan& T = P.spot(r0.spot);
+    head[sg] = uint4{P.seg_mem_off[sg], P.seg_mem_cnt[sg], r0.node, T.n[0] | T.n[1] << 16};
+  }
+  const size_t o_big = put(blob, big), o_mid = put(blob, mid), o_head = put(blob, head);
   const size_t o_spots = (blob.size() + 255) & ~size_t(255);
   blob.resize(o_spots + K * sizeof(SpotTab));
   DeviceBuf b;
@@ -1361,11 +1557,22 @@ Status Engine::create(int device, Plan&& plan, const float* regrets, const float
   e->run_length_ = default_run_length(P);
   d.active = AT(uint32_t, o_act);
   e->active_ = (uint32_t*)(base + o_act);
+  e->dp_->big = std::move(big);
+  e->dp_->mid = std::move(mid);
+  d.big = AT(uint32_t, o_big);
+  d.mid = AT(uint32_t, o_mid);
+  d.seg_head = AT(uint4, o_head);
+  d.big_host = e->dp_->big.data();
+  d.num_big = (uint32_t)e->dp_->big.size();
+  d.mid_host = e->dp_->mid.data();
+  d.num_mid = (uint32_t)e->dp_->mid.size();
 #undef AT
   // storage, then the scratch rows: chunk groups sized to what is left
-  DeviceBuf br, bs, bre, bp;
+  DeviceBuf br, bs, bre, bp, bm;
   if ((st = e->alloc(br, P.storage_len * sizeof(float))) != kOk) return st;
   if ((st = e->alloc(bs, P.storage_len * sizeof(float))) != kOk) return st;
+  if ((st = e->alloc(bm, P.mrec.size() + 256)) != kOk) return st;
+  d.malive = (uint8_t*)bm.p;
   if ((st = e->alloc(bp, sizeof(IterParams) + (size_t)K * 2 * rowlen * 8)) != kOk) return st;
   HIPCHK(hipMemGetInfo(&fr, &tot));
   uint64_t budget = fr > margin ? fr - margin : 0;
@@ -1540,10 +1747,39 @@ static void launch_span(const DPlan& d, const Plan& P, hipStream_t st, int p, Sp
                          (const uint32_t*)alive);
       PROF_STOP();
     }
-    if (b > a) {
+    if (b <= a) return;
+    if constexpr (std::is_same<V, float>::value) {
+      // f32 walks: segments of up to REG_S members in registers, then those
+      // of up to REG_M among [a, b) in registers at lower occupancy, then the
+      // larger ones in LDS
+      PROF_START(kKUp);
+      hipLaunchKernelGGL((up_reg_kernel<UMODE, (int)REG_S>), dim3(b - a, tiles), dim3(UP_T), 0, st, d, p, a, span, B,
+                         (float*)vals, rowlen, (const uint32_t*)alive, regrets, (const float*)sums, prm, (float*)qout,
+                         (const uint32_t*)nullptr);
+      PROF_STOP();
+      auto cut = [&](const uint32_t* l, uint32_t n, uint32_t& i0, uint32_t& i1) {
+        i0 = (uint32_t)(std::lower_bound(l, l + n, a) - l);
+        i1 = (uint32_t)(std::lower_bound(l, l + n, b) - l);
+      };
+      uint32_t i0, i1;
+      cut(d.mid_host, d.num_mid, i0, i1);
+      if (i1 > i0) {
+        PROF_START(kKUp);
+        hipLaunchKernelGGL((up_reg_kernel<UMODE, (int)REG_M>), dim3(i1 - i0, tiles), dim3(UP_T), 0, st, d, p, i0, span, B,
+                           (float*)vals, rowlen, (const uint32_t*)alive, regrets, (const float*)sums, prm, (float*)qout, d.mid);
+        PROF_STOP();
+      }
+      cut(d.big_host, d.num_big, i0, i1);
+      if (i1 > i0) {
+        PROF_START(kKUp);
+        hipLaunchKernelGGL((up_kernel<V, UMODE>), dim3(i1 - i0, tiles), dim3(UP_T), lds_up, st, d, p, i0, span, B, (V*)vals,
+                           rowlen, (const uint32_t*)alive, regrets, (const float*)sums, prm, qout, d.big);
+        PROF_STOP();
+      }
+    } else {
       PROF_START(kKUp);
       hipLaunchKernelGGL((up_kernel<V, UMODE>), dim3(b - a, tiles), dim3(UP_T), lds_up, st, d, p, a, span, B, (V*)vals, rowlen,
-                         (const uint32_t*)alive, regrets, (const float*)sums, prm, qout);
+                         (const uint32_t*)alive, regrets, (const float*)sums, prm, qout, (const uint32_t*)nullptr);
       PROF_STOP();
     }
   };
