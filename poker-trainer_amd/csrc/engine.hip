@@ -604,7 +604,7 @@ static_assert(offsetof(TermRec, invest) == 20 && offsetof(TermRec, rake) == 28 &
               "TermRec word layout");
 
 template <typename V, int p>  // p: the traverser (a constant, so per-player tables are static fields)
-__global__ void __launch_bounds__(FT) terminal_fast_kernel(DPlan P, uint32_t b0, Span span, Bias B,
+__global__ void __launch_bounds__(FT, sizeof(V) == 4 ? 6 : 5) terminal_fast_kernel(DPlan P, uint32_t b0, Span span, Bias B,
                                                               const V* __restrict__ reach, const uint32_t* __restrict__ alive,
                                                               V* __restrict__ vals, uint32_t rowlen) {
   extern __shared__ double sh[];
@@ -680,6 +680,19 @@ __global__ void __launch_bounds__(FT) terminal_fast_kernel(DPlan P, uint32_t b0,
     }
   }
   for (;;) {
+    // The per-lane lists are re-declared live each terminal (empty asm), so
+    // their range tests are recomputed here (a compare each) instead of
+    // being hoisted out of the loop as lane masks that spill.
+#pragma unroll
+    for (int k = 0; k < CL; k++) asm volatile("" : "+v"(jc[k]));
+#pragma unroll
+    for (int k = 0; k < FE; k++) asm volatile("" : "+v"(ja[k]));
+#pragma unroll
+    for (int k = 0; k < FK; k++) asm volatile("" : "+v"(hcs[k]), "+v"(hc[k].x), "+v"(hc[k].y), "+v"(hc[k].z));
+    asm volatile("" : "+v"(cs), "+v"(ce));
+    uint32_t tl = tid;  // the thread index, re-declared likewise
+    asm volatile("" : "+v"(tl));
+    const uint32_t ll = tl & 63, ql = tl & (CG - 1);
     // fetch the next terminal while this one is summed (its slot is in
     // range whether or not it is alive)
     const bool more = t + 1 < tend;
@@ -693,14 +706,14 @@ __global__ void __launch_bounds__(FT) terminal_fast_kernel(DPlan P, uint32_t b0,
       const V* R = reach + (size_t)sn * rowlen;
 #pragma unroll
       for (int k = 0; k < FK; k++) {
-        const uint32_t j = tid + k * FT;
+        const uint32_t j = tl + k * FT;
         rn[k] = j < nq ? R[j] : V(0);
       }
     }
     if (live) {  // a dead slot's value row is never read
 #pragma unroll
       for (int k = 0; k < FK; k++) {
-        const uint32_t j = tid + k * FT;
+        const uint32_t j = tl + k * FT;
         if (j < nq) rv[j] = r[k];
       }
       __syncthreads();
@@ -712,7 +725,7 @@ __global__ void __launch_bounds__(FT) terminal_fast_kernel(DPlan P, uint32_t b0,
       V* out = vals + (size_t)B.v(tr.vslot) * rowlen;
 
       // per-card scans: 8 lanes per card, prefix and suffix
-      if (tid < 52 * CG) {
+      if (tl < 52 * CG) {
         double v[CL];
         double tsum = 0.0;
 #pragma unroll
@@ -725,14 +738,14 @@ __global__ void __launch_bounds__(FT) terminal_fast_kernel(DPlan P, uint32_t b0,
         for (int d = 1; d < CG; d <<= 1) {
           const double y = __shfl_up(inc, d, CG);
           const double z = __shfl_down(incs, d, CG);
-          if ((int)cq >= d) inc += y;
-          if ((int)cq + d < CG) incs += z;
+          if ((int)ql >= d) inc += y;
+          if ((int)ql + d < CG) incs += z;
         }
-        if (cq == 0) tot[card] = incs;  // the card's total
+        if (ql == 0) tot[card] = incs;  // the card's total
         if (showdown) {
           double ex = __shfl_up(inc, 1, CG), exs = __shfl_down(incs, 1, CG);
-          if (cq == 0) ex = 0.0;
-          if (cq == CG - 1) exs = 0.0;
+          if (ql == 0) ex = 0.0;
+          if (ql == CG - 1) exs = 0.0;
 #pragma unroll
           for (int k = 0; k < CL; k++) {
             ex += v[k];
@@ -758,10 +771,10 @@ __global__ void __launch_bounds__(FT) terminal_fast_kernel(DPlan P, uint32_t b0,
       for (int d = 1; d < 64; d <<= 1) {
         const double y = __shfl_up(inc, d, 64);
         const double z = __shfl_down(incs, d, 64);
-        if ((int)lane >= d) inc += y;
-        if ((int)lane + d < 64) incs += z;
+        if ((int)ll >= d) inc += y;
+        if ((int)ll + d < 64) incs += z;
       }
-      if (lane == 63) wtot[wave] = inc;
+      if (ll == 63) wtot[wave] = inc;
       __syncthreads();
       if (!showdown) {  // cfr.rs:224-235, eval.rs:39-64,170-206
         double total = 0.0;
@@ -771,7 +784,7 @@ __global__ void __launch_bounds__(FT) terminal_fast_kernel(DPlan P, uint32_t b0,
         const double pay = (f == p ? -(hp + (double)inv_p) : hp + (double)(f ? tr.inv1 : tr.inv0) - rake) * norm;
 #pragma unroll
         for (int k = 0; k < FK; k++) {
-          const uint32_t i = tid + k * FT;
+          const uint32_t i = tl + k * FT;
           if (i < np) {
             const int a = hcs[k] & 0xFF, c = (hcs[k] >> 8) & 0xFF;
             const uint32_t sm = hcs[k] >> 16;
@@ -791,18 +804,18 @@ __global__ void __launch_bounds__(FT) terminal_fast_kernel(DPlan P, uint32_t b0,
           if (w > (int)wave) offs_ += x;
         }
         double ex = __shfl_up(inc, 1, 64), exs = __shfl_down(incs, 1, 64);
-        if (lane == 0) ex = 0.0;
-        if (lane == 63) exs = 0.0;
+        if (ll == 0) ex = 0.0;
+        if (ll == 63) exs = 0.0;
         double bp = offp + ex, bs = offs_ + exs;
 #pragma unroll
         for (int k = 0; k < FE; k++) {
           bp += v[k];
-          if (tid * FE + k < nA) Pa[tid * FE + k] = bp;
+          if (tl * FE + k < nA) Pa[tl * FE + k] = bp;
         }
 #pragma unroll
         for (int k = FE - 1; k >= 0; k--) {
           bs += v[k];
-          if (tid * FE + k < nA) Sa[tid * FE + k] = bs;
+          if (tl * FE + k < nA) Sa[tl * FE + k] = bs;
         }
         __syncthreads();
         const double win = (hp + (double)inv_o - rake) * norm;
@@ -811,7 +824,7 @@ __global__ void __launch_bounds__(FT) terminal_fast_kernel(DPlan P, uint32_t b0,
         const uint32_t mo = nA;
 #pragma unroll
         for (int k = 0; k < FK; k++) {
-          const uint32_t i = tid + k * FT;
+          const uint32_t i = tl + k * FT;
           if (i >= np) continue;
           const uint32_t lt = hc[k].x & 0xFFFF, le = hc[k].x >> 16;
           V x = V(0);
