@@ -176,6 +176,7 @@ __global__ void __launch_bounds__(1024) down_kernel(DPlan P, int p, uint32_t seg
                                                       uint32_t rowlen) {
   extern __shared__ uint4 smem_down[];
   __shared__ uint32_t red;
+  __shared__ uint32_t red3[3];  // per-node ORs, used in turn (each zeroed two nodes ahead)
   const uint32_t seg = seg0 + blockIdx.x;
   const int o = p ^ 1;
   const uint32_t moff = P.seg_mem_off[seg], mcnt = P.seg_mem_cnt[seg];
@@ -186,7 +187,9 @@ __global__ void __launch_bounds__(1024) down_kernel(DPlan P, int p, uint32_t seg
   MemRec* rec = (MemRec*)smem_down;
   uint8_t* m_alive = (uint8_t*)(rec + P.max_members);
   stage_records(P.mrec + moff, mcnt, rec);
+  if (tid < 3) red3[tid] = 0;
   __syncthreads();
+  const uint32_t lane = tid & 63;
   const uint32_t sp_raw = P.seg_pslot[(size_t)p * P.num_segs + seg];
   if (seg == span.seg || sp_raw == NONE) {
     if (tid == 0) {
@@ -232,17 +235,18 @@ __global__ void __launch_bounds__(1024) down_kernel(DPlan P, int p, uint32_t seg
     g = prm->strat;
     weighted = prm->reach_weighted != 0;
   }
+  uint32_t ri = 0;
   for (uint32_t m = 0; m < mcnt; m++) {
     const MemRec& r = rec[m];
     if ((int)r.kind != o || !span.has(r.node)) continue;
     const uint32_t na = r.na, cs = B.s(r.cslot[p]), cl = r.clocal;
+    // Every wave keeps its own copy of the members' alive flags up to date
+    // (the same values at the same LDS bytes in every wave), so a flag a wave
+    // reads was written by that wave: no barrier is needed for them.
     if (!m_alive[r.src[p]]) {
       // A dead node's children are dead (cfr.rs:271-274, river.rs:103-107).
-      if (tid < na) {
-        alive[cs + tid] = 0;
-        m_alive[cl + tid] = 0;
-      }
-      __syncthreads();
+      if (lane < na) m_alive[cl + lane] = 0;
+      if (tid < na) alive[cs + tid] = 0;
       continue;
     }
     const V* rrow = reach + (size_t)B.s(r.slot[p]) * rowlen;
@@ -292,13 +296,19 @@ __global__ void __launch_bounds__(1024) down_kernel(DPlan P, int p, uint32_t seg
           if (v != V(0)) bits |= 1u << a;
         }
     }
-    bits = block_or(bits, &red);
-    if (tid < na) {
-      const uint32_t b = (bits >> tid) & 1u;
-      alive[cs + tid] = b;
-      m_alive[cl + tid] = (uint8_t)b;
-    }
+    // the children's alive bits: a ballot per action in each wave, one LDS
+    // OR per wave, one barrier
+    uint32_t wb = 0;
+#pragma unroll
+    for (int a = 0; a < MAXA; a++)
+      if (a < (int)na && __ballot((bits >> a) & 1u)) wb |= 1u << a;
+    if (lane == 0 && wb) atomicOr(&red3[ri], wb);
     __syncthreads();
+    const uint32_t all = red3[ri];
+    if (tid == 0) red3[ri == 0 ? 2 : ri - 1] = 0;  // the buffer two nodes ahead
+    ri = ri == 2 ? 0 : ri + 1;
+    if (lane < na) m_alive[cl + lane] = (uint8_t)((all >> lane) & 1u);
+    if (tid < na) alive[cs + tid] = (all >> tid) & 1u;
   }
   // every member's alive flag, for the up pass
   for (uint32_t m = tid; m < mcnt; m += blockDim.x) P.malive[moff + m] = m_alive[rec[m].src[p]];
