@@ -83,11 +83,12 @@ struct DPlan {
   // per member: its alive flag for the current traverser (written by the
   // down pass, read by the up pass)
   uint8_t* malive;
-  // segments of more than REG_M members, and of REG_S + 1 to REG_M members
-  // (ascending): device copies, and the host copies launches are cut from
-  const uint32_t *big, *mid;
-  const uint32_t *big_host, *mid_host;
-  uint32_t num_big, num_mid;
+  // segments of more than REG_M members, of REG_S + 1 to REG_M members and
+  // of REG_T + 1 to REG_S members (ascending): device copies, and the host
+  // copies launches are cut from
+  const uint32_t *big, *mid, *m48;
+  const uint32_t *big_host, *mid_host, *m48_host;
+  uint32_t num_big, num_mid, num_m48;
 };
 
 // The nodes a traversal covers: {root} and [lo, hi) (the whole tree, or the
@@ -743,13 +744,16 @@ __global__ void __launch_bounds__(FT, sizeof(V) == 4 ? 6 : 5) terminal_fast_kern
           v[k] = jc[k] != NONE ? (double)rv[jc[k]] : 0.0;
           tsum += v[k];
         }
+        // (a fold needs only the suffix: the card's total)
         double inc = tsum, incs = tsum;
 #pragma unroll
         for (int d = 1; d < CG; d <<= 1) {
-          const double y = __shfl_up(inc, d, CG);
           const double z = __shfl_down(incs, d, CG);
-          if ((int)ql >= d) inc += y;
           if ((int)ql + d < CG) incs += z;
+          if (showdown) {
+            const double y = __shfl_up(inc, d, CG);
+            if ((int)ql >= d) inc += y;
+          }
         }
         if (ql == 0) tot[card] = incs;  // the card's total
         if (showdown) {
@@ -776,13 +780,16 @@ __global__ void __launch_bounds__(FT, sizeof(V) == 4 ? 6 : 5) terminal_fast_kern
         v[k] = ja[k] != NONE ? (double)rv[ja[k]] : 0.0;
         tsum += v[k];
       }
+      // (a fold needs only the prefix: the wave's total at lane 63)
       double inc = tsum, incs = tsum;
 #pragma unroll
       for (int d = 1; d < 64; d <<= 1) {
         const double y = __shfl_up(inc, d, 64);
-        const double z = __shfl_down(incs, d, 64);
         if ((int)ll >= d) inc += y;
-        if ((int)ll + d < 64) incs += z;
+        if (showdown) {
+          const double z = __shfl_down(incs, d, 64);
+          if ((int)ll + d < 64) incs += z;
+        }
       }
       if (ll == 63) wtot[wave] = inc;
       __syncthreads();
@@ -928,9 +935,10 @@ __global__ void __launch_bounds__(UP_T) chance_kernel(DPlan P, int p, uint32_t c
 
 // ------------------------------------------------------------------ up
 // f32 walks keep a segment's member values in VGPRs (up_reg_kernel<CAP>:
-// up to 48 members at four waves per SIMD, up to 64 at two); larger
-// segments, and the f64 walks, keep them in LDS (up_kernel).
-constexpr uint32_t REG_S = 48, REG_M = 64;
+// up to 32 members in one vector register range, up to 48 at four waves per
+// SIMD, up to 64 at two); larger segments, and the f64 walks, keep them in
+// LDS (up_kernel).
+constexpr uint32_t REG_T = 32, REG_S = 48, REG_M = 64;
 constexpr uint64_t kNoRow = ~0ull;
 
 // CAP f32 values per lane in two vector registers (32 + CAP - 32 lanes of
@@ -941,16 +949,18 @@ constexpr uint64_t kNoRow = ~0ull;
 // selects take values, not addresses.
 template <int CAP>
 struct MemberVec {
-  static_assert(CAP == 48 || CAP == 64, "48 or 64 members");
-  static constexpr int W1 = CAP - 32;
+  static_assert(CAP == 32 || CAP == 48 || CAP == 64, "32, 48 or 64 members");
+  static constexpr int W1 = CAP == 32 ? 2 : CAP - 32;  // (CAP 32: an unused pair)
   typedef float lo __attribute__((ext_vector_type(32)));
   typedef float hi __attribute__((ext_vector_type(W1)));
   __device__ static __forceinline__ float get(const lo& r0, const hi& r1, uint32_t i) {
-    const float x0 = r0[i & 31], x1 = r1[i & (W1 - 1)];
+    const float x0 = r0[i & 31];
+    if (CAP == 32) return x0;
+    const float x1 = r1[i & (W1 - 1)];
     return i < 32 ? x0 : x1;
   }
   __device__ static __forceinline__ void set(lo& r0, hi& r1, uint32_t i, float v) {
-    if (i < 32) r0[i & 31] = v;
+    if (CAP == 32 || i < 32) r0[i & 31] = v;
     else r1[i & (W1 - 1)] = v;
   }
 };
@@ -970,11 +980,11 @@ constexpr uint32_t kFLive = 8, kFSpan = 16, kFRoot = 32;
 // node's action, kept at the index of the child it leads to) is loaded
 // before the walk, all loads in flight together; the walk then runs on
 // registers, in reverse breadth-first order, with the arithmetic of
-// up_kernel below. CAP = 48 runs over a range of segments and leaves those
-// above 48 members; CAP = 64 runs over the plan's list of segments of 49 to
-// 64 members (list[seg0 + blockIdx.x]).
+// up_kernel below. CAP = 32 (or 48) runs over a range of segments and leaves
+// those above CAP members; CAP = 48 and 64 also run over the plan's lists of
+// segments of 33 to 48 and 49 to 64 members (list[seg0 + blockIdx.x]).
 template <int MODE, int CAP>
-__global__ void __launch_bounds__(UP_T, CAP == 48 ? 4 : 2) up_reg_kernel(DPlan P, int p, uint32_t seg0, Span span, Bias B, float* __restrict__ vals,
+__global__ void __launch_bounds__(UP_T, CAP == 64 ? 2 : 4) up_reg_kernel(DPlan P, int p, uint32_t seg0, Span span, Bias B, float* __restrict__ vals,
                                                       uint32_t rowlen, const uint32_t* __restrict__ alive,
                                                       float* __restrict__ regrets, const float* __restrict__ sums,
                                                       const IterParams* __restrict__ prm, float* __restrict__ qout,
@@ -1409,6 +1419,7 @@ struct Engine::DPlanHolder {
   DPlan d;
   std::vector<uint32_t> big;  // segments above REG_M members, ascending
   std::vector<uint32_t> mid;  // segments of REG_S + 1 to REG_M members, ascending
+  std::vector<uint32_t> m48;  // segments of REG_T + 1 to REG_S members, ascending
 };
 
 Engine::~Engine() {
@@ -1522,16 +1533,17 @@ Status Engine::create(int device, Plan&& plan, const float* regrets, const float
   }
   std::vector<uint32_t> ones(K, 1u);
   const size_t o_act = put(blob, ones);
-  std::vector<uint32_t> big, mid;
+  std::vector<uint32_t> big, mid, m48;
   std::vector<uint4> head(P.seg_mem_cnt.size());
   for (uint32_t sg = 0; sg < (uint32_t)P.seg_mem_cnt.size(); sg++) {
     if (P.seg_mem_cnt[sg] > REG_M) big.push_back(sg);
     else if (P.seg_mem_cnt[sg] > REG_S) mid.push_back(sg);
+    else if (P.seg_mem_cnt[sg] > REG_T) m48.push_back(sg);
     const MemRec& r0 = P.mrec[P.seg_mem_off[sg]];
     const Plan& T = P.spot(r0.spot);
     head[sg] = uint4{P.seg_mem_off[sg], P.seg_mem_cnt[sg], r0.node, T.n[0] | T.n[1] << 16};
   }
-  const size_t o_big = put(blob, big), o_mid = put(blob, mid), o_head = put(blob, head);
+  const size_t o_big = put(blob, big), o_mid = put(blob, mid), o_m48 = put(blob, m48), o_head = put(blob, head);
   const size_t o_spots = (blob.size() + 255) & ~size_t(255);
   blob.resize(o_spots + K * sizeof(SpotTab));
   DeviceBuf b;
@@ -1582,13 +1594,17 @@ Status Engine::create(int device, Plan&& plan, const float* regrets, const float
   e->active_ = (uint32_t*)(base + o_act);
   e->dp_->big = std::move(big);
   e->dp_->mid = std::move(mid);
+  e->dp_->m48 = std::move(m48);
   d.big = AT(uint32_t, o_big);
   d.mid = AT(uint32_t, o_mid);
+  d.m48 = AT(uint32_t, o_m48);
   d.seg_head = AT(uint4, o_head);
   d.big_host = e->dp_->big.data();
   d.num_big = (uint32_t)e->dp_->big.size();
   d.mid_host = e->dp_->mid.data();
   d.num_mid = (uint32_t)e->dp_->mid.size();
+  d.m48_host = e->dp_->m48.data();
+  d.num_m48 = (uint32_t)e->dp_->m48.size();
 #undef AT
   // storage, then the scratch rows: chunk groups sized to what is left
   DeviceBuf br, bs, bre, bp, bm;
@@ -1772,19 +1788,32 @@ static void launch_span(const DPlan& d, const Plan& P, hipStream_t st, int p, Sp
     }
     if (b <= a) return;
     if constexpr (std::is_same<V, float>::value) {
-      // f32 walks: segments of up to REG_S members in registers, then those
-      // of up to REG_M among [a, b) in registers at lower occupancy, then the
-      // larger ones in LDS
-      PROF_START(kKUp);
-      hipLaunchKernelGGL((up_reg_kernel<UMODE, (int)REG_S>), dim3(b - a, tiles), dim3(UP_T), 0, st, d, p, a, span, B,
-                         (float*)vals, rowlen, (const uint32_t*)alive, regrets, (const float*)sums, prm, (float*)qout,
-                         (const uint32_t*)nullptr);
-      PROF_STOP();
+      // f32 walks: segments of up to REG_T (or, when more than a quarter of
+      // them are larger, REG_S) members in registers over [a, b), then those
+      // of up to REG_S and REG_M among [a, b), then the larger ones in LDS
       auto cut = [&](const uint32_t* l, uint32_t n, uint32_t& i0, uint32_t& i1) {
         i0 = (uint32_t)(std::lower_bound(l, l + n, a) - l);
         i1 = (uint32_t)(std::lower_bound(l, l + n, b) - l);
       };
       uint32_t i0, i1;
+      cut(d.m48_host, d.num_m48, i0, i1);
+      const bool wide = (uint64_t)(i1 - i0) * 4 > (uint64_t)(b - a);
+      PROF_START(kKUp);
+      if (wide)
+        hipLaunchKernelGGL((up_reg_kernel<UMODE, (int)REG_S>), dim3(b - a, tiles), dim3(UP_T), 0, st, d, p, a, span, B,
+                           (float*)vals, rowlen, (const uint32_t*)alive, regrets, (const float*)sums, prm, (float*)qout,
+                           (const uint32_t*)nullptr);
+      else
+        hipLaunchKernelGGL((up_reg_kernel<UMODE, (int)REG_T>), dim3(b - a, tiles), dim3(UP_T), 0, st, d, p, a, span, B,
+                           (float*)vals, rowlen, (const uint32_t*)alive, regrets, (const float*)sums, prm, (float*)qout,
+                           (const uint32_t*)nullptr);
+      PROF_STOP();
+      if (!wide && i1 > i0) {
+        PROF_START(kKUp);
+        hipLaunchKernelGGL((up_reg_kernel<UMODE, (int)REG_S>), dim3(i1 - i0, tiles), dim3(UP_T), 0, st, d, p, i0, span, B,
+                           (float*)vals, rowlen, (const uint32_t*)alive, regrets, (const float*)sums, prm, (float*)qout, d.m48);
+        PROF_STOP();
+      }
       cut(d.mid_host, d.num_mid, i0, i1);
       if (i1 > i0) {
         PROF_START(kKUp);
