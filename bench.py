@@ -45,6 +45,19 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+def measured_traffic(workload, precision, kernel):
+    """HBM bytes per launch of a kernel class as rocprofv3 counted them
+    (FETCH_SIZE and WRITE_SIZE, scaled by a plain copy's counts), recorded in
+    profiles/traffic.json with the commands that measured them; None when
+    that workload was not counted."""
+    try:
+        with open(os.path.join(ROOT, "profiles", "traffic.json")) as f:
+            e = json.load(f).get(f"{workload}/{precision}/{kernel}")
+    except (OSError, ValueError):
+        return None
+    return None if e is None else e.get("hbm_bytes_per_launch")
+
+
 def cpu_baseline(cfg, iters_to_target, checks, seconds):
     """The oracle (C++ restatement of the reference CPU solve, parallel over
     chance children like the reference's rayon split) on the host cores: a
@@ -84,6 +97,11 @@ def main():
     ap.add_argument("--target", type=float, default=0.5)
     ap.add_argument("--check-every", type=int, default=10)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--max-iterations", type=int, default=0,
+                    help="cap the solve at this many iterations (0: the workload's own count; a capped run that "
+                         "stops short of the target says so in target_reached)")
+    ap.add_argument("--net-only", action="store_true",
+                    help="valuenet: only time the network's forward and training kernels on a synthetic batch")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="only run a plain device copy of this many MiB (5 launches) and exit: the known-size "
@@ -116,6 +134,8 @@ def main():
     if args.workload == "valuenet":
         return bench_valuenet(args)
     desc, maker, max_iters = WORKLOADS[args.workload]
+    if args.max_iterations:
+        max_iters = min(max_iters, args.max_iterations)
     cfg = getattr(spots, maker)(args.seed)
 
     # steps: W warmup iterations, then K timed ones
@@ -150,7 +170,8 @@ def main():
     dom = max(("down", "terminal", "up"), key=lambda k: ms[k])
     achieved = bytes_cls[dom] / (ms[dom] * 1e-3) / 1e9 if ms[dom] > 0 else 0.0
     roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None, "kernel": dom,
+                "frac": round(achieved / HBM_PEAK_GBS, 5),
+                "traffic": measured_traffic(args.workload, args.precision, dom), "kernel": dom,
                 "launches_per_iteration": launches[dom], "avg_launch_ms": round(ms[dom] / max(launches[dom], 1), 5),
                 "algorithmic_bytes_per_launch": round(bytes_cls[dom] / max(launches[dom], 1)),
                 "iteration_ms_by_class": {k: round(v, 4) for k, v in ms.items()}}
@@ -251,6 +272,23 @@ def bench_valuenet(args):
     from pts import valuenet as V
     from pts.net import ValueNet
     from pts.solver import depth_limited
+    if args.net_only:
+        net = ValueNet(1024, seed=1)
+        net.profile(1024)
+        fwd_ms, train_ms = min(net.profile(1024) for _ in range(5))
+        flops = ValueNet.forward_flops_per_row() * 1024
+        tf = flops / (fwd_ms * 1e-3) / 1e12
+        line = {"metric": "value-network forward, BF16 MFMA with FP32 accumulation, 1024 rows", "value": round(tf, 2),
+                "unit": "TFLOP/s", "n_gpus": 1, "steps": 5, "warmup": 1, "ms_per_step": round(fwd_ms, 4),
+                "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "bf16", "data": "synthetic",
+                "config": {"workload": "network 2706-4x1024-2652, He-normal weights, one batch of 1024 rows",
+                           "train_step_ms": round(train_ms, 4),
+                           "train_step_tflops": round(3 * flops / (train_ms * 1e-3) / 1e12, 2)},
+                "roofline": {"bound": "mfma", "achieved": round(tf, 2), "peak": 2500.0, "unit": "TFLOP/s",
+                             "frac": round(tf / 2500.0, 5), "traffic": None, "kernel": "gemm_kernel (forward)"},
+                "cpu_baseline": None}
+        print(json.dumps(line), flush=True)
+        return
     cfg = V.eval_flop(args.seed, args.classes)
     pots = V.leaf_pots(cfg)
     X, Y, M, info = V.generate(args.subgames, args.classes, pots, batch=64, log=log, base=cfg.ranges)

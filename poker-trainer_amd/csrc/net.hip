@@ -10,9 +10,10 @@
 //
 // Every product is one hand-written GEMM kernel on gfx950 MFMA
 // (v_mfma_f32_16x16x32_bf16): 64x64 output tiles per 256-thread workgroup,
-// each wave a 32x32 quadrant (2x2 MFMA blocks), K in steps of 32 staged
-// through LDS. Widths are padded (input 2752, output 2688) so every dimension
-// is a multiple of 64 (32 for K); padded weights stay zero.
+// each wave a 32x32 quadrant (2x2 MFMA blocks), K in steps of 64 staged
+// through LDS (the next four steps' tiles in flight in registers). Widths are
+// padded (input 2752, output 2688) so every dimension is a multiple of 64;
+// padded weights stay zero.
 #include <hip/hip_runtime.h>
 
 #include <chrono>
@@ -32,8 +33,8 @@ typedef __bf16 bf16;
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
-constexpr int TM = 64, TN = 64, TK = 32;
-constexpr int LDS_K = 40;  // padded LDS row (bf16): 80 bytes, 16-byte aligned
+constexpr int TM = 64, TN = 64, TK = 64;
+constexpr int LDS_K = 72;  // padded LDS row (bf16): 144 bytes, 16-byte aligned
 
 enum Epi : int {
   kEpiF32 = 0,       // C (f32) = A B
@@ -44,8 +45,8 @@ enum Epi : int {
 
 // C[M x N] = A[M x K] B[K x N]. A is K-contiguous (A[m * lda + k]) when AK,
 // else M-contiguous (A[k * lda + m]); B is given K-contiguous as Bt[n * ldb +
-// k] when BK, else N-contiguous (B[k * ldb + n]). M, N are multiples of 64
-// and K of 32 (the host checks); grid (N / 64, M / 64).
+// k] when BK, else N-contiguous (B[k * ldb + n]). M, N and K are multiples of
+// 64 (the host checks); grid (N / 64, M / 64).
 template <bool AK, bool BK, int EPI>
 __global__ void __launch_bounds__(256) gemm_kernel(const bf16* __restrict__ A, int lda, const bf16* __restrict__ B, int ldb,
                                                    void* __restrict__ C, int ldc, int K, const float* __restrict__ bias,
@@ -61,52 +62,83 @@ __global__ void __launch_bounds__(256) gemm_kernel(const bf16* __restrict__ A, i
 #pragma unroll
     for (int j = 0; j < 2; j++) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
 
-  // one 16-byte global load per operand per thread per K step
-  auto load_a = [&](int k0) -> bf16x8 {
-    if (AK) return *(const bf16x8*)(A + (size_t)(m0 + (tid >> 2)) * lda + k0 + (tid & 3) * 8);
-    return *(const bf16x8*)(A + (size_t)(k0 + (tid >> 3)) * lda + m0 + (tid & 7) * 8);
+  // two 16-byte global loads per operand per thread per K step: chunk
+  // c = tid + 256 q of the 64 x 64 tile, 8 elements each
+  auto load_a = [&](int k0, int q) -> bf16x8 {
+    const int c = tid + 256 * q;
+    if (AK) return *(const bf16x8*)(A + (size_t)(m0 + (c >> 3)) * lda + k0 + (c & 7) * 8);
+    return *(const bf16x8*)(A + (size_t)(k0 + (c >> 3)) * lda + m0 + (c & 7) * 8);
   };
-  auto load_b = [&](int k0) -> bf16x8 {
-    if (BK) return *(const bf16x8*)(B + (size_t)(n0 + (tid >> 2)) * ldb + k0 + (tid & 3) * 8);
-    return *(const bf16x8*)(B + (size_t)(k0 + (tid >> 3)) * ldb + n0 + (tid & 7) * 8);
+  auto load_b = [&](int k0, int q) -> bf16x8 {
+    const int c = tid + 256 * q;
+    if (BK) return *(const bf16x8*)(B + (size_t)(n0 + (c >> 3)) * ldb + k0 + (c & 7) * 8);
+    return *(const bf16x8*)(B + (size_t)(k0 + (c >> 3)) * ldb + n0 + (c & 7) * 8);
   };
-  auto store_a = [&](bf16x8 v) {
+  auto store_a = [&](bf16x8 v, int q) {
+    const int c = tid + 256 * q;
     if (AK) {
-      *(bf16x8*)&As[tid >> 2][(tid & 3) * 8] = v;
+      *(bf16x8*)&As[c >> 3][(c & 7) * 8] = v;
     } else {
 #pragma unroll
-      for (int j = 0; j < 8; j++) As[(tid & 7) * 8 + j][tid >> 3] = v[j];
+      for (int j = 0; j < 8; j++) As[(c & 7) * 8 + j][c >> 3] = v[j];
     }
   };
-  auto store_b = [&](bf16x8 v) {
+  auto store_b = [&](bf16x8 v, int q) {
+    const int c = tid + 256 * q;
     if (BK) {
-      *(bf16x8*)&Bs[tid >> 2][(tid & 3) * 8] = v;
+      *(bf16x8*)&Bs[c >> 3][(c & 7) * 8] = v;
     } else {
 #pragma unroll
-      for (int j = 0; j < 8; j++) Bs[(tid & 7) * 8 + j][tid >> 3] = v[j];
+      for (int j = 0; j < 8; j++) Bs[(c & 7) * 8 + j][c >> 3] = v[j];
     }
   };
 
-  bf16x8 ra = load_a(0), rb = load_b(0);
-  for (int k0 = 0; k0 < K; k0 += TK) {
-    store_a(ra);
-    store_b(rb);
-    __syncthreads();
-    if (k0 + TK < K) {  // the next tile is in flight during the MFMAs
-      ra = load_a(k0 + TK);
-      rb = load_b(k0 + TK);
+  // KS K steps in flight: the tiles of step k + KS are loaded into the
+  // registers step k has just stored to LDS (the stage index is a constant
+  // in the unrolled inner loop, so the stages stay in registers)
+  constexpr int KS = 4;
+  bf16x8 pa[KS][2], pb[KS][2];
+#pragma unroll
+  for (int q = 0; q < KS; q++)
+    if (q * TK < K) {
+      pa[q][0] = load_a(q * TK, 0);
+      pa[q][1] = load_a(q * TK, 1);
+      pb[q][0] = load_b(q * TK, 0);
+      pb[q][1] = load_b(q * TK, 1);
     }
-    const int r = lane & 15, kg = (lane >> 4) * 8;
-    bf16x8 fa[2], fb[2];
+  const int r = lane & 15, kg = (lane >> 4) * 8;
+  for (int k0 = 0; k0 < K; k0 += KS * TK) {
 #pragma unroll
-    for (int i = 0; i < 2; i++) fa[i] = *(const bf16x8*)&As[wm + i * 16 + r][kg];
+    for (int q = 0; q < KS; q++) {
+      const int k = k0 + q * TK;
+      if (k < K) {
+        store_a(pa[q][0], 0);
+        store_a(pa[q][1], 1);
+        store_b(pb[q][0], 0);
+        store_b(pb[q][1], 1);
+        __syncthreads();
+        if (k + KS * TK < K) {
+          pa[q][0] = load_a(k + KS * TK, 0);
+          pa[q][1] = load_a(k + KS * TK, 1);
+          pb[q][0] = load_b(k + KS * TK, 0);
+          pb[q][1] = load_b(k + KS * TK, 1);
+        }
 #pragma unroll
-    for (int j = 0; j < 2; j++) fb[j] = *(const bf16x8*)&Bs[wn + j * 16 + r][kg];
+        for (int s = 0; s < TK; s += 32) {
+          bf16x8 fa[2], fb[2];
 #pragma unroll
-    for (int i = 0; i < 2; i++)
+          for (int i = 0; i < 2; i++) fa[i] = *(const bf16x8*)&As[wm + i * 16 + r][s + kg];
 #pragma unroll
-      for (int j = 0; j < 2; j++) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa[i], fb[j], acc[i][j], 0, 0, 0);
-    __syncthreads();
+          for (int j = 0; j < 2; j++) fb[j] = *(const bf16x8*)&Bs[wn + j * 16 + r][s + kg];
+#pragma unroll
+          for (int i = 0; i < 2; i++)
+#pragma unroll
+            for (int j = 0; j < 2; j++)
+              acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa[i], fb[j], acc[i][j], 0, 0, 0);
+        }
+        __syncthreads();
+      }
+    }
   }
   // D[4 * (lane / 16) + e][lane % 16] of every 16x16 block
   const int col = lane & 15, rowb = (lane >> 4) * 4;
