@@ -34,7 +34,7 @@ WORKLOADS = {
               "3 bets; ranges U(0,1) per combo with 40% zeroed; f32, alpha 1.5 beta 0 gamma 2", "river_spot", 1000),
     "flop": ("flop HU exact DCFR: seeded 3-card board, pot 6bb, stacks 97bb, bets 33%/75%/all-in, raises 2.5x/all-in, "
              "3 bets per street, 49 turn x 48 river cards; ranges from 169-class weights sigmoid(score + N(0,0.5)); f32 "
-             "regrets and sums (16.19 G cells), alpha 1.5 beta 0 gamma 2", "flop_spot", 300),
+             "regrets and sums (16.19 G cells), alpha 1.5 beta 0 gamma 2", "flop_spot", 1000),
 }
 BATCH_DESC = ("batched HU exact DCFR: 256 river + 64 turn spots, seeded boards, per-range nonzero combos U[80,1326], "
               "SPR log-uniform 0.5-10, 1-3 bet sizes from {25,33,50,75,100,150}% + all-in, raises 2.5x/all-in, 3 bets; "
