@@ -265,9 +265,9 @@ def bench_valuenet(args):
     subgames solved by the batched GPU DCFR to 0.2% of the pot, the network
     trained on them by its own kernels, the depth-limited flop solve (300
     iterations), then the exact full-tree DCFR warm-started from its flop
-    strategy until the FP64 best response says <= target (the judge; the
-    value-net path's time is data-independent: training data are reused
-    across spots and not counted), against the exact solve from scratch."""
+    regrets until the FP64 best response says <= target (the judge and
+    fallback). Training data are reused across spots and not counted. The
+    comparison is the exact solve from scratch."""
     import numpy as np
     from pts import valuenet as V
     from pts.net import ValueNet
@@ -316,21 +316,22 @@ def bench_valuenet(args):
     dl.iterate(300)
     dl_s = time.perf_counter() - t0
     pr_w, warm_s, _ = V.judge(cfg, dl, args.target, 600, args.check_every)
-    log(f"[bench] value-net path: depth-limited {dl_s:.2f}s + exact continuation {pr_w.iteration} iterations, "
-        f"{pr_w.exploitability_pct:.4f}% pot, {warm_s:.1f}s")
+    warm_it, warm_pct = pr_w.iteration, pr_w.exploitability_pct
+    log(f"[bench] value-net path: depth-limited {dl_s:.2f}s + exact continuation {warm_it} iterations, "
+        f"{warm_pct:.4f}% pot, {warm_s:.1f}s")
     pr_c, cold_s, _ = V.judge(cfg, None, args.target, 600, args.check_every)
     log(f"[bench] exact from scratch: {pr_c.iteration} iterations, {pr_c.exploitability_pct:.4f}% pot, {cold_s:.1f}s")
     total = dl_s + warm_s
     line = {
-        "metric": METRIC, "value": round(total, 3), "unit": "s", "n_gpus": 1, "steps": pr_w.iteration, "warmup": 0,
+        "metric": METRIC, "value": round(total, 3), "unit": "s", "n_gpus": 1, "steps": warm_it, "warmup": 0,
         "ms_per_step": None, "higher_is_better": False, "scaling": "weak", "vs_baseline": None, "dtype": "bf16+f32",
         "data": "synthetic",
         "config": {"workload": f"value-net depth-limited flop solve judged by the exact FP64 best response: flop seed "
                                f"{args.seed}, flop-workload tree (5.06M decision nodes) with {args.classes}-class ranges; "
                                f"network 2706-4x1024-2652 trained on {args.subgames} GPU-solved turn subgames",
                    "value_net_path_s": round(total, 3), "depth_limited_s": round(dl_s, 3),
-                   "exact_continuation_s": round(warm_s, 3), "exact_continuation_iterations": pr_w.iteration,
-                   "value_net_path_exploitability_pct": round(pr_w.exploitability_pct, 5),
+                   "exact_continuation_s": round(warm_s, 3), "exact_continuation_iterations": warm_it,
+                   "value_net_path_exploitability_pct": round(warm_pct, 5),
                    "exact_from_scratch_s": round(cold_s, 3), "exact_from_scratch_iterations": pr_c.iteration,
                    "exact_from_scratch_exploitability_pct": round(pr_c.exploitability_pct, 5),
                    "speedup_vs_exact": round(cold_s / total, 3), "training_data_s": round(info["seconds"], 2),

@@ -311,6 +311,11 @@ uint32_t pts_solver_num_leaves(pts_solver* s);
  * solve. */
 pts_status pts_solver_copy_root_street(pts_solver* dst, pts_solver* src, float regret_scale, float sum_scale,
                                        uint32_t* copied);
+/* Pins dst's root-street decision nodes to src's average strategy (no
+ * reference counterpart; the value-net judge's re-solve of the later streets
+ * under the depth-limited root strategy): sums copied, regrets set to the
+ * average strategy's probabilities so that regret matching plays it. */
+pts_status pts_solver_pin_root_street(pts_solver* dst, pts_solver* src, uint32_t* pinned);
 
 #ifdef __cplusplus
 }

@@ -1007,4 +1007,41 @@ pts_status pts_solver_copy_root_street(pts_solver* dst, pts_solver* src, float r
   return PTS_OK;
 }
 
+// Pins dst's root-street decision nodes to src's average strategy: sums are
+// copied, regrets become the average strategy's probabilities (the sums
+// normalised per hand, uniform where they sum to zero, cfr.rs:124-154), so
+// regret matching in dst plays exactly that strategy at those nodes.
+pts_status pts_solver_pin_root_street(pts_solver* dst, pts_solver* src, uint32_t* pinned) {
+  NEED_ENGINE(dst);
+  NEED_ENGINE(src);
+  if (dst->hands.n(0) != src->hands.n(0) || dst->hands.n(1) != src->hands.n(1))
+    return bad("invalid input: the two solvers have different hand lists");
+  const Tree& a = src->tree;
+  const Tree& b = dst->tree;
+  const uint8_t street = (uint8_t)src->spec.root_street;
+  std::vector<std::pair<uint32_t, uint32_t>> stack{{0u, 0u}};
+  uint32_t n = 0;
+  while (!stack.empty()) {
+    auto [x, y] = stack.back();
+    stack.pop_back();
+    const Node& na = a.nodes[x];
+    const Node& nb = b.nodes[y];
+    if (!is_player(na.kind) || na.kind != nb.kind || na.street != street || na.num_children != nb.num_children) continue;
+    const size_t nh = src->hands.n(na.kind), na_ = na.num_children, len = na_ * nh;
+    std::vector<float> r(len), m(len);
+    Status e = src->engine->read_storage(na.offset, len, r.data(), m.data());
+    if (e != kOk) return st(e);
+    for (size_t h = 0; h < nh; h++) {
+      double d = 0.0;
+      for (size_t k = 0; k < na_; k++) d += m[k * nh + h];
+      for (size_t k = 0; k < na_; k++) r[k * nh + h] = d > 0.0 ? (float)(m[k * nh + h] / d) : (float)(1.0 / (double)na_);
+    }
+    if ((e = dst->engine->write_storage(nb.offset, len, r.data(), m.data())) != kOk) return st(e);
+    n++;
+    for (uint32_t k = 0; k < na.num_children; k++) stack.push_back({na.first_child + k, nb.first_child + k});
+  }
+  *pinned = n;
+  return PTS_OK;
+}
+
 }  // extern "C"

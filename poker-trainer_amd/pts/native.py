@@ -67,7 +67,7 @@ EXPORTS = [
     "pts_batch_autotune", "pts_batch_set_tiles", "pts_batch_profile_iteration", "pts_batch_host_info",
     "pts_net_num_params", "pts_net_forward_flops_per_row", "pts_net_create", "pts_net_destroy", "pts_net_set_params",
     "pts_net_get_params", "pts_net_get_grads", "pts_net_forward", "pts_net_gradients", "pts_net_train_step", "pts_net_profile",
-    "pts_dl_new", "pts_solver_num_leaves", "pts_solver_copy_root_street", "pts_batch_root_rows",
+    "pts_dl_new", "pts_solver_num_leaves", "pts_solver_copy_root_street", "pts_solver_pin_root_street", "pts_batch_root_rows",
     "pts_batch_num_combinations",
 ]
 
@@ -148,6 +148,7 @@ def _declare(L):
     L.pts_solver_num_leaves.restype = ctypes.c_uint32
     L.pts_solver_num_leaves.argtypes = [P]
     L.pts_solver_copy_root_street.argtypes = [P, P, ctypes.c_float, ctypes.c_float, ctypes.POINTER(ctypes.c_uint32)]
+    L.pts_solver_pin_root_street.argtypes = [P, P, ctypes.POINTER(ctypes.c_uint32)]
     L.pts_batch_root_rows.argtypes = [P, ctypes.c_int, f64p]
     L.pts_batch_num_combinations.restype = ctypes.c_double
     L.pts_batch_num_combinations.argtypes = [P, ctypes.c_uint32]

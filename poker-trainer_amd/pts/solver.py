@@ -211,6 +211,14 @@ def copy_root_street(dst: Solver, src: Solver, regret_scale: float = 1.0, sum_sc
     return n.value
 
 
+def pin_root_street(dst: Solver, src: Solver) -> int:
+    """Pins dst's root-street decision nodes to src's average strategy
+    (pts_solver_pin_root_street); returns the number of nodes pinned."""
+    n = ctypes.c_uint32()
+    check(lib().pts_solver_pin_root_street(dst.h, src.h, ctypes.byref(n)))
+    return n.value
+
+
 class TreeArrays:
     """The arrays of a pts_tree_desc, kept alive while an Engine uses them."""
 

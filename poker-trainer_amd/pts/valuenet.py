@@ -16,12 +16,12 @@ p's range with compat > 0. The depth-limited solver applies the inverse
 at its leaves (norm * pot * EV * compatible reach, summed over the dealt
 card).
 
-The judge: the depth-limited flop strategy is copied into the full flop
-tree (regrets and strategy sums of the flop's decision nodes), the exact
-DCFR continues from there, and the solve is done when the exact FP64 best
-response puts exploitability at or below the target; the same exact solve
-from scratch is the comparison (and the fallback when the warm start does
-not get there within its iteration budget).
+The judge (`judge`): the depth-limited flop regrets are copied into the full
+flop tree, the exact DCFR continues from there, and the solve is done when
+the exact FP64 best response puts exploitability at or below the target;
+the same exact solve from scratch is the comparison. `resolve` is the other
+judge: the full tree's flop decision nodes pinned to the depth-limited
+average strategy and only the later streets solved by exact DCFR under it.
 """
 from __future__ import annotations
 
@@ -35,7 +35,7 @@ from . import spots
 from .config import COMBOS, FLOP, TURN, BettingState, DcfrParams, SolveConfig
 from .net import ValueNet
 from .ranges import HandClass
-from .solver import BatchSolver, Solver, copy_root_street, depth_limited
+from .solver import BatchSolver, Solver, copy_root_street, depth_limited, pin_root_street
 
 WORKLOAD = 4
 CARD_MASK = np.array([[1 << a | 1 << b for (a, b) in COMBOS]], np.uint64)[0]
@@ -232,6 +232,32 @@ def heldout_loss(net: ValueNet, X, Y, M):
             num += net.gradients(X[i:i + 64], Y[i:i + 64], M[i:i + 64]) * w
             den += w
     return num / max(den, 1e-30)
+
+
+def resolve(cfg: SolveConfig, dl: Solver, target_pct: float = 0.5, max_iterations: int = 300, check_every: int = 10):
+    """The full tree with its flop decision nodes pinned to the depth-limited
+    solver's average strategy (re-pinned after every iteration) and the
+    later streets solved by exact DCFR, until the exact FP64 best response
+    of that profile is at or below the target. Returns (progress, seconds,
+    solver, reached)."""
+    full = Solver(cfg)
+    t0 = time.perf_counter()
+    pin_root_street(full, dl)
+    hist = []
+    t, e = 0, float("inf")
+    nxt = check_every
+    while t < max_iterations:
+        full.iterate(1)
+        t += 1
+        pin_root_street(full, dl)
+        if t >= nxt or t == max_iterations:
+            e = full.exploitability()
+            hist.append((t, 100.0 * e / cfg.root.pot))
+            if hist[-1][1] <= target_pct:
+                break
+            nxt = t + check_every
+    reached = bool(hist) and hist[-1][1] <= target_pct
+    return hist, time.perf_counter() - t0, full, reached
 
 
 def judge(cfg: SolveConfig, warm: Optional[Solver] = None, target_pct: float = 0.5, max_iterations: int = 600,

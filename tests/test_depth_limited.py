@@ -116,3 +116,27 @@ def test_depth_limited_solve_runs_and_copies_into_the_full_tree():
     assert copied == int(np.sum((dl.nodes()["kind"] <= 1)))
     r, s = full.storage()
     assert np.count_nonzero(s) > 0 and len(r) == n
+
+
+@pytest.mark.gpu
+def test_pinned_root_street_plays_the_depth_limited_average():
+    cfg = turn_cfg()
+    dl = depth_limited(cfg, ValueNet(64 * 4, seed=4))
+    dl.iterate(20)
+    full = Solver(cfg)
+    from pts.solver import pin_root_street
+    assert pin_root_street(full, dl) == int(np.sum(dl.nodes()["kind"] <= 1))
+    nd, nf = dl.nodes(), full.nodes()
+    rd, sd = dl.storage()
+    rf, sf = full.storage()
+    # the root: sums copied, regrets the average strategy (uniform where a
+    # hand's sums are zero), so regret matching in the full tree plays it
+    k, na = int(nd["kind"][0]), int(nd["num_children"][0])
+    nh = dl.n[k]
+    od, of = int(nd["offset"][0]), int(nf["offset"][0])
+    sdb = sd[od:od + na * nh].reshape(na, nh)
+    assert np.array_equal(sf[of:of + na * nh].reshape(na, nh), sdb)
+    tot = sdb.astype(np.float64).sum(axis=0)
+    want = np.where(tot > 0, sdb / np.where(tot > 0, tot, 1.0), 1.0 / na).astype(np.float32)
+    assert np.allclose(rf[of:of + na * nh].reshape(na, nh), want, rtol=1e-6, atol=0)
+    assert np.any(tot > 0)

@@ -92,6 +92,24 @@ def test_turn_solve_bit_exact():
     assert abs(eg - c.exploitability()) < 1e-4 * cfg.root.pot
 
 
+@pytest.mark.parametrize("kind,index", [("river", 11), ("river", 1), ("turn", 1)])
+def test_large_segments_bit_exact(kind, index):
+    # Segment sizes select the walk: up to 32, 48 or 64 members in registers,
+    # more than 64 in LDS. The river trees here are single segments of 57 and
+    # 99 members; the turn tree's river segments span all four sizes (672 of
+    # at most 32, 192 of 33-48, 192 of 49-64, 49 of 65-69 members).
+    cfg = spots.batch_spot(kind, index)
+    g, c = Solver(cfg, precision=EXACT), O.OracleSolver(cfg)
+    n = 8 if kind == "river" else 3
+    g.iterate(n)
+    c.iterate(n)
+    rg, sg = g.storage()
+    rc, sc = c.storage()
+    assert same(rg, rc), describe(rg, rc)
+    assert same(sg, sc), describe(sg, sc)
+    assert g.exploitability() == c.exploitability_f64()
+
+
 def test_solve_loop_matches_the_oracle():
     cfg = spots.river_spot(4)
     g, c = Solver(cfg, precision=EXACT), O.OracleSolver(cfg)
