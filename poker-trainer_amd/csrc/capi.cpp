@@ -24,6 +24,7 @@ struct pts_solver {
   TreeSpec spec;
   Tree tree;
   Hands hands;
+  Iso iso;
   TreeCounts counts;
   std::unique_ptr<Engine> engine;
   Plan host_plan;  // kept by host-only solvers (device -1)
@@ -104,57 +105,6 @@ static Menu menu_of(const pts_menu_c& m) {
   return r;
 }
 
-static const int kPairs[6][2] = {{0, 1}, {0, 2}, {0, 3}, {1, 2}, {1, 3}, {2, 3}};
-
-// Whether suit isomorphism would merge any chance outcomes: some suit pair
-// leaves both ranges unchanged (crates/solver/src/iso.rs:245-273) and has
-// equal rank sets on the board of some chance node (iso.rs:277-300): the
-// root board, and for a flop root also every flop + turn board.
-static bool isomorphism_applies(const Hands& h, const int* board, int n) {
-  auto masks = [](const std::vector<int>& b, uint16_t m[4]) {
-    m[0] = m[1] = m[2] = m[3] = 0;
-    for (int c : b) m[c & 3] |= (uint16_t)(1u << (c >> 2));
-  };
-  std::vector<std::vector<int>> boards;
-  if (n == 5) return false;  // a river root has no chance nodes
-  std::vector<int> root(board, board + n);
-  boards.push_back(root);
-  if (n == 3) {
-    uint64_t used = 0;
-    for (int c : root) used |= 1ull << c;
-    for (int t = 0; t < 52; t++)
-      if (!(used >> t & 1)) {
-        boards.push_back(root);
-        boards.back().push_back(t);
-      }
-  }
-  for (auto& pr : kPairs) {
-    bool ok = true;
-    for (int p = 0; p < 2 && ok; p++) {
-      float mx = 0.0f;
-      for (float w : h.weights[p]) mx = std::max(mx, w);
-      mx = std::max(mx, 1e-30f);
-      for (size_t i = 0; i < h.n(p) && ok; i++) {
-        int c[2];
-        for (int k = 0; k < 2; k++) {
-          int x = h.cards[p][2 * i + k], s = x & 3;
-          int s2 = s == pr[0] ? pr[1] : s == pr[1] ? pr[0] : s;
-          c[k] = (x >> 2) * 4 + s2;
-        }
-        uint32_t j = h.lookup[p][combo_of(c[0], c[1])];
-        if (j == kNone || std::fabs(h.weights[p][i] - h.weights[p][j]) > 1e-4f * mx) ok = false;
-      }
-    }
-    if (!ok) continue;
-    for (auto& b : boards) {
-      uint16_t m[4];
-      masks(b, m);
-      if (m[pr[0]] == m[pr[1]]) return true;
-    }
-  }
-  return false;
-}
-
 // lib.rs:307-329 + the parts of Solver::new the GPU build supports
 static std::string validate(const pts_config_c& c, TreeSpec& spec, Hands& hands) {
   int n = c.board_len;
@@ -174,9 +124,6 @@ static std::string validate(const pts_config_c& c, TreeSpec& spec, Hands& hands)
   for (int p = 0; p < 2; p++)
     if (hands.n(p) == 0) return "player " + std::to_string(p) + " has an empty range";
   if (hands.num_combinations <= 0.0) return "player 0 has an empty range";
-  if (c.isomorphism && isomorphism_applies(hands, c.board, n))
-    return "invalid input: suit isomorphism would merge chance outcomes of this spot, which the GPU plan does not "
-           "support (set isomorphism = false)";
   spec.root_street = (Street)street;
   spec.board.assign(c.board, c.board + n);
   Betting& b = spec.root;
@@ -341,6 +288,8 @@ pts_status pts_solver_estimate(const pts_config_c* c, pts_stats_c* out) {  // li
   Hands hands;
   std::string err = validate(*c, spec, hands);
   if (!err.empty()) return bad(err);
+  const Iso iso = make_iso(hands, c->isomorphism != 0);
+  spec.iso = &iso;
   TreeCounts t = count_tree(spec);
   *out = {t.player, t.chance, t.terminal, t.cells, memory_bytes(t)};
   return PTS_OK;
@@ -349,15 +298,17 @@ pts_status pts_solver_estimate(const pts_config_c* c, pts_stats_c* out) {  // li
 static PlanInput plan_input(pts_solver* s, std::vector<uint8_t>& kind, std::vector<uint8_t>& street, std::vector<uint16_t>& nc,
                             std::vector<uint32_t>& fc, std::vector<int32_t>& inv, std::vector<uint64_t>& off,
                             std::vector<float>& rake, std::vector<float>& fac, std::vector<int8_t>& card,
-                            std::vector<int32_t>& boards, std::vector<uint32_t>* same) {
+                            std::vector<int32_t>& boards, std::vector<uint32_t>* same, std::vector<uint32_t>& isf,
+                            std::vector<uint16_t>& isc, std::vector<uint16_t>* perm) {
   size_t N = s->tree.nodes.size();
   kind.resize(N); street.resize(N); nc.resize(N); fc.resize(N); inv.resize(2 * N); off.resize(N);
-  rake.resize(N); fac.resize(N); card.resize(N);
+  rake.resize(N); fac.resize(N); card.resize(N); isf.resize(N); isc.resize(N);
   for (size_t i = 0; i < N; i++) {
     const Node& n = s->tree.nodes[i];
     kind[i] = n.kind; street[i] = n.street; nc[i] = n.num_children; fc[i] = n.first_child;
     inv[2 * i] = n.invest[0]; inv[2 * i + 1] = n.invest[1]; off[i] = n.offset;
     rake[i] = n.rake; fac[i] = n.chance_factor; card[i] = n.card;
+    isf[i] = n.iso_first; isc[i] = n.iso_cnt;
   }
   for (auto& b : s->tree.boards) boards.insert(boards.end(), b.begin(), b.end());
   PlanInput in;
@@ -379,6 +330,19 @@ static PlanInput plan_input(pts_solver* s, std::vector<uint8_t>& kind, std::vect
   in.batch_river = true;
   in.root_stack[0] = s->cfg.stack[0];
   in.root_stack[1] = s->cfg.stack[1];
+  if (!s->tree.iso_refs.empty()) {
+    // every pair's permutation, identity where the pair is not symmetric
+    for (int q = 0; q < 2; q++) {
+      const size_t n = s->hands.n(q);
+      perm[q].resize(6 * n);
+      for (int pi = 0; pi < 6; pi++)
+        for (size_t h = 0; h < n; h++) perm[q][pi * n + h] = s->iso.symmetric[pi] ? s->iso.perm[q][pi][h] : (uint16_t)h;
+      in.iso_perm[q] = perm[q].data();
+    }
+    in.iso_first = isf.data();
+    in.iso_cnt = isc.data();
+    in.iso_refs = s->tree.iso_refs.data();
+  }
   return in;
 }
 
@@ -391,6 +355,8 @@ static std::string host_build(const pts_config_c* c, pts_solver* s, Plan& plan, 
   std::string err = validate(*c, s->spec, s->hands);
   if (!err.empty()) return err;
   s->spec.depth_limit = depth_limit;
+  s->iso = make_iso(s->hands, c->isomorphism != 0);  // iso.rs:245-273
+  s->spec.iso = &s->iso;
   s->counts = count_tree(s->spec);
   uint64_t need = memory_bytes(s->counts);
   if (need > c->max_memory_bytes) {
@@ -407,8 +373,9 @@ static std::string host_build(const pts_config_c* c, pts_solver* s, Plan& plan, 
   std::vector<uint64_t> off;
   std::vector<float> rake, fac;
   std::vector<int8_t> card;
-  std::vector<uint32_t> same[2];
-  PlanInput in = plan_input(s, kind, street, nc, fc, inv, off, rake, fac, card, boards, same);
+  std::vector<uint32_t> same[2], isf;
+  std::vector<uint16_t> isc, perm[2];
+  PlanInput in = plan_input(s, kind, street, nc, fc, inv, off, rake, fac, card, boards, same, isf, isc, perm);
   return build_plan(in, plan);
 }
 
@@ -558,7 +525,8 @@ void pts_solver_nodes(pts_solver* s, uint8_t* kind, uint8_t* street, uint16_t* n
   }
 }
 
-// lib.rs:746-825 (no isomorphism): reach of both players along `path`.
+// lib.rs:746-825: reach of both players along `path` (a card merged by suit
+// isomorphism is not followed: its canonical card's path is the query).
 static pts_status walk(pts_solver* s, const pts_step_c* path, int32_t np, uint32_t& id, std::vector<float> reach[2]) {
   NEED_ENGINE(s);
   id = 0;
@@ -586,7 +554,13 @@ static pts_status walk(pts_solver* s, const pts_step_c* path, int32_t np, uint32
       uint32_t ch = kNone;
       for (uint32_t c = node.first_child; c < node.first_child + node.num_children; c++)
         if (s->tree.nodes[c].card == stp.card) { ch = c; break; }
-      if (ch == kNone) return bad("bad path: card not in the tree at depth " + std::to_string(d));
+      if (ch == kNone) {
+        for (uint32_t k = 0; k < node.iso_cnt; k++)
+          if (s->tree.iso_refs[node.iso_first + k].card == stp.card)
+            return bad("invalid input: the card at depth " + std::to_string(d) +
+                       " is merged by suit isomorphism; query the isomorphic canonical card's path");
+        return bad("bad path: card not in the tree at depth " + std::to_string(d));
+      }
       id = ch;
       for (int x = 0; x < 2; x++)
         for (size_t h = 0; h < s->hands.n(x); h++)

@@ -61,6 +61,7 @@ struct SpotTab {
   const uint16_t *ent_idx[2], *cl_off[2], *cl_idx[2];
   const uint4* hand_cnt[2];
   const uint32_t* lookup[2];  // combo -> hand, or NONE
+  const uint16_t* perm[2];    // 6 suit-pair hand permutations of n each (null without isomorphism)
 };
 
 // Device view of the plan (all pointers into one HBM blob).
@@ -75,6 +76,8 @@ struct DPlan {
   const TermBatch* tbatch;
   const MemRec* crec;
   const uint32_t* chance_child_vslot;
+  const uint32_t* ciso_off;  // per chance record: its cards merged by isomorphism, ciso[ciso_off[i], ciso_off[i + 1])
+  const IsoRef* ciso;
   const SpotTab* spots;
   const uint32_t* active;  // per spot: 0 freezes the spot (its root is dead)
   // per segment: {first member, member count, root node, hands of player
@@ -930,6 +933,15 @@ __global__ void __launch_bounds__(UP_T) chance_kernel(DPlan P, int p, uint32_t c
     const V x = row[(size_t)a * rowlen];
     acc += (c == c1 || c == c2) ? 0.0 : (double)x;
   }
+  // cards merged by suit isomorphism, after the canonical ones: the
+  // canonical child's row at the permuted hand (cfr.rs:457-478)
+  const V* base = row - h;
+  for (uint32_t i = P.ciso_off[ci]; i < P.ciso_off[ci + 1]; i++) {
+    const IsoRef e = P.ciso[i];
+    const uint32_t hp = S.perm[p][(size_t)e.pair * np + h];
+    const V x = base[(size_t)e.child * rowlen + hp];
+    acc += ((int)e.card == c1 || (int)e.card == c2) ? 0.0 : (double)x;
+  }
   *out = (V)acc;
 }
 
@@ -1504,13 +1516,14 @@ Status Engine::create(int device, Plan&& plan, const float* regrets, const float
   size_t o_card = put(blob, P.card), o_smo = put(blob, P.seg_mem_off), o_smc = put(blob, P.seg_mem_cnt);
   size_t o_sps = put(blob, P.seg_pslot), o_sf = put(blob, P.seg_factor), o_mr = put(blob, P.mrec), o_tr = put(blob, P.trec);
   size_t o_cr = put(blob, P.crec), o_ccv = put(blob, P.chance_child_vslot);
+  size_t o_cio = put(blob, P.ciso_off), o_cis = put(blob, P.ciso);
   // room for the most runs (runs of one terminal), so the run length can be retuned
   std::vector<TermBatch> tb_room(std::max<size_t>(P.trec.size(), P.tbatch.size()) + 1);
   std::copy(P.tbatch.begin(), P.tbatch.end(), tb_room.begin());
   size_t o_tb = put(blob, tb_room);
   const uint32_t K = P.num_spots;
   struct SpotOff {
-    size_t cards[2], same[2], bo[2], bi[2], eo[2], ec[2], cb[2], ei[2], co[2], ci[2], hc[2], w[2], lk[2];
+    size_t cards[2], same[2], bo[2], bi[2], eo[2], ec[2], cb[2], ei[2], co[2], ci[2], hc[2], w[2], lk[2], pm[2];
   };
   std::vector<SpotOff> so(K);
   for (uint32_t k = 0; k < K; k++) {
@@ -1529,6 +1542,7 @@ Status Engine::create(int device, Plan&& plan, const float* regrets, const float
       so[k].hc[q] = put(blob, T.hand_cnt[q]);
       so[k].w[q] = put(blob, T.weights[q]);
       so[k].lk[q] = put(blob, T.lookup[q]);
+      so[k].pm[q] = put(blob, T.iso_perm[q]);
     }
   }
   std::vector<uint32_t> ones(K, 1u);
@@ -1572,6 +1586,7 @@ Status Engine::create(int device, Plan&& plan, const float* regrets, const float
       t.cl_off[q] = AT(uint16_t, so[k].co[q]); t.cl_idx[q] = AT(uint16_t, so[k].ci[q]);
       t.hand_cnt[q] = AT(uint4, so[k].hc[q]);
       t.lookup[q] = AT(uint32_t, so[k].lk[q]);
+      t.perm[q] = T.iso_perm[q].empty() ? nullptr : AT(uint16_t, so[k].pm[q]);
     }
   }
   std::memcpy(blob.data() + o_spots, tabs.data(), K * sizeof(SpotTab));
@@ -1587,6 +1602,7 @@ Status Engine::create(int device, Plan&& plan, const float* regrets, const float
   d.seg_mem_off = AT(uint32_t, o_smo); d.seg_mem_cnt = AT(uint32_t, o_smc); d.seg_pslot = AT(uint32_t, o_sps);
   d.seg_factor = AT(float, o_sf); d.mrec = AT(MemRec, o_mr); d.trec = AT(TermRec, o_tr);
   d.crec = AT(MemRec, o_cr); d.chance_child_vslot = AT(uint32_t, o_ccv); d.tbatch = AT(TermBatch, o_tb);
+  d.ciso_off = AT(uint32_t, o_cio); d.ciso = AT(IsoRef, o_cis);
   d.spots = AT(SpotTab, o_spots);
   e->tbatch_dev_ = (TermBatch*)(base + o_tb);
   e->run_length_ = default_run_length(P);

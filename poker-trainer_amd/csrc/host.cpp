@@ -230,8 +230,32 @@ struct TreeWalker {
     uint64_t used = 0;
     for (int c : spec.board) used |= 1ull << c;
     for (int c : dealt) used |= 1ull << c;
-    std::vector<int> cards;
-    for (int c = 0; c < 52; c++) if (!(used >> c & 1)) cards.push_back(c);
+    std::vector<int> avail, cards;
+    for (int c = 0; c < 52; c++) if (!(used >> c & 1)) avail.push_back(c);
+    // iso.rs:277-300: a card whose suit has a lower symmetric partner with
+    // the same board ranks is merged onto that partner's card
+    std::vector<IsoRef> refs;
+    std::vector<int> ref_canon;
+    uint16_t m[4] = {0, 0, 0, 0};
+    for (int c : spec.board) m[c & 3] |= (uint16_t)(1u << (c >> 2));
+    for (int c : dealt) m[c & 3] |= (uint16_t)(1u << (c >> 2));
+    const bool iso = spec.iso && spec.iso->enabled && spec.iso->any();
+    for (int c : avail) {
+      int found = -1, pair = -1;
+      if (iso)
+        for (int s0 = 0; s0 < (c & 3) && found < 0; s0++)
+          for (int pi = 0; pi < 6; pi++)
+            if (kSuitPairs[pi][0] == s0 && kSuitPairs[pi][1] == (c & 3) && spec.iso->symmetric[pi] && m[s0] == m[c & 3]) {
+              found = s0;
+              pair = pi;
+            }
+      if (found >= 0) {
+        refs.push_back({0, (uint8_t)pair, (int8_t)c});
+        ref_canon.push_back((c >> 2) * 4 + found);
+      } else {
+        cards.push_back(c);
+      }
+    }
     size_t first = 0;
     if (tree) {
       first = alloc(cards.size());
@@ -241,7 +265,13 @@ struct TreeWalker {
       Node& n = tree->nodes[id];
       n.kind = kChance; n.street = st.street; n.num_children = (uint16_t)cards.size(); n.first_child = (uint32_t)first;
       n.invest[0] = inv[0]; n.invest[1] = inv[1];
-      n.chance_factor = 1.0f / ((float)cards.size() * compatible / (float)cards.size());
+      n.chance_factor = 1.0f / ((float)(cards.size() + refs.size()) * compatible / (float)avail.size());
+      n.iso_first = (uint32_t)tree->iso_refs.size();
+      n.iso_cnt = (uint16_t)refs.size();
+      for (size_t r = 0; r < refs.size(); r++) {
+        refs[r].child = (uint16_t)(std::find(cards.begin(), cards.end(), ref_canon[r]) - cards.begin());
+        tree->iso_refs.push_back(refs[r]);
+      }
     }
     for (size_t i = 0; i < cards.size(); i++) {
       size_t child = first + i;
@@ -278,6 +308,38 @@ Tree build_tree(const TreeSpec& spec) {
   int32_t inv[2] = {0, 0};
   w.player(0, spec.root, inv);
   return t;
+}
+
+// ----------------------------------------------------------------- isomorphism
+Iso make_iso(const Hands& h, bool enabled) {  // iso.rs:245-273
+  Iso iso;
+  iso.enabled = enabled;
+  if (!enabled) return iso;
+  for (int pi = 0; pi < 6; pi++) {
+    const int a = kSuitPairs[pi][0], b = kSuitPairs[pi][1];
+    bool ok = true;
+    std::vector<uint16_t> pp[2];
+    for (int p = 0; p < 2 && ok; p++) {
+      float mx = 0.0f;
+      for (float w : h.weights[p]) mx = std::max(mx, w);
+      mx = std::max(mx, 1e-30f);
+      for (size_t i = 0; i < h.n(p) && ok; i++) {
+        int c[2];
+        for (int k = 0; k < 2; k++) {
+          const int x = h.cards[p][2 * i + k], s = x & 3;
+          c[k] = (x >> 2) * 4 + (s == a ? b : s == b ? a : s);
+        }
+        const uint32_t j = h.lookup[p][combo_of(c[0], c[1])];
+        if (j == kNone || std::fabs(h.weights[p][i] - h.weights[p][j]) > 1e-4f * mx) ok = false;
+        else pp[p].push_back((uint16_t)j);
+      }
+    }
+    if (!ok) continue;
+    iso.symmetric[pi] = true;
+    iso.perm[0][pi] = std::move(pp[0]);
+    iso.perm[1][pi] = std::move(pp[1]);
+  }
+  return iso;
 }
 
 // ----------------------------------------------------------------- hands

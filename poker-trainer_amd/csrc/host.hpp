@@ -63,6 +63,33 @@ struct Node {
   float rake, chance_factor;
   int8_t card;      // chance child: the dealt card, else -1
   Action action;    // action child: the action taken
+  // chance node: its cards merged by suit isomorphism, Tree::iso_refs[iso_first, + iso_cnt)
+  uint32_t iso_first;
+  uint16_t iso_cnt;
+};
+
+// ----------------------------------------------------------------- isomorphism
+// Suit pairs (crates/solver/src/iso.rs:174) under which both ranges are
+// unchanged (weights within 1e-4 of the largest, iso.rs:245-273), with the
+// hand permutation each one induces, per player.
+constexpr int kSuitPairs[6][2] = {{0, 1}, {0, 2}, {0, 3}, {1, 2}, {1, 3}, {2, 3}};
+struct Hands;
+struct Iso {
+  bool enabled = false;
+  bool symmetric[6] = {false, false, false, false, false, false};
+  std::vector<uint16_t> perm[2][6];  // hand h -> the hand with the pair's suits swapped
+  bool any() const {
+    for (bool b : symmetric) if (b) return true;
+    return false;
+  }
+};
+Iso make_iso(const Hands& h, bool enabled);
+// A dealt card merged onto a canonical child: the child's values, with the
+// hands permuted by suit pair `pair`, stand for the card's (iso.rs:277-300).
+struct IsoRef {
+  uint16_t child;  // canonical child index at the chance node
+  uint8_t pair;
+  int8_t card;
 };
 
 struct TreeSpec {
@@ -76,12 +103,15 @@ struct TreeSpec {
   // cut the tree where the root street's betting ends with chips behind:
   // such a node becomes a leaf (its chance factor as the chance node's)
   bool depth_limit = false;
+  // suit isomorphism: chance nodes keep one child per class of equivalent cards
+  const Iso* iso = nullptr;
 };
 
 struct TreeCounts { uint64_t player = 0, chance = 0, terminal = 0, cells = 0; };
 
 struct Tree {
   std::vector<Node> nodes;
+  std::vector<IsoRef> iso_refs;
   std::vector<std::vector<int>> boards;  // full 5-card board of every showdown table
   uint64_t storage_len = 0;
   uint32_t player_nodes = 0;

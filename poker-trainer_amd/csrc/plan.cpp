@@ -37,6 +37,12 @@ std::string build_plan(const PlanInput& in, Plan& P) {
     P.cards[q].assign(in.hand_cards[q], in.hand_cards[q] + 2 * n);
     P.weights[q].assign(in.hand_weights[q], in.hand_weights[q] + n);
     P.same[q].assign(in.hand_same[q], in.hand_same[q] + n);
+    P.iso_perm[q].clear();
+    if (in.iso_refs && in.iso_perm[q]) {
+      P.iso_perm[q].assign(in.iso_perm[q], in.iso_perm[q] + 6 * (size_t)n);
+      for (uint16_t x : P.iso_perm[q])
+        if (x >= n) return fail("bad isomorphism permutation", x);
+    }
     for (uint32_t i = 0; i < n; i++) {
       uint8_t a = P.cards[q][2 * i], b = P.cards[q][2 * i + 1];
       if (a >= 52 || b >= 52 || a == b) return fail("bad hand cards", i);
@@ -375,6 +381,8 @@ std::string build_plan(const PlanInput& in, Plan& P) {
     P.chance_off.assign(D + 1, 0);
     P.crec.clear();
     P.chance_child_vslot.clear();
+    P.ciso_off.assign(1, 0);
+    P.ciso.clear();
     P.ch_coff.assign((size_t)D * (C + 1), 0);
     for (uint32_t d = 0; d < D; d++) {
       P.chance_off[d] = (uint32_t)P.crec.size();
@@ -388,6 +396,14 @@ std::string build_plan(const PlanInput& in, Plan& P) {
           if (r.kind != kChance) continue;
           P.crec.push_back(r);
           P.chance_child_vslot.push_back(P.vslot[r.offset]);
+          if (in.iso_refs)
+            for (uint32_t k = 0; k < in.iso_cnt[r.node]; k++) {
+              const IsoRef& e = in.iso_refs[in.iso_first[r.node] + k];
+              if (e.child >= r.na || e.pair >= 6 || e.card < 0 || e.card >= 52 || P.iso_perm[0].empty())
+                return fail("bad isomorphism reference", r.node);
+              P.ciso.push_back(e);
+            }
+          P.ciso_off.push_back((uint32_t)P.ciso.size());
         }
       }
       if (d >= 1)
@@ -635,6 +651,8 @@ std::string merge_plans(std::vector<Plan>&& parts, Plan& M) {
   M.seg_pslot.insert(M.seg_pslot.end(), pslot[1].begin(), pslot[1].end());
   // chance records by depth
   M.chance_off.assign(D + 1, 0);
+  M.ciso_off.assign(1, 0);
+  M.ciso.clear();
   for (uint32_t d = 0; d < D; d++) {
     M.chance_off[d] = (uint32_t)M.crec.size();
     for (uint32_t k = 0; k < K; k++) {
@@ -643,6 +661,8 @@ std::string merge_plans(std::vector<Plan>&& parts, Plan& M) {
       for (uint32_t i = T.chance_off[d]; i < T.chance_off[d + 1]; i++) {
         M.crec.push_back(shift(T.crec[i], k));
         M.chance_child_vslot.push_back(T.chance_child_vslot[i] + (uint32_t)vslot_off[k]);
+        M.ciso.insert(M.ciso.end(), T.ciso.begin() + T.ciso_off[i], T.ciso.begin() + T.ciso_off[i + 1]);
+        M.ciso_off.push_back((uint32_t)M.ciso.size());
       }
     }
   }

@@ -105,6 +105,12 @@ struct PlanInput {
   uint64_t storage_len = 0;
   bool batch_river = true;
   int64_t root_stack[2] = {0, 0};  // chips behind at the root (for leaves)
+  // suit isomorphism (none when iso_refs is null): per node its merged cards
+  // (chance nodes), and per player 6 hand permutations of num_hands each
+  const uint32_t* iso_first = nullptr;
+  const uint16_t* iso_cnt = nullptr;
+  const IsoRef* iso_refs = nullptr;
+  const uint16_t* iso_perm[2] = {nullptr, nullptr};
 };
 
 struct Plan {
@@ -136,6 +142,11 @@ struct Plan {
   std::vector<uint32_t> chance_off;  // [depth + 1]
   std::vector<MemRec> crec;          // one per chance node (vslot: its own row; offset: first child; cslot: children)
   std::vector<uint32_t> chance_child_vslot;  // first child's value row, parallel to crec
+  // the cards each chance record's node merges by suit isomorphism:
+  // ciso[ciso_off[i], ciso_off[i + 1]) (ciso_off has crec.size() + 1 entries)
+  std::vector<uint32_t> ciso_off;
+  std::vector<IsoRef> ciso;
+  std::vector<uint16_t> iso_perm[2];  // 6 x n[p] hand permutations (empty without isomorphism)
   uint32_t num_slots[2] = {0, 0}, num_vslots = 0, max_members = 0;
   // chunks: the subtrees below the root street's chance nodes (the depth-1
   // segments and their descendants), run in groups that share scratch.

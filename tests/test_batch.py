@@ -12,9 +12,11 @@ from pts.solver import BatchSolver, Solver
 
 
 def small_batch():
-    # mixed streets and hand counts, one set of DCFR parameters
+    # mixed streets and hand counts, one set of DCFR parameters; the last
+    # spot's chance nodes merge suit-isomorphic cards
+    from test_host_cpu import iso_turn
     return [spots.river_spot(0), spots.turn_spot(1), spots.river_spot(2), spots.batch_spot("river", 3),
-            spots.batch_spot("turn", 0), spots.river_spot(5)]
+            spots.batch_spot("turn", 0), spots.river_spot(5), iso_turn()]
 
 
 def test_batch_plan_merges_and_checks_on_the_host():

@@ -12,7 +12,7 @@ from pts import spots, PostflopMenu, make_config, parse_cards, combo_index, Dcfr
 from pts.solver import Engine, Solver, SolveOptions
 from pts.native import EXACT, FAST
 
-from test_host_cpu import oracle_tree
+from test_host_cpu import iso_flop, iso_turn, oracle_tree
 
 pytestmark = pytest.mark.gpu
 
@@ -108,6 +108,25 @@ def test_large_segments_bit_exact(kind, index):
     assert same(rg, rc), describe(rg, rc)
     assert same(sg, sc), describe(sg, sc)
     assert g.exploitability() == c.exploitability_f64()
+
+
+@pytest.mark.parametrize("make", [iso_turn, iso_flop], ids=["iso-turn", "iso-flop"])
+def test_isomorphism_bit_exact(make):
+    # chance nodes merge suit-isomorphic cards (iso.rs): the chance sums add
+    # the canonical child's row at the permuted hands after the canonical
+    # children, in the oracle's order
+    cfg = make()
+    g, c = Solver(cfg, precision=EXACT), O.OracleSolver(cfg)
+    g.iterate(10)
+    c.iterate(10)
+    rg, sg = g.storage()
+    rc, sc = c.storage()
+    assert same(rg, rc), describe(rg, rc)
+    assert same(sg, sc), describe(sg, sc)
+    assert g.exploitability() == c.exploitability_f64()
+    f = Solver(cfg)  # fast precision: the same chance kernel
+    f.iterate(10)
+    assert abs(f.exploitability() - c.exploitability_f64()) <= 1e-4 * cfg.root.pot
 
 
 def test_solve_loop_matches_the_oracle():

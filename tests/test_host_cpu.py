@@ -56,6 +56,20 @@ def product_nodes(h):
     return a
 
 
+def iso_turn():
+    # diamonds and spades are absent from the board: every river card of one
+    # of them is merged onto the other's
+    return make_config("2c7c9hTh", "AA, KK, QQ, AKs, AQs, KQs, JTs, 98s, 76s", "TT, 99, 88, AJs, KJs, QJs, T9s, 87s, 65s",
+                       1000, 5000, PostflopMenu.one(["50%"], [], max_raises=1), isomorphism=True)
+
+
+def iso_flop():
+    # one card of three suits: no merge at the turn; after a turn card that
+    # repeats a rank in spades, that pair of suits merges river cards
+    return make_config("2c7d9h", "AA, KK, QQ, AKs, JTs, 98s", "TT, 99, AJs, KQs, 87s, 65s", 600, 1200,
+                       PostflopMenu.one(["50%"], [], max_raises=1), isomorphism=True)
+
+
 CONFIGS = [
     ("river", lambda: spots.river_spot(0)),
     ("river7", lambda: spots.river_spot(7)),
@@ -64,6 +78,10 @@ CONFIGS = [
     ("allin", lambda: make_config("2c7d9hTs", "AhAd", "KhKd", 1000, 1000, PostflopMenu.one(["allin"], []))),
     ("default-menu", lambda: make_config("Qh9d4c2s", "AQ, KQ, QJs, JTs, 99, 44, T9s, 87s, A5s, KJs",
                                          "AA, KK, AQ, QT, 99, JJ, TT, 98s, KJs, 65s, AJs", 600, 4000, PostflopMenu())),
+    # suit isomorphism (iso.rs): class ranges are suit-symmetric, and the
+    # board leaves suits with equal rank sets, so chance nodes merge cards
+    ("iso-turn", lambda: iso_turn()),
+    ("iso-flop", lambda: iso_flop()),
 ]
 
 
@@ -97,7 +115,7 @@ def test_tree_arena_equals_the_oracle(name, make):
 
 
 def test_estimate_matches_the_oracle():
-    for make in (lambda: spots.turn_spot(1), lambda: spots.river_spot(2)):
+    for make in (lambda: spots.turn_spot(1), lambda: spots.river_spot(2), iso_turn, iso_flop):
         cfg = make()
         from pts.solver import Solver
         a = Solver.estimate(cfg)
