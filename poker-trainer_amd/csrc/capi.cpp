@@ -525,13 +525,21 @@ void pts_solver_nodes(pts_solver* s, uint8_t* kind, uint8_t* street, uint16_t* n
   }
 }
 
-// lib.rs:746-825: reach of both players along `path` (a card merged by suit
-// isomorphism is not followed: its canonical card's path is the query).
-static pts_status walk(pts_solver* s, const pts_step_c* path, int32_t np, uint32_t& id, std::vector<float> reach[2]) {
+// lib.rs:746-825: reach of both players along `path`. A card merged by suit
+// isomorphism continues at its canonical child: later cards are read through
+// the swapped suits, the reach arrays are permuted into the canonical hands,
+// and to_canon maps each hand to the canonical hand that stands for it.
+static pts_status walk(pts_solver* s, const pts_step_c* path, int32_t np, uint32_t& id, std::vector<float> reach[2],
+                       std::vector<uint32_t> to_canon[2]) {
   NEED_ENGINE(s);
   id = 0;
   reach[0] = s->hands.weights[0];
   reach[1] = s->hands.weights[1];
+  int suit_map[4] = {0, 1, 2, 3};
+  for (int x = 0; x < 2; x++) {
+    to_canon[x].resize(s->hands.n(x));
+    for (size_t h = 0; h < to_canon[x].size(); h++) to_canon[x][h] = (uint32_t)h;
+  }
   for (int32_t d = 0; d < np; d++) {
     const Node& node = s->tree.nodes[id];
     const pts_step_c& stp = path[d];
@@ -551,20 +559,32 @@ static pts_status walk(pts_solver* s, const pts_step_c* path, int32_t np, uint32
       for (size_t i = 0; i < n; i++) reach[q][i] *= strat[(size_t)ai * n + i];
       id = node.first_child + ai;
     } else if (node.kind == kChance && stp.kind == 1) {
+      if (stp.card < 0 || stp.card >= 52) return bad("bad path: bad card at depth " + std::to_string(d));
+      const int cc = (stp.card >> 2) * 4 + suit_map[stp.card & 3];
       uint32_t ch = kNone;
       for (uint32_t c = node.first_child; c < node.first_child + node.num_children; c++)
-        if (s->tree.nodes[c].card == stp.card) { ch = c; break; }
+        if (s->tree.nodes[c].card == cc) { ch = c; break; }
       if (ch == kNone) {
+        const IsoRef* e = nullptr;
         for (uint32_t k = 0; k < node.iso_cnt; k++)
-          if (s->tree.iso_refs[node.iso_first + k].card == stp.card)
-            return bad("invalid input: the card at depth " + std::to_string(d) +
-                       " is merged by suit isomorphism; query the isomorphic canonical card's path");
-        return bad("bad path: card not in the tree at depth " + std::to_string(d));
+          if (s->tree.iso_refs[node.iso_first + k].card == cc) e = &s->tree.iso_refs[node.iso_first + k];
+        if (!e) return bad("bad path: card not in the tree at depth " + std::to_string(d));
+        const int a = kSuitPairs[e->pair][0], b = kSuitPairs[e->pair][1];
+        for (int& x : suit_map) x = x == a ? b : x == b ? a : x;
+        for (int x = 0; x < 2; x++) {
+          const std::vector<uint16_t>& perm = s->iso.perm[x][e->pair];
+          std::vector<float> r2(reach[x].size(), 0.0f);
+          for (size_t h = 0; h < reach[x].size(); h++) r2[perm[h]] = reach[x][h];
+          reach[x] = std::move(r2);
+          for (uint32_t& t : to_canon[x]) t = perm[t];
+        }
+        ch = node.first_child + e->child;
       }
       id = ch;
+      const int canonical = s->tree.nodes[ch].card;
       for (int x = 0; x < 2; x++)
         for (size_t h = 0; h < s->hands.n(x); h++)
-          if (s->hands.cards[x][2 * h] == stp.card || s->hands.cards[x][2 * h + 1] == stp.card) reach[x][h] = 0.0f;
+          if (s->hands.cards[x][2 * h] == canonical || s->hands.cards[x][2 * h + 1] == canonical) reach[x][h] = 0.0f;
     } else {
       return bad("bad path: step doesn't match node kind at depth " + std::to_string(d));
     }
@@ -577,7 +597,8 @@ pts_status pts_solver_query(pts_solver* s, const pts_step_c* path, int32_t npath
                             double reach_fraction[2], int64_t* pot, int64_t invest[2], int32_t cap_actions, int32_t cap_hands) {
   uint32_t id;
   std::vector<float> reach[2];
-  pts_status r = walk(s, path, npath, id, reach);
+  std::vector<uint32_t> to_canon[2];
+  pts_status r = walk(s, path, npath, id, reach, to_canon);
   if (r != PTS_OK) return r;
   const Node node = s->tree.nodes[id];
   if (!is_player(node.kind)) return bad("bad path: not a decision node");
@@ -594,11 +615,12 @@ pts_status pts_solver_query(pts_solver* s, const pts_step_c* path, int32_t npath
   double half_pot = (double)s->cfg.pot / 2.0, norm = 1.0 / s->hands.num_combinations;
   double offset = half_pot + (double)node.invest[q];
   for (size_t h = 0; h < nh; h++) {
-    reach_out[h] = reach[q][h];
-    double denom = compat[h] * norm;
+    const size_t hc = to_canon[q][h];  // the canonical hand standing for h
+    reach_out[h] = reach[q][hc];
+    double denom = compat[hc] * norm;
     for (size_t a = 0; a < A; a++) {
-      strategy[a * nh + h] = strat[a * nh + h];
-      ev[a * nh + h] = denom > 0.0 ? (float)((double)cv[a * nh + h] / denom + offset) : 0.0f;
+      strategy[a * nh + h] = strat[a * nh + hc];
+      ev[a * nh + h] = denom > 0.0 ? (float)((double)cv[a * nh + hc] / denom + offset) : 0.0f;
     }
   }
   for (int p = 0; p < 2; p++) {
@@ -625,12 +647,13 @@ pts_status pts_solver_query(pts_solver* s, const pts_step_c* path, int32_t npath
 pts_status pts_solver_ranges_after(pts_solver* s, const pts_step_c* path, int32_t npath, float* out0, float* out1) {
   uint32_t id;
   std::vector<float> reach[2];
-  pts_status r = walk(s, path, npath, id, reach);
+  std::vector<uint32_t> to_canon[2];
+  pts_status r = walk(s, path, npath, id, reach, to_canon);
   if (r != PTS_OK) return r;
   float* outs[2] = {out0, out1};
   for (int p = 0; p < 2; p++) {
     std::memset(outs[p], 0, 1326 * sizeof(float));
-    for (size_t h = 0; h < s->hands.n(p); h++) outs[p][s->hands.combos[p][h]] = reach[p][h];
+    for (size_t h = 0; h < s->hands.n(p); h++) outs[p][s->hands.combos[p][h]] = reach[p][to_canon[p][h]];
   }
   return PTS_OK;
 }

@@ -163,6 +163,41 @@ def test_queries_match_the_oracle():
         assert same(ra[0], rb[0]) and same(ra[1], rb[1])
 
 
+def test_queries_through_isomorphic_cards():
+    # lib.rs:746-825 with iso.rs: a river card merged onto its suit-swapped
+    # canonical card is answered from the canonical subtree through the hand
+    # permutation, as the oracle does; and it agrees with a solve that merges
+    # nothing (tests.rs:234-271)
+    cfg = iso_turn()
+    g, c = Solver(cfg, precision=EXACT), O.OracleSolver(cfg)
+    g.iterate(8)
+    c.iterate(8)
+    cfg_plain = iso_turn()
+    cfg_plain.isomorphism = False
+    plain = Solver(cfg_plain, precision=EXACT)
+    plain.iterate(8)
+    path = [c.query([]).actions[0]]
+    path = path + [c.query(path).actions[0]]
+    assert path[0] == "check" and path[1] == "check"
+    # spades are merged onto diamonds at the river: the merged tree is smaller
+    assert g.stats().player_nodes < plain.stats().player_nodes
+    for card in range(52):
+        if card in cfg.board:
+            continue
+        pth = path + [("card", card)]
+        a, b = g.query(pth), c.query(pth)
+        assert a.actions == b.actions
+        assert same(a.strategy, b.strategy) and same(a.reach, b.reach)
+        assert same(a.ev, b.ev), describe(a.ev, b.ev)
+        ra, rb = g.ranges_after(pth), c.ranges_after(pth)
+        assert same(ra[0], rb[0]) and same(ra[1], rb[1])
+        p = plain.query(pth)
+        assert np.max(np.abs(a.strategy - p.strategy)) < 2e-3
+        assert np.max(np.abs(a.ev - p.ev)) < 0.5
+        pr = plain.ranges_after(pth)
+        assert np.max(np.abs(ra[0] - pr[0])) < 1e-6 and np.max(np.abs(ra[1] - pr[1])) < 1e-6
+
+
 REF = DcfrParams.reference_default()
 
 
