@@ -109,6 +109,9 @@ def main():
     ap.add_argument("--precision", default="fast", choices=["fast", "exact"],
                     help="fast: block-scan terminal sums (default, as the reference's GPU default); exact: the "
                          "reference's summation order, bit-identical to the CPU solve")
+    ap.add_argument("--storage", default="f32", choices=["f32", "16bit"],
+                    help="regret / strategy-sum storage: f32 (default), or 16-bit (i16 regrets, u16 sums, one f32 "
+                         "scale per decision node: half the bytes per cell)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (this build has no multi-GPU path); got --gpus {args.gpus}",
@@ -137,6 +140,10 @@ def main():
     if args.max_iterations:
         max_iters = min(max_iters, args.max_iterations)
     cfg = getattr(spots, maker)(args.seed)
+    q16 = args.storage == "16bit"
+    if q16:
+        cfg.storage_mode = 1
+        desc = desc.replace("f32 regrets and sums", "16-bit regrets and sums").replace("; f32,", "; 16-bit storage,")
 
     # steps: W warmup iterations, then K timed ones
     t0 = time.perf_counter()
@@ -166,12 +173,15 @@ def main():
         f"{len(hist)} checks")
 
     cells = st.storage_cells
-    bytes_cls = {"down": 12.0 * cells, "up": 8.0 * cells, "terminal": 4.0 * st.terminal_nodes * (n0 + n1)}
+    # per cell: down reads regrets and reads and writes sums, up reads and
+    # writes regrets (4 bytes each, 2 with 16-bit storage)
+    cb = 2.0 if q16 else 4.0
+    bytes_cls = {"down": 3 * cb * cells, "up": 2 * cb * cells, "terminal": 4.0 * st.terminal_nodes * (n0 + n1)}
     dom = max(("down", "terminal", "up"), key=lambda k: ms[k])
     achieved = bytes_cls[dom] / (ms[dom] * 1e-3) / 1e9 if ms[dom] > 0 else 0.0
     roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 5),
-                "traffic": measured_traffic(args.workload, args.precision, dom), "kernel": dom,
+                "traffic": None if q16 else measured_traffic(args.workload, args.precision, dom), "kernel": dom,
                 "launches_per_iteration": launches[dom], "avg_launch_ms": round(ms[dom] / max(launches[dom], 1), 5),
                 "algorithmic_bytes_per_launch": round(bytes_cls[dom] / max(launches[dom], 1)),
                 "iteration_ms_by_class": {k: round(v, 4) for k, v in ms.items()}}
@@ -188,7 +198,8 @@ def main():
         "vs_baseline": None, "dtype": "f32", "data": "synthetic",
         "config": {"workload": desc, "seed": args.seed, "hands": [n0, n1], "decision_nodes": st.player_nodes,
                    "terminals": st.terminal_nodes, "storage_cells": cells, "target_pct": args.target,
-                   "check_every": args.check_every, "precision": args.precision, "iterations_to_target": pr.iteration,
+                   "check_every": args.check_every, "precision": args.precision, "storage": args.storage,
+                   "iterations_to_target": pr.iteration,
                    "target_reached": reached, "final_exploitability_pct": round(pr.exploitability_pct, 5),
                    "iterations_per_s": round(1.0 / step_s, 2), "build_s": round(build_s, 3),
                    "parallelism": "1 GPU"},

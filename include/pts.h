@@ -160,7 +160,8 @@ typedef struct { /* crates/solver/src/lib.rs:73-105 (exact chance, f32 storage) 
   double rake_rate, rake_cap;
   uint64_t max_memory_bytes;
   float prune;        /* must be 0 */
-  int32_t storage_mode; /* must be 0 (f32) */
+  int32_t storage_mode; /* storage.rs:74-81: 0 f32, 1 16-bit ("compressed"), 2 auto (16-bit above 1 GB of f32);
+                           batched and depth-limited solves take f32 only */
   int32_t isomorphism;  /* solved without suit isomorphism; rejected if it would apply */
   int32_t beta_some;
   double alpha, beta, gamma; /* crates/solver/src/cfr.rs:47-63 */
@@ -211,8 +212,19 @@ uint32_t pts_solver_iterations(pts_solver* s);
 void pts_solver_set_iterations(pts_solver* s, uint32_t t);
 uint64_t pts_solver_storage_len(pts_solver* s);
 int32_t pts_solver_num_hands(pts_solver* s, int player);
+/* f32 storage as stored; 16-bit storage decoded, q * (scale / MAX) (storage.rs:209-246).
+ * Setting f32 values is refused for 16-bit storage. */
 pts_status pts_solver_get_storage(pts_solver* s, float* regrets, float* sums);
 pts_status pts_solver_set_storage(pts_solver* s, const float* regrets, const float* sums);
+/* 16-bit storage (storage.rs:87-106; the reference's HostStorage::Compressed,
+ * gpu/mod.rs, which its engine uploads and downloads): storage_len i16
+ * regrets and u16 strategy sums, and one regret scale and one strategy-sum
+ * scale per player node (player_nodes of pts_solver_stats, in the tree
+ * builder's order, tree.rs:463-469). */
+int32_t pts_solver_is_compressed(pts_solver* s);
+pts_status pts_solver_get_storage_q(pts_solver* s, int16_t* regrets, uint16_t* sums, float* regret_scale, float* strat_scale);
+pts_status pts_solver_set_storage_q(pts_solver* s, const int16_t* regrets, const uint16_t* sums, const float* regret_scale,
+                                    const float* strat_scale);
 /* The next DCFR iteration (advancing the iteration count), profiled as
  * pts_engine_profile_iteration. */
 pts_status pts_solver_profile_iteration(pts_solver* s, double ms[4], uint32_t launches[4], double* wall_ms);

@@ -41,7 +41,7 @@ typedef struct {
   double rake_rate, rake_cap;
   uint64_t max_memory_bytes;
   float prune;
-  int32_t storage_mode; /* 0 f32 */
+  int32_t storage_mode; /* 0 f32, 1 16-bit, 2 16-bit above 1 GB of f32 */
   int32_t isomorphism;
   int32_t beta_some;
   double alpha, beta, gamma;
@@ -99,7 +99,13 @@ int32_t orc_solver_num_hands(void* s, int32_t p);
 double orc_solver_num_combinations(void* s);
 void orc_solver_hands(void* s, int32_t p, uint16_t* combos, float* weights);
 void orc_solver_get_storage(void* s, float* regrets, float* sums);
-void orc_solver_set_storage(void* s, const float* regrets, const float* sums);
+int32_t orc_solver_set_storage(void* s, const float* regrets, const float* sums); /* -1: 16-bit storage */
+int32_t orc_solver_is_compressed(void* s);
+/* one block through the 16-bit codec from zero storage (rq: regrets, sq: sums) */
+void orc_encode_block(int32_t sums, const float* v, uint64_t n, int16_t* rq, uint16_t* sq, float* scale);
+int32_t orc_solver_get_storage_q(void* s, int16_t* regrets, uint16_t* sums, float* regret_scale, float* strat_scale);
+int32_t orc_solver_set_storage_q(void* s, const int16_t* regrets, const uint16_t* sums, const float* regret_scale,
+                                 const float* strat_scale);
 uint32_t orc_solver_iterations(void* s);
 void orc_solver_set_iterations(void* s, uint32_t t);
 /* runs `count` DCFR iterations (no exploitability checks) */

@@ -225,15 +225,54 @@ void orc_solver_hands(void* s, int32_t p, uint16_t* combos, float* weights) {
   }
 }
 
+// f32 storage as is; 16-bit storage decoded (storage.rs:209-246)
 void orc_solver_get_storage(void* s, float* regrets, float* sums) {
   Solver* v = (Solver*)s;
+  if (v->compressed()) {
+    for (const Node& n : v->tree.nodes) {
+      if (!is_player(n.kind)) continue;
+      const size_t len = (size_t)n.num_children * v->hands.len(n.kind);
+      const float rs = v->q.regret_scale[n.slot] / 32767.0f, ss = v->q.strat_scale[n.slot] / 65535.0f;
+      for (size_t i = 0; i < len; i++) {
+        regrets[n.offset + i] = (float)v->q.regrets[n.offset + i] * rs;
+        sums[n.offset + i] = (float)v->q.strat[n.offset + i] * ss;
+      }
+    }
+    return;
+  }
   std::memcpy(regrets, v->regrets.data(), v->regrets.size() * sizeof(float));
   std::memcpy(sums, v->strat.data(), v->strat.size() * sizeof(float));
 }
-void orc_solver_set_storage(void* s, const float* regrets, const float* sums) {
+int32_t orc_solver_set_storage(void* s, const float* regrets, const float* sums) {
   Solver* v = (Solver*)s;
+  if (v->compressed()) return -1;
   std::memcpy(v->regrets.data(), regrets, v->regrets.size() * sizeof(float));
   std::memcpy(v->strat.data(), sums, v->strat.size() * sizeof(float));
+  return 0;
+}
+void orc_encode_block(int32_t sums, const float* v, uint64_t n, int16_t* rq, uint16_t* sq, float* scale) {
+  encode_block(sums != 0, v, n, rq, sq, scale);
+}
+int32_t orc_solver_is_compressed(void* s) { return ((Solver*)s)->compressed() ? 1 : 0; }
+// 16-bit storage: the stored integers and the per-player-node scales
+int32_t orc_solver_get_storage_q(void* s, int16_t* regrets, uint16_t* sums, float* regret_scale, float* strat_scale) {
+  Solver* v = (Solver*)s;
+  if (!v->compressed()) return -1;
+  std::memcpy(regrets, v->q.regrets.data(), v->q.regrets.size() * sizeof(int16_t));
+  std::memcpy(sums, v->q.strat.data(), v->q.strat.size() * sizeof(uint16_t));
+  std::memcpy(regret_scale, v->q.regret_scale.data(), v->q.regret_scale.size() * sizeof(float));
+  std::memcpy(strat_scale, v->q.strat_scale.data(), v->q.strat_scale.size() * sizeof(float));
+  return 0;
+}
+int32_t orc_solver_set_storage_q(void* s, const int16_t* regrets, const uint16_t* sums, const float* regret_scale,
+                                 const float* strat_scale) {
+  Solver* v = (Solver*)s;
+  if (!v->compressed()) return -1;
+  std::memcpy(v->q.regrets.data(), regrets, v->q.regrets.size() * sizeof(int16_t));
+  std::memcpy(v->q.strat.data(), sums, v->q.strat.size() * sizeof(uint16_t));
+  std::memcpy(v->q.regret_scale.data(), regret_scale, v->q.regret_scale.size() * sizeof(float));
+  std::memcpy(v->q.strat_scale.data(), strat_scale, v->q.strat_scale.size() * sizeof(float));
+  return 0;
 }
 uint32_t orc_solver_iterations(void* s) { return ((Solver*)s)->iterations; }
 void orc_solver_set_iterations(void* s, uint32_t t) { ((Solver*)s)->iterations = t; }

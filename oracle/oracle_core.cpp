@@ -1042,6 +1042,114 @@ template <typename V> static bool all_zero(const V* x, size_t n) {
 
 template <typename V> static void traverse_t(const Ctx& ctx, size_t id, int p, const V* opp, size_t nq, Mode mode, V* out, size_t n);
 
+// ---- 16-bit storage (storage.rs:108-128, 236-330)
+// Rust's saturating `f32 as i16` / `as u16`: truncation toward zero, NaN to 0.
+static int16_t sat_i16(float y) {
+  if (!(y == y)) return 0;
+  if (y >= 32767.0f) return 32767;
+  if (y <= -32768.0f) return -32768;
+  return (int16_t)(int32_t)y;
+}
+static uint16_t sat_u16(float y) {
+  if (!(y == y)) return 0;
+  if (y >= 65535.0f) return 65535;
+  if (y <= 0.0f) return 0;
+  return (uint16_t)(uint32_t)y;
+}
+// max(|x|) from 0 (f32::max ignores NaN, as the comparison below does)
+static float max_abs(const float* v, size_t n) {
+  float m = 0.0f;
+  for (size_t i = 0; i < n; i++) {
+    float a = std::fabs(v[i]);
+    if (a > m) m = a;
+  }
+  return m;
+}
+// storage.rs:236-246: decoded strategy sums
+static void load_strat(const Ctx& ctx, const Node& node, size_t len, float* out) {
+  if (!ctx.q) {
+    std::copy(ctx.strat->data() + node.offset, ctx.strat->data() + node.offset + len, out);
+    return;
+  }
+  const float step = ctx.q->strat_scale[node.slot] / 65535.0f;
+  const uint16_t* q = ctx.q->strat.data() + node.offset;
+  for (size_t i = 0; i < len; i++) out[i] = (float)q[i] * step;
+}
+// storage.rs:253-289: f applied to the decoded regrets, then re-encoded with
+// the block's new largest |value| as its scale (round half away from zero)
+template <typename F> static void update_regrets(const Ctx& ctx, const Node& node, size_t len, F f) {
+  if (!ctx.q) {
+    f(ctx.regrets->data() + node.offset);
+    return;
+  }
+  float& scale = ctx.q->regret_scale[node.slot];
+  int16_t* q = ctx.q->regrets.data() + node.offset;
+  std::vector<float> buf(len);
+  const float step = scale / 32767.0f;
+  for (size_t i = 0; i < len; i++) buf[i] = (float)q[i] * step;
+  f(buf.data());
+  const float m = max_abs(buf.data(), len);
+  scale = m;
+  if (m > 0.0f) {
+    const float inv = 32767.0f / m;
+    for (size_t i = 0; i < len; i++) {
+      const float y = buf[i] * inv;
+      q[i] = sat_i16(y + std::copysign(0.5f, y));
+    }
+  } else {
+    std::fill(q, q + len, (int16_t)0);
+  }
+}
+// storage.rs:291-325: the same for the strategy sums (u16, values >= 0)
+template <typename F> static void update_strat(const Ctx& ctx, const Node& node, size_t len, F f) {
+  if (!ctx.q) {
+    f(ctx.strat->data() + node.offset);
+    return;
+  }
+  float& scale = ctx.q->strat_scale[node.slot];
+  uint16_t* q = ctx.q->strat.data() + node.offset;
+  std::vector<float> buf(len);
+  const float step = scale / 65535.0f;
+  for (size_t i = 0; i < len; i++) buf[i] = (float)q[i] * step;
+  f(buf.data());
+  const float m = max_abs(buf.data(), len);
+  scale = m;
+  if (m > 0.0f) {
+    const float inv = 65535.0f / m;
+    for (size_t i = 0; i < len; i++) {
+      const float v = buf[i] > 0.0f ? buf[i] : 0.0f;
+      q[i] = sat_u16(v * inv + 0.5f);
+    }
+  } else {
+    std::fill(q, q + len, (uint16_t)0);
+  }
+}
+
+// One block through the 16-bit codec from zero storage (storage.rs:333-357):
+// the stored integers and the new scale.
+void encode_block(bool sums, const float* v, size_t n, int16_t* rq, uint16_t* sq, float* scale) {
+  Storage16 st;
+  st.regrets.assign(n, 0);
+  st.strat.assign(n, 0);
+  st.regret_scale.assign(1, 0.0f);
+  st.strat_scale.assign(1, 0.0f);
+  Ctx c{};
+  c.q = &st;
+  Node nd{};
+  nd.offset = 0;
+  nd.slot = 0;
+  auto copy_in = [&](float* b) { std::copy(v, v + n, b); };
+  if (sums) {
+    update_strat(c, nd, n, copy_in);
+    std::copy(st.strat.begin(), st.strat.end(), sq);
+    *scale = st.strat_scale[0];
+  } else {
+    update_regrets(c, nd, n, copy_in);
+    std::copy(st.regrets.begin(), st.regrets.end(), rq);
+    *scale = st.regret_scale[0];
+  }
+}
+
 // cfr.rs:315-382: value of a traverser node from its children (action-major
 // cfv); in CFR mode also the regret update. V = float for CFR.
 template <typename V> static void own_node_update(const Ctx& ctx, const Node& node, size_t n, const V* cfv, Mode mode, V* out) {
@@ -1054,7 +1162,8 @@ template <typename V> static void own_node_update(const Ctx& ctx, const Node& no
   }
   if (mode.kind == MAvg) {
     std::vector<V> s(na * n);
-    const float* src = ctx.strat->data() + node.offset;
+    std::vector<float> src(na * n);
+    load_strat(ctx, node, na * n, src.data());
     for (size_t i = 0; i < na * n; i++) s[i] = (V)src[i];
     normalize_in_place<V>(s.data(), na, n);
     for (size_t h = 0; h < n; h++) out[h] = V(0);
@@ -1063,7 +1172,7 @@ template <typename V> static void own_node_update(const Ctx& ctx, const Node& no
     return;
   }
   if constexpr (std::is_same<V, float>::value) {
-    float* r = ctx.regrets->data() + node.offset;
+   update_regrets(ctx, node, na * n, [&](float* r) {
     const Discount& d = mode.d;
     std::vector<float> inv(n);
     for (size_t h = 0; h < n; h++) inv[h] = pos0(r[h]);
@@ -1089,6 +1198,7 @@ template <typename V> static void own_node_update(const Ctx& ctx, const Node& no
         float disc = x > 0.0f ? d.pos : d.neg;
         r[a * n + h] = x * disc + cfv[a * n + h] - out[h];
       }
+   });
   }
 }
 
@@ -1098,19 +1208,29 @@ template <typename V> static void opponent_strategy(const Ctx& ctx, const Node& 
   size_t na = node.num_children;
   if (mode.kind == MCfr) {
     if constexpr (std::is_same<V, float>::value) {
-      float* r = ctx.regrets->data() + node.offset;
-      float* ss = ctx.strat->data() + node.offset;
-      regret_match_into(r, na, nq, s);
+      // storage.rs:221-234: 16-bit regrets are matched undecoded (as f32
+      // integers; the scale cancels)
+      if (ctx.q) {
+        std::vector<float> r(na * nq);
+        const int16_t* q = ctx.q->regrets.data() + node.offset;
+        for (size_t i = 0; i < na * nq; i++) r[i] = (float)q[i];
+        regret_match_into(r.data(), na, nq, s);
+      } else {
+        regret_match_into(ctx.regrets->data() + node.offset, na, nq, s);
+      }
       float g = mode.d.strat;
-      for (size_t a = 0; a < na; a++)
-        for (size_t h = 0; h < nq; h++) {
-          float& x = ss[a * nq + h];
-          if (mode.d.reach_weighted) x = x * g + s[a * nq + h] * reach[h];
-          else x = x * g + s[a * nq + h];
-        }
+      update_strat(ctx, node, na * nq, [&](float* ss) {
+        for (size_t a = 0; a < na; a++)
+          for (size_t h = 0; h < nq; h++) {
+            float& x = ss[a * nq + h];
+            if (mode.d.reach_weighted) x = x * g + s[a * nq + h] * reach[h];
+            else x = x * g + s[a * nq + h];
+          }
+      });
     }
   } else {
-    const float* src = ctx.strat->data() + node.offset;
+    std::vector<float> src(na * nq);
+    load_strat(ctx, node, na * nq, src.data());
     for (size_t i = 0; i < na * nq; i++) s[i] = (V)src[i];
     normalize_in_place<V>(s, na, nq);
   }
@@ -1371,6 +1491,13 @@ static TreeParams params_for(const SolveConfig& cfg, const HandSet& hs, const Is
   return P;
 }
 
+// lib.rs:346-352: Auto compresses when the f32 tree would exceed 1 GB
+static bool storage_compressed(const SolveConfig& cfg, const TreeStats& st) {
+  if (cfg.storage_mode == 1) return true;
+  if (cfg.storage_mode == 2) return st.memory_bytes_f32() > (1ull << 30);
+  return false;
+}
+
 std::string estimate(const SolveConfig& cfg, TreeStats& out) {  // lib.rs:355-362
   HandSet hs;
   std::string e = validate(cfg, hs);
@@ -1380,11 +1507,11 @@ std::string estimate(const SolveConfig& cfg, TreeStats& out) {  // lib.rs:355-36
   std::vector<int> dealt;
   out = TreeStats();
   count_player(P, cfg.root, dealt, out);
-  out.compressed = false;
+  out.compressed = storage_compressed(cfg, out);
   return "";
 }
 
-std::string make_solver(const SolveConfig& cfg, Solver& s) {  // lib.rs:365-396 (f32 storage)
+std::string make_solver(const SolveConfig& cfg, Solver& s) {  // lib.rs:365-396
   s.config = cfg;
   std::string e = validate(s.config, s.hands);
   if (!e.empty()) return e;
@@ -1393,7 +1520,7 @@ std::string make_solver(const SolveConfig& cfg, Solver& s) {  // lib.rs:365-396 
   std::vector<int> dealt;
   s.stats = TreeStats();
   count_player(P, s.config.root, dealt, s.stats);
-  s.stats.compressed = false;
+  s.stats.compressed = storage_compressed(s.config, s.stats);
   uint64_t needed = s.stats.memory_bytes();
   if (needed > s.config.max_memory_bytes) {
     char buf[160];
@@ -1405,8 +1532,18 @@ std::string make_solver(const SolveConfig& cfg, Solver& s) {  // lib.rs:365-396 
   int32_t inv0[2] = {0, 0};
   b.fill_player(0, s.config.root, inv0, dealt);
   s.tree = std::move(b.t);
-  s.regrets.assign(s.tree.storage_len, 0.0f);
-  s.strat.assign(s.tree.storage_len, 0.0f);
+  s.regrets.clear();
+  s.strat.clear();
+  s.q = Storage16();
+  if (s.stats.compressed) {  // storage.rs:144-156
+    s.q.regrets.assign(s.tree.storage_len, 0);
+    s.q.strat.assign(s.tree.storage_len, 0);
+    s.q.regret_scale.assign(s.tree.num_player_nodes, 0.0f);
+    s.q.strat_scale.assign(s.tree.num_player_nodes, 0.0f);
+  } else {
+    s.regrets.assign(s.tree.storage_len, 0.0f);
+    s.strat.assign(s.tree.storage_len, 0.0f);
+  }
   s.tables.resize(s.tree.boards.size());
 #pragma omp parallel for schedule(dynamic, 16)
   for (long i = 0; i < (long)s.tree.boards.size(); i++) s.tables[i] = make_table(s.hands, s.tree.boards[i]);
@@ -1418,6 +1555,7 @@ std::string make_solver(const SolveConfig& cfg, Solver& s) {  // lib.rs:365-396 
 Ctx Solver::ctx() {  // lib.rs:444-455
   Ctx c;
   c.tree = &tree; c.regrets = &regrets; c.strat = &strat; c.hands = &hands; c.tables = &tables; c.iso = &iso;
+  c.q = stats.compressed ? &q : nullptr;
   c.half_pot = (double)config.root.pot / 2.0;
   c.norm = 1.0 / hands.num_combinations;
   c.batch_river = batch_river;
@@ -1517,7 +1655,8 @@ std::vector<float> Solver::average_strategy(size_t id, size_t n) {  // lib.rs:73
   const Node& node = tree.nodes[id];
   std::vector<float> s(node.num_children * n, 0.0f);
   if (n > 0) {
-    std::copy(strat.begin() + node.offset, strat.begin() + node.offset + s.size(), s.begin());
+    Ctx c = const_cast<Solver*>(this)->ctx();
+    orc::load_strat(c, node, s.size(), s.data());
     normalize_in_place<float>(s.data(), node.num_children, n);
   }
   return s;

@@ -218,7 +218,7 @@ struct SolveConfig {  // crates/solver/src/lib.rs:73-135
   std::vector<int> keep[2];
   uint64_t max_memory_bytes = 8ull << 30;
   double rake_rate = 0.0, rake_cap = 0.0;
-  int storage_mode = 0;  // 0 F32 (the only mode restated here)
+  int storage_mode = 0;  // storage.rs:74-81: 0 F32, 1 Compressed, 2 Auto (compressed above 1 GB of f32)
   bool isomorphism = true;
   DcfrParams dcfr;
   Backend backend = Backend::Cpu;
@@ -241,9 +241,19 @@ struct NodeInfo {  // lib.rs:219-275
 enum StepKind { StepAction = 0, StepCard = 1 };
 struct Step { StepKind kind; Action action; int card; };
 
+// 16-bit storage (storage.rs:87-106): regrets as i16 and strategy sums as
+// u16, each player node's block with one f32 scale (its largest |value|);
+// a stored q stands for q * (scale / MAX).
+struct Storage16 {
+  std::vector<int16_t> regrets;
+  std::vector<uint16_t> strat;
+  std::vector<float> regret_scale, strat_scale;  // per player node (Node::slot)
+};
+
 struct Solver;
 struct Ctx {
   const Tree* tree; std::vector<float>* regrets; std::vector<float>* strat;
+  Storage16* q;  // null: f32 storage in regrets / strat
   const HandSet* hands; const std::vector<StrengthTable>* tables; const Isomorphism* iso;
   double half_pot, norm; bool batch_river; bool parallel;
 };
@@ -253,7 +263,8 @@ struct Solver {
   HandSet hands;
   Isomorphism iso;
   Tree tree;
-  std::vector<float> regrets, strat;
+  std::vector<float> regrets, strat;  // f32 storage (empty when compressed)
+  Storage16 q;                        // 16-bit storage (empty when f32)
   std::vector<StrengthTable> tables;
   TreeStats stats;
   uint32_t iterations = 0;
@@ -262,6 +273,7 @@ struct Solver {
   bool parallel = true;
 
   Ctx ctx();
+  bool compressed() const { return stats.compressed; }
   bool raked() const { return config.rake_rate > 0.0 && config.rake_cap > 0.0; }
   void cpu_iteration();
   double exploitability();
@@ -276,6 +288,10 @@ struct Solver {
   std::string query(const std::vector<Step>& path, NodeInfo& out);
   std::string ranges_after(const std::vector<Step>& path, std::vector<float> out[2]);
 };
+
+// One block through the 16-bit codec from zero storage: the stored integers
+// (rq for regrets, sq for sums) and the block's new scale.
+void encode_block(bool sums, const float* v, size_t n, int16_t* rq, uint16_t* sq, float* scale);
 
 // Returns "" on success, else an error (SolverError Display text).
 std::string estimate(const SolveConfig& cfg, TreeStats& out);

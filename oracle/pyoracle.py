@@ -87,7 +87,16 @@ def _declare(L):
     L.orc_solver_num_combinations.argtypes = [P]
     L.orc_solver_hands.argtypes = [P, ctypes.c_int32, np.ctypeslib.ndpointer(dtype=np.uint16), fp]
     L.orc_solver_get_storage.argtypes = [P, fp, fp]
+    L.orc_solver_set_storage.restype = ctypes.c_int32
     L.orc_solver_set_storage.argtypes = [P, fp, fp]
+    L.orc_solver_is_compressed.restype = ctypes.c_int32
+    L.orc_solver_is_compressed.argtypes = [P]
+    i16p, u16p = np.ctypeslib.ndpointer(dtype=np.int16), np.ctypeslib.ndpointer(dtype=np.uint16)
+    L.orc_solver_get_storage_q.restype = ctypes.c_int32
+    L.orc_solver_get_storage_q.argtypes = [P, i16p, u16p, fp, fp]
+    L.orc_solver_set_storage_q.restype = ctypes.c_int32
+    L.orc_solver_set_storage_q.argtypes = [P, i16p, u16p, fp, fp]
+    L.orc_encode_block.argtypes = [ctypes.c_int32, fp, ctypes.c_uint64, i16p, u16p, fp]
     L.orc_solver_iterations.restype = ctypes.c_uint32
     L.orc_solver_iterations.argtypes = [P]
     L.orc_solver_set_iterations.argtypes = [P, ctypes.c_uint32]
@@ -231,7 +240,27 @@ class OracleSolver:
         return r, s
 
     def set_storage(self, r, s):
-        lib().orc_solver_set_storage(self.h, np.ascontiguousarray(r, np.float32), np.ascontiguousarray(s, np.float32))
+        if lib().orc_solver_set_storage(self.h, np.ascontiguousarray(r, np.float32), np.ascontiguousarray(s, np.float32)):
+            raise ValueError("16-bit storage: use set_storage_q")
+
+    @property
+    def compressed(self):
+        return bool(lib().orc_solver_is_compressed(self.h))
+
+    def storage_q(self):
+        """16-bit storage (storage.rs:87-106): i16 regrets, u16 sums, and the
+        regret and strategy-sum scales of every player node."""
+        n, k = self.storage_len, self.stats().player_nodes
+        r, s = np.zeros(n, np.int16), np.zeros(n, np.uint16)
+        rs, ss = np.zeros(k, np.float32), np.zeros(k, np.float32)
+        if lib().orc_solver_get_storage_q(self.h, r, s, rs, ss):
+            raise ValueError("f32 storage")
+        return r, s, rs, ss
+
+    def set_storage_q(self, r, s, rs, ss):
+        if lib().orc_solver_set_storage_q(self.h, np.ascontiguousarray(r, np.int16), np.ascontiguousarray(s, np.uint16),
+                                          np.ascontiguousarray(rs, np.float32), np.ascontiguousarray(ss, np.float32)):
+            raise ValueError("f32 storage")
 
     @property
     def iterations(self):
@@ -326,6 +355,15 @@ class OracleSolver:
         if lib().orc_solver_ranges_after(self.h, steps_c(path), len(path), o0, o1, err, 256):
             raise ValueError(err.value.decode())
         return o0, o1
+
+
+def encode_block(values, sums):
+    """One block through the 16-bit codec (storage.rs:253-325) from zero
+    storage: (stored integers, scale)."""
+    v = np.ascontiguousarray(values, np.float32)
+    rq, sq, sc = np.zeros(len(v), np.int16), np.zeros(len(v), np.uint16), np.zeros(1, np.float32)
+    lib().orc_encode_block(int(sums), v, len(v), rq, sq, sc)
+    return (sq if sums else rq), float(sc[0])
 
 
 class OracleHandSet:

@@ -28,6 +28,7 @@ struct pts_solver {
   TreeCounts counts;
   std::unique_ptr<Engine> engine;
   Plan host_plan;  // kept by host-only solvers (device -1)
+  bool compressed = false;  // 16-bit storage (storage.rs:74-81)
   uint32_t iterations = 0;
   double exploitability = std::numeric_limits<double>::infinity();
 };
@@ -118,7 +119,7 @@ static std::string validate(const pts_config_c& c, TreeSpec& spec, Hands& hands)
   int street = n == 3 ? kFlop : n == 4 ? kTurn : kRiver;
   if (street != c.street) return "invalid board: board doesn't match the root street";
   if (c.prune != 0.0f) return "invalid input: range pruning is not supported";
-  if (c.storage_mode != 0) return "invalid input: only f32 storage is supported";
+  if (c.storage_mode < 0 || c.storage_mode > 2) return "invalid input: storage mode";
   const float* r[2] = {c.ranges[0], c.ranges[1]};
   hands = make_hands(r, bits);
   for (int p = 0; p < 2; p++)
@@ -146,7 +147,15 @@ static std::string validate(const pts_config_c& c, TreeSpec& spec, Hands& hands)
   return "";
 }
 
-static uint64_t memory_bytes(const TreeCounts& t) { return t.cells * 8 + (t.player + t.chance + t.terminal) * 56; }
+// tree.rs:264-272: storage bytes per (action, hand) value pair (8 f32, 4
+// 16-bit) plus the arena's 56-byte nodes
+static uint64_t memory_bytes(const TreeCounts& t, bool compressed = false) {
+  return t.cells * (compressed ? 4 : 8) + (t.player + t.chance + t.terminal) * 56;
+}
+// lib.rs:346-352: Compressed, or Auto when the f32 tree would exceed 1 GB
+static bool compressed_for(const pts_config_c& c, const TreeCounts& t) {
+  return c.storage_mode == 1 || (c.storage_mode == 2 && memory_bytes(t) > (1ull << 30));
+}
 
 // ------------------------------------------------------------------ engine
 extern "C" {
@@ -291,7 +300,7 @@ pts_status pts_solver_estimate(const pts_config_c* c, pts_stats_c* out) {  // li
   const Iso iso = make_iso(hands, c->isomorphism != 0);
   spec.iso = &iso;
   TreeCounts t = count_tree(spec);
-  *out = {t.player, t.chance, t.terminal, t.cells, memory_bytes(t)};
+  *out = {t.player, t.chance, t.terminal, t.cells, memory_bytes(t, compressed_for(*c, t))};
   return PTS_OK;
 }
 
@@ -299,16 +308,17 @@ static PlanInput plan_input(pts_solver* s, std::vector<uint8_t>& kind, std::vect
                             std::vector<uint32_t>& fc, std::vector<int32_t>& inv, std::vector<uint64_t>& off,
                             std::vector<float>& rake, std::vector<float>& fac, std::vector<int8_t>& card,
                             std::vector<int32_t>& boards, std::vector<uint32_t>* same, std::vector<uint32_t>& isf,
-                            std::vector<uint16_t>& isc, std::vector<uint16_t>* perm) {
+                            std::vector<uint16_t>& isc, std::vector<uint16_t>* perm, std::vector<uint32_t>& ps) {
   size_t N = s->tree.nodes.size();
   kind.resize(N); street.resize(N); nc.resize(N); fc.resize(N); inv.resize(2 * N); off.resize(N);
-  rake.resize(N); fac.resize(N); card.resize(N); isf.resize(N); isc.resize(N);
+  rake.resize(N); fac.resize(N); card.resize(N); isf.resize(N); isc.resize(N); ps.assign(N, 0);
   for (size_t i = 0; i < N; i++) {
     const Node& n = s->tree.nodes[i];
     kind[i] = n.kind; street[i] = n.street; nc[i] = n.num_children; fc[i] = n.first_child;
     inv[2 * i] = n.invest[0]; inv[2 * i + 1] = n.invest[1]; off[i] = n.offset;
     rake[i] = n.rake; fac[i] = n.chance_factor; card[i] = n.card;
     isf[i] = n.iso_first; isc[i] = n.iso_cnt;
+    if (is_player(n.kind)) ps[i] = n.pslot;
   }
   for (auto& b : s->tree.boards) boards.insert(boards.end(), b.begin(), b.end());
   PlanInput in;
@@ -327,6 +337,8 @@ static PlanInput plan_input(pts_solver* s, std::vector<uint8_t>& kind, std::vect
   in.half_pot = (double)s->cfg.pot / 2.0;
   in.num_combinations = s->hands.num_combinations;
   in.storage_len = s->tree.storage_len;
+  in.pslot = ps.data();
+  in.num_pslots = s->tree.player_nodes;
   in.batch_river = true;
   in.root_stack[0] = s->cfg.stack[0];
   in.root_stack[1] = s->cfg.stack[1];
@@ -358,7 +370,8 @@ static std::string host_build(const pts_config_c* c, pts_solver* s, Plan& plan, 
   s->iso = make_iso(s->hands, c->isomorphism != 0);  // iso.rs:245-273
   s->spec.iso = &s->iso;
   s->counts = count_tree(s->spec);
-  uint64_t need = memory_bytes(s->counts);
+  s->compressed = compressed_for(*c, s->counts);
+  uint64_t need = memory_bytes(s->counts, s->compressed);
   if (need > c->max_memory_bytes) {
     char m[160];
     std::snprintf(m, sizeof m, "game tree needs ~%.2f GB but the limit is %.2f GB", (double)need / 1e9,
@@ -373,9 +386,9 @@ static std::string host_build(const pts_config_c* c, pts_solver* s, Plan& plan, 
   std::vector<uint64_t> off;
   std::vector<float> rake, fac;
   std::vector<int8_t> card;
-  std::vector<uint32_t> same[2], isf;
+  std::vector<uint32_t> same[2], isf, ps;
   std::vector<uint16_t> isc, perm[2];
-  PlanInput in = plan_input(s, kind, street, nc, fc, inv, off, rake, fac, card, boards, same, isf, isc, perm);
+  PlanInput in = plan_input(s, kind, street, nc, fc, inv, off, rake, fac, card, boards, same, isf, isc, perm, ps);
   return build_plan(in, plan);
 }
 
@@ -393,7 +406,7 @@ pts_status pts_solver_new(const pts_config_c* c, int device, int precision, uint
     *out = s.release();
     return PTS_OK;
   }
-  Status e = Engine::create(device, std::move(plan), nullptr, nullptr, precision, scratch_budget, s->engine);
+  Status e = Engine::create(device, std::move(plan), nullptr, nullptr, precision, scratch_budget, s->engine, s->compressed);
   if (e != kOk) return st(e);
   *out = s.release();
   return PTS_OK;
@@ -403,8 +416,10 @@ void pts_solver_free(pts_solver* s) { delete s; }
 
 void pts_solver_stats(pts_solver* s, pts_stats_c* out) {
   const TreeCounts& t = s->counts;
-  *out = {t.player, t.chance, t.terminal, t.cells, memory_bytes(t)};
+  *out = {t.player, t.chance, t.terminal, t.cells, memory_bytes(t, s->compressed)};
 }
+
+int32_t pts_solver_is_compressed(pts_solver* s) { return s->compressed ? 1 : 0; }
 
 static pts_status need_engine(pts_solver* s) {
   if (!s->engine) {
@@ -494,8 +509,43 @@ pts_status pts_solver_plan_groups(pts_solver* s, uint64_t scratch_budget, int f6
   return PTS_OK;
 }
 int32_t pts_solver_num_hands(pts_solver* s, int p) { return (int32_t)s->hands.n(p); }
-pts_status pts_solver_get_storage(pts_solver* s, float* r, float* m) { NEED_ENGINE(s); return st(s->engine->download(r, m)); }
-pts_status pts_solver_set_storage(pts_solver* s, const float* r, const float* m) { NEED_ENGINE(s); return st(s->engine->upload(r, m)); }
+// f32 storage as is; 16-bit storage decoded (storage.rs:209-246)
+pts_status pts_solver_get_storage(pts_solver* s, float* r, float* m) {
+  NEED_ENGINE(s);
+  if (!s->compressed) return st(s->engine->download(r, m));
+  const size_t len = s->tree.storage_len, k = s->tree.player_nodes;
+  std::vector<int16_t> rq(len);
+  std::vector<uint16_t> sq(len);
+  std::vector<float> rs(k), ss(k);
+  Status e = s->engine->download_q(rq.data(), sq.data(), rs.data(), ss.data());
+  if (e != kOk) return st(e);
+  for (const Node& n : s->tree.nodes) {
+    if (!is_player(n.kind)) continue;
+    const size_t cells = (size_t)n.num_children * s->hands.n(n.kind);
+    const float rstep = rs[n.pslot] / 32767.0f, sstep = ss[n.pslot] / 65535.0f;
+    for (size_t i = n.offset; i < n.offset + cells; i++) {
+      if (r) r[i] = (float)rq[i] * rstep;
+      if (m) m[i] = (float)sq[i] * sstep;
+    }
+  }
+  return PTS_OK;
+}
+pts_status pts_solver_set_storage(pts_solver* s, const float* r, const float* m) {
+  NEED_ENGINE(s);
+  if (s->compressed) return bad("invalid input: 16-bit storage is set through pts_solver_set_storage_q");
+  return st(s->engine->upload(r, m));
+}
+pts_status pts_solver_get_storage_q(pts_solver* s, int16_t* r, uint16_t* m, float* regret_scale, float* strat_scale) {
+  NEED_ENGINE(s);
+  if (!s->compressed) return bad("invalid input: the solver has f32 storage");
+  return st(s->engine->download_q(r, m, regret_scale, strat_scale));
+}
+pts_status pts_solver_set_storage_q(pts_solver* s, const int16_t* r, const uint16_t* m, const float* regret_scale,
+                                    const float* strat_scale) {
+  NEED_ENGINE(s);
+  if (!s->compressed) return bad("invalid input: the solver has f32 storage");
+  return st(s->engine->upload_q(r, m, regret_scale, strat_scale));
+}
 pts_status pts_solver_profile_iteration(pts_solver* s, double ms[4], uint32_t launches[4], double* wall_ms) {
   NEED_ENGINE(s);
   s->iterations += 1;
@@ -553,7 +603,7 @@ static pts_status walk(pts_solver* s, const pts_step_c* path, int32_t np, uint32
       int q = node.kind;
       size_t n = s->hands.n(q);
       std::vector<float> strat((size_t)node.num_children * n);
-      Status e = s->engine->read_sums(node.offset, strat.size(), strat.data());
+      Status e = s->engine->read_sums(node.offset, strat.size(), strat.data(), node.pslot);
       if (e != kOk) return st(e);
       normalize(strat.data(), node.num_children, n);
       for (size_t i = 0; i < n; i++) reach[q][i] *= strat[(size_t)ai * n + i];
@@ -610,7 +660,7 @@ pts_status pts_solver_query(pts_solver* s, const pts_step_c* path, int32_t npath
   Status e = s->engine->subtree_values(id, q, reach[o], cv);
   if (e != kOk) return st(e);
   std::vector<float> strat(A * nh);
-  if ((e = s->engine->read_sums(node.offset, strat.size(), strat.data())) != kOk) return st(e);
+  if ((e = s->engine->read_sums(node.offset, strat.size(), strat.data(), node.pslot)) != kOk) return st(e);
   normalize(strat.data(), A, nh);
   double half_pot = (double)s->cfg.pot / 2.0, norm = 1.0 / s->hands.num_combinations;
   double offset = half_pot + (double)node.invest[q];
@@ -711,6 +761,7 @@ pts_status pts_batch_new(const pts_config_c* cfgs, uint32_t num_spots, int devic
     std::unique_ptr<pts_solver> s(new pts_solver());
     std::string err = host_build(&cfgs[k], s.get(), plans[k]);
     if (!err.empty()) return bad("spot " + std::to_string(k) + ": " + err);
+    if (s->compressed) return bad("spot " + std::to_string(k) + ": batched spots need f32 storage");
     b->store_off.push_back(off);
     off += s->tree.storage_len;
     b->spots.push_back(std::move(s));
@@ -952,6 +1003,7 @@ pts_status pts_dl_new(const pts_config_c* cfg, int device, int precision, pts_ne
   Plan plan;
   std::string err = host_build(cfg, s.get(), plan, true);
   if (!err.empty()) return bad(err);
+  if (s->compressed) return bad("invalid input: depth-limited solves need f32 storage");
   if (plan.leaves.empty()) return bad("invalid input: the depth-limited tree has no leaves");
   if (device < 0) {  // host only: the cut tree and its plan
     s->host_plan = std::move(plan);

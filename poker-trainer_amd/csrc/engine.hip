@@ -110,6 +110,42 @@ struct Bias {
   __device__ __forceinline__ uint32_t v(uint32_t x) const { return x < v0 ? x : x - vb; }
 };
 
+// Regret and strategy-sum storage (crates/solver/src/storage.rs:87-106):
+// f32 cells, or 16-bit cells (i16 regrets, u16 sums) with one f32 scale per
+// player node, a stored q standing for q * (scale / MAX).
+struct Store {
+  float* r;      // f32 regrets (null with 16-bit storage)
+  float* s;      // f32 strategy sums
+  int16_t* rq;   // 16-bit regrets
+  uint16_t* sq;  // 16-bit strategy sums
+  float* rsc;    // per player node: the regrets' scale
+  float* ssc;    // per player node: the strategy sums' scale
+  int32_t* rnx;  // per player node: the regrets' new scale (float bits) while a CFR traversal runs, else -1
+};
+constexpr float kRegMax = 32767.0f, kSumMax = 65535.0f;  // i16::MAX, u16::MAX as f32
+// Rust's saturating `f32 as i16` / `as u16`: truncation toward zero, NaN to 0
+__device__ __forceinline__ int16_t sat_i16(float y) {
+  if (!(y == y)) return 0;
+  return (int16_t)(int32_t)fminf(fmaxf(y, -32768.0f), 32767.0f);
+}
+__device__ __forceinline__ uint16_t sat_u16(float y) {
+  if (!(y == y)) return 0;
+  return (uint16_t)(uint32_t)fminf(fmaxf(y, 0.0f), 65535.0f);
+}
+// storage.rs:274-281: y = v * (MAX / max), rounded half away from zero
+__device__ __forceinline__ int16_t enc_regret(float v, float inv) {
+  const float y = v * inv;
+  return sat_i16(y + copysignf(0.5f, y));
+}
+// storage.rs:312-317: max(v, 0) * (MAX / max) + 0.5, truncated
+__device__ __forceinline__ uint16_t enc_sum(float v, float inv) { return sat_u16(fmaxf(v, 0.0f) * inv + 0.5f); }
+// the largest of the wave's 64 values (every lane gets it)
+__device__ __forceinline__ float wave_max(float x) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) x = fmaxf(x, __shfl_xor(x, o));
+  return x;
+}
+
 // Per-player fields of a record held in registers: selects, not a runtime
 // index (which would put the record in scratch memory).
 __device__ __forceinline__ uint32_t pick(const uint32_t (&a)[2], int p) { return p ? a[1] : a[0]; }
@@ -173,14 +209,14 @@ __global__ void collect_kernel(DPlan P, int p, const V* __restrict__ vals, V* __
 // members in breadth-first order. CFR mode (V = float): regret matching and
 // the strategy-sum update (cfr.rs:156-207,389-419; river.rs:116-160).
 // Otherwise the normalized average strategy (cfr.rs:124-154).
-template <typename V, int MODE>
+template <typename V, int MODE, bool Q>
 __global__ void __launch_bounds__(1024) down_kernel(DPlan P, int p, uint32_t seg0, Span span, Bias B, V* __restrict__ reach,
-                                                      uint32_t* __restrict__ alive, const float* __restrict__ regrets,
-                                                      float* __restrict__ sums, const IterParams* __restrict__ prm,
+                                                      uint32_t* __restrict__ alive, Store sto, const IterParams* __restrict__ prm,
                                                       uint32_t rowlen) {
   extern __shared__ uint4 smem_down[];
   __shared__ uint32_t red;
   __shared__ uint32_t red3[3];  // per-node ORs, used in turn (each zeroed two nodes ahead)
+  __shared__ int32_t mx3[3];    // 16-bit storage: per-node maxima of the new sums (float bits), likewise
   const uint32_t seg = seg0 + blockIdx.x;
   const int o = p ^ 1;
   const uint32_t moff = P.seg_mem_off[seg], mcnt = P.seg_mem_cnt[seg];
@@ -191,7 +227,10 @@ __global__ void __launch_bounds__(1024) down_kernel(DPlan P, int p, uint32_t seg
   MemRec* rec = (MemRec*)smem_down;
   uint8_t* m_alive = (uint8_t*)(rec + P.max_members);
   stage_records(P.mrec + moff, mcnt, rec);
-  if (tid < 3) red3[tid] = 0;
+  if (tid < 3) {
+    red3[tid] = 0;
+    mx3[tid] = 0;
+  }
   __syncthreads();
   const uint32_t lane = tid & 63;
   const uint32_t sp_raw = P.seg_pslot[(size_t)p * P.num_segs + seg];
@@ -257,32 +296,53 @@ __global__ void __launch_bounds__(1024) down_kernel(DPlan P, int p, uint32_t seg
     V* crow = reach + (size_t)cs * rowlen;
     const uint64_t off = r.offset;
     uint32_t bits = 0;
+    // 16-bit storage: the sums' decoding step (storage.rs:296-300) and the
+    // largest |new sum| of this thread's cells
+    const float sstep = Q ? sto.ssc[r.pslot] / kSumMax : 0.0f;
+    float lmax = 0.0f;
+    // cfr.rs:389-414: regret matching (16-bit regrets undecoded, the scale
+    // cancels: storage.rs:221-234), then the sums x * g + s (* reach)
+    auto matched = [&](uint32_t h, float (&sv)[MAXA]) {
+      float rg[MAXA];
+#pragma unroll
+      for (int a = 0; a < MAXA; a++)
+        if (a < (int)na) rg[a] = Q ? (float)sto.rq[off + (size_t)a * nq + h] : sto.r[off + (size_t)a * nq + h];
+      float d = pos0(rg[0]);
+#pragma unroll
+      for (int a = 1; a < MAXA; a++)
+        if (a < (int)na) d += pos0(rg[a]);
+      const float inv = recip(d), uni = 1.0f / (float)na;
+#pragma unroll
+      for (int a = 0; a < MAXA; a++)
+        if (a < (int)na) sv[a] = inv == 0.0f ? uni : pos0(rg[a]) * inv;
+    };
+    auto new_sum = [&](uint32_t h, int a, float sv, float rr) {
+      const size_t i = off + (size_t)a * nq + h;
+      const float x = Q ? (float)sto.sq[i] * sstep : sto.s[i];
+      return weighted ? x * g + sv * rr : x * g + sv;
+    };
     for (uint32_t h = tid; h < nq; h += blockDim.x) {
       const V rr = rrow[h];
       V sa[MAXA];
       if (MODE == kCfr) {
-        float rg[MAXA];
-#pragma unroll
-        for (int a = 0; a < MAXA; a++)
-          if (a < (int)na) rg[a] = regrets[off + (size_t)a * nq + h];
-        float d = pos0(rg[0]);
-#pragma unroll
-        for (int a = 1; a < MAXA; a++)
-          if (a < (int)na) d += pos0(rg[a]);
-        const float inv = recip(d), uni = 1.0f / (float)na;
+        float sv[MAXA];
+        matched(h, sv);
 #pragma unroll
         for (int a = 0; a < MAXA; a++)
           if (a < (int)na) {
-            const float sv = inv == 0.0f ? uni : pos0(rg[a]) * inv;
-            float* x = sums + off + (size_t)a * nq + h;
-            *x = weighted ? *x * g + sv * (float)rr : *x * g + sv;
-            sa[a] = (V)sv;
+            const float y = new_sum(h, a, sv[a], (float)rr);
+            if (Q) lmax = fmaxf(lmax, fabsf(y));
+            else sto.s[off + (size_t)a * nq + h] = y;
+            sa[a] = (V)sv[a];
           }
       } else {
         V sv[MAXA];
 #pragma unroll
         for (int a = 0; a < MAXA; a++)
-          if (a < (int)na) sv[a] = (V)sums[off + (size_t)a * nq + h];
+          if (a < (int)na) {
+            const size_t i = off + (size_t)a * nq + h;
+            sv[a] = Q ? (V)((float)sto.sq[i] * sstep) : (V)sto.s[i];
+          }
         V d = sv[0];
 #pragma unroll
         for (int a = 1; a < MAXA; a++)
@@ -307,12 +367,38 @@ __global__ void __launch_bounds__(1024) down_kernel(DPlan P, int p, uint32_t seg
     for (int a = 0; a < MAXA; a++)
       if (a < (int)na && __ballot((bits >> a) & 1u)) wb |= 1u << a;
     if (lane == 0 && wb) atomicOr(&red3[ri], wb);
+    if (Q && MODE == kCfr) {
+      const float wm = wave_max(lmax);
+      if (lane == 0) atomicMax(&mx3[ri], __float_as_int(wm));
+    }
     __syncthreads();
     const uint32_t all = red3[ri];
-    if (tid == 0) red3[ri == 0 ? 2 : ri - 1] = 0;  // the buffer two nodes ahead
+    const float mx = Q && MODE == kCfr ? __int_as_float(mx3[ri]) : 0.0f;
+    if (tid == 0) {  // the buffers two nodes ahead
+      red3[ri == 0 ? 2 : ri - 1] = 0;
+      if (Q) mx3[ri == 0 ? 2 : ri - 1] = 0;
+    }
     ri = ri == 2 ? 0 : ri + 1;
     if (lane < na) m_alive[cl + lane] = (uint8_t)((all >> lane) & 1u);
     if (tid < na) alive[cs + tid] = (all >> tid) & 1u;
+    if (Q && MODE == kCfr) {
+      // the same new sums again, encoded with the block's largest as the new
+      // scale (storage.rs:306-320); every thread decoded with the old scale
+      // before the barrier above
+      const float inv = mx > 0.0f ? kSumMax / mx : 0.0f;
+      for (uint32_t h = tid; h < nq; h += blockDim.x) {
+        const float rr = (float)rrow[h];
+        float sv[MAXA];
+        matched(h, sv);
+#pragma unroll
+        for (int a = 0; a < MAXA; a++)
+          if (a < (int)na) {
+            const float y = new_sum(h, a, sv[a], rr);
+            sto.sq[off + (size_t)a * nq + h] = mx > 0.0f ? enc_sum(y, inv) : (uint16_t)0;
+          }
+      }
+      if (tid == 0) sto.ssc[r.pslot] = mx;
+    }
   }
   // every member's alive flag, for the up pass
   for (uint32_t m = tid; m < mcnt; m += blockDim.x) P.malive[moff + m] = m_alive[rec[m].src[p]];
@@ -978,6 +1064,7 @@ struct MemberVec {
 };
 
 __device__ __forceinline__ uint32_t lane_u32(uint32_t x, uint32_t m) { return (uint32_t)__builtin_amdgcn_readlane((int)x, (int)m); }
+__device__ __forceinline__ float lane_f32(float x, uint32_t m) { return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(x), (int)m)); }
 __device__ __forceinline__ uint64_t lane_u64(uint64_t x, uint32_t m) {
   return (uint64_t)lane_u32((uint32_t)x, m) | (uint64_t)lane_u32((uint32_t)(x >> 32), m) << 32;
 }
@@ -995,15 +1082,22 @@ constexpr uint32_t kFLive = 8, kFSpan = 16, kFRoot = 32;
 // up_kernel below. CAP = 32 (or 48) runs over a range of segments and leaves
 // those above CAP members; CAP = 48 and 64 also run over the plan's lists of
 // segments of 33 to 48 and 49 to 64 members (list[seg0 + blockIdx.x]).
-template <int MODE, int CAP>
+//
+// QP, the storage pass: 0 f32 storage; with 16-bit storage 3 reads (average
+// mode), and a CFR walk runs twice, 1 for the values and every own node's
+// largest |new regret| (atomically maxed over the hand tiles into rnx), 2 for
+// the same regrets encoded with that maximum as the node's new scale
+// (storage.rs:253-289: the scale is the whole block's maximum).
+template <int MODE, int CAP, int QP>
 __global__ void __launch_bounds__(UP_T, CAP == 64 ? 2 : 4) up_reg_kernel(DPlan P, int p, uint32_t seg0, Span span, Bias B, float* __restrict__ vals,
-                                                      uint32_t rowlen, const uint32_t* __restrict__ alive,
-                                                      float* __restrict__ regrets, const float* __restrict__ sums,
+                                                      uint32_t rowlen, const uint32_t* __restrict__ alive, Store sto,
                                                       const IterParams* __restrict__ prm, float* __restrict__ qout,
                                                       const uint32_t* __restrict__ list) {
   typedef float V;
   constexpr bool ROWS = MODE != kBest;
+  constexpr bool Q = QP != 0;
   __shared__ uint64_t src_lds[CAP];
+  __shared__ float stp_lds[Q ? CAP : 1];  // 16-bit storage: the decoding step of each storage row, by child
   const uint32_t seg = list != nullptr ? list[seg0 + blockIdx.x] : seg0 + blockIdx.x;
   const uint4 head = P.seg_head[seg];
   const uint32_t moff = head.x, mcnt = head.y, root = head.z;
@@ -1013,7 +1107,7 @@ __global__ void __launch_bounds__(UP_T, CAP == 64 ? 2 : 4) up_reg_kernel(DPlan P
   const uint32_t h = blockIdx.y * UP_T + lane;
   const bool act = h < np;
   const uint32_t hh = act ? h : 0;  // idle lanes load row element 0 and store nothing
-  uint32_t f = 0, vslot = 0;
+  uint32_t f = 0, vslot = 0, psr = 0;
   uint64_t off = 0;
   if (lane < mcnt) {
     const MemRec& r = P.mrec[moff + lane];
@@ -1022,15 +1116,26 @@ __global__ void __launch_bounds__(UP_T, CAP == 64 ? 2 : 4) up_reg_kernel(DPlan P
         (uint32_t)r.na << 8 | (uint32_t)r.clocal << 16;
     vslot = r.vslot;
     off = r.offset;
+    psr = r.pslot;
     if (ROWS) src_lds[lane] = kNoRow;
   }
   uint64_t src = kNoRow;
+  float stp = 0.0f;
   if (ROWS) {
     __syncthreads();
-    if (lane < mcnt && (f & (kFLive | kFSpan)) == (kFLive | kFSpan) && (int)(f & 7) == p)
-      for (uint32_t a = 0; a < ((f >> 8) & 0xFF); a++) src_lds[(f >> 16) + a] = off + (uint64_t)a * np;
+    if (lane < mcnt && (f & (kFLive | kFSpan)) == (kFLive | kFSpan) && (int)(f & 7) == p) {
+      float step = 0.0f;
+      if (Q) step = MODE == kCfr ? sto.rsc[psr] / kRegMax : sto.ssc[psr] / kSumMax;
+      for (uint32_t a = 0; a < ((f >> 8) & 0xFF); a++) {
+        src_lds[(f >> 16) + a] = off + (uint64_t)a * np;
+        if (Q) stp_lds[(f >> 16) + a] = step;
+      }
+    }
     __syncthreads();
-    if (lane < mcnt) src = src_lds[lane];
+    if (lane < mcnt) {
+      src = src_lds[lane];
+      if (Q) stp = stp_lds[lane];
+    }
   }
   typedef MemberVec<CAP> MV;
   typename MV::lo l0 = typename MV::lo(0.0f), s0 = typename MV::lo(0.0f);  // member values; storage rows, by child
@@ -1039,7 +1144,7 @@ __global__ void __launch_bounds__(UP_T, CAP == 64 ? 2 : 4) up_reg_kernel(DPlan P
 #define LV_SET(i, x) MV::set(l0, l1, (i), (x))
 #define SL_GET(i) MV::get(s0, s1, (i))
 #define SL_SET(i, x) MV::set(s0, s1, (i), (x))
-  const float* store = MODE == kCfr ? regrets : sums;
+  const float* store = MODE == kCfr ? sto.r : sto.s;
 #pragma unroll
   for (uint32_t m = 0; m < (uint32_t)CAP; m++) {  // unrolled: constant indices, all loads in flight
     if (m < mcnt) {
@@ -1048,7 +1153,10 @@ __global__ void __launch_bounds__(UP_T, CAP == 64 ? 2 : 4) up_reg_kernel(DPlan P
         LV_SET(m, vals[(size_t)B.v(lane_u32(vslot, m)) * rowlen + hh]);
       if (ROWS) {
         const uint64_t sm = lane_u64(src, m);
-        if (sm != kNoRow) SL_SET(m, store[sm + hh]);
+        if (sm != kNoRow) {
+          if (Q) SL_SET(m, (MODE == kCfr ? (float)sto.rq[sm + hh] : (float)sto.sq[sm + hh]) * lane_f32(stp, m));
+          else SL_SET(m, store[sm + hh]);
+        }
       }
     }
   }
@@ -1087,13 +1195,29 @@ __global__ void __launch_bounds__(UP_T, CAP == 64 ? 2 : 4) up_reg_kernel(DPlan P
             for (uint32_t a = 0; a < na; a++) sm += LV_GET(cl + a);
             outv = sm * (1.0f / (float)na);
           }
-          if (act)
+          // cfr.rs:374-379; the regret row of action a is the storage row of
+          // child cl + a (a macro, not a lambda: the member vectors stay
+          // plain values)
+#define UPDATED(a) (SL_GET(cl + (a)) * (SL_GET(cl + (a)) > 0.0f ? pos : neg) + LV_GET(cl + (a)) - outv)
+          if (QP == 0) {
+            if (act)
+              for (uint32_t a = 0; a < na; a++) sto.r[lane_u64(src, cl + a) + h] = UPDATED(a);
+          } else if (QP == 1) {
+            float lm = 0.0f;
             for (uint32_t a = 0; a < na; a++) {
-              const float x = SL_GET(cl + a);
-              const float disc = x > 0.0f ? pos : neg;
-              // the regret row of action a is the storage row of child cl + a
-              regrets[lane_u64(src, cl + a) + h] = x * disc + LV_GET(cl + a) - outv;
+              const float y = UPDATED(a);
+              if (act) lm = fmaxf(lm, fabsf(y));
             }
+            lm = wave_max(lm);
+            if (lane == 0) atomicMax(&sto.rnx[lane_u32(psr, m)], __float_as_int(lm));
+          } else {
+            const float mx = __int_as_float(sto.rnx[lane_u32(psr, m)]);
+            const float inv = mx > 0.0f ? kRegMax / mx : 0.0f;
+            if (act)
+              for (uint32_t a = 0; a < na; a++)
+                sto.rq[lane_u64(src, cl + a) + h] = mx > 0.0f ? enc_regret(UPDATED(a), inv) : (int16_t)0;
+          }
+#undef UPDATED
           v = outv;
         } else {
           float d = SL_GET(cl);
@@ -1110,7 +1234,7 @@ __global__ void __launch_bounds__(UP_T, CAP == 64 ? 2 : 4) up_reg_kernel(DPlan P
     LV_SET(m, v);
   }
   const V top = LV_GET(0);
-  if (act) vals[(size_t)B.v(lane_u32(vslot, 0)) * rowlen + h] = top;
+  if (act && QP != 2) vals[(size_t)B.v(lane_u32(vslot, 0)) * rowlen + h] = top;
 #undef LV_GET
 #undef LV_SET
 #undef SL_GET
@@ -1126,13 +1250,13 @@ __global__ void __launch_bounds__(UP_T, CAP == 64 ? 2 : 4) up_reg_kernel(DPlan P
 // an exact zero (cfr.rs:421-432, river.rs:264-271); chance nodes: f64 sum
 // over dealt cards, zero for own hands holding the card (cfr.rs:457-478).
 // Every segment root's row is written (zeros when dead), so a chance node
-// reads its children's rows unconditionally.
-template <typename V, int MODE>
+// reads its children's rows unconditionally. QP as in up_reg_kernel.
+template <typename V, int MODE, int QP>
 __global__ void __launch_bounds__(UP_T) up_kernel(DPlan P, int p, uint32_t seg0, Span span, Bias B, V* __restrict__ vals,
-                                                  uint32_t rowlen, const uint32_t* __restrict__ alive,
-                                                  float* __restrict__ regrets, const float* __restrict__ sums,
+                                                  uint32_t rowlen, const uint32_t* __restrict__ alive, Store sto,
                                                   const IterParams* __restrict__ prm, V* __restrict__ qout,
                                                   const uint32_t* __restrict__ list) {
+  constexpr bool Q = QP != 0;
   extern __shared__ uint4 smem_up[];
   const uint32_t seg = list != nullptr ? list[seg0 + blockIdx.x] : seg0 + blockIdx.x;
   const uint32_t moff = P.seg_mem_off[seg], mcnt = P.seg_mem_cnt[seg];
@@ -1148,7 +1272,11 @@ __global__ void __launch_bounds__(UP_T) up_kernel(DPlan P, int p, uint32_t seg0,
   __syncthreads();
   for (uint32_t m = tid; m < mcnt; m += UP_T) live[m] = alive[B.s(rec[m].slot[p])] != 0;
   __syncthreads();
-  if (h >= np) return;
+  // the max pass keeps every lane for its wave reductions: lanes past the
+  // hands read hand 0 and store nothing
+  const bool act = h < np;
+  if (QP != 1 && !act) return;
+  const uint32_t hc = act ? h : 0;
   // leaves first: terminal and chance values, 8 rows in flight per lane
   for (uint32_t m0 = 0; m0 < mcnt; m0 += 8) {
     V x[8];
@@ -1157,7 +1285,7 @@ __global__ void __launch_bounds__(UP_T) up_kernel(DPlan P, int p, uint32_t seg0,
       const uint32_t m = m0 + k;
       x[k] = V(0);
       if (m < mcnt && rec[m].kind >= kChance && live[m] && span.has(rec[m].node))
-        x[k] = vals[(size_t)B.v(rec[m].vslot) * rowlen + h];
+        x[k] = vals[(size_t)B.v(rec[m].vslot) * rowlen + hc];
     }
 #pragma unroll
     for (int k = 0; k < 8; k++) {
@@ -1175,7 +1303,7 @@ __global__ void __launch_bounds__(UP_T) up_kernel(DPlan P, int p, uint32_t seg0,
     if (!span.has(r.node)) continue;
     const uint8_t k = r.kind;
     const uint32_t na = r.na, cl = r.clocal;
-    if (qout != nullptr && r.node == span.root) {
+    if (QP != 1 && qout != nullptr && r.node == span.root) {
       // query: the values of the decision node's children (lib.rs:854-863)
       for (uint32_t a = 0; a < na; a++) qout[(size_t)a * np + h] = lv[(size_t)(cl + a) * UP_T + tid];
       return;
@@ -1191,9 +1319,10 @@ __global__ void __launch_bounds__(UP_T) up_kernel(DPlan P, int p, uint32_t seg0,
         const uint64_t off = r.offset;
         if (MODE == kCfr) {
           float rg[MAXA];
+          const float rstep = Q ? sto.rsc[r.pslot] / kRegMax : 0.0f;
 #pragma unroll
           for (int a = 0; a < MAXA; a++)
-            if (a < (int)na) rg[a] = regrets[off + (size_t)a * np + h];
+            if (a < (int)na) rg[a] = Q ? (float)sto.rq[off + (size_t)a * np + hc] * rstep : sto.r[off + (size_t)a * np + hc];
           float d = pos0(rg[0]);
 #pragma unroll
           for (int a = 1; a < MAXA; a++)
@@ -1211,13 +1340,32 @@ __global__ void __launch_bounds__(UP_T) up_kernel(DPlan P, int p, uint32_t seg0,
               if (a < (int)na) sm += (float)cv[a];
             outv = sm * (1.0f / (float)na);
           }
+          float y[MAXA];
 #pragma unroll
           for (int a = 0; a < MAXA; a++)
             if (a < (int)na) {
               const float x = rg[a];
               const float disc = x > 0.0f ? pos : neg;
-              regrets[off + (size_t)a * np + h] = x * disc + (float)cv[a] - outv;
+              y[a] = x * disc + (float)cv[a] - outv;
             }
+          if (QP == 0) {
+#pragma unroll
+            for (int a = 0; a < MAXA; a++)
+              if (a < (int)na) sto.r[off + (size_t)a * np + h] = y[a];
+          } else if (QP == 1) {
+            float lm = 0.0f;
+#pragma unroll
+            for (int a = 0; a < MAXA; a++)
+              if (a < (int)na && act) lm = fmaxf(lm, fabsf(y[a]));
+            lm = wave_max(lm);
+            if (tid == 0) atomicMax(&sto.rnx[r.pslot], __float_as_int(lm));
+          } else {
+            const float mx = __int_as_float(sto.rnx[r.pslot]);
+            const float inv = mx > 0.0f ? kRegMax / mx : 0.0f;
+#pragma unroll
+            for (int a = 0; a < MAXA; a++)
+              if (a < (int)na) sto.rq[off + (size_t)a * np + h] = mx > 0.0f ? enc_regret(y[a], inv) : (int16_t)0;
+          }
           v = (V)outv;
         } else if (MODE == kBest) {
           V b = cv[0];
@@ -1227,9 +1375,10 @@ __global__ void __launch_bounds__(UP_T) up_kernel(DPlan P, int p, uint32_t seg0,
           v = b;
         } else {
           V sv[MAXA];
+          const float sstep = Q ? sto.ssc[r.pslot] / kSumMax : 0.0f;
 #pragma unroll
           for (int a = 0; a < MAXA; a++)
-            if (a < (int)na) sv[a] = (V)sums[off + (size_t)a * np + h];
+            if (a < (int)na) sv[a] = Q ? (V)((float)sto.sq[off + (size_t)a * np + h] * sstep) : (V)sto.s[off + (size_t)a * np + h];
           V d = sv[0];
 #pragma unroll
           for (int a = 1; a < MAXA; a++)
@@ -1247,7 +1396,21 @@ __global__ void __launch_bounds__(UP_T) up_kernel(DPlan P, int p, uint32_t seg0,
     }
     lv[(size_t)m * UP_T + tid] = v;
   }
-  vals[(size_t)B.v(rec[0].vslot) * rowlen + h] = lv[tid];
+  if (act && QP != 2) vals[(size_t)B.v(rec[0].vslot) * rowlen + h] = lv[tid];
+}
+
+// After a CFR traversal with 16-bit storage: every own node's new regret
+// scale (its largest |new regret|) replaces the old; nodes the traversal
+// left alone (dead) keep theirs (storage.rs:253-289 runs only for updated
+// nodes).
+__global__ void commit_scales_kernel(Store sto, uint32_t n) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int32_t b = sto.rnx[i];
+  if (b >= 0) {
+    sto.rsc[i] = __int_as_float(b);
+    sto.rnx[i] = -1;
+  }
 }
 
 // ------------------------------------------------------------------ leaves
@@ -1466,13 +1629,20 @@ static size_t put(std::vector<unsigned char>& blob, const std::vector<T>& v) {
   return off;
 }
 
+Store Engine::store() const { return Store{regrets_, sums_, rq_, sq_, rsc_, ssc_, rnx_}; }
+
 Status Engine::create(int device, Plan&& plan, const float* regrets, const float* sums, int precision,
-                      uint64_t scratch_budget, std::unique_ptr<Engine>& out) {
+                      uint64_t scratch_budget, std::unique_ptr<Engine>& out, bool q16) {
   std::unique_ptr<Engine> e(new Engine());
   if (precision != kFast && precision != kExact) {
     set_error("invalid input: precision");
     return kInvalidInput;
   }
+  if (q16 && (plan.num_pslots == 0 || regrets || sums)) {
+    set_error("invalid input: 16-bit storage needs player slots in the plan and starts from zeros");
+    return kInvalidInput;
+  }
+  e->q16_ = q16;
   e->precision_ = precision;
   e->plan_ = std::move(plan);
   e->device_ = device;
@@ -1623,9 +1793,21 @@ Status Engine::create(int device, Plan&& plan, const float* regrets, const float
   d.num_m48 = (uint32_t)e->dp_->m48.size();
 #undef AT
   // storage, then the scratch rows: chunk groups sized to what is left
-  DeviceBuf br, bs, bre, bp, bm;
-  if ((st = e->alloc(br, P.storage_len * sizeof(float))) != kOk) return st;
-  if ((st = e->alloc(bs, P.storage_len * sizeof(float))) != kOk) return st;
+  DeviceBuf br, bs, bre, bp, bm, bsc;
+  const size_t cell = q16 ? 2 : sizeof(float);
+  if ((st = e->alloc(br, P.storage_len * cell)) != kOk) return st;
+  if ((st = e->alloc(bs, P.storage_len * cell)) != kOk) return st;
+  if (q16) {
+    if ((st = e->alloc(bsc, (size_t)P.num_pslots * 12)) != kOk) return st;
+    e->rq_ = (int16_t*)br.p;
+    e->sq_ = (uint16_t*)bs.p;
+    e->rsc_ = (float*)bsc.p;
+    e->ssc_ = e->rsc_ + P.num_pslots;
+    e->rnx_ = (int32_t*)(e->ssc_ + P.num_pslots);
+  } else {
+    e->regrets_ = (float*)br.p;
+    e->sums_ = (float*)bs.p;
+  }
   if ((st = e->alloc(bm, P.mrec.size() + 256)) != kOk) return st;
   d.malive = (uint8_t*)bm.p;
   if ((st = e->alloc(bp, sizeof(IterParams) + (size_t)K * 2 * rowlen * 8)) != kOk) return st;
@@ -1651,8 +1833,6 @@ Status Engine::create(int device, Plan&& plan, const float* regrets, const float
   }
   const size_t pool = std::max(e->pool_bytes_[0], e->pool_bytes_[1]);
   if ((st = e->alloc(bre, pool)) != kOk) return st;
-  e->regrets_ = (float*)br.p;
-  e->sums_ = (float*)bs.p;
   for (int f64 = 0; f64 < 2; f64++) {
     e->reach_[f64] = (char*)bre.p + lay[f64][0];
     e->vals_[f64] = (char*)bre.p + lay[f64][1];
@@ -1691,13 +1871,22 @@ Status Engine::create(int device, Plan&& plan, const float* regrets, const float
                      e->lds_term_[0])) != kOk ||
       (st = lds_attr(fastp ? (const void*)terminal_fast_kernel<float, 1> : (const void*)terminal_kernel<float>,
                      e->lds_term_[0])) != kOk ||
-      (st = lds_attr((const void*)up_kernel<double, kBest>, e->lds_up_[1])) != kOk ||
-      (st = lds_attr((const void*)up_kernel<double, kAverage>, e->lds_up_[1])) != kOk ||
-      (st = lds_attr((const void*)up_kernel<float, kAverage>, e->lds_up_[0])) != kOk ||
-      (st = lds_attr((const void*)up_kernel<float, kCfr>, e->lds_up_[0])) != kOk ||
-      (st = lds_attr((const void*)down_kernel<float, kCfr>, e->lds_down_)) != kOk ||
-      (st = lds_attr((const void*)down_kernel<double, kAverage>, e->lds_down_)) != kOk ||
-      (st = lds_attr((const void*)down_kernel<float, kAverage>, e->lds_down_)) != kOk)
+      (st = lds_attr((const void*)up_kernel<double, kBest, 0>, e->lds_up_[1])) != kOk ||
+      (st = lds_attr((const void*)up_kernel<double, kAverage, 0>, e->lds_up_[1])) != kOk ||
+      (st = lds_attr((const void*)up_kernel<float, kAverage, 0>, e->lds_up_[0])) != kOk ||
+      (st = lds_attr((const void*)up_kernel<float, kCfr, 0>, e->lds_up_[0])) != kOk ||
+      (st = lds_attr((const void*)down_kernel<float, kCfr, false>, e->lds_down_)) != kOk ||
+      (st = lds_attr((const void*)down_kernel<double, kAverage, false>, e->lds_down_)) != kOk ||
+      (st = lds_attr((const void*)down_kernel<float, kAverage, false>, e->lds_down_)) != kOk)
+    return st;
+  if (q16 && ((st = lds_attr((const void*)up_kernel<double, kBest, 3>, e->lds_up_[1])) != kOk ||
+              (st = lds_attr((const void*)up_kernel<double, kAverage, 3>, e->lds_up_[1])) != kOk ||
+              (st = lds_attr((const void*)up_kernel<float, kAverage, 3>, e->lds_up_[0])) != kOk ||
+              (st = lds_attr((const void*)up_kernel<float, kCfr, 1>, e->lds_up_[0])) != kOk ||
+              (st = lds_attr((const void*)up_kernel<float, kCfr, 2>, e->lds_up_[0])) != kOk ||
+              (st = lds_attr((const void*)down_kernel<float, kCfr, true>, e->lds_down_)) != kOk ||
+              (st = lds_attr((const void*)down_kernel<double, kAverage, true>, e->lds_down_)) != kOk ||
+              (st = lds_attr((const void*)down_kernel<float, kAverage, true>, e->lds_down_)) != kOk))
     return st;
   const char* ng = std::getenv("PTS_NO_GRAPH");
   e->use_graph_ = !(ng && ng[0] == '1');
@@ -1707,6 +1896,18 @@ Status Engine::create(int device, Plan&& plan, const float* regrets, const float
 }
 
 Status Engine::upload(const float* regrets, const float* sums) {
+  if (q16_) {
+    if (regrets || sums) {
+      set_error("invalid input: 16-bit storage takes its integers and scales (upload_q)");
+      return kInvalidInput;
+    }
+    const size_t bytes = plan_.storage_len * 2;
+    HIPCHK(hipMemset(rq_, 0, bytes));
+    HIPCHK(hipMemset(sq_, 0, bytes));
+    HIPCHK(hipMemset(rsc_, 0, (size_t)plan_.num_pslots * 8));
+    HIPCHK(hipMemset(rnx_, 0xFF, (size_t)plan_.num_pslots * 4));  // -1: no new scale
+    return kOk;
+  }
   const size_t bytes = plan_.storage_len * sizeof(float);
   if (regrets) HIPCHK(hipMemcpy(regrets_, regrets, bytes, hipMemcpyHostToDevice));
   else HIPCHK(hipMemset(regrets_, 0, bytes));
@@ -1716,19 +1917,59 @@ Status Engine::upload(const float* regrets, const float* sums) {
 }
 
 
+Status Engine::download_q(int16_t* regrets, uint16_t* sums, float* regret_scale, float* strat_scale) const {
+  if (!q16_) {
+    set_error("invalid input: f32 storage");
+    return kInvalidInput;
+  }
+  HIPCHK(hipStreamSynchronize(stream_));
+  if (regrets) HIPCHK(hipMemcpy(regrets, rq_, plan_.storage_len * 2, hipMemcpyDeviceToHost));
+  if (sums) HIPCHK(hipMemcpy(sums, sq_, plan_.storage_len * 2, hipMemcpyDeviceToHost));
+  if (regret_scale) HIPCHK(hipMemcpy(regret_scale, rsc_, (size_t)plan_.num_pslots * 4, hipMemcpyDeviceToHost));
+  if (strat_scale) HIPCHK(hipMemcpy(strat_scale, ssc_, (size_t)plan_.num_pslots * 4, hipMemcpyDeviceToHost));
+  return kOk;
+}
+
+Status Engine::upload_q(const int16_t* regrets, const uint16_t* sums, const float* regret_scale, const float* strat_scale) {
+  if (!q16_ || !regrets || !sums || !regret_scale || !strat_scale) {
+    set_error("invalid input: 16-bit storage arrays");
+    return kInvalidInput;
+  }
+  HIPCHK(hipStreamSynchronize(stream_));
+  HIPCHK(hipMemcpy(rq_, regrets, plan_.storage_len * 2, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(sq_, sums, plan_.storage_len * 2, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(rsc_, regret_scale, (size_t)plan_.num_pslots * 4, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(ssc_, strat_scale, (size_t)plan_.num_pslots * 4, hipMemcpyHostToDevice));
+  HIPCHK(hipMemset(rnx_, 0xFF, (size_t)plan_.num_pslots * 4));
+  return kOk;
+}
+
 Status Engine::download(float* regrets, float* sums) const {
+  if (q16_) {
+    set_error("invalid input: 16-bit storage (download_q)");
+    return kInvalidInput;
+  }
   HIPCHK(hipStreamSynchronize(stream_));
   HIPCHK(hipMemcpy(regrets, regrets_, plan_.storage_len * sizeof(float), hipMemcpyDeviceToHost));
   HIPCHK(hipMemcpy(sums, sums_, plan_.storage_len * sizeof(float), hipMemcpyDeviceToHost));
   return kOk;
 }
 
-Status Engine::read_sums(uint64_t offset, size_t len, float* out) const {
-  if (offset + len > plan_.storage_len) {
+Status Engine::read_sums(uint64_t offset, size_t len, float* out, uint32_t pslot) const {
+  if (offset + len > plan_.storage_len || (q16_ && pslot >= plan_.num_pslots)) {
     set_error("invalid input: storage range");
     return kInvalidInput;
   }
   HIPCHK(hipStreamSynchronize(stream_));
+  if (q16_) {  // storage.rs:236-246
+    std::vector<uint16_t> q(len);
+    float scale = 0.0f;
+    HIPCHK(hipMemcpy(q.data(), sq_ + offset, len * 2, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(&scale, ssc_ + pslot, 4, hipMemcpyDeviceToHost));
+    const float step = scale / 65535.0f;
+    for (size_t i = 0; i < len; i++) out[i] = (float)q[i] * step;
+    return kOk;
+  }
   HIPCHK(hipMemcpy(out, sums_ + offset, len * sizeof(float), hipMemcpyDeviceToHost));
   return kOk;
 }
@@ -1763,9 +2004,10 @@ struct ProfRec {
 #define PROF_STOP() \
   if (prof) prof->stop()
 
-template <typename V, int DMODE, int UMODE>
+// Q: 16-bit storage (a CFR up pass then walks twice, QP 1 and 2; reads use QP 3)
+template <typename V, int DMODE, int UMODE, bool Q>
 static void launch_span(const DPlan& d, const Plan& P, hipStream_t st, int p, Span span, uint32_t first_depth,
-                        const std::vector<ChunkGroup>& groups, void* reach, uint32_t* alive, float* regrets, float* sums,
+                        const std::vector<ChunkGroup>& groups, void* reach, uint32_t* alive, const Store& sto,
                         const IterParams* prm, void* vals, uint32_t rowlen, size_t lds_down, size_t lds_term, size_t lds_up,
                         V* qout, bool fast, uint32_t down_t, ProfRec* prof = nullptr,
                         const std::function<void()>* leaf_hook = nullptr) {
@@ -1777,8 +2019,8 @@ static void launch_span(const DPlan& d, const Plan& P, hipStream_t st, int p, Sp
     // few segments (the top of the tree): one lane per opponent hand
     const uint32_t threads = (b - a) < 64 ? 1024 : down_t;
     PROF_START(kKDown);
-    hipLaunchKernelGGL((down_kernel<V, DMODE>), dim3(b - a), dim3(threads), lds_down, st, d, p, a, span, B, (V*)reach, alive,
-                       regrets, sums, prm, rowlen);
+    hipLaunchKernelGGL((down_kernel<V, DMODE, Q>), dim3(b - a), dim3(threads), lds_down, st, d, p, a, span, B, (V*)reach, alive,
+                       sto, prm, rowlen);
     PROF_STOP();
   };
   auto terminals = [&](uint32_t a, uint32_t b, const Bias& B) {
@@ -1803,52 +2045,62 @@ static void launch_span(const DPlan& d, const Plan& P, hipStream_t st, int p, Sp
       PROF_STOP();
     }
     if (b <= a) return;
-    if constexpr (std::is_same<V, float>::value) {
-      // f32 walks: segments of up to REG_T (or, when more than a quarter of
-      // them are larger, REG_S) members in registers over [a, b), then those
-      // of up to REG_S and REG_M among [a, b), then the larger ones in LDS
-      auto cut = [&](const uint32_t* l, uint32_t n, uint32_t& i0, uint32_t& i1) {
-        i0 = (uint32_t)(std::lower_bound(l, l + n, a) - l);
-        i1 = (uint32_t)(std::lower_bound(l, l + n, b) - l);
-      };
-      uint32_t i0, i1;
-      cut(d.m48_host, d.num_m48, i0, i1);
-      const bool wide = (uint64_t)(i1 - i0) * 4 > (uint64_t)(b - a);
-      PROF_START(kKUp);
-      if (wide)
-        hipLaunchKernelGGL((up_reg_kernel<UMODE, (int)REG_S>), dim3(b - a, tiles), dim3(UP_T), 0, st, d, p, a, span, B,
-                           (float*)vals, rowlen, (const uint32_t*)alive, regrets, (const float*)sums, prm, (float*)qout,
-                           (const uint32_t*)nullptr);
-      else
-        hipLaunchKernelGGL((up_reg_kernel<UMODE, (int)REG_T>), dim3(b - a, tiles), dim3(UP_T), 0, st, d, p, a, span, B,
-                           (float*)vals, rowlen, (const uint32_t*)alive, regrets, (const float*)sums, prm, (float*)qout,
-                           (const uint32_t*)nullptr);
-      PROF_STOP();
-      if (!wide && i1 > i0) {
+    // the walks of one storage pass over the segments [a, b)
+    auto walks = [&](auto qp) {
+      constexpr int QP = decltype(qp)::value;
+      if constexpr (std::is_same<V, float>::value) {
+        // f32 walks: segments of up to REG_T (or, when more than a quarter of
+        // them are larger, REG_S) members in registers over [a, b), then those
+        // of up to REG_S and REG_M among [a, b), then the larger ones in LDS
+        auto cut = [&](const uint32_t* l, uint32_t n, uint32_t& i0, uint32_t& i1) {
+          i0 = (uint32_t)(std::lower_bound(l, l + n, a) - l);
+          i1 = (uint32_t)(std::lower_bound(l, l + n, b) - l);
+        };
+        uint32_t i0, i1;
+        cut(d.m48_host, d.num_m48, i0, i1);
+        const bool wide = (uint64_t)(i1 - i0) * 4 > (uint64_t)(b - a);
         PROF_START(kKUp);
-        hipLaunchKernelGGL((up_reg_kernel<UMODE, (int)REG_S>), dim3(i1 - i0, tiles), dim3(UP_T), 0, st, d, p, i0, span, B,
-                           (float*)vals, rowlen, (const uint32_t*)alive, regrets, (const float*)sums, prm, (float*)qout, d.m48);
+        if (wide)
+          hipLaunchKernelGGL((up_reg_kernel<UMODE, (int)REG_S, QP>), dim3(b - a, tiles), dim3(UP_T), 0, st, d, p, a, span, B,
+                             (float*)vals, rowlen, (const uint32_t*)alive, sto, prm, (float*)qout, (const uint32_t*)nullptr);
+        else
+          hipLaunchKernelGGL((up_reg_kernel<UMODE, (int)REG_T, QP>), dim3(b - a, tiles), dim3(UP_T), 0, st, d, p, a, span, B,
+                             (float*)vals, rowlen, (const uint32_t*)alive, sto, prm, (float*)qout, (const uint32_t*)nullptr);
+        PROF_STOP();
+        if (!wide && i1 > i0) {
+          PROF_START(kKUp);
+          hipLaunchKernelGGL((up_reg_kernel<UMODE, (int)REG_S, QP>), dim3(i1 - i0, tiles), dim3(UP_T), 0, st, d, p, i0, span, B,
+                             (float*)vals, rowlen, (const uint32_t*)alive, sto, prm, (float*)qout, d.m48);
+          PROF_STOP();
+        }
+        cut(d.mid_host, d.num_mid, i0, i1);
+        if (i1 > i0) {
+          PROF_START(kKUp);
+          hipLaunchKernelGGL((up_reg_kernel<UMODE, (int)REG_M, QP>), dim3(i1 - i0, tiles), dim3(UP_T), 0, st, d, p, i0, span, B,
+                             (float*)vals, rowlen, (const uint32_t*)alive, sto, prm, (float*)qout, d.mid);
+          PROF_STOP();
+        }
+        cut(d.big_host, d.num_big, i0, i1);
+        if (i1 > i0) {
+          PROF_START(kKUp);
+          hipLaunchKernelGGL((up_kernel<V, UMODE, QP>), dim3(i1 - i0, tiles), dim3(UP_T), lds_up, st, d, p, i0, span, B, (V*)vals,
+                             rowlen, (const uint32_t*)alive, sto, prm, qout, d.big);
+          PROF_STOP();
+        }
+      } else {
+        PROF_START(kKUp);
+        hipLaunchKernelGGL((up_kernel<V, UMODE, QP>), dim3(b - a, tiles), dim3(UP_T), lds_up, st, d, p, a, span, B, (V*)vals, rowlen,
+                           (const uint32_t*)alive, sto, prm, qout, (const uint32_t*)nullptr);
         PROF_STOP();
       }
-      cut(d.mid_host, d.num_mid, i0, i1);
-      if (i1 > i0) {
-        PROF_START(kKUp);
-        hipLaunchKernelGGL((up_reg_kernel<UMODE, (int)REG_M>), dim3(i1 - i0, tiles), dim3(UP_T), 0, st, d, p, i0, span, B,
-                           (float*)vals, rowlen, (const uint32_t*)alive, regrets, (const float*)sums, prm, (float*)qout, d.mid);
-        PROF_STOP();
-      }
-      cut(d.big_host, d.num_big, i0, i1);
-      if (i1 > i0) {
-        PROF_START(kKUp);
-        hipLaunchKernelGGL((up_kernel<V, UMODE>), dim3(i1 - i0, tiles), dim3(UP_T), lds_up, st, d, p, i0, span, B, (V*)vals,
-                           rowlen, (const uint32_t*)alive, regrets, (const float*)sums, prm, qout, d.big);
-        PROF_STOP();
-      }
+    };
+    if constexpr (!Q) {
+      walks(std::integral_constant<int, 0>());
+    } else if constexpr (UMODE == kCfr) {
+      walks(std::integral_constant<int, 1>());
+      walks(std::integral_constant<int, 2>());
     } else {
-      PROF_START(kKUp);
-      hipLaunchKernelGGL((up_kernel<V, UMODE>), dim3(b - a, tiles), dim3(UP_T), lds_up, st, d, p, a, span, B, (V*)vals, rowlen,
-                         (const uint32_t*)alive, regrets, (const float*)sums, prm, qout, (const uint32_t*)nullptr);
-      PROF_STOP();
+      walks(std::integral_constant<int, 3>());
     }
   };
   if (first_depth == 0) down(P.depth_off[0], P.depth_off[1], B0);
@@ -1897,21 +2149,29 @@ Status Engine::launch_traversal(int p, Mode mode, bool f64, ProfRec* prof) {
   std::function<void()> hook;
   if (!P.leaves.empty()) hook = [&, p, mode, f64]() { launch_leaves(p, mode, f64, reach, vals, alive); };
   const std::function<void()>* hk = P.leaves.empty() ? nullptr : &hook;
-  if (mode == kCfr)
-    launch_span<float, kCfr, kCfr>(d, P, stream_, p, span, 0, groups_[0], reach, alive, regrets_, sums_, params_, vals, rowlen_, lds_down_,
-                                   lds_term_[0], lds_up_[0], (float*)nullptr, precision_ == kFast, down_threads_, prof, hk);
-  else if (mode == kBest)
-    launch_span<double, kAverage, kBest>(d, P, stream_, p, span, 0, groups_[1], reach, alive, regrets_, sums_, params_, vals, rowlen_,
-                                         lds_down_, lds_term_[1], lds_up_[1], (double*)nullptr, precision_ == kFast, down_threads_,
-                                         nullptr, hk);
-  else if (f64)
-    launch_span<double, kAverage, kAverage>(d, P, stream_, p, span, 0, groups_[1], reach, alive, regrets_, sums_, params_, vals, rowlen_,
-                                            lds_down_, lds_term_[1], lds_up_[1], (double*)nullptr, precision_ == kFast, down_threads_,
-                                            nullptr, hk);
-  else
-    launch_span<float, kAverage, kAverage>(d, P, stream_, p, span, 0, groups_[0], reach, alive, regrets_, sums_, params_, vals, rowlen_,
-                                           lds_down_, lds_term_[0], lds_up_[0], (float*)nullptr, precision_ == kFast, down_threads_,
-                                           nullptr, hk);
+  const Store sto = store();
+  auto run = [&](auto qc) {
+    constexpr bool Q = decltype(qc)::value;
+    if (mode == kCfr)
+      launch_span<float, kCfr, kCfr, Q>(d, P, stream_, p, span, 0, groups_[0], reach, alive, sto, params_, vals, rowlen_, lds_down_,
+                                        lds_term_[0], lds_up_[0], (float*)nullptr, precision_ == kFast, down_threads_, prof, hk);
+    else if (mode == kBest)
+      launch_span<double, kAverage, kBest, Q>(d, P, stream_, p, span, 0, groups_[1], reach, alive, sto, params_, vals, rowlen_,
+                                              lds_down_, lds_term_[1], lds_up_[1], (double*)nullptr, precision_ == kFast,
+                                              down_threads_, nullptr, hk);
+    else if (f64)
+      launch_span<double, kAverage, kAverage, Q>(d, P, stream_, p, span, 0, groups_[1], reach, alive, sto, params_, vals, rowlen_,
+                                                 lds_down_, lds_term_[1], lds_up_[1], (double*)nullptr, precision_ == kFast,
+                                                 down_threads_, nullptr, hk);
+    else
+      launch_span<float, kAverage, kAverage, Q>(d, P, stream_, p, span, 0, groups_[0], reach, alive, sto, params_, vals, rowlen_,
+                                                lds_down_, lds_term_[0], lds_up_[0], (float*)nullptr, precision_ == kFast,
+                                                down_threads_, nullptr, hk);
+  };
+  if (q16_) run(std::true_type());
+  else run(std::false_type());
+  if (q16_ && mode == kCfr && P.num_pslots > 0)
+    hipLaunchKernelGGL(commit_scales_kernel, dim3((P.num_pslots + 255) / 256), dim3(256), 0, stream_, sto, P.num_pslots);
   // keep every spot's root row of p
   void* res = (char*)params_ + sizeof(IterParams);
   if (f64) hipLaunchKernelGGL((collect_kernel<double>), dim3(tiles, P.num_spots), dim3(256), 0, stream_, d, p, (const double*)vals, (double*)res, rowlen_);
@@ -1965,6 +2225,10 @@ Status Engine::attach_net(net::Net* n, const std::vector<int>& board) {
   const Plan& P = plan_;
   if (P.leaves.empty()) {
     set_error("invalid input: the tree has no leaves for a value network");
+    return kInvalidInput;
+  }
+  if (q16_) {
+    set_error("invalid input: value-network leaves need f32 storage");
     return kInvalidInput;
   }
   if (P.num_spots != 1) {
@@ -2024,6 +2288,10 @@ Status Engine::attach_net(net::Net* n, const std::vector<int>& board) {
 }
 
 Status Engine::write_storage(uint64_t offset, size_t len, const float* regrets, const float* sums) {
+  if (q16_) {
+    set_error("invalid input: 16-bit storage");
+    return kInvalidInput;
+  }
   if (offset + len > plan_.storage_len) {
     set_error("invalid input: storage range");
     return kInvalidInput;
@@ -2186,6 +2454,10 @@ Status Engine::retune(uint32_t run_length, uint32_t down_threads) {
 }
 
 Status Engine::read_storage(uint64_t offset, size_t len, float* regrets, float* sums) const {
+  if (q16_) {
+    set_error("invalid input: 16-bit storage");
+    return kInvalidInput;
+  }
   if (offset + len > plan_.storage_len) {
     set_error("invalid input: storage range");
     return kInvalidInput;
@@ -2260,8 +2532,14 @@ Status Engine::subtree_values(uint32_t node, int q, const std::vector<float>& op
   // depths from the node's segment on
   uint32_t seg = P.seg_of[node], first_depth = 0;
   while (first_depth + 1 < P.depth_off.size() && P.depth_off[first_depth + 1] <= seg) first_depth++;
-  launch_span<float, kAverage, kAverage>(dp_->d, P, stream_, q, span, first_depth, groups_[0], reach_[0], alive_[0], regrets_, sums_, params_, vals_[0],
-                                         rowlen_, lds_down_, lds_term_[0], lds_up_[0], dout, precision_ == kFast, down_threads_);
+  if (q16_)
+    launch_span<float, kAverage, kAverage, true>(dp_->d, P, stream_, q, span, first_depth, groups_[0], reach_[0], alive_[0], store(),
+                                                 params_, vals_[0], rowlen_, lds_down_, lds_term_[0], lds_up_[0], dout,
+                                                 precision_ == kFast, down_threads_);
+  else
+    launch_span<float, kAverage, kAverage, false>(dp_->d, P, stream_, q, span, first_depth, groups_[0], reach_[0], alive_[0], store(),
+                                                  params_, vals_[0], rowlen_, lds_down_, lds_term_[0], lds_up_[0], dout,
+                                                  precision_ == kFast, down_threads_);
   HIPCHK(hipGetLastError());
   child_vals.assign((size_t)na * np, 0.0f);
   HIPCHK(hipMemcpyAsync(child_vals.data(), dout, (size_t)na * np * 4, hipMemcpyDeviceToHost, stream_));

@@ -15,6 +15,7 @@ namespace pts {
 namespace net {
 class Net;
 }
+struct Store;
 
 enum Status : int {
   kOk = 0,
@@ -62,9 +63,13 @@ class Engine {
   // regrets / sums may be null (zeros). scratch_budget caps the reach and
   // value rows in bytes (0: all free HBM after the storage, less a margin);
   // the chunks below the root street's chance nodes run in groups that fit.
+  // q16: 16-bit storage (crates/solver/src/storage.rs:87-106: i16 regrets,
+  // u16 sums, one f32 scale per player node; the plan must carry player
+  // slots), starting from zeros (regrets / sums must be null).
   static Status create(int device, Plan&& plan, const float* regrets, const float* sums, int precision,
-                       uint64_t scratch_budget, std::unique_ptr<Engine>& out);
+                       uint64_t scratch_budget, std::unique_ptr<Engine>& out, bool q16 = false);
   int precision() const { return precision_; }
+  bool compressed() const { return q16_; }
 
   // One DCFR iteration: player 0's update, then player 1's (crates/solver/src/lib.rs:577-585).
   Status cfr_iteration(const IterParams& d);
@@ -88,8 +93,11 @@ class Engine {
   Status retune(uint32_t run_length, uint32_t down_threads);
   uint32_t run_length() const { return run_length_; }
   uint32_t down_threads() const { return down_threads_; }
-  // A storage range to the host (either pointer may be null).
+  // A storage range to the host (either pointer may be null); f32 storage.
   Status read_storage(uint64_t offset, size_t len, float* regrets, float* sums) const;
+  // 16-bit storage: the stored integers and the per-player-node scales.
+  Status download_q(int16_t* regrets, uint16_t* sums, float* regret_scale, float* strat_scale) const;
+  Status upload_q(const int16_t* regrets, const uint16_t* sums, const float* regret_scale, const float* strat_scale);
   Status zero_storage();
   Status write_storage(uint64_t offset, size_t len, const float* regrets, const float* sums);
   // Depth-limited trees: the value network that values the leaves (null:
@@ -99,8 +107,9 @@ class Engine {
   Status exploitability(bool raked, double* out);
   Status download(float* regrets, float* sums) const;
   Status upload(const float* regrets, const float* sums);
-  // A slice of the strategy sums (e.g. one node's block) to the host.
-  Status read_sums(uint64_t offset, size_t len, float* out) const;
+  // A slice of the strategy sums (e.g. one node's block) to the host; with
+  // 16-bit storage decoded with player node pslot's scale (storage.rs:236-246).
+  Status read_sums(uint64_t offset, size_t len, float* out, uint32_t pslot = 0xFFFFFFFFu) const;
   // Subtree traversal for queries: player q at decision node `node`, opponent
   // reach given; writes the node's children values (na * n_q, action-major).
   Status subtree_values(uint32_t node, int q, const std::vector<float>& opp_reach, std::vector<float>& child_vals);
@@ -131,6 +140,13 @@ class Engine {
   void* dplan_blob_ = nullptr;
   float* regrets_ = nullptr;
   float* sums_ = nullptr;
+  // 16-bit storage
+  bool q16_ = false;
+  int16_t* rq_ = nullptr;
+  uint16_t* sq_ = nullptr;
+  float *rsc_ = nullptr, *ssc_ = nullptr;
+  int32_t* rnx_ = nullptr;
+  Store store() const;
   // scratch pool: reach rows, value rows and alive flags for f32 [0] and f64 [1] rows
   void* reach_[2] = {nullptr, nullptr};
   void* vals_[2] = {nullptr, nullptr};

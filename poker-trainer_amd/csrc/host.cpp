@@ -161,6 +161,7 @@ struct TreeWalker {
       n.first_child = (uint32_t)first;
       n.invest[0] = inv[0]; n.invest[1] = inv[1];
       n.offset = tree->storage_len;
+      n.pslot = tree->player_nodes;  // tree.rs:463-469
       tree->storage_len += acts.size() * spec.hands[st.to_act];
       tree->player_nodes++;
     }

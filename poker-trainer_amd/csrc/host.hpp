@@ -60,6 +60,7 @@ struct Node {
   uint32_t first_child;
   int32_t invest[2];
   uint64_t offset;  // player: storage offset; showdown: board index; leaf: pot (chips)
+  uint32_t pslot;   // player: its index among the player nodes (the 16-bit storage's scale slot)
   float rake, chance_factor;
   int8_t card;      // chance child: the dealt card, else -1
   Action action;    // action child: the action taken

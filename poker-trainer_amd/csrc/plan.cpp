@@ -28,6 +28,7 @@ std::string build_plan(const PlanInput& in, Plan& P) {
   P.half_pot = in.half_pot;
   P.norm = 1.0 / in.num_combinations;
   P.storage_len = in.storage_len;
+  P.num_pslots = in.pslot ? in.num_pslots : 0;
   P.batch_river = in.batch_river;
   P.num_boards = in.num_boards;
 
@@ -101,6 +102,7 @@ std::string build_plan(const PlanInput& in, Plan& P) {
     if (is_player(k)) {
       uint64_t cells = (uint64_t)nc * in.num_hands[k];
       if (P.offset[i] + cells > in.storage_len) return fail("storage block out of range", i);
+      if (in.pslot && in.pslot[i] >= in.num_pslots) return fail("player slot out of range", i);
     } else {
       if (!(P.factor[i] > 0.0f)) return fail("bad chance factor", i);
       for (uint32_t a = 0; a < nc; a++)
@@ -371,6 +373,7 @@ std::string build_plan(const PlanInput& in, Plan& P) {
       if (r.vslot == kNone) r.vslot = chance_vslot[id];
     } else {
       r.offset = P.offset[id];
+      r.pslot = in.pslot ? in.pslot[id] : 0;
       r.clocal = P.local[fc];
       for (uint32_t a = 0; a < r.na; a++)
         if (P.local[fc + a] != P.local[fc] + a || P.seg_of[fc + a] != P.seg_of[id]) return fail("children not consecutive members", id);
@@ -572,7 +575,9 @@ std::string merge_plans(std::vector<Plan>&& parts, Plan& M) {
   std::vector<uint64_t> node_off(K), vslot_off(K), store_off(K), term_off(K), slot_off[2];
   slot_off[0].resize(K);
   slot_off[1].resize(K);
-  uint64_t nn = 0, nv = 0, ns = 0, nt = 0, sl[2] = {0, 0}, nb = 0;
+  uint64_t nn = 0, nv = 0, ns = 0, nt = 0, sl[2] = {0, 0}, nb = 0, nps = 0;
+  std::vector<uint64_t> ps_off(K);
+  bool q16 = true;
   uint32_t D = 0;
   for (uint32_t k = 0; k < K; k++) {
     const Plan& T = parts[k];
@@ -580,6 +585,9 @@ std::string merge_plans(std::vector<Plan>&& parts, Plan& M) {
     vslot_off[k] = nv;
     store_off[k] = ns;
     term_off[k] = nt;
+    ps_off[k] = nps;
+    nps += T.num_pslots;
+    q16 = q16 && T.num_pslots > 0;
     for (int p = 0; p < 2; p++) {
       slot_off[p][k] = sl[p];
       sl[p] += T.num_slots[p];
@@ -600,6 +608,7 @@ std::string merge_plans(std::vector<Plan>&& parts, Plan& M) {
   M.num_nodes = nn;
   M.num_vslots = (uint32_t)nv;
   M.storage_len = ns;
+  M.num_pslots = q16 && nps < 0xFFFFFFFFull ? (uint32_t)nps : 0;
   M.num_slots[0] = (uint32_t)sl[0];
   M.num_slots[1] = (uint32_t)sl[1];
   M.num_boards = (uint32_t)nb;
@@ -620,7 +629,10 @@ std::string merge_plans(std::vector<Plan>&& parts, Plan& M) {
       if (r.cslot[p] != kNone) r.cslot[p] += (uint32_t)slot_off[p][k];
     }
     if (r.kind == kChance) r.offset += node_off[k];  // first child's node id
-    else if (is_player(r.kind)) r.offset += store_off[k];
+    else if (is_player(r.kind)) {
+      r.offset += store_off[k];
+      r.pslot += (uint32_t)ps_off[k];
+    }
     r.spot = (uint16_t)k;
     return r;
   };

@@ -49,7 +49,7 @@ struct MemRec {
   int8_t card;         // chance child: the dealt card
   uint8_t tflags;
   uint16_t spot;       // the spot (tree) of a batch it belongs to
-  uint8_t pad[4];
+  uint32_t pslot;      // decision: its index among the player nodes (16-bit storage scale slot)
 };
 static_assert(sizeof(MemRec) == 48, "MemRec layout");
 
@@ -111,6 +111,9 @@ struct PlanInput {
   const uint16_t* iso_cnt = nullptr;
   const IsoRef* iso_refs = nullptr;
   const uint16_t* iso_perm[2] = {nullptr, nullptr};
+  // per node its player-node index (player nodes; null: no 16-bit storage)
+  const uint32_t* pslot = nullptr;
+  uint32_t num_pslots = 0;
 };
 
 struct Plan {
@@ -198,6 +201,7 @@ struct Plan {
   uint32_t max_entries = 0;
   double half_pot = 0, norm = 0;
   uint64_t storage_len = 0;
+  uint32_t num_pslots = 0;  // player nodes with a scale slot (0: no 16-bit storage)
   bool batch_river = true;
   uint32_t depths() const { return (uint32_t)depth_off.size() - 1; }
 };

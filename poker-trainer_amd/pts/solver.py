@@ -155,6 +155,24 @@ class Solver:
     def set_storage(self, r, s):
         check(lib().pts_solver_set_storage(self.h, np.ascontiguousarray(r, np.float32), np.ascontiguousarray(s, np.float32)))
 
+    @property
+    def compressed(self) -> bool:
+        """16-bit storage (storage.rs:74-81; lib.rs:346-352 for Auto)."""
+        return bool(lib().pts_solver_is_compressed(self.h))
+
+    def storage_q(self):
+        """16-bit storage: (i16 regrets, u16 sums, regret scales, strategy-sum
+        scales), one scale of each per player node."""
+        n, k = self.storage_len, self.stats().player_nodes
+        r, s = np.zeros(n, np.int16), np.zeros(n, np.uint16)
+        rs, ss = np.zeros(k, np.float32), np.zeros(k, np.float32)
+        check(lib().pts_solver_get_storage_q(self.h, r, s, rs, ss))
+        return r, s, rs, ss
+
+    def set_storage_q(self, r, s, rs, ss):
+        check(lib().pts_solver_set_storage_q(self.h, np.ascontiguousarray(r, np.int16), np.ascontiguousarray(s, np.uint16),
+                                             np.ascontiguousarray(rs, np.float32), np.ascontiguousarray(ss, np.float32)))
+
     def nodes(self):
         m = lib().pts_solver_num_nodes(self.h)
         a = dict(kind=np.zeros(m, np.uint8), street=np.zeros(m, np.uint8), num_children=np.zeros(m, np.uint16),

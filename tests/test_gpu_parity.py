@@ -129,6 +129,86 @@ def test_isomorphism_bit_exact(make):
     assert abs(f.exploitability() - c.exploitability_f64()) <= 1e-4 * cfg.root.pot
 
 
+def compressed(cfg):
+    cfg.storage_mode = 1  # 16-bit storage (storage.rs:87-106)
+    return cfg
+
+
+Q_SPOTS = [("river0", lambda: spots.river_spot(0), 12), ("turn0", lambda: spots.turn_spot(0), 4),
+           ("iso-turn", iso_turn, 6), ("river-big", lambda: spots.batch_spot("river", 11), 6)]
+
+
+@pytest.mark.parametrize("name,make,iters", Q_SPOTS, ids=[s[0] for s in Q_SPOTS])
+def test_compressed_solves_bit_exact(name, make, iters):
+    # i16 regrets and u16 sums with per-node scales: every node's new scale is
+    # its block's largest |value| over all hands (the down pass reduces over
+    # its workgroup, the up pass over the hand tiles in a first walk), then
+    # the block is re-encoded in a second walk
+    cfg = compressed(make())
+    g, c = Solver(cfg, precision=EXACT), O.OracleSolver(cfg)
+    assert g.compressed and c.compressed
+    g.iterate(iters)
+    c.iterate(iters)
+    for x, y, what in zip(g.storage_q(), c.storage_q(), ("regrets", "sums", "regret scales", "sum scales")):
+        assert same(x, y), f"{what}: {describe(x, y)}"
+    assert g.exploitability() == c.exploitability_f64()
+    rg, sg = g.storage()  # decoded (storage.rs:209-246)
+    rc, sc = c.storage()
+    assert same(rg, rc) and same(sg, sc)
+
+
+def test_compressed_traversals_from_random_storage():
+    # every code path of the codec from arbitrary 16-bit storage: integers
+    # over their whole range, some blocks with scale 0
+    cfg = compressed(spots.turn_spot(1))
+    g, c = Solver(cfg, precision=EXACT), O.OracleSolver(cfg)
+    n, k = c.storage_len, c.stats().player_nodes
+    rng = np.random.default_rng(5)
+    r = rng.integers(-32768, 32768, n).astype(np.int16)
+    s = rng.integers(0, 65536, n).astype(np.uint16)
+    rs = (rng.random(k) * 50).astype(np.float32)
+    ss = (rng.random(k) * 20).astype(np.float32)
+    rs[rng.random(k) < 0.1] = 0.0
+    ss[rng.random(k) < 0.1] = 0.0
+    g.set_storage_q(r, s, rs, ss)
+    c.set_storage_q(r, s, rs, ss)
+    assert g.exploitability() == c.exploitability_f64()
+    g.iterations = 5
+    c.iterations = 5
+    g.iterate(2)
+    c.iterate(2)
+    for x, y, what in zip(g.storage_q(), c.storage_q(), ("regrets", "sums", "regret scales", "sum scales")):
+        assert same(x, y), f"{what}: {describe(x, y)}"
+    qa, qb = g.query([]), c.query([])
+    assert same(qa.strategy, qb.strategy) and same(qa.ev, qb.ev), describe(qa.ev, qb.ev)
+
+
+def test_compressed_queries_and_fast_precision():
+    cfg = compressed(spots.turn_spot(2))
+    g, c = Solver(cfg, precision=EXACT), O.OracleSolver(cfg)
+    g.iterate(8)
+    c.iterate(8)
+    path = [c.query([]).actions[0]]
+    path = path + [c.query(path).actions[0]]
+    card = next(x for x in range(52) if x not in cfg.board)
+    for pth in ([], path[:1], path + [("card", card)]):
+        a, b = g.query(pth), c.query(pth)
+        assert a.actions == b.actions
+        assert same(a.strategy, b.strategy) and same(a.reach, b.reach)
+        assert same(a.ev, b.ev), describe(a.ev, b.ev)
+        ra, rb = g.ranges_after(pth), c.ranges_after(pth)
+        assert same(ra[0], rb[0]) and same(ra[1], rb[1])
+    f = Solver(cfg)  # fast precision terminal sums, same storage kernels
+    f.iterate(8)
+    assert f.compressed
+    assert abs(f.exploitability() - c.exploitability_f64()) <= 1e-3 * cfg.root.pot
+    # the f32 solve it approximates (tests.rs:273-302)
+    plain = spots.turn_spot(2)
+    p32 = Solver(plain, precision=EXACT)
+    p32.iterate(8)
+    assert np.max(np.abs(p32.query([]).range_strategy() - g.query([]).range_strategy())) < 0.03
+
+
 def test_solve_loop_matches_the_oracle():
     cfg = spots.river_spot(4)
     g, c = Solver(cfg, precision=EXACT), O.OracleSolver(cfg)
