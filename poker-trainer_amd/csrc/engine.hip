@@ -139,11 +139,21 @@ __device__ __forceinline__ int16_t enc_regret(float v, float inv) {
 }
 // storage.rs:312-317: max(v, 0) * (MAX / max) + 0.5, truncated
 __device__ __forceinline__ uint16_t enc_sum(float v, float inv) { return sat_u16(fmaxf(v, 0.0f) * inv + 0.5f); }
-// the largest of the wave's 64 values (every lane gets it)
+// The largest of the wave's 64 values, all >= 0 (0 is the identity), in
+// every lane: inclusive max-scans within each row of 16 lanes (DPP
+// row_shr 1, 2, 4, 8), row 0's and row 2's last lanes into rows 1 and 3
+// (row_bcast:15), then row 1's last lane into rows 2 and 3 (row_bcast:31):
+// lane 63 ends with the maximum. Lanes a DPP move leaves without a source
+// take 0.
+__device__ __forceinline__ float dpp_max(float x, int src) { return fmaxf(x, __int_as_float(src)); }
 __device__ __forceinline__ float wave_max(float x) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) x = fmaxf(x, __shfl_xor(x, o));
-  return x;
+  x = dpp_max(x, __builtin_amdgcn_update_dpp(0, __float_as_int(x), 0x111, 0xF, 0xF, false));  // row_shr:1
+  x = dpp_max(x, __builtin_amdgcn_update_dpp(0, __float_as_int(x), 0x112, 0xF, 0xF, false));  // row_shr:2
+  x = dpp_max(x, __builtin_amdgcn_update_dpp(0, __float_as_int(x), 0x114, 0xF, 0xF, false));  // row_shr:4
+  x = dpp_max(x, __builtin_amdgcn_update_dpp(0, __float_as_int(x), 0x118, 0xF, 0xF, false));  // row_shr:8
+  x = dpp_max(x, __builtin_amdgcn_update_dpp(0, __float_as_int(x), 0x142, 0xA, 0xF, false));  // row_bcast:15
+  x = dpp_max(x, __builtin_amdgcn_update_dpp(0, __float_as_int(x), 0x143, 0xC, 0xF, false));  // row_bcast:31
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(x), 63));
 }
 
 // Per-player fields of a record held in registers: selects, not a runtime
@@ -226,6 +236,9 @@ __global__ void __launch_bounds__(1024) down_kernel(DPlan P, int p, uint32_t seg
   if (!span.has(root) && seg != span.seg) return;
   MemRec* rec = (MemRec*)smem_down;
   uint8_t* m_alive = (uint8_t*)(rec + P.max_members);
+  // 16-bit storage: a node's new sums (action-major, nq per action) between
+  // their maximum and their encoding; every thread reads back only its own
+  float* ybuf = (float*)(m_alive + ((P.max_members + 15) & ~15u));
   stage_records(P.mrec + moff, mcnt, rec);
   if (tid < 3) {
     red3[tid] = 0;
@@ -331,8 +344,12 @@ __global__ void __launch_bounds__(1024) down_kernel(DPlan P, int p, uint32_t seg
         for (int a = 0; a < MAXA; a++)
           if (a < (int)na) {
             const float y = new_sum(h, a, sv[a], (float)rr);
-            if (Q) lmax = fmaxf(lmax, fabsf(y));
-            else sto.s[off + (size_t)a * nq + h] = y;
+            if (Q) {
+              lmax = fmaxf(lmax, fabsf(y));
+              ybuf[a * nq + h] = y;
+            } else {
+              sto.s[off + (size_t)a * nq + h] = y;
+            }
             sa[a] = (V)sv[a];
           }
       } else {
@@ -382,21 +399,14 @@ __global__ void __launch_bounds__(1024) down_kernel(DPlan P, int p, uint32_t seg
     if (lane < na) m_alive[cl + lane] = (uint8_t)((all >> lane) & 1u);
     if (tid < na) alive[cs + tid] = (all >> tid) & 1u;
     if (Q && MODE == kCfr) {
-      // the same new sums again, encoded with the block's largest as the new
-      // scale (storage.rs:306-320); every thread decoded with the old scale
-      // before the barrier above
+      // the new sums, encoded with the block's largest as the new scale
+      // (storage.rs:306-320); every thread decoded with the old scale before
+      // the barrier above
       const float inv = mx > 0.0f ? kSumMax / mx : 0.0f;
-      for (uint32_t h = tid; h < nq; h += blockDim.x) {
-        const float rr = (float)rrow[h];
-        float sv[MAXA];
-        matched(h, sv);
+      for (uint32_t h = tid; h < nq; h += blockDim.x)
 #pragma unroll
         for (int a = 0; a < MAXA; a++)
-          if (a < (int)na) {
-            const float y = new_sum(h, a, sv[a], rr);
-            sto.sq[off + (size_t)a * nq + h] = mx > 0.0f ? enc_sum(y, inv) : (uint16_t)0;
-          }
-      }
+          if (a < (int)na) sto.sq[off + (size_t)a * nq + h] = mx > 0.0f ? enc_sum(ybuf[a * nq + h], inv) : (uint16_t)0;
       if (tid == 0) sto.ssc[r.pslot] = mx;
     }
   }
@@ -1844,7 +1854,7 @@ Status Engine::create(int device, Plan&& plan, const float* regrets, const float
 
   // LDS sizes
   const size_t mm = std::max<size_t>(P.max_members, 1);
-  e->lds_down_ = mm * sizeof(MemRec) + mm;
+  e->lds_down_ = mm * sizeof(MemRec) + ((mm + 15) & ~size_t(15)) + (q16 ? (size_t)MAXA * rowlen * 4 : 0);
   for (int f64 = 0; f64 < 2; f64++) {
     size_t vs = f64 ? 8 : 4;
     e->lds_term_[f64] = precision == kExact ? (size_t)rowlen * 3 * 8 + (size_t)rowlen * vs
