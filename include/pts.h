@@ -96,6 +96,16 @@ typedef struct {
  * street's chance nodes run in groups that fit it. */
 pts_status pts_engine_create(int device, const pts_tree_desc* tree, const float* regrets, const float* sums, int precision,
                              uint64_t scratch_budget, pts_engine** out);
+/* The same engine over 16-bit storage (storage.rs:87-106; the reference's
+ * HostStorage::Compressed, gpu/engine.rs:292-343): storage_len i16 regrets
+ * and u16 strategy sums; slot[i] is decision node i's scale slot (tree.rs:
+ * 164-165, Node::slot; ignored for other nodes), num_slots regret scales and
+ * num_slots strategy-sum scales. */
+pts_status pts_engine_create_q(int device, const pts_tree_desc* tree, const uint32_t* slot, uint32_t num_slots,
+                               const int16_t* regrets, const uint16_t* sums, const float* regret_scale,
+                               const float* strat_scale, int precision, uint64_t scratch_budget, pts_engine** out);
+/* engine.rs:347-382 for 16-bit storage: the integers and the scales. */
+pts_status pts_engine_download_q(pts_engine* e, int16_t* regrets, uint16_t* sums, float* regret_scale, float* strat_scale);
 /* Engine::cfr_iteration (engine.rs:386-392): player 0's update, then player 1's,
  * with the discounts of cfr.rs:83-107 computed by the caller. */
 pts_status pts_engine_iterate(pts_engine* e, float pos, float neg, float strat, int reach_weighted);

@@ -194,10 +194,7 @@ int pts_backend_wants_gpu(int backend, int street) {
   return 0;
 }
 
-pts_status pts_engine_create(int device, const pts_tree_desc* t, const float* regrets, const float* sums, int precision,
-                             uint64_t scratch_budget, pts_engine** out) {
-  *out = nullptr;
-  if (!t || !regrets || !sums) return bad("invalid input: null argument");
+static PlanInput desc_input(const pts_tree_desc* t) {
   PlanInput in;
   in.num_nodes = t->num_nodes;
   in.kind = t->kind;
@@ -221,6 +218,14 @@ pts_status pts_engine_create(int device, const pts_tree_desc* t, const float* re
   in.num_combinations = t->num_combinations;
   in.storage_len = t->storage_len;
   in.batch_river = t->batch_river != 0;
+  return in;
+}
+
+pts_status pts_engine_create(int device, const pts_tree_desc* t, const float* regrets, const float* sums, int precision,
+                             uint64_t scratch_budget, pts_engine** out) {
+  *out = nullptr;
+  if (!t || !regrets || !sums) return bad("invalid input: null argument");
+  PlanInput in = desc_input(t);
   Plan plan;
   std::string err = build_plan(in, plan);
   if (!err.empty()) return bad(err);
@@ -229,6 +234,30 @@ pts_status pts_engine_create(int device, const pts_tree_desc* t, const float* re
   if (s != kOk) return st(s);
   *out = new pts_engine{std::move(e)};
   return PTS_OK;
+}
+
+pts_status pts_engine_create_q(int device, const pts_tree_desc* t, const uint32_t* slot, uint32_t num_slots,
+                               const int16_t* regrets, const uint16_t* sums, const float* regret_scale,
+                               const float* strat_scale, int precision, uint64_t scratch_budget, pts_engine** out) {
+  *out = nullptr;
+  if (!t || !slot || !regrets || !sums || !regret_scale || !strat_scale) return bad("invalid input: null argument");
+  if (num_slots == 0) return bad("invalid input: no player slots");
+  PlanInput in = desc_input(t);
+  in.pslot = slot;
+  in.num_pslots = num_slots;
+  Plan plan;
+  std::string err = build_plan(in, plan);
+  if (!err.empty()) return bad(err);
+  std::unique_ptr<Engine> e;
+  Status s = Engine::create(device, std::move(plan), nullptr, nullptr, precision, scratch_budget, e, true);
+  if (s != kOk) return st(s);
+  if ((s = e->upload_q(regrets, sums, regret_scale, strat_scale)) != kOk) return st(s);
+  *out = new pts_engine{std::move(e)};
+  return PTS_OK;
+}
+
+pts_status pts_engine_download_q(pts_engine* e, int16_t* regrets, uint16_t* sums, float* regret_scale, float* strat_scale) {
+  return st(e->e->download_q(regrets, sums, regret_scale, strat_scale));
 }
 
 pts_status pts_engine_iterate(pts_engine* e, float pos, float neg, float strat, int rw) {

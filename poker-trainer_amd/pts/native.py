@@ -69,6 +69,7 @@ EXPORTS = [
     "pts_net_get_params", "pts_net_get_grads", "pts_net_forward", "pts_net_gradients", "pts_net_train_step", "pts_net_profile",
     "pts_dl_new", "pts_solver_num_leaves", "pts_solver_copy_root_street", "pts_solver_pin_root_street", "pts_batch_root_rows",
     "pts_batch_num_combinations", "pts_solver_is_compressed", "pts_solver_get_storage_q", "pts_solver_set_storage_q",
+    "pts_engine_create_q", "pts_engine_download_q",
 ]
 
 
@@ -175,6 +176,10 @@ def _declare(L):
     L.pts_solver_is_compressed.argtypes = [P]
     L.pts_solver_get_storage_q.argtypes = [P, i16p, u16p, f32p, f32p]
     L.pts_solver_set_storage_q.argtypes = [P, i16p, u16p, f32p, f32p]
+    L.pts_engine_create_q.argtypes = [ctypes.c_int, ctypes.c_void_p, np.ctypeslib.ndpointer(dtype=np.uint32),
+                                      ctypes.c_uint32, i16p, u16p, f32p, f32p, ctypes.c_int, ctypes.c_uint64,
+                                      ctypes.POINTER(ctypes.c_void_p)]
+    L.pts_engine_download_q.argtypes = [P, i16p, u16p, f32p, f32p]
     L.pts_solver_profile_iteration.argtypes = [P, f64p, ctypes.POINTER(ctypes.c_uint32), ctypes.POINTER(ctypes.c_double)]
     L.pts_engine_profile_iteration.argtypes = [P, ctypes.c_float, ctypes.c_float, ctypes.c_float, ctypes.c_int, f64p,
                                                ctypes.POINTER(ctypes.c_uint32), ctypes.POINTER(ctypes.c_double)]

@@ -272,11 +272,25 @@ class TreeArrays:
 class Engine:
     """pts_engine_*: the GPU backend boundary (crates/solver/src/gpu/engine.rs)."""
 
-    def __init__(self, tree: TreeArrays, regrets, sums, device: int = 0, precision: int = 0, scratch_budget: int = 0):
+    def __init__(self, tree: TreeArrays, regrets, sums, device: int = 0, precision: int = 0, scratch_budget: int = 0,
+                 q16=None):
+        """f32 storage (regrets, sums), or with q16 = (slot per node, i16
+        regrets, u16 sums, regret scales, strategy-sum scales) 16-bit storage
+        (HostStorage::Compressed)."""
         self.tree = tree
         h = ctypes.c_void_p()
-        check(lib().pts_engine_create(device, ctypes.byref(tree.desc), np.ascontiguousarray(regrets, np.float32),
-                                      np.ascontiguousarray(sums, np.float32), precision, scratch_budget, ctypes.byref(h)))
+        if q16 is None:
+            check(lib().pts_engine_create(device, ctypes.byref(tree.desc), np.ascontiguousarray(regrets, np.float32),
+                                          np.ascontiguousarray(sums, np.float32), precision, scratch_budget,
+                                          ctypes.byref(h)))
+            self.slots = 0
+        else:
+            slot, r, s, rs, ss = q16
+            self.slots = len(rs)
+            check(lib().pts_engine_create_q(device, ctypes.byref(tree.desc), np.ascontiguousarray(slot, np.uint32),
+                                            self.slots, np.ascontiguousarray(r, np.int16), np.ascontiguousarray(s, np.uint16),
+                                            np.ascontiguousarray(rs, np.float32), np.ascontiguousarray(ss, np.float32),
+                                            precision, scratch_budget, ctypes.byref(h)))
         self.h = h
         self.storage_len = tree.desc.storage_len
         self.n = [tree.desc.num_hands[0], tree.desc.num_hands[1]]
@@ -309,6 +323,12 @@ class Engine:
         s = np.zeros(self.storage_len, np.float32)
         check(lib().pts_engine_download(self.h, r, s, self.storage_len))
         return r, s
+
+    def download_q(self):
+        r, s = np.zeros(self.storage_len, np.int16), np.zeros(self.storage_len, np.uint16)
+        rs, ss = np.zeros(self.slots, np.float32), np.zeros(self.slots, np.float32)
+        check(lib().pts_engine_download_q(self.h, r, s, rs, ss))
+        return r, s, rs, ss
 
     def upload(self, r, s):
         check(lib().pts_engine_upload(self.h, np.ascontiguousarray(r, np.float32), np.ascontiguousarray(s, np.float32),

@@ -183,6 +183,34 @@ def test_compressed_traversals_from_random_storage():
     assert same(qa.strategy, qb.strategy) and same(qa.ev, qb.ev), describe(qa.ev, qb.ev)
 
 
+def test_compressed_engine_layer_traversals():
+    # pts_engine_create_q (HostStorage::Compressed behind the engine boundary):
+    # one traversal per mode from random 16-bit storage, against the oracle
+    cfg = compressed(spots.turn_spot(0))
+    o, t = oracle_tree(cfg)
+    nodes = o.nodes()
+    slot = np.where(nodes["kind"] <= 1, nodes["slot"], 0).astype(np.uint32)
+    n, k = o.storage_len, o.stats().player_nodes
+    rng = np.random.default_rng(9)
+    r = rng.integers(-32768, 32768, n).astype(np.int16)
+    s = rng.integers(0, 65536, n).astype(np.uint16)
+    rs = (rng.random(k) * 30).astype(np.float32)
+    ss = (rng.random(k) * 10).astype(np.float32)
+    ss[rng.random(k) < 0.1] = 0.0
+    o.set_storage_q(r, s, rs, ss)
+    e = Engine(t, None, None, precision=EXACT, q16=(slot, r, s, rs, ss))
+    for p in (0, 1):
+        for mode in (1, 2):
+            assert same(e.traverse(p, mode), o.traverse_root_f64(p, mode)), f"p{p} mode{mode}"
+    d = O.discount(4)
+    for p in (0, 1):
+        g = e.traverse(p, 0, (d[0], d[1], d[2], 1))
+        c = o.traverse_root(p, 0, 4)
+        assert same(g.astype(np.float32), c), f"cfr p{p}: {describe(g, c)}"
+        for x, y in zip(e.download_q(), o.storage_q()):
+            assert same(x, y), describe(x, y)
+
+
 def test_compressed_queries_and_fast_precision():
     cfg = compressed(spots.turn_spot(2))
     g, c = Solver(cfg, precision=EXACT), O.OracleSolver(cfg)
