@@ -237,6 +237,21 @@ def test_compressed_queries_and_fast_precision():
     assert np.max(np.abs(p32.query([]).range_strategy() - g.query([]).range_strategy())) < 0.03
 
 
+def test_dumped_trainer_spot_bit_exact():
+    # a spot in the trainer's POKER_TRAINER_DUMP format (review.rs:1176-1213),
+    # facing a bet mid-street with rake, replayed
+    from pts.dump import read_dump
+    from test_dump import TURN_FACING
+    _, cfg = read_dump(TURN_FACING)
+    g, c = Solver(cfg, precision=EXACT), O.OracleSolver(cfg)
+    g.iterate(6)
+    c.iterate(6)
+    rg, sg = g.storage()
+    rc, sc = c.storage()
+    assert same(rg, rc) and same(sg, sc), describe(rg, rc)
+    assert g.exploitability() == c.exploitability_f64()
+
+
 def test_solve_loop_matches_the_oracle():
     cfg = spots.river_spot(4)
     g, c = Solver(cfg, precision=EXACT), O.OracleSolver(cfg)
