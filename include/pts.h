@@ -155,7 +155,7 @@ typedef struct { /* crates/holdem/src/menu.rs:100-119 */
   double add_allin_threshold, force_allin_threshold, merge_threshold;
 } pts_menu_c;
 
-typedef struct { /* crates/solver/src/lib.rs:73-105 (exact chance, f32 storage) */
+typedef struct { /* crates/solver/src/lib.rs:73-105 */
   int32_t board[5];
   int32_t board_len;
   float ranges[2][1326];
@@ -172,12 +172,18 @@ typedef struct { /* crates/solver/src/lib.rs:73-105 (exact chance, f32 storage) 
   float prune;        /* must be 0 */
   int32_t storage_mode; /* storage.rs:74-81: 0 f32, 1 16-bit ("compressed"), 2 auto (16-bit above 1 GB of f32);
                            batched and depth-limited solves take f32 only */
-  int32_t isomorphism;  /* solved without suit isomorphism; rejected if it would apply */
+  int32_t isomorphism;  /* suit isomorphism (iso.rs) */
   int32_t beta_some;
   double alpha, beta, gamma; /* crates/solver/src/cfr.rs:47-63 */
   int32_t restart_average, unweighted_average;
   int32_t backend;
-  int32_t _pad;
+  /* tree.rs:128-136 ChanceMode: 0 Exact; 1 Sampled { turn, river, seed }: a
+   * stratified sample of sample_turn turn cards and sample_river river cards
+   * per branch (sampling.rs; all-in runouts stay exact), drawn from rand
+   * 0.9's StdRng seeded with sample_seed */
+  int32_t chance_mode;
+  uint32_t sample_turn, sample_river;
+  uint64_t sample_seed;
 } pts_config_c;
 
 typedef struct { uint64_t player_nodes, chance_nodes, terminal_nodes, storage_cells, memory_bytes; } pts_stats_c;

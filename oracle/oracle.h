@@ -47,7 +47,9 @@ typedef struct {
   double alpha, beta, gamma;
   int32_t restart_average, unweighted_average;
   int32_t backend; /* 0 cpu, 1 gpu, 2 auto */
-  int32_t _pad;
+  int32_t chance_mode; /* 0 exact, 1 sampled (tree.rs:128-136) */
+  uint32_t sample_turn, sample_river;
+  uint64_t sample_seed;
 } orc_config_c;
 
 typedef struct {
@@ -101,6 +103,10 @@ void orc_solver_hands(void* s, int32_t p, uint16_t* combos, float* weights);
 void orc_solver_get_storage(void* s, float* regrets, float* sums);
 int32_t orc_solver_set_storage(void* s, const float* regrets, const float* sums); /* -1: 16-bit storage */
 int32_t orc_solver_is_compressed(void* s);
+/* rand 0.9.5's StdRng pieces and sampling.rs's card sample (test hooks) */
+void orc_chacha_block(int32_t rounds, const uint32_t* in, uint32_t* out);
+void orc_stdrng_u64(uint64_t seed, uint64_t n, uint64_t* out);
+int32_t orc_sample_cards(const int32_t* available, int32_t n, int32_t k, uint64_t seed, int32_t* out);
 /* one block through the 16-bit codec from zero storage (rq: regrets, sq: sums) */
 void orc_encode_block(int32_t sums, const float* v, uint64_t n, int16_t* rq, uint16_t* sq, float* scale);
 int32_t orc_solver_get_storage_q(void* s, int16_t* regrets, uint16_t* sums, float* regret_scale, float* strat_scale);

@@ -66,6 +66,10 @@ static SolveConfig config_from(const orc_config_c* c) {
   s.dcfr.restart_average = c->restart_average != 0;
   s.dcfr.unweighted_average = c->unweighted_average != 0;
   s.backend = (Backend)c->backend;
+  s.chance_mode = c->chance_mode;
+  s.sample_turn = c->sample_turn;
+  s.sample_river = c->sample_river;
+  s.sample_seed = c->sample_seed;
   return s;
 }
 
@@ -249,6 +253,18 @@ int32_t orc_solver_set_storage(void* s, const float* regrets, const float* sums)
   std::memcpy(v->regrets.data(), regrets, v->regrets.size() * sizeof(float));
   std::memcpy(v->strat.data(), sums, v->strat.size() * sizeof(float));
   return 0;
+}
+void orc_chacha_block(int32_t rounds, const uint32_t* in, uint32_t* out) { chacha_block(rounds, in, out); }
+void orc_stdrng_u64(uint64_t seed, uint64_t n, uint64_t* out) {
+  StdRng r(seed);
+  for (uint64_t i = 0; i < n; i++) out[i] = r.next_u64();
+}
+int32_t orc_sample_cards(const int32_t* available, int32_t n, int32_t k, uint64_t seed, int32_t* out) {
+  StdRng r(seed);
+  std::vector<int> av(available, available + n);
+  std::vector<int> s = sample_cards(av, (size_t)k, r);
+  for (size_t i = 0; i < s.size(); i++) out[i] = s[i];
+  return (int32_t)s.size();
 }
 void orc_encode_block(int32_t sums, const float* v, uint64_t n, int16_t* rq, uint16_t* sq, float* scale) {
   encode_block(sums != 0, v, n, rq, sq, scale);

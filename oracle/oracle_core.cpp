@@ -488,8 +488,121 @@ std::vector<Action> Tree::actions(size_t id) const {
   return out;
 }
 
+// ---- rand 0.9.5's StdRng and the sampled chance cards. The crate is not in
+// the reference tree (Cargo.lock pins rand 0.9.5, rand_chacha 0.9.0,
+// rand_core 0.9.5); restated from the published algorithms:
+//   StdRng = ChaCha12Rng: the ChaCha block function (D. J. Bernstein; RFC
+//     8439's constants and key layout) with 12 rounds, a 64-bit block
+//     counter in words 12-13 and stream 0 in words 14-15; 16-word blocks in
+//     counter order, read as u32 words; next_u64 = two words, the first as
+//     the low half (rand_core BlockRng);
+//   seed_from_u64: PCG32 (M. O'Neill, XSH RR output; multiplier
+//     6364136223846793005, increment 11634580027462260723), the state
+//     advanced before each of the eight 32-bit key words;
+//   random::<f64>() = (next_u64 >> 11) * 2^-53.
+static inline uint32_t rotl32(uint32_t x, int r) { return (x << r) | (x >> (32 - r)); }
+void chacha_block(int rounds, const uint32_t in[16], uint32_t out[16]) {
+  uint32_t x[16];
+  std::memcpy(x, in, sizeof x);
+  auto qr = [&](int a, int b, int c, int d) {
+    x[a] += x[b]; x[d] = rotl32(x[d] ^ x[a], 16);
+    x[c] += x[d]; x[b] = rotl32(x[b] ^ x[c], 12);
+    x[a] += x[b]; x[d] = rotl32(x[d] ^ x[a], 8);
+    x[c] += x[d]; x[b] = rotl32(x[b] ^ x[c], 7);
+  };
+  for (int i = 0; i < rounds; i += 2) {
+    qr(0, 4, 8, 12); qr(1, 5, 9, 13); qr(2, 6, 10, 14); qr(3, 7, 11, 15);
+    qr(0, 5, 10, 15); qr(1, 6, 11, 12); qr(2, 7, 8, 13); qr(3, 4, 9, 14);
+  }
+  for (int i = 0; i < 16; i++) out[i] = x[i] + in[i];
+}
+
+StdRng::StdRng(uint64_t state) {
+  const uint64_t MUL = 6364136223846793005ull, INC = 11634580027462260723ull;
+  for (int i = 0; i < 8; i++) {
+    state = state * MUL + INC;
+    const uint32_t xs = (uint32_t)(((state >> 18) ^ state) >> 27);
+    const uint32_t rot = (uint32_t)(state >> 59);
+    key[i] = (xs >> rot) | (xs << ((32 - rot) & 31));
+  }
+}
+void StdRng::refill() {
+  for (int b = 0; b < 4; b++) {
+    const uint32_t in[16] = {0x61707865u, 0x3320646eu, 0x79622d32u, 0x6b206574u, key[0], key[1], key[2], key[3],
+                             key[4], key[5], key[6], key[7], (uint32_t)counter, (uint32_t)(counter >> 32), 0u, 0u};
+    chacha_block(12, in, buf + 16 * b);
+    counter++;
+  }
+}
+uint32_t StdRng::next_u32() {
+  if (idx >= 64) {
+    refill();
+    idx = 0;
+  }
+  return buf[idx++];
+}
+uint64_t StdRng::next_u64() {
+  if (idx < 63) {
+    const uint64_t v = (uint64_t)buf[idx + 1] << 32 | buf[idx];
+    idx += 2;
+    return v;
+  }
+  if (idx >= 64) {
+    refill();
+    idx = 2;
+    return (uint64_t)buf[1] << 32 | buf[0];
+  }
+  const uint64_t x = buf[63];
+  refill();
+  idx = 1;
+  return (uint64_t)buf[0] << 32 | x;
+}
+double StdRng::next_f64() { return (double)(next_u64() >> 11) * (1.0 / 9007199254740992.0); }
+
+// sampling.rs:9-56: k of the available cards, every suit in proportion to its
+// remaining cards (largest remainder, ties broken by a random draw), ranks
+// spread evenly within a suit from a random offset; sorted.
+std::vector<int> sample_cards(const std::vector<int>& available, size_t k, StdRng& rng) {
+  if (k >= available.size()) return available;
+  std::vector<int> by_suit[4];
+  for (int c : available) by_suit[c & 3].push_back(c);
+  for (auto& v : by_suit) std::sort(v.begin(), v.end(), [](int a, int b) { return (a >> 2) < (b >> 2); });
+  const double total = (double)available.size();
+  size_t quotas[4];
+  std::vector<std::tuple<double, double, int>> fracs;
+  for (int su = 0; su < 4; su++) {
+    const double exact = (double)k * (double)by_suit[su].size() / total;
+    quotas[su] = (size_t)std::floor(exact);
+    fracs.emplace_back(exact - std::floor(exact), rng.next_f64(), su);
+  }
+  size_t assigned = quotas[0] + quotas[1] + quotas[2] + quotas[3];
+  std::sort(fracs.begin(), fracs.end(), [](const auto& a, const auto& b) { return b < a; });
+  for (const auto& f : fracs) {
+    if (assigned >= k) break;
+    const int su = std::get<2>(f);
+    if (quotas[su] < by_suit[su].size()) {
+      quotas[su]++;
+      assigned++;
+    }
+  }
+  std::vector<int> out;
+  for (int su = 0; su < 4; su++) {
+    const size_t m = by_suit[su].size(), q = quotas[su];
+    if (q == 0) continue;
+    const double step = (double)m / (double)q;
+    const double offset = rng.next_f64() * step;
+    for (size_t j = 0; j < q; j++) {
+      const size_t idx = std::min((size_t)std::floor(offset + (double)j * step), m - 1);
+      out.push_back(by_suit[su][idx]);
+    }
+  }
+  std::sort(out.begin(), out.end());
+  out.erase(std::unique(out.begin(), out.end()), out.end());
+  return out;
+}
+
 namespace {
-struct TreeParams {  // tree.rs:276-335 (exact chance only)
+struct TreeParams {  // tree.rs:276-335
   Street root_street;
   const std::vector<int>* root_board;
   const PostflopMenu* menu;
@@ -498,10 +611,16 @@ struct TreeParams {  // tree.rs:276-335 (exact chance only)
   size_t num_hands[2];
   double rake_rate, rake_cap;
   const Isomorphism* iso;
+  int chance_mode = 0;  // tree.rs:128-136
+  size_t sample_turn = 0, sample_river = 0;
   const PostflopMenu& menu_for(Street s) const { return s == root_street ? *menu : *future_menu; }
   std::vector<Action> actions(const BettingState& st) const { return postflop_actions(st.situation(big_blind), menu_for(st.street)); }
   float rake(Chips pot) const { return (float)std::min((double)pot * rake_rate, rake_cap); }
-  void chance_cards(const std::vector<int>& dealt, std::vector<int>& canon, std::vector<std::array<int, 3>>& refs, size_t& n_avail) const {
+  // tree.rs:314-334: a sample of the available cards when sampling applies
+  // (not an all-in runout, fewer cards than available), else every card up
+  // to suit isomorphism
+  void chance_cards(const std::vector<int>& dealt, bool all_in, Street next, StdRng& rng, std::vector<int>& canon,
+                    std::vector<std::array<int, 3>>& refs, size_t& n_avail) const {
     std::vector<int> board = *root_board;
     board.insert(board.end(), dealt.begin(), dealt.end());
     uint64_t used = 0;
@@ -509,23 +628,29 @@ struct TreeParams {  // tree.rs:276-335 (exact chance only)
     std::vector<int> available;
     for (int c = 0; c < 52; c++) if (!(used & card_bit(c))) available.push_back(c);
     n_avail = available.size();
+    const size_t k = next == Turn ? sample_turn : sample_river;
+    if (chance_mode == 1 && !all_in && k < n_avail) {
+      canon = sample_cards(available, k, rng);
+      refs.clear();
+      return;
+    }
     iso->chance_cards(board, available, canon, refs);
   }
 };
 
-void count_after_street(const TreeParams& P, const BettingState& st, std::vector<int>& dealt, TreeStats& stats);
-void count_player(const TreeParams& P, const BettingState& st, std::vector<int>& dealt, TreeStats& stats) {  // tree.rs:345-362
+void count_after_street(const TreeParams& P, const BettingState& st, std::vector<int>& dealt, StdRng& rng, TreeStats& stats);
+void count_player(const TreeParams& P, const BettingState& st, std::vector<int>& dealt, StdRng& rng, TreeStats& stats) {  // tree.rs:345-362
   stats.player_nodes += 1;
   std::vector<Action> acts = P.actions(st);
   stats.storage_cells += (uint64_t)(acts.size() * P.num_hands[st.to_act]);
   for (const Action& a : acts) {
     Transition t = apply(st, a);
-    if (t.kind == Continue) count_player(P, t.st, dealt, stats);
+    if (t.kind == Continue) count_player(P, t.st, dealt, rng, stats);
     else if (t.kind == FoldT) stats.terminal_nodes += 1;
-    else count_after_street(P, t.st, dealt, stats);
+    else count_after_street(P, t.st, dealt, rng, stats);
   }
 }
-void count_after_street(const TreeParams& P, const BettingState& st, std::vector<int>& dealt, TreeStats& stats) {  // tree.rs:364-401
+void count_after_street(const TreeParams& P, const BettingState& st, std::vector<int>& dealt, StdRng& rng, TreeStats& stats) {  // tree.rs:364-401
   if (st.street == River) { stats.terminal_nodes += 1; return; }
   stats.chance_nodes += 1;
   bool all_in = st.stack[0] == 0 || st.stack[1] == 0;
@@ -533,7 +658,7 @@ void count_after_street(const TreeParams& P, const BettingState& st, std::vector
   std::vector<int> cards;
   std::vector<std::array<int, 3>> refs;
   size_t n_avail;
-  P.chance_cards(dealt, cards, refs, n_avail);
+  P.chance_cards(dealt, all_in, next, rng, cards, refs, n_avail);
   bool uniform = refs.empty() && !P.iso->enabled;
   size_t reps = uniform ? 1 : cards.size();
   TreeStats sub;
@@ -542,10 +667,10 @@ void count_after_street(const TreeParams& P, const BettingState& st, std::vector
     if (all_in) {
       BettingState s2 = st;
       s2.street = next;
-      count_after_street(P, s2, dealt, sub);
+      count_after_street(P, s2, dealt, rng, sub);
     } else {
       BettingState s2 = BettingState::street_start(next, st.pot, st.stack[0], st.stack[1], P.big_blind);
-      count_player(P, s2, dealt, sub);
+      count_player(P, s2, dealt, rng, sub);
     }
     dealt.pop_back();
   }
@@ -568,7 +693,8 @@ struct Builder {  // tree.rs:403-574
   const TreeParams& P;
   Tree t;
   std::unordered_map<uint64_t, uint32_t> board_index;
-  explicit Builder(const TreeParams& p) : P(p) {}
+  StdRng rng;  // tree.rs:415-429: seeded with the sampled mode's seed, else 0
+  Builder(const TreeParams& p, uint64_t seed) : P(p), rng(seed) {}
   size_t alloc(size_t k) { size_t f = t.nodes.size(); t.nodes.resize(f + k, placeholder()); return f; }
   void fill_player(size_t id, const BettingState& st, const int32_t invest[2], std::vector<int>& dealt) {
     std::vector<Action> acts = P.actions(st);
@@ -621,7 +747,7 @@ struct Builder {  // tree.rs:403-574
     std::vector<int> cards;
     std::vector<std::array<int, 3>> refs;
     size_t n_avail;
-    P.chance_cards(dealt, cards, refs, n_avail);
+    P.chance_cards(dealt, all_in, next, rng, cards, refs, n_avail);
     size_t blen = P.root_board->size() + dealt.size();
     float compatible = (float)(52 - blen - 4);
     size_t total_cards = cards.size() + refs.size();
@@ -1488,6 +1614,9 @@ static TreeParams params_for(const SolveConfig& cfg, const HandSet& hs, const Is
   P.rake_rate = cfg.rake_rate;
   P.rake_cap = cfg.rake_cap;
   P.iso = &iso;
+  P.chance_mode = cfg.chance_mode;
+  P.sample_turn = cfg.sample_turn;
+  P.sample_river = cfg.sample_river;
   return P;
 }
 
@@ -1506,7 +1635,8 @@ std::string estimate(const SolveConfig& cfg, TreeStats& out) {  // lib.rs:355-36
   TreeParams P = params_for(cfg, hs, iso);
   std::vector<int> dealt;
   out = TreeStats();
-  count_player(P, cfg.root, dealt, out);
+  StdRng rng(0);  // tree.rs:337-343
+  count_player(P, cfg.root, dealt, rng, out);
   out.compressed = storage_compressed(cfg, out);
   return "";
 }
@@ -1519,7 +1649,8 @@ std::string make_solver(const SolveConfig& cfg, Solver& s) {  // lib.rs:365-396
   TreeParams P = params_for(s.config, s.hands, s.iso);
   std::vector<int> dealt;
   s.stats = TreeStats();
-  count_player(P, s.config.root, dealt, s.stats);
+  StdRng rng(0);
+  count_player(P, s.config.root, dealt, rng, s.stats);
   s.stats.compressed = storage_compressed(s.config, s.stats);
   uint64_t needed = s.stats.memory_bytes();
   if (needed > s.config.max_memory_bytes) {
@@ -1527,7 +1658,7 @@ std::string make_solver(const SolveConfig& cfg, Solver& s) {  // lib.rs:365-396
     snprintf(buf, sizeof buf, "game tree needs ~%.2f GB but the limit is %.2f GB", (double)needed / 1e9, (double)s.config.max_memory_bytes / 1e9);
     return buf;
   }
-  Builder b(P);
+  Builder b(P, s.config.chance_mode == 1 ? s.config.sample_seed : 0);
   b.t.nodes.push_back(placeholder());
   int32_t inv0[2] = {0, 0};
   b.fill_player(0, s.config.root, inv0, dealt);

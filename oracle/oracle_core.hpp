@@ -220,6 +220,10 @@ struct SolveConfig {  // crates/solver/src/lib.rs:73-135
   double rake_rate = 0.0, rake_cap = 0.0;
   int storage_mode = 0;  // storage.rs:74-81: 0 F32, 1 Compressed, 2 Auto (compressed above 1 GB of f32)
   bool isomorphism = true;
+  // tree.rs:128-136: 0 Exact, 1 Sampled { turn, river, seed }
+  int chance_mode = 0;
+  uint32_t sample_turn = 0, sample_river = 0;
+  uint64_t sample_seed = 0;
   DcfrParams dcfr;
   Backend backend = Backend::Cpu;
 };
@@ -288,6 +292,22 @@ struct Solver {
   std::string query(const std::vector<Step>& path, NodeInfo& out);
   std::string ranges_after(const std::vector<Step>& path, std::vector<float> out[2]);
 };
+
+// rand 0.9.5's StdRng (ChaCha12Rng seeded by seed_from_u64) and the
+// reference's stratified card sample (sampling.rs)
+void chacha_block(int rounds, const uint32_t in[16], uint32_t out[16]);
+struct StdRng {
+  uint32_t key[8];
+  uint64_t counter = 0;
+  uint32_t buf[64];
+  size_t idx = 64;
+  explicit StdRng(uint64_t seed);
+  void refill();
+  uint32_t next_u32();
+  uint64_t next_u64();
+  double next_f64();
+};
+std::vector<int> sample_cards(const std::vector<int>& available, size_t k, StdRng& rng);
 
 // One block through the 16-bit codec from zero storage: the stored integers
 // (rq for regrets, sq for sums) and the block's new scale.

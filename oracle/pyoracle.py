@@ -97,6 +97,11 @@ def _declare(L):
     L.orc_solver_set_storage_q.restype = ctypes.c_int32
     L.orc_solver_set_storage_q.argtypes = [P, i16p, u16p, fp, fp]
     L.orc_encode_block.argtypes = [ctypes.c_int32, fp, ctypes.c_uint64, i16p, u16p, fp]
+    u32p, u64p, i32p = (np.ctypeslib.ndpointer(dtype=t) for t in (np.uint32, np.uint64, np.int32))
+    L.orc_chacha_block.argtypes = [ctypes.c_int32, u32p, u32p]
+    L.orc_stdrng_u64.argtypes = [ctypes.c_uint64, ctypes.c_uint64, u64p]
+    L.orc_sample_cards.restype = ctypes.c_int32
+    L.orc_sample_cards.argtypes = [i32p, ctypes.c_int32, ctypes.c_int32, ctypes.c_uint64, i32p]
     L.orc_solver_iterations.restype = ctypes.c_uint32
     L.orc_solver_iterations.argtypes = [P]
     L.orc_solver_set_iterations.argtypes = [P, ctypes.c_uint32]
@@ -364,6 +369,28 @@ def encode_block(values, sums):
     rq, sq, sc = np.zeros(len(v), np.int16), np.zeros(len(v), np.uint16), np.zeros(1, np.float32)
     lib().orc_encode_block(int(sums), v, len(v), rq, sq, sc)
     return (sq if sums else rq), float(sc[0])
+
+
+def chacha_block(words, rounds):
+    """The ChaCha block function on 16 input words (RFC 8439 section 2.3)."""
+    out = np.zeros(16, np.uint32)
+    lib().orc_chacha_block(rounds, np.ascontiguousarray(words, np.uint32), out)
+    return out
+
+
+def stdrng_u64(seed, n):
+    """The first n next_u64 draws of rand 0.9's StdRng::seed_from_u64(seed)."""
+    out = np.zeros(n, np.uint64)
+    lib().orc_stdrng_u64(seed, n, out)
+    return out
+
+
+def sample_cards(available, k, seed):
+    """sampling.rs's stratified sample with a fresh StdRng::seed_from_u64(seed)."""
+    av = np.ascontiguousarray(available, np.int32)
+    out = np.zeros(max(len(av), 1), np.int32)
+    n = lib().orc_sample_cards(av, len(av), k, seed, out)
+    return out[:n].tolist()
 
 
 class OracleHandSet:

@@ -120,6 +120,13 @@ static std::string validate(const pts_config_c& c, TreeSpec& spec, Hands& hands)
   if (street != c.street) return "invalid board: board doesn't match the root street";
   if (c.prune != 0.0f) return "invalid input: range pruning is not supported";
   if (c.storage_mode < 0 || c.storage_mode > 2) return "invalid input: storage mode";
+  if (c.chance_mode < 0 || c.chance_mode > 1) return "invalid input: chance mode";
+  if (c.chance_mode == 1 && (c.sample_turn == 0 || c.sample_river == 0))
+    return "invalid input: sampled chance deals at least one turn and one river card";
+  spec.sampled = c.chance_mode == 1;
+  spec.sample_turn = c.sample_turn;
+  spec.sample_river = c.sample_river;
+  spec.sample_seed = c.sample_seed;
   const float* r[2] = {c.ranges[0], c.ranges[1]};
   hands = make_hands(r, bits);
   for (int p = 0; p < 2; p++)

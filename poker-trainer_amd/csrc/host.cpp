@@ -3,9 +3,97 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstring>
+#include <tuple>
 #include <unordered_map>
 
 namespace pts {
+
+// ----------------------------------------------------------------- sampled chance
+void ChaChaRng::block(const uint32_t in[16], uint32_t out[16], int rounds) {
+  uint32_t v[16];
+  std::memcpy(v, in, sizeof v);
+  auto rot = [](uint32_t x, int r) { return (x << r) | (x >> (32 - r)); };
+  auto quarter = [&](int a, int b, int c, int d) {
+    v[a] += v[b]; v[d] ^= v[a]; v[d] = rot(v[d], 16);
+    v[c] += v[d]; v[b] ^= v[c]; v[b] = rot(v[b], 12);
+    v[a] += v[b]; v[d] ^= v[a]; v[d] = rot(v[d], 8);
+    v[c] += v[d]; v[b] ^= v[c]; v[b] = rot(v[b], 7);
+  };
+  for (int r = 0; r < rounds; r += 2) {
+    for (int col = 0; col < 4; col++) quarter(col, col + 4, col + 8, col + 12);
+    for (int dg = 0; dg < 4; dg++) quarter(dg, 4 + (dg + 1) % 4, 8 + (dg + 2) % 4, 12 + (dg + 3) % 4);
+  }
+  for (int w = 0; w < 16; w++) out[w] = v[w] + in[w];
+}
+
+ChaChaRng::ChaChaRng(uint64_t seed) {
+  uint64_t st = seed;
+  for (uint32_t& k : key_) {
+    st = st * 6364136223846793005ull + 11634580027462260723ull;
+    const uint32_t x = (uint32_t)(((st >> 18) ^ st) >> 27);
+    const unsigned r = (unsigned)(st >> 59);
+    k = r ? (x >> r) | (x << (32 - r)) : x;
+  }
+}
+
+uint32_t ChaChaRng::next_word() {
+  if (pos_ == 16) {
+    const uint32_t in[16] = {0x61707865u, 0x3320646eu, 0x79622d32u, 0x6b206574u,
+                             key_[0], key_[1], key_[2], key_[3], key_[4], key_[5], key_[6], key_[7],
+                             (uint32_t)blocks_, (uint32_t)(blocks_ >> 32), 0u, 0u};
+    block(in, words_, 12);
+    blocks_++;
+    pos_ = 0;
+  }
+  return words_[pos_++];
+}
+
+// The stream is one sequence of words whatever the buffering, so two
+// consecutive words make a u64 (rand_core's BlockRng reads the same pair,
+// also across its 4-block buffer's end).
+uint64_t ChaChaRng::next_u64() {
+  const uint64_t lo = next_word();
+  return (uint64_t)next_word() << 32 | lo;
+}
+
+std::vector<int> stratified_cards(const std::vector<int>& available, size_t k, ChaChaRng& rng) {
+  if (k >= available.size()) return available;
+  std::vector<int> suit[4];
+  for (int c : available) suit[c & 3].push_back(c);
+  for (auto& v : suit) std::sort(v.begin(), v.end());  // by rank: one suit's cards differ in rank only
+  const double n = (double)available.size();
+  size_t quota[4];
+  std::tuple<double, double, int> rem[4];
+  size_t given = 0;
+  for (int s = 0; s < 4; s++) {
+    const double share = (double)k * (double)suit[s].size() / n;
+    const double whole = std::floor(share);
+    quota[s] = (size_t)whole;
+    given += quota[s];
+    rem[s] = std::make_tuple(share - whole, rng.next_f64(), s);
+  }
+  std::sort(rem, rem + 4, [](const auto& a, const auto& b) { return a > b; });
+  for (const auto& r : rem) {
+    if (given >= k) break;
+    const int s = std::get<2>(r);
+    if (quota[s] < suit[s].size()) {
+      quota[s]++;
+      given++;
+    }
+  }
+  std::vector<int> out;
+  for (int s = 0; s < 4; s++) {
+    if (quota[s] == 0) continue;
+    const size_t m = suit[s].size();
+    const double step = (double)m / (double)quota[s];
+    const double start = rng.next_f64() * step;
+    for (size_t j = 0; j < quota[s]; j++) out.push_back(suit[s][std::min((size_t)std::floor(start + (double)j * step), m - 1)]);
+  }
+  std::sort(out.begin(), out.end());
+  out.erase(std::unique(out.begin(), out.end()), out.end());
+  return out;
+}
 
 void combo_pair(int combo, int& hi, int& lo) {
   int h = 1;
@@ -133,6 +221,7 @@ struct TreeWalker {
   TreeCounts counts;
   std::vector<int> dealt;
   std::unordered_map<uint64_t, uint32_t> board_of;
+  ChaChaRng rng;  // sampled chance (tree.rs:415-429)
 
   const Menu& menu_for(Street s) const { return s == spec.root_street ? spec.menu : spec.future_menu; }
   float rake(Chips pot) const { return (float)std::min((double)pot * spec.rake_rate, spec.rake_cap); }
@@ -233,6 +322,8 @@ struct TreeWalker {
     for (int c : dealt) used |= 1ull << c;
     std::vector<int> avail, cards;
     for (int c = 0; c < 52; c++) if (!(used >> c & 1)) avail.push_back(c);
+    const size_t k = next == kTurn ? spec.sample_turn : spec.sample_river;
+    const bool sample = spec.sampled && !all_in && k < avail.size();  // tree.rs:326-333
     // iso.rs:277-300: a card whose suit has a lower symmetric partner with
     // the same board ranks is merged onto that partner's card
     std::vector<IsoRef> refs;
@@ -240,8 +331,9 @@ struct TreeWalker {
     uint16_t m[4] = {0, 0, 0, 0};
     for (int c : spec.board) m[c & 3] |= (uint16_t)(1u << (c >> 2));
     for (int c : dealt) m[c & 3] |= (uint16_t)(1u << (c >> 2));
-    const bool iso = spec.iso && spec.iso->enabled && spec.iso->any();
-    for (int c : avail) {
+    const bool iso = !sample && spec.iso && spec.iso->enabled && spec.iso->any();
+    if (sample) cards = stratified_cards(avail, k, rng);
+    for (int c : sample ? std::vector<int>() : avail) {
       int found = -1, pair = -1;
       if (iso)
         for (int s0 = 0; s0 < (c & 3) && found < 0; s0++)
@@ -296,7 +388,7 @@ struct TreeWalker {
 }  // namespace
 
 TreeCounts count_tree(const TreeSpec& spec) {
-  TreeWalker w{spec, nullptr, {}, {}, {}};
+  TreeWalker w{spec, nullptr, {}, {}, {}, ChaChaRng(0)};  // tree.rs:337-343
   int32_t inv[2] = {0, 0};
   w.player(0, spec.root, inv);
   return w.counts;
@@ -304,7 +396,7 @@ TreeCounts count_tree(const TreeSpec& spec) {
 
 Tree build_tree(const TreeSpec& spec) {
   Tree t;
-  TreeWalker w{spec, &t, {}, {}, {}};
+  TreeWalker w{spec, &t, {}, {}, {}, ChaChaRng(spec.sampled ? spec.sample_seed : 0)};
   t.nodes.push_back(w.blank());
   int32_t inv[2] = {0, 0};
   w.player(0, spec.root, inv);

@@ -106,7 +106,40 @@ struct TreeSpec {
   bool depth_limit = false;
   // suit isomorphism: chance nodes keep one child per class of equivalent cards
   const Iso* iso = nullptr;
+  // tree.rs:128-136, ChanceMode::Sampled: chance nodes deal a stratified
+  // sample of sample_turn / sample_river cards (not in all-in runouts)
+  bool sampled = false;
+  uint32_t sample_turn = 0, sample_river = 0;
+  uint64_t sample_seed = 0;
 };
+
+// ----------------------------------------------------------------- sampled chance
+// The random stream of the reference's sampled mode: rand 0.9.5's StdRng
+// (rand_chacha 0.9.0's ChaCha12Rng) seeded by rand_core 0.9.5's
+// seed_from_u64, and its f64 draw. ChaCha with 12 rounds (RFC 8439 layout,
+// 64-bit block counter in words 12-13, stream 0 in words 14-15), key words
+// from PCG32 (XSH RR) steps of the seed; u64 draws take two consecutive
+// words, low first; f64 draws are (u64 >> 11) * 2^-53.
+class ChaChaRng {
+ public:
+  explicit ChaChaRng(uint64_t seed);
+  uint64_t next_u64();
+  double next_f64() { return (double)(next_u64() >> 11) * 0x1.0p-53; }
+  // the block function for `rounds` rounds (20 is RFC 8439's ChaCha20)
+  static void block(const uint32_t in[16], uint32_t out[16], int rounds);
+
+ private:
+  uint32_t next_word();
+  uint32_t key_[8];
+  uint64_t blocks_ = 0;  // blocks generated so far (the next block's counter)
+  uint32_t words_[16];
+  uint32_t pos_ = 16;    // next unread word of words_
+};
+
+// sampling.rs:9-56: k of the available cards (ascending), each suit given
+// its proportional share (largest remainders first, ties broken by a random
+// draw), spread over its ranks from a random offset.
+std::vector<int> stratified_cards(const std::vector<int>& available, size_t k, ChaChaRng& rng);
 
 struct TreeCounts { uint64_t player = 0, chance = 0, terminal = 0, cells = 0; };
 

@@ -154,7 +154,7 @@ BACKEND_CPU, BACKEND_GPU, BACKEND_AUTO = 0, 1, 2
 
 @dataclass
 class SolveConfig:
-    """lib.rs:73-135 (exact chance; F32 storage)."""
+    """lib.rs:73-135."""
     board: List[int]
     ranges: List[List[float]]
     root: BettingState
@@ -169,6 +169,8 @@ class SolveConfig:
     isomorphism: bool = False
     dcfr: DcfrParams = field(default_factory=DcfrParams)
     backend: int = BACKEND_GPU
+    # tree.rs:128-136: None (Exact) or (turn cards, river cards, seed) sampled per branch
+    sampled: Optional[tuple] = None
 
     def to_c(self) -> "ConfigC":
         c = ConfigC()
@@ -199,6 +201,9 @@ class SolveConfig:
         c.alpha, c.beta, c.gamma = d.alpha, (d.beta or 0.0), d.gamma
         c.restart_average, c.unweighted_average = int(d.restart_average), int(d.unweighted_average)
         c.backend = self.backend
+        if self.sampled is not None:
+            c.chance_mode = 1
+            c.sample_turn, c.sample_river, c.sample_seed = int(self.sampled[0]), int(self.sampled[1]), int(self.sampled[2])
         return c
 
 
@@ -228,7 +233,8 @@ class ConfigC(ctypes.Structure):
         ("isomorphism", ctypes.c_int32), ("beta_some", ctypes.c_int32),
         ("alpha", ctypes.c_double), ("beta", ctypes.c_double), ("gamma", ctypes.c_double),
         ("restart_average", ctypes.c_int32), ("unweighted_average", ctypes.c_int32),
-        ("backend", ctypes.c_int32), ("_pad", ctypes.c_int32),
+        ("backend", ctypes.c_int32), ("chance_mode", ctypes.c_int32),
+        ("sample_turn", ctypes.c_uint32), ("sample_river", ctypes.c_uint32), ("sample_seed", ctypes.c_uint64),
     ]
 
 

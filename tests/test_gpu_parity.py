@@ -252,6 +252,36 @@ def test_dumped_trainer_spot_bit_exact():
     assert g.exploitability() == c.exploitability_f64()
 
 
+@pytest.mark.parametrize("storage", [0, 1], ids=["f32", "16bit"])
+def test_sampled_flop_bit_exact(storage):
+    # chance-sampled mode (sampling.rs; gpu/testutil.rs:120-131 and its
+    # compressed variant): 3 turn and 2 river cards per branch, all-in
+    # runouts exact with isomorphism
+    from test_sampled import sampled_flop
+    cfg = sampled_flop()
+    cfg.storage_mode = storage
+    g, c = Solver(cfg, precision=EXACT), O.OracleSolver(cfg)
+    g.iterate(8)
+    c.iterate(8)
+    if storage:
+        for x, y in zip(g.storage_q(), c.storage_q()):
+            assert same(x, y), describe(x, y)
+    else:
+        rg, sg = g.storage()
+        rc, sc = c.storage()
+        assert same(rg, rc) and same(sg, sc), describe(rg, rc)
+    assert g.exploitability() == c.exploitability_f64()
+    q = [c.query([]).actions[0]]
+    q = q + [c.query(q).actions[0]]
+    assert q == ["check", "check"]
+    nodes = c.nodes()
+    chance = nodes["first_child"][nodes["first_child"][0]]  # check, check: the turn's chance node
+    turn = int(nodes["edge_card"][nodes["first_child"][chance]])  # its first sampled card
+    for pth in ([], q + [("card", turn)]):
+        a, b = g.query(pth), c.query(pth)
+        assert same(a.strategy, b.strategy) and same(a.ev, b.ev), describe(a.ev, b.ev)
+
+
 def test_solve_loop_matches_the_oracle():
     cfg = spots.river_spot(4)
     g, c = Solver(cfg, precision=EXACT), O.OracleSolver(cfg)
