@@ -44,6 +44,8 @@ constexpr int TERM_T = 64;    // terminal workgroup: one wave
 constexpr int UP_T = 64;      // up workgroup: one wave, one hand per lane
 constexpr int DOWN_T = 512;   // down workgroup
 constexpr int MAX_K = 21;     // ceil(1326 / 64): hands per lane in the terminal kernel
+constexpr int DG_A = 16;      // actions per down-pass group (the 16-bit pass stages DG_A rows in LDS)
+constexpr int DG_N = 8;       // nodes per down-pass group
 
 // One spot's hands and strength tables on the device (a batch holds many
 // spots; every record names its spot).
@@ -92,6 +94,12 @@ struct DPlan {
   const uint32_t *big, *mid, *m48;
   const uint32_t *big_host, *mid_host, *m48_host;
   uint32_t num_big, num_mid, num_m48;
+  // per traverser p: each segment's opponent decision nodes in groups the
+  // down pass runs together (one level of the segment, at most DG_A actions
+  // in all): entries dg_ent[p][dg_off[p][seg] .. dg_off[p][seg + 1]), each
+  // the member's local index | its group's number << 16
+  const uint32_t* dg_off[2];
+  const uint32_t* dg_ent[2];
 };
 
 // The nodes a traversal covers: {root} and [lo, hi) (the whole tree, or the
@@ -216,7 +224,7 @@ __global__ void collect_kernel(DPlan P, int p, const V* __restrict__ vals, V* __
 
 // ------------------------------------------------------------------ down
 // One workgroup per segment: the opponent's reach pushed down the segment's
-// members in breadth-first order. CFR mode (V = float): regret matching and
+// opponent decision nodes, a level at a time. CFR mode (V = float): regret matching and
 // the strategy-sum update (cfr.rs:156-207,389-419; river.rs:116-160).
 // Otherwise the normalized average strategy (cfr.rs:124-154).
 template <typename V, int MODE, bool Q>
@@ -225,8 +233,6 @@ __global__ void __launch_bounds__(1024) down_kernel(DPlan P, int p, uint32_t seg
                                                       uint32_t rowlen) {
   extern __shared__ uint4 smem_down[];
   __shared__ uint32_t red;
-  __shared__ uint32_t red3[3];  // per-node ORs, used in turn (each zeroed two nodes ahead)
-  __shared__ int32_t mx3[3];    // 16-bit storage: per-node maxima of the new sums (float bits), likewise
   const uint32_t seg = seg0 + blockIdx.x;
   const int o = p ^ 1;
   const uint32_t moff = P.seg_mem_off[seg], mcnt = P.seg_mem_cnt[seg];
@@ -236,14 +242,11 @@ __global__ void __launch_bounds__(1024) down_kernel(DPlan P, int p, uint32_t seg
   if (!span.has(root) && seg != span.seg) return;
   MemRec* rec = (MemRec*)smem_down;
   uint8_t* m_alive = (uint8_t*)(rec + P.max_members);
-  // 16-bit storage: a node's new sums (action-major, nq per action) between
-  // their maximum and their encoding; every thread reads back only its own
-  float* ybuf = (float*)(m_alive + ((P.max_members + 15) & ~15u));
+  // 16-bit storage: a group's new sums (its nodes' rows, action-major, nq
+  // per row) between their maxima and their encoding (after the staged
+  // group entries)
+  float* ybuf = (float*)(m_alive + ((P.max_members + 15) & ~15u) + 4 * (size_t)P.max_members);
   stage_records(P.mrec + moff, mcnt, rec);
-  if (tid < 3) {
-    red3[tid] = 0;
-    mx3[tid] = 0;
-  }
   __syncthreads();
   const uint32_t lane = tid & 63;
   const uint32_t sp_raw = P.seg_pslot[(size_t)p * P.num_segs + seg];
@@ -291,124 +294,166 @@ __global__ void __launch_bounds__(1024) down_kernel(DPlan P, int p, uint32_t seg
     g = prm->strat;
     weighted = prm->reach_weighted != 0;
   }
-  uint32_t ri = 0;
-  for (uint32_t m = 0; m < mcnt; m++) {
-    const MemRec& r = rec[m];
-    if ((int)r.kind != o || !span.has(r.node)) continue;
-    const uint32_t na = r.na, cs = B.s(r.cslot[p]), cl = r.clocal;
-    // Every wave keeps its own copy of the members' alive flags up to date
-    // (the same values at the same LDS bytes in every wave), so a flag a wave
-    // reads was written by that wave: no barrier is needed for them.
-    if (!m_alive[r.src[p]]) {
-      // A dead node's children are dead (cfr.rs:271-274, river.rs:103-107).
-      if (lane < na) m_alive[cl + lane] = 0;
-      if (tid < na) alive[cs + tid] = 0;
-      continue;
-    }
-    const V* rrow = reach + (size_t)B.s(r.slot[p]) * rowlen;
-    V* crow = reach + (size_t)cs * rowlen;
-    const uint64_t off = r.offset;
-    uint32_t bits = 0;
-    // 16-bit storage: the sums' decoding step (storage.rs:296-300) and the
-    // largest |new sum| of this thread's cells
-    const float sstep = Q ? sto.ssc[r.pslot] / kSumMax : 0.0f;
-    float lmax = 0.0f;
-    // cfr.rs:389-414: regret matching (16-bit regrets undecoded, the scale
-    // cancels: storage.rs:221-234), then the sums x * g + s (* reach)
-    auto matched = [&](uint32_t h, float (&sv)[MAXA]) {
-      float rg[MAXA];
+  // The opponent's decision nodes in groups of one level (their reach rows
+  // were written by earlier groups): the waves share the group's
+  // (node, 64-hand tile) units, two units' loads in flight per wave, then
+  // one barrier settles every node's children alive bits (a ballot per
+  // action per unit, an LDS OR per node) and, with 16-bit storage, every
+  // node's largest new sum.
+  __shared__ uint32_t gor[3][DG_N];  // per group node: its children's alive bits, buffers used in turn
+  __shared__ int32_t gmx[3][DG_N];   // 16-bit storage: per group node: the largest |new sum| (float bits)
+  uint32_t* ents = (uint32_t*)(m_alive + ((P.max_members + 15) & ~15u));  // the segment's groups, staged
+  const uint32_t e0 = P.dg_off[p][seg], ne = P.dg_off[p][seg + 1] - e0;
+  for (uint32_t k = tid; k < ne; k += blockDim.x) ents[k] = P.dg_ent[p][e0 + k];
+  if (tid < 3 * DG_N) {
+    (&gor[0][0])[tid] = 0;
+    (&gmx[0][0])[tid] = 0;
+  }
+  __syncthreads();
+  const uint32_t wave = tid >> 6, nw = blockDim.x >> 6;
+  const uint32_t tiles = (nq + 63) >> 6;
+  uint32_t gi = 0;
+  for (uint32_t e = 0; e < ne;) {
+    const uint32_t gid = ents[e] >> 16;
+    uint32_t gn = 1;
+    while (e + gn < ne && (ents[e + gn] >> 16) == gid) gn++;
+    const uint32_t units = gn * tiles;
+    for (uint32_t u0 = wave; u0 < units; u0 += 2 * nw) {
+      constexpr int UB = 2;
+      // the units' nodes, hands and inputs, every load issued before any store
+      uint32_t jj[UB], hx[UB], na[UB];
+      bool on[UB], act[UB];
+      uint64_t off[UB];
+      const MemRec* rp[UB];
+      V rr[UB];
+      float rg[UB][MAXA], xs[UB][MAXA];
+      V sv[UB][MAXA];
 #pragma unroll
-      for (int a = 0; a < MAXA; a++)
-        if (a < (int)na) rg[a] = Q ? (float)sto.rq[off + (size_t)a * nq + h] : sto.r[off + (size_t)a * nq + h];
-      float d = pos0(rg[0]);
-#pragma unroll
-      for (int a = 1; a < MAXA; a++)
-        if (a < (int)na) d += pos0(rg[a]);
-      const float inv = recip(d), uni = 1.0f / (float)na;
-#pragma unroll
-      for (int a = 0; a < MAXA; a++)
-        if (a < (int)na) sv[a] = inv == 0.0f ? uni : pos0(rg[a]) * inv;
-    };
-    auto new_sum = [&](uint32_t h, int a, float sv, float rr) {
-      const size_t i = off + (size_t)a * nq + h;
-      const float x = Q ? (float)sto.sq[i] * sstep : sto.s[i];
-      return weighted ? x * g + sv * rr : x * g + sv;
-    };
-    for (uint32_t h = tid; h < nq; h += blockDim.x) {
-      const V rr = rrow[h];
-      V sa[MAXA];
-      if (MODE == kCfr) {
-        float sv[MAXA];
-        matched(h, sv);
+      for (int k = 0; k < UB; k++) {
+        const uint32_t u = u0 + k * nw;
+        const uint32_t j = u < units ? u / tiles : 0;
+        const MemRec& r = rec[ents[e + j] & 0xFFFF];
+        rp[k] = &r;
+        jj[k] = j;
+        on[k] = u < units && span.has(r.node) && m_alive[r.src[p]];  // dead: no rows, no updates (cfr.rs:271-274)
+        const uint32_t h = (u - j * tiles) * 64 + lane;
+        act[k] = on[k] && h < nq;
+        hx[k] = act[k] ? h : 0;
+        na[k] = on[k] ? r.na : 0;
+        off[k] = r.offset;
+        rr[k] = on[k] ? reach[(size_t)B.s(r.slot[p]) * rowlen + hx[k]] : V(0);
+        const float sstep = Q && on[k] ? sto.ssc[r.pslot] / kSumMax : 0.0f;
 #pragma unroll
         for (int a = 0; a < MAXA; a++)
-          if (a < (int)na) {
-            const float y = new_sum(h, a, sv[a], (float)rr);
-            if (Q) {
-              lmax = fmaxf(lmax, fabsf(y));
-              ybuf[a * nq + h] = y;
-            } else {
-              sto.s[off + (size_t)a * nq + h] = y;
-            }
-            sa[a] = (V)sv[a];
+          if (a < (int)na[k]) {
+            const size_t i = off[k] + (size_t)a * nq + hx[k];
+            if (MODE == kCfr) rg[k][a] = Q ? (float)sto.rq[i] : sto.r[i];
+            xs[k][a] = Q ? (float)sto.sq[i] * sstep : sto.s[i];
           }
-      } else {
-        V sv[MAXA];
-#pragma unroll
-        for (int a = 0; a < MAXA; a++)
-          if (a < (int)na) {
-            const size_t i = off + (size_t)a * nq + h;
-            sv[a] = Q ? (V)((float)sto.sq[i] * sstep) : (V)sto.s[i];
-          }
-        V d = sv[0];
-#pragma unroll
-        for (int a = 1; a < MAXA; a++)
-          if (a < (int)na) d += sv[a];
-        const V inv = recip(d), uni = V(1) / (V)na;
-#pragma unroll
-        for (int a = 0; a < MAXA; a++)
-          if (a < (int)na) sa[a] = inv == V(0) ? uni : sv[a] * inv;
       }
 #pragma unroll
-      for (int a = 0; a < MAXA; a++)
-        if (a < (int)na) {
-          const V v = sa[a] * rr;
-          crow[(size_t)a * rowlen + h] = v;
-          if (v != V(0)) bits |= 1u << a;
-        }
-    }
-    // the children's alive bits: a ballot per action in each wave, one LDS
-    // OR per wave, one barrier
-    uint32_t wb = 0;
+      for (int k = 0; k < UB; k++) {
+        if (!on[k]) continue;
+        const MemRec& r = *rp[k];
+        const uint32_t n = na[k], h = hx[k];
+        V sa[MAXA];
+        float lmax = 0.0f;
+        if (MODE == kCfr) {
+          // cfr.rs:389-414: regret matching (16-bit regrets undecoded, the
+          // scale cancels: storage.rs:221-234), then the sums x * g + s (* reach)
+          float d = pos0(rg[k][0]);
 #pragma unroll
-    for (int a = 0; a < MAXA; a++)
-      if (a < (int)na && __ballot((bits >> a) & 1u)) wb |= 1u << a;
-    if (lane == 0 && wb) atomicOr(&red3[ri], wb);
-    if (Q && MODE == kCfr) {
-      const float wm = wave_max(lmax);
-      if (lane == 0) atomicMax(&mx3[ri], __float_as_int(wm));
-    }
-    __syncthreads();
-    const uint32_t all = red3[ri];
-    const float mx = Q && MODE == kCfr ? __int_as_float(mx3[ri]) : 0.0f;
-    if (tid == 0) {  // the buffers two nodes ahead
-      red3[ri == 0 ? 2 : ri - 1] = 0;
-      if (Q) mx3[ri == 0 ? 2 : ri - 1] = 0;
-    }
-    ri = ri == 2 ? 0 : ri + 1;
-    if (lane < na) m_alive[cl + lane] = (uint8_t)((all >> lane) & 1u);
-    if (tid < na) alive[cs + tid] = (all >> tid) & 1u;
-    if (Q && MODE == kCfr) {
-      // the new sums, encoded with the block's largest as the new scale
-      // (storage.rs:306-320); every thread decoded with the old scale before
-      // the barrier above
-      const float inv = mx > 0.0f ? kSumMax / mx : 0.0f;
-      for (uint32_t h = tid; h < nq; h += blockDim.x)
+          for (int a = 1; a < MAXA; a++)
+            if (a < (int)n) d += pos0(rg[k][a]);
+          const float inv = recip(d), uni = 1.0f / (float)n;
+          uint32_t yo = 0;  // this node's rows in the group's staging rows
+          if (Q)
+            for (uint32_t q = 0; q < jj[k]; q++) yo += rec[ents[e + q] & 0xFFFF].na;
+#pragma unroll
+          for (int a = 0; a < MAXA; a++)
+            if (a < (int)n) {
+              const float s = inv == 0.0f ? uni : pos0(rg[k][a]) * inv;
+              const float x = xs[k][a];
+              const float y = weighted ? x * g + s * (float)rr[k] : x * g + s;
+              if (Q) {
+                if (act[k]) {
+                  lmax = fmaxf(lmax, fabsf(y));
+                  ybuf[(yo + a) * nq + h] = y;
+                }
+              } else if (act[k]) {
+                sto.s[off[k] + (size_t)a * nq + h] = y;
+              }
+              sa[a] = (V)s;
+            }
+        } else {
+#pragma unroll
+          for (int a = 0; a < MAXA; a++)
+            if (a < (int)n) sv[k][a] = (V)xs[k][a];
+          V d = sv[k][0];
+#pragma unroll
+          for (int a = 1; a < MAXA; a++)
+            if (a < (int)n) d += sv[k][a];
+          const V inv = recip(d), uni = V(1) / (V)n;
+#pragma unroll
+          for (int a = 0; a < MAXA; a++)
+            if (a < (int)n) sa[a] = inv == V(0) ? uni : sv[k][a] * inv;
+        }
+        V* crow = reach + (size_t)B.s(r.cslot[p]) * rowlen;
+        uint32_t wb = 0;
 #pragma unroll
         for (int a = 0; a < MAXA; a++)
-          if (a < (int)na) sto.sq[off + (size_t)a * nq + h] = mx > 0.0f ? enc_sum(ybuf[a * nq + h], inv) : (uint16_t)0;
-      if (tid == 0) sto.ssc[r.pslot] = mx;
+          if (a < (int)n) {
+            const V v = sa[a] * rr[k];
+            if (act[k]) crow[(size_t)a * rowlen + h] = v;
+            if (__ballot(act[k] && v != V(0))) wb |= 1u << a;
+          }
+        if (lane == 0 && wb) atomicOr(&gor[gi][jj[k]], wb);
+        if (Q && MODE == kCfr) {
+          const float wm = wave_max(lmax);
+          if (lane == 0) atomicMax(&gmx[gi][jj[k]], __float_as_int(wm));
+        }
+      }
     }
+    __syncthreads();
+    // the group's children: alive flags (every wave keeps its own copy of
+    // the members' flags, the same values at the same LDS bytes, so a wave
+    // reads flags it wrote itself), and the buffers two groups ahead zeroed
+    for (uint32_t j = 0; j < gn; j++) {
+      const MemRec& r = rec[ents[e + j] & 0xFFFF];
+      if (!span.has(r.node)) continue;
+      const uint32_t all = m_alive[r.src[p]] ? gor[gi][j] : 0u;
+      if (lane < r.na) m_alive[r.clocal + lane] = (uint8_t)((all >> lane) & 1u);
+      if (tid < r.na) alive[B.s(r.cslot[p]) + tid] = (all >> tid) & 1u;
+    }
+    const uint32_t gz = gi == 0 ? 2 : gi - 1;
+    if (Q && MODE == kCfr) {
+      // the new sums, encoded with each node's largest as its new scale
+      // (storage.rs:306-320); every thread decoded with the old scales
+      // before the barrier above
+      for (uint32_t u = wave; u < units; u += nw) {
+        const uint32_t j = u / tiles, h = (u - j * tiles) * 64 + lane;
+        const MemRec& r = rec[ents[e + j] & 0xFFFF];
+        // (a dead node was never staged: it keeps its sums and scale)
+        if (!span.has(r.node) || !m_alive[r.src[p]] || h >= nq) continue;
+        uint32_t yo = 0;
+        for (uint32_t k = 0; k < j; k++) yo += rec[ents[e + k] & 0xFFFF].na;
+        const float mx = __int_as_float(gmx[gi][j]);
+        const float inv = mx > 0.0f ? kSumMax / mx : 0.0f;
+        for (uint32_t a = 0; a < r.na; a++)
+          sto.sq[r.offset + (size_t)a * nq + h] = mx > 0.0f ? enc_sum(ybuf[(yo + a) * nq + h], inv) : (uint16_t)0;
+      }
+      if (tid < gn) {
+        const MemRec& r = rec[ents[e + tid] & 0xFFFF];
+        if (span.has(r.node) && m_alive[r.src[p]]) sto.ssc[r.pslot] = __int_as_float(gmx[gi][tid]);
+      }
+      __syncthreads();  // the staging rows and gmx serve the next group
+    }
+    if (tid < DG_N) {
+      gor[gz][tid] = 0;
+      gmx[gz][tid] = 0;
+    }
+    gi = gi == 2 ? 0 : gi + 1;
+    e += gn;
   }
   // every member's alive flag, for the up pass
   for (uint32_t m = tid; m < mcnt; m += blockDim.x) P.malive[moff + m] = m_alive[rec[m].src[p]];
@@ -1727,6 +1772,46 @@ Status Engine::create(int device, Plan&& plan, const float* regrets, const float
   }
   std::vector<uint32_t> ones(K, 1u);
   const size_t o_act = put(blob, ones);
+  // down-pass groups: per segment and traverser, the opponent's decision
+  // nodes by level (the number of the opponent's decision nodes above them
+  // in the segment: a level's nodes read only rows of earlier levels), cut
+  // into groups of at most DG_N nodes and DG_A actions
+  std::vector<uint32_t> dg_off[2], dg_ent[2];
+  for (int q = 0; q < 2; q++) {
+    const int opp = q ^ 1;
+    dg_off[q].assign(P.seg_mem_cnt.size() + 1, 0);
+    std::vector<uint32_t> lvl;
+    std::vector<std::pair<uint32_t, uint32_t>> ent;  // (level, local)
+    for (uint32_t sg = 0; sg < (uint32_t)P.seg_mem_cnt.size(); sg++) {
+      dg_off[q][sg] = (uint32_t)dg_ent[q].size();
+      const MemRec* rr = P.mrec.data() + P.seg_mem_off[sg];
+      const uint32_t mc = P.seg_mem_cnt[sg];
+      lvl.assign(mc, 0);
+      ent.clear();
+      for (uint32_t m = 0; m < mc; m++) {  // members in breadth-first order: parents first
+        if (!is_player(rr[m].kind)) continue;
+        const uint32_t up = lvl[m] + ((int)rr[m].kind == opp ? 1u : 0u);
+        for (uint32_t a = 0; a < rr[m].na; a++) lvl[rr[m].clocal + a] = up;
+        if ((int)rr[m].kind == opp) ent.push_back({lvl[m], m});
+      }
+      std::stable_sort(ent.begin(), ent.end(), [](const auto& x, const auto& y) { return x.first < y.first; });
+      uint32_t grp = 0, acts = 0, nodes = 0, cur = ~0u;
+      for (const auto& x : ent) {
+        const uint32_t na = rr[x.second].na;
+        if (nodes > 0 && (x.first != cur || acts + na > (uint32_t)DG_A || nodes == (uint32_t)DG_N)) {
+          grp++;
+          acts = nodes = 0;
+        }
+        cur = x.first;
+        acts += na;
+        nodes++;
+        dg_ent[q].push_back(x.second | grp << 16);
+      }
+    }
+    dg_off[q][P.seg_mem_cnt.size()] = (uint32_t)dg_ent[q].size();
+  }
+  const size_t o_dgo0 = put(blob, dg_off[0]), o_dgo1 = put(blob, dg_off[1]);
+  const size_t o_dge0 = put(blob, dg_ent[0]), o_dge1 = put(blob, dg_ent[1]);
   std::vector<uint32_t> big, mid, m48;
   std::vector<uint4> head(P.seg_mem_cnt.size());
   for (uint32_t sg = 0; sg < (uint32_t)P.seg_mem_cnt.size(); sg++) {
@@ -1795,6 +1880,10 @@ Status Engine::create(int device, Plan&& plan, const float* regrets, const float
   d.mid = AT(uint32_t, o_mid);
   d.m48 = AT(uint32_t, o_m48);
   d.seg_head = AT(uint4, o_head);
+  d.dg_off[0] = AT(uint32_t, o_dgo0);
+  d.dg_off[1] = AT(uint32_t, o_dgo1);
+  d.dg_ent[0] = AT(uint32_t, o_dge0);
+  d.dg_ent[1] = AT(uint32_t, o_dge1);
   d.big_host = e->dp_->big.data();
   d.num_big = (uint32_t)e->dp_->big.size();
   d.mid_host = e->dp_->mid.data();
@@ -1854,7 +1943,7 @@ Status Engine::create(int device, Plan&& plan, const float* regrets, const float
 
   // LDS sizes
   const size_t mm = std::max<size_t>(P.max_members, 1);
-  e->lds_down_ = mm * sizeof(MemRec) + ((mm + 15) & ~size_t(15)) + (q16 ? (size_t)MAXA * rowlen * 4 : 0);
+  e->lds_down_ = mm * sizeof(MemRec) + ((mm + 15) & ~size_t(15)) + 4 * mm + (q16 ? (size_t)DG_A * rowlen * 4 : 0);
   for (int f64 = 0; f64 < 2; f64++) {
     size_t vs = f64 ? 8 : 4;
     e->lds_term_[f64] = precision == kExact ? (size_t)rowlen * 3 * 8 + (size_t)rowlen * vs
