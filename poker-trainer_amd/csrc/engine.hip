@@ -758,6 +758,20 @@ static_assert(offsetof(TermRec, invest) == 20 && offsetof(TermRec, rake) == 28 &
                   offsetof(TermRec, tflags) == 33,
               "TermRec word layout");
 
+// f64 lane moves by DPP (two 32-bit moves): lane i reads the lane CTRL
+// names; lanes without a source (or in rows ROW_MASK leaves out) read 0.
+// row_shr:n 0x110 + n, row_shl:n 0x100 + n (within rows of 16 lanes);
+// wave_shr:1 0x138, wave_shl:1 0x130; row_bcast:15 0x142, row_bcast:31 0x143.
+template <int CTRL, int ROW_MASK = 0xF>
+__device__ __forceinline__ double dpp_f64(double x) {
+  const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(x), CTRL, ROW_MASK, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(x), CTRL, ROW_MASK, 0xF, false);
+  return __hiloint2double(hi, lo);
+}
+__device__ __forceinline__ double lane_f64(double x, int l) {
+  return __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(x), l), __builtin_amdgcn_readlane(__double2loint(x), l));
+}
+
 template <typename V, int p>  // p: the traverser (a constant, so per-player tables are static fields)
 __global__ void __launch_bounds__(FT, sizeof(V) == 4 ? 6 : 5) terminal_fast_kernel(DPlan P, uint32_t b0, Span span, Bias B,
                                                               const V* __restrict__ reach, const uint32_t* __restrict__ alive,
@@ -889,19 +903,29 @@ __global__ void __launch_bounds__(FT, sizeof(V) == 4 ? 6 : 5) terminal_fast_kern
           tsum += v[k];
         }
         // (a fold needs only the suffix: the card's total)
+        // inclusive suffix and prefix scans over the card's 8 lanes
+        // (Hillis-Steele steps 1, 2, 4 by DPP row shifts; a value from the
+        // neighbouring group is masked out)
         double inc = tsum, incs = tsum;
-#pragma unroll
-        for (int d = 1; d < CG; d <<= 1) {
-          const double z = __shfl_down(incs, d, CG);
-          if ((int)ql + d < CG) incs += z;
+        {
+          const double z1 = dpp_f64<0x101>(incs);
+          if ((int)ql + 1 < CG) incs += z1;
+          const double z2 = dpp_f64<0x102>(incs);
+          if ((int)ql + 2 < CG) incs += z2;
+          const double z4 = dpp_f64<0x104>(incs);
+          if ((int)ql + 4 < CG) incs += z4;
           if (showdown) {
-            const double y = __shfl_up(inc, d, CG);
-            if ((int)ql >= d) inc += y;
+            const double y1 = dpp_f64<0x111>(inc);
+            if ((int)ql >= 1) inc += y1;
+            const double y2 = dpp_f64<0x112>(inc);
+            if ((int)ql >= 2) inc += y2;
+            const double y4 = dpp_f64<0x114>(inc);
+            if ((int)ql >= 4) inc += y4;
           }
         }
         if (ql == 0) tot[card] = incs;  // the card's total
         if (showdown) {
-          double ex = __shfl_up(inc, 1, CG), exs = __shfl_down(incs, 1, CG);
+          double ex = dpp_f64<0x111>(inc), exs = dpp_f64<0x101>(incs);
           if (ql == 0) ex = 0.0;
           if (ql == CG - 1) exs = 0.0;
 #pragma unroll
@@ -924,16 +948,27 @@ __global__ void __launch_bounds__(FT, sizeof(V) == 4 ? 6 : 5) terminal_fast_kern
         v[k] = ja[k] != NONE ? (double)rv[ja[k]] : 0.0;
         tsum += v[k];
       }
-      // (a fold needs only the prefix: the wave's total at lane 63)
+      // (a fold needs only the prefix: the wave's total at lane 63).
+      // Inclusive prefix: DPP scans within rows of 16 lanes, then each row's
+      // last lane broadcast into the next rows. Inclusive suffix: scans
+      // within rows the other way, then the later rows' totals (their first
+      // lanes) added. Terms that are all zero sum to an exact zero either way.
       double inc = tsum, incs = tsum;
-#pragma unroll
-      for (int d = 1; d < 64; d <<= 1) {
-        const double y = __shfl_up(inc, d, 64);
-        if ((int)ll >= d) inc += y;
-        if (showdown) {
-          const double z = __shfl_down(incs, d, 64);
-          if ((int)ll + d < 64) incs += z;
-        }
+      inc += dpp_f64<0x111>(inc);
+      inc += dpp_f64<0x112>(inc);
+      inc += dpp_f64<0x114>(inc);
+      inc += dpp_f64<0x118>(inc);
+      inc += dpp_f64<0x142, 0xA>(inc);
+      inc += dpp_f64<0x143, 0xC>(inc);
+      if (showdown) {
+        incs += dpp_f64<0x101>(incs);
+        incs += dpp_f64<0x102>(incs);
+        incs += dpp_f64<0x104>(incs);
+        incs += dpp_f64<0x108>(incs);
+        const double t1 = lane_f64(incs, 16), t2 = lane_f64(incs, 32), t3 = lane_f64(incs, 48);
+        const double t23 = t2 + t3;
+        const uint32_t row = ll >> 4;
+        if (row < 3) incs += row == 2 ? t3 : row == 1 ? t23 : t1 + t23;
       }
       if (ll == 63) wtot[wave] = inc;
       __syncthreads();
@@ -964,7 +999,7 @@ __global__ void __launch_bounds__(FT, sizeof(V) == 4 ? 6 : 5) terminal_fast_kern
           if (w < (int)wave) offp += x;
           if (w > (int)wave) offs_ += x;
         }
-        double ex = __shfl_up(inc, 1, 64), exs = __shfl_down(incs, 1, 64);
+        double ex = dpp_f64<0x138>(inc), exs = dpp_f64<0x130>(incs);  // wave_shr:1, wave_shl:1
         if (ll == 0) ex = 0.0;
         if (ll == 63) exs = 0.0;
         double bp = offp + ex, bs = offs_ + exs;
