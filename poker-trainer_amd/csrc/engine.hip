@@ -104,8 +104,13 @@ struct DPlan {
 
 // The nodes a traversal covers: {root} and [lo, hi) (the whole tree, or the
 // subtree below a queried decision node).
+// A whole-tree traversal (init = 1) starts every spot's root reach row from
+// the opponent's range in its down pass and copies every spot's root value
+// row into res[(spot * 2 + p) * rowlen] in its up pass.
 struct Span {
   uint32_t root, lo, hi, seg;  // seg: segment holding `root`
+  uint32_t init;
+  void* res;
   __device__ bool has(uint32_t id) const { return id == root || (id >= lo && id < hi); }
 };
 
@@ -195,16 +200,6 @@ __device__ __forceinline__ void stage_records(const MemRec* __restrict__ g, uint
 }
 
 // ------------------------------------------------------------------ begin
-// Every spot's root reach row: the opponent's range weights (blockIdx.y: spot).
-template <typename V>
-__global__ void begin_kernel(DPlan P, int p, V* __restrict__ reach, uint32_t* __restrict__ alive, uint32_t rowlen) {
-  const SpotTab& S = P.spots[blockIdx.y];
-  const uint32_t nq = S.n[p ^ 1], slot = S.root_slot[p];
-  const uint32_t h = blockIdx.x * blockDim.x + threadIdx.x;
-  if (h < nq) reach[(size_t)slot * rowlen + h] = (V)S.weights[p ^ 1][h];
-  if (h == 0) alive[slot] = P.active[blockIdx.y];
-}
-
 // One reach row for a query: the caller's opponent reach at `slot`.
 template <typename V>
 __global__ void begin_row_kernel(uint32_t nq, const float* __restrict__ w, V* __restrict__ reach, uint32_t slot,
@@ -212,14 +207,6 @@ __global__ void begin_row_kernel(uint32_t nq, const float* __restrict__ w, V* __
   const uint32_t h = blockIdx.x * blockDim.x + threadIdx.x;
   if (h < nq) reach[(size_t)slot * rowlen + h] = (V)w[h];
   if (h == 0) alive[slot] = is_alive;
-}
-
-// Every spot's root value row of traverser p into out[(spot * 2 + p) * rowlen].
-template <typename V>
-__global__ void collect_kernel(DPlan P, int p, const V* __restrict__ vals, V* __restrict__ out, uint32_t rowlen) {
-  const SpotTab& S = P.spots[blockIdx.y];
-  const uint32_t h = blockIdx.x * blockDim.x + threadIdx.x;
-  if (h < S.n[p]) out[((size_t)blockIdx.y * 2 + p) * rowlen + h] = vals[(size_t)S.root_vslot * rowlen + h];
 }
 
 // ------------------------------------------------------------------ down
@@ -250,7 +237,17 @@ __global__ void __launch_bounds__(1024) down_kernel(DPlan P, int p, uint32_t seg
   __syncthreads();
   const uint32_t lane = tid & 63;
   const uint32_t sp_raw = P.seg_pslot[(size_t)p * P.num_segs + seg];
-  if (seg == span.seg || sp_raw == NONE) {
+  if (span.init && sp_raw == NONE) {
+    // a spot's root: its reach row is the opponent's range weights, alive
+    // unless the spot is frozen
+    const uint32_t slot = S.root_slot[p];
+    for (uint32_t h = tid; h < nq; h += blockDim.x) reach[(size_t)slot * rowlen + h] = (V)S.weights[o][h];
+    if (tid == 0) {
+      const uint32_t on = P.active[P.mrec[moff].spot];
+      alive[slot] = on;
+      m_alive[rec[0].src[p]] = (uint8_t)(on != 0);
+    }
+  } else if (seg == span.seg || sp_raw == NONE) {
     if (tid == 0) {
       // the span root's (or a spot root's) row was written by a begin kernel:
       // a spot's root segment starts at member 0; the span's segment at the
@@ -1324,7 +1321,11 @@ __global__ void __launch_bounds__(UP_T, CAP == 64 ? 2 : 4) up_reg_kernel(DPlan P
     LV_SET(m, v);
   }
   const V top = LV_GET(0);
-  if (act && QP != 2) vals[(size_t)B.v(lane_u32(vslot, 0)) * rowlen + h] = top;
+  if (act && QP != 2) {
+    vals[(size_t)B.v(lane_u32(vslot, 0)) * rowlen + h] = top;
+    if (span.init && P.seg_pslot[(size_t)p * P.num_segs + seg] == NONE)  // a spot's root: its result row
+      ((V*)span.res)[((size_t)P.mrec[moff].spot * 2 + p) * rowlen + h] = top;
+  }
 #undef LV_GET
 #undef LV_SET
 #undef SL_GET
@@ -1486,7 +1487,11 @@ __global__ void __launch_bounds__(UP_T) up_kernel(DPlan P, int p, uint32_t seg0,
     }
     lv[(size_t)m * UP_T + tid] = v;
   }
-  if (act && QP != 2) vals[(size_t)B.v(rec[0].vslot) * rowlen + h] = lv[tid];
+  if (act && QP != 2) {
+    vals[(size_t)B.v(rec[0].vslot) * rowlen + h] = lv[tid];
+    if (span.init && P.seg_pslot[(size_t)p * P.num_segs + seg] == NONE)  // a spot's root: its result row
+      ((V*)span.res)[((size_t)rec[0].spot * 2 + p) * rowlen + h] = lv[tid];
+  }
 }
 
 // After a CFR traversal with 16-bit storage: every own node's new regret
@@ -2269,17 +2274,15 @@ Status Engine::launch_traversal(int p, Mode mode, bool f64, ProfRec* prof) {
   const Plan& P = plan_;
   const DPlan& d = dp_->d;
   const int o = p ^ 1;
-  Span span{0, 1, (uint32_t)P.num_nodes, 0};
-  const uint32_t tiles = (std::max(P.n[0], P.n[1]) + 255) / 256;
+  // whole tree: the root segments' down pass starts every spot's reach row
+  // and their up pass keeps every spot's root value row of p (after the
+  // iteration parameters, where root_rows reads them)
+  Span span{0, 1, (uint32_t)P.num_nodes, 0, 1, (char*)params_ + sizeof(IterParams)};
   (void)o;
-  PROF_START(kKBegin);
   const int x = f64 ? 1 : 0;
   void* reach = reach_[x];
   void* vals = vals_[x];
   uint32_t* alive = alive_[x];
-  if (f64) hipLaunchKernelGGL((begin_kernel<double>), dim3(tiles, P.num_spots), dim3(256), 0, stream_, d, p, (double*)reach, alive, rowlen_);
-  else hipLaunchKernelGGL((begin_kernel<float>), dim3(tiles, P.num_spots), dim3(256), 0, stream_, d, p, (float*)reach, alive, rowlen_);
-  PROF_STOP();
   std::function<void()> hook;
   if (!P.leaves.empty()) hook = [&, p, mode, f64]() { launch_leaves(p, mode, f64, reach, vals, alive); };
   const std::function<void()>* hk = P.leaves.empty() ? nullptr : &hook;
@@ -2306,10 +2309,6 @@ Status Engine::launch_traversal(int p, Mode mode, bool f64, ProfRec* prof) {
   else run(std::false_type());
   if (q16_ && mode == kCfr && P.num_pslots > 0)
     hipLaunchKernelGGL(commit_scales_kernel, dim3((P.num_pslots + 255) / 256), dim3(256), 0, stream_, sto, P.num_pslots);
-  // keep every spot's root row of p
-  void* res = (char*)params_ + sizeof(IterParams);
-  if (f64) hipLaunchKernelGGL((collect_kernel<double>), dim3(tiles, P.num_spots), dim3(256), 0, stream_, d, p, (const double*)vals, (double*)res, rowlen_);
-  else hipLaunchKernelGGL((collect_kernel<float>), dim3(tiles, P.num_spots), dim3(256), 0, stream_, d, p, (const float*)vals, (float*)res, rowlen_);
   HIPCHK(hipGetLastError());
   return kOk;
 }
