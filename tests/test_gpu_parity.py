@@ -486,6 +486,21 @@ def test_small_flop_solve_bit_exact(seed):
     assert g.exploitability() == c.exploitability_f64()
 
 
+def test_grouped_compressed_flop_bit_exact():
+    # 16-bit storage under chunk groups: every group runs both up walks (the
+    # maxima, then the encoding) before its rows are reused, and the regret
+    # scales are committed once per traversal
+    cfg = spots.small_flop_spot(0)
+    cfg.storage_mode = 1
+    g, c = Solver(cfg, precision=EXACT, scratch_budget=4 << 20), O.OracleSolver(cfg)
+    assert g.compressed and g.num_groups() > 10
+    g.iterate(4)
+    c.iterate(4)
+    for x, y, what in zip(g.storage_q(), c.storage_q(), ("regrets", "sums", "regret scales", "sum scales")):
+        assert same(x, y), f"{what}: {describe(x, y)}"
+    assert g.exploitability() == c.exploitability_f64()
+
+
 def test_grouping_does_not_change_the_fast_solve():
     cfg = spots.small_flop_spot(1)
     a, b = Solver(cfg, precision=FAST), Solver(cfg, precision=FAST, scratch_budget=4 << 20)
