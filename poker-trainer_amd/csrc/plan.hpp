@@ -6,10 +6,10 @@
 // crossing another chance node (chance nodes are the segment's leaves; their
 // children root deeper segments). Segments are grouped by depth (the number
 // of chance nodes above them). One traversal of player p runs
-//   begin -> down(depth 0..D-1) -> terminals -> up(depth D-1..0)
+//   down(depth 0..D-1) -> terminals -> up(depth D-1..0)
 // where a down kernel gives one workgroup per segment (the opponent's reach
-// pushed down in breadth-first member order, with whole-row "alive" tests as
-// block reductions), the terminal kernel gives one workgroup per terminal,
+// pushed down a level of the opponent's nodes at a time, with whole-row
+// "alive" tests as block reductions), the terminal kernel gives one workgroup per terminal,
 // and an up kernel gives one workgroup per (segment, 64-hand tile) (values
 // pulled up in reverse member order, regret update fused).
 //
