@@ -1176,10 +1176,10 @@ constexpr uint32_t kFLive = 8, kFSpan = 16, kFRoot = 32;
 // the same regrets encoded with that maximum as the node's new scale
 // (storage.rs:253-289: the scale is the whole block's maximum).
 template <int MODE, int CAP, int QP>
-__global__ void __launch_bounds__(UP_T, CAP == 64 ? 2 : 4) up_reg_kernel(DPlan P, int p, uint32_t seg0, Span span, Bias B, float* __restrict__ vals,
-                                                      uint32_t rowlen, const uint32_t* __restrict__ alive, Store sto,
-                                                      const IterParams* __restrict__ prm, float* __restrict__ qout,
-                                                      const uint32_t* __restrict__ list) {
+__device__ __forceinline__ void up_reg_walk(const DPlan& P, int p, uint32_t seg0, const Span& span, const Bias& B,
+                                            float* __restrict__ vals, uint32_t rowlen, const uint32_t* __restrict__ alive,
+                                            const Store& sto, const IterParams* __restrict__ prm, float* __restrict__ qout,
+                                            const uint32_t* __restrict__ list) {
   typedef float V;
   constexpr bool ROWS = MODE != kBest;
   constexpr bool Q = QP != 0;
@@ -1330,6 +1330,28 @@ __global__ void __launch_bounds__(UP_T, CAP == 64 ? 2 : 4) up_reg_kernel(DPlan P
 #undef LV_SET
 #undef SL_GET
 #undef SL_SET
+}
+
+template <int MODE, int CAP, int QP>
+__global__ void __launch_bounds__(UP_T, CAP == 64 ? 2 : 4) up_reg_kernel(DPlan P, int p, uint32_t seg0, Span span, Bias B, float* __restrict__ vals,
+                                                      uint32_t rowlen, const uint32_t* __restrict__ alive, Store sto,
+                                                      const IterParams* __restrict__ prm, float* __restrict__ qout,
+                                                      const uint32_t* __restrict__ list) {
+  up_reg_walk<MODE, CAP, QP>(P, p, seg0, span, B, vals, rowlen, alive, sto, prm, qout, list);
+}
+
+// A range with many segments of 33 to 48 members: one launch, each
+// workgroup walking its segment with the 32-member code when it fits (one
+// indexed move per member read) and the 48-member code otherwise (at the
+// 48-member kernel's occupancy).
+template <int MODE, int QP>
+__global__ void __launch_bounds__(UP_T, 4) up_reg_dual_kernel(DPlan P, int p, uint32_t seg0, Span span, Bias B, float* __restrict__ vals,
+                                                          uint32_t rowlen, const uint32_t* __restrict__ alive, Store sto,
+                                                          const IterParams* __restrict__ prm, float* __restrict__ qout) {
+  if (P.seg_head[seg0 + blockIdx.x].y <= REG_T)
+    up_reg_walk<MODE, (int)REG_T, QP>(P, p, seg0, span, B, vals, rowlen, alive, sto, prm, qout, nullptr);
+  else
+    up_reg_walk<MODE, (int)REG_S, QP>(P, p, seg0, span, B, vals, rowlen, alive, sto, prm, qout, nullptr);
 }
 
 // Values in LDS: the f64 walks, and f32 segments above REG_M members
@@ -2200,8 +2222,8 @@ static void launch_span(const DPlan& d, const Plan& P, hipStream_t st, int p, Sp
         const bool wide = (uint64_t)(i1 - i0) * 4 > (uint64_t)(b - a);
         PROF_START(kKUp);
         if (wide)
-          hipLaunchKernelGGL((up_reg_kernel<UMODE, (int)REG_S, QP>), dim3(b - a, tiles), dim3(UP_T), 0, st, d, p, a, span, B,
-                             (float*)vals, rowlen, (const uint32_t*)alive, sto, prm, (float*)qout, (const uint32_t*)nullptr);
+          hipLaunchKernelGGL((up_reg_dual_kernel<UMODE, QP>), dim3(b - a, tiles), dim3(UP_T), 0, st, d, p, a, span, B,
+                             (float*)vals, rowlen, (const uint32_t*)alive, sto, prm, (float*)qout);
         else
           hipLaunchKernelGGL((up_reg_kernel<UMODE, (int)REG_T, QP>), dim3(b - a, tiles), dim3(UP_T), 0, st, d, p, a, span, B,
                              (float*)vals, rowlen, (const uint32_t*)alive, sto, prm, (float*)qout, (const uint32_t*)nullptr);
