@@ -54,6 +54,12 @@ typedef struct pts_solver pts_solver;
 size_t pts_abi_sizeof(int what);
 /* Text of the last error on this thread; returns its full length. */
 size_t pts_last_error(char* buf, size_t len);
+/* The last failure's numbers (thread-local; zeros when it has none): the
+ * hipError_t of a PTS_DRIVER (or PTS_NO_DEVICE) status, and the bytes
+ * needed and available of a PTS_OUT_OF_MEMORY status, the fields of the
+ * reference's GpuError::Driver and GpuError::OutOfMemory
+ * (crates/solver/src/gpu/mod.rs:86-136). */
+void pts_last_error_detail(int32_t* hip_error, uint64_t* needed, uint64_t* available);
 /* Number of usable GPUs (crates/solver/src/gpu/mod.rs:242-282 opens device 0). */
 pts_status pts_device_count(int* out);
 /* Whether `backend` puts a solve rooted on `street` (1 flop, 2 turn, 3 river)

@@ -178,6 +178,13 @@ size_t pts_abi_sizeof(int what) {
   }
 }
 
+void pts_last_error_detail(int32_t* hip_error, uint64_t* needed, uint64_t* available) {
+  const ErrorDetail& d = last_error_detail();
+  if (hip_error) *hip_error = d.hip_error;
+  if (needed) *needed = d.needed;
+  if (available) *available = d.available;
+}
+
 size_t pts_last_error(char* buf, size_t len) {
   const std::string& e = last_error();
   if (buf && len) std::snprintf(buf, len, "%s", e.c_str());
@@ -190,6 +197,7 @@ pts_status pts_device_count(int* out) {
   if (e != hipSuccess) {
     *out = 0;
     set_error(std::string("no GPU: ") + hipGetErrorString(e));
+    set_error_detail(ErrorDetail{(int32_t)e, 0, 0});
     return PTS_NO_DEVICE;
   }
   return PTS_OK;

@@ -25,14 +25,21 @@
 namespace pts {
 
 static thread_local std::string g_err;
-void set_error(const std::string& m) { g_err = m; }
+static thread_local ErrorDetail g_detail;
+void set_error(const std::string& m) {
+  g_err = m;
+  g_detail = ErrorDetail{};
+}
 const std::string& last_error() { return g_err; }
+void set_error_detail(const ErrorDetail& d) { g_detail = d; }
+const ErrorDetail& last_error_detail() { return g_detail; }
 
 #define HIPCHK(x)                                                                                  \
   do {                                                                                             \
     hipError_t e_ = (x);                                                                           \
     if (e_ != hipSuccess) {                                                                        \
       set_error(std::string("HIP error in ") + #x + ": " + hipGetErrorString(e_));                 \
+      set_error_detail(ErrorDetail{(int32_t)e_, 0, 0});                                            \
       return e_ == hipErrorOutOfMemory ? kOutOfMemory : kDriver;                                   \
     }                                                                                              \
   } while (0)
@@ -1731,6 +1738,7 @@ Status Engine::alloc(DeviceBuf& b, size_t bytes) {
     char m[200];
     std::snprintf(m, sizeof m, "out of GPU memory: needed %zu more bytes, %zu available", bytes, fr);
     set_error(m);
+    set_error_detail(ErrorDetail{(int32_t)e, (uint64_t)bytes, (uint64_t)fr});
     return kOutOfMemory;
   }
   bufs_.push_back(b);

@@ -29,6 +29,15 @@ enum Status : int {
 
 void set_error(const std::string& msg);
 const std::string& last_error();
+// The last failure's numbers (thread-local, cleared by set_error): the HIP
+// error code of a driver failure, and the bytes needed / available of an
+// out-of-memory failure (crates/solver/src/gpu/mod.rs:86-136).
+struct ErrorDetail {
+  int32_t hip_error = 0;
+  uint64_t needed = 0, available = 0;
+};
+void set_error_detail(const ErrorDetail& d);
+const ErrorDetail& last_error_detail();
 
 // Traversal modes (crates/solver/src/cfr.rs:248-256)
 enum Mode : int { kCfr = 0, kBest = 1, kAverage = 2 };

@@ -22,9 +22,12 @@ class PtsError(RuntimeError):
     NAMES = {1: "DriverMissing", 2: "DriverTooOld", 3: "NoDevice", 4: "Driver", 5: "InvalidInput",
              6: "OutOfMemory", 7: "Cancelled"}
 
-    def __init__(self, status, msg):
+    def __init__(self, status, msg, hip_error=0, needed=0, available=0):
         super().__init__(f"{self.NAMES.get(status, status)}: {msg}")
         self.status = status
+        self.hip_error = hip_error  # GpuError::Driver's error code
+        self.needed = needed        # GpuError::OutOfMemory { needed, available }
+        self.available = available
 
 
 class StatsC(ctypes.Structure):
@@ -69,7 +72,7 @@ EXPORTS = [
     "pts_net_get_params", "pts_net_get_grads", "pts_net_forward", "pts_net_gradients", "pts_net_train_step", "pts_net_profile",
     "pts_dl_new", "pts_solver_num_leaves", "pts_solver_copy_root_street", "pts_solver_pin_root_street", "pts_batch_root_rows",
     "pts_batch_num_combinations", "pts_solver_is_compressed", "pts_solver_get_storage_q", "pts_solver_set_storage_q",
-    "pts_engine_create_q", "pts_engine_download_q",
+    "pts_engine_create_q", "pts_engine_download_q", "pts_last_error_detail",
 ]
 
 
@@ -93,6 +96,8 @@ def _declare(L):
     L.pts_abi_sizeof.argtypes = [ctypes.c_int]
     L.pts_last_error.restype = ctypes.c_size_t
     L.pts_last_error.argtypes = [ctypes.c_char_p, ctypes.c_size_t]
+    L.pts_last_error_detail.argtypes = [ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_uint64),
+                                        ctypes.POINTER(ctypes.c_uint64)]
     L.pts_device_count.argtypes = [ctypes.POINTER(ctypes.c_int)]
     L.pts_backend_wants_gpu.argtypes = [ctypes.c_int, ctypes.c_int]
     L.pts_engine_create.argtypes = [ctypes.c_int, ctypes.POINTER(TreeDescC), f32p, f32p, ctypes.c_int, ctypes.c_uint64,
@@ -200,9 +205,16 @@ def last_error() -> str:
     return buf.value.decode(errors="replace")
 
 
+def last_error_detail():
+    """(hip_error, needed, available) of the last failure on this thread."""
+    e, n, a = ctypes.c_int32(), ctypes.c_uint64(), ctypes.c_uint64()
+    lib().pts_last_error_detail(ctypes.byref(e), ctypes.byref(n), ctypes.byref(a))
+    return e.value, n.value, a.value
+
+
 def check(status: int):
     if status != 0:
-        raise PtsError(status, last_error())
+        raise PtsError(status, last_error(), *last_error_detail())
 
 
 def device_count() -> int:

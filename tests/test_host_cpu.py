@@ -230,6 +230,19 @@ def test_valid_tree_passes_validation_without_a_gpu():
     assert st == 3  # PTS_NO_DEVICE: validation passed, the device is missing
 
 
+def test_error_details():
+    # gpu/mod.rs:86-136: the numbers behind Driver and OutOfMemory errors;
+    # a validation error carries none
+    _, t = oracle_tree(spots.river_spot(0))
+    t.desc.storage_len = 10
+    assert create_status(t, 10) == 5
+    assert native.last_error_detail() == (0, 0, 0)
+    if native.device_count() == 0:
+        with pytest.raises(native.PtsError) as e:
+            native.check(lib().pts_device_count(ctypes.byref(ctypes.c_int())))
+        assert e.value.status == 3 and e.value.hip_error != 0  # the HIP error of the failed device query
+
+
 def test_backend_switch():
     # lib.rs:497-503
     L = lib()
