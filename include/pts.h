@@ -12,6 +12,13 @@
  *    callers without a tree builder of their own.
  * All functions return a pts_status; pts_last_error() describes the last
  * failure on the calling thread. No function aborts the process.
+ *
+ * Threads: a pts_solver may be used from several threads; its calls that
+ * run the GPU (solve, iterate, exploitability, root values, storage,
+ * queries) take one lock per solver, so queries from reader threads wait
+ * for a solve in progress (the reference queries after solves from other
+ * threads, crates/trainer/src/study.rs:995-1051). A pts_engine, like the
+ * reference's gpu::Engine, is used from one thread at a time.
  */
 #ifndef PTS_H
 #define PTS_H

@@ -6,6 +6,7 @@
 #include <cstring>
 #include <limits>
 #include <memory>
+#include <mutex>
 
 #include "../../include/pts.h"
 #include "engine.hpp"
@@ -29,6 +30,9 @@ struct pts_solver {
   std::unique_ptr<Engine> engine;
   Plan host_plan;  // kept by host-only solvers (device -1)
   bool compressed = false;  // 16-bit storage (storage.rs:74-81)
+  // One engine call at a time: queries may come from reader threads while
+  // (or after) another thread solves (crates/trainer/src/study.rs:995-1051)
+  std::recursive_mutex mu;
   uint32_t iterations = 0;
   double exploitability = std::numeric_limits<double>::infinity();
 };
@@ -478,6 +482,7 @@ static bool raked(const pts_solver* s) { return s->cfg.rake_rate > 0.0 && s->cfg
 
 pts_status pts_solver_solve(pts_solver* s, uint32_t max_iterations, double target_pct, uint32_t check_every, pts_progress_c* out,
                             pts_progress_c* history, int32_t history_cap, int32_t* history_len) {  // lib.rs:594-642
+  std::lock_guard<std::recursive_mutex> lock_(s->mu);
   NEED_ENGINE(s);
   double pot = (double)s->cfg.pot;
   uint32_t ce = std::max(check_every, 1u);
@@ -519,6 +524,7 @@ pts_status pts_solver_solve(pts_solver* s, uint32_t max_iterations, double targe
 }
 
 pts_status pts_solver_iterate(pts_solver* s, uint32_t count) {
+  std::lock_guard<std::recursive_mutex> lock_(s->mu);
   NEED_ENGINE(s);
   for (uint32_t i = 0; i < count; i++) {
     s->iterations += 1;
@@ -528,8 +534,13 @@ pts_status pts_solver_iterate(pts_solver* s, uint32_t count) {
   return PTS_OK;
 }
 
-pts_status pts_solver_exploitability(pts_solver* s, double* out) { NEED_ENGINE(s); return st(s->engine->exploitability(raked(s), out)); }
+pts_status pts_solver_exploitability(pts_solver* s, double* out) {
+  std::lock_guard<std::recursive_mutex> lock_(s->mu);
+  NEED_ENGINE(s);
+  return st(s->engine->exploitability(raked(s), out));
+}
 pts_status pts_solver_root_values(pts_solver* s, int mode, double out[2]) {
+  std::lock_guard<std::recursive_mutex> lock_(s->mu);
   NEED_ENGINE(s);
   if (mode != PTS_MODE_BEST_RESPONSE && mode != PTS_MODE_AVERAGE) return bad("invalid input: mode");
   return st(s->engine->root_values((Mode)mode, out));
@@ -555,6 +566,7 @@ pts_status pts_solver_plan_groups(pts_solver* s, uint64_t scratch_budget, int f6
 int32_t pts_solver_num_hands(pts_solver* s, int p) { return (int32_t)s->hands.n(p); }
 // f32 storage as is; 16-bit storage decoded (storage.rs:209-246)
 pts_status pts_solver_get_storage(pts_solver* s, float* r, float* m) {
+  std::lock_guard<std::recursive_mutex> lock_(s->mu);
   NEED_ENGINE(s);
   if (!s->compressed) return st(s->engine->download(r, m));
   const size_t len = s->tree.storage_len, k = s->tree.player_nodes;
@@ -575,22 +587,26 @@ pts_status pts_solver_get_storage(pts_solver* s, float* r, float* m) {
   return PTS_OK;
 }
 pts_status pts_solver_set_storage(pts_solver* s, const float* r, const float* m) {
+  std::lock_guard<std::recursive_mutex> lock_(s->mu);
   NEED_ENGINE(s);
   if (s->compressed) return bad("invalid input: 16-bit storage is set through pts_solver_set_storage_q");
   return st(s->engine->upload(r, m));
 }
 pts_status pts_solver_get_storage_q(pts_solver* s, int16_t* r, uint16_t* m, float* regret_scale, float* strat_scale) {
+  std::lock_guard<std::recursive_mutex> lock_(s->mu);
   NEED_ENGINE(s);
   if (!s->compressed) return bad("invalid input: the solver has f32 storage");
   return st(s->engine->download_q(r, m, regret_scale, strat_scale));
 }
 pts_status pts_solver_set_storage_q(pts_solver* s, const int16_t* r, const uint16_t* m, const float* regret_scale,
                                     const float* strat_scale) {
+  std::lock_guard<std::recursive_mutex> lock_(s->mu);
   NEED_ENGINE(s);
   if (!s->compressed) return bad("invalid input: the solver has f32 storage");
   return st(s->engine->upload_q(r, m, regret_scale, strat_scale));
 }
 pts_status pts_solver_profile_iteration(pts_solver* s, double ms[4], uint32_t launches[4], double* wall_ms) {
+  std::lock_guard<std::recursive_mutex> lock_(s->mu);
   NEED_ENGINE(s);
   s->iterations += 1;
   IterParams d = discount(s->iterations, s->cfg);
@@ -689,6 +705,7 @@ static pts_status walk(pts_solver* s, const pts_step_c* path, int32_t np, uint32
 pts_status pts_solver_query(pts_solver* s, const pts_step_c* path, int32_t npath, int32_t* player, int32_t* na, int32_t* n,
                             int32_t* act_kinds, int64_t* act_tos, uint16_t* combos, float* strategy, float* ev, float* reach_out,
                             double reach_fraction[2], int64_t* pot, int64_t invest[2], int32_t cap_actions, int32_t cap_hands) {
+  std::lock_guard<std::recursive_mutex> lock_(s->mu);
   uint32_t id;
   std::vector<float> reach[2];
   std::vector<uint32_t> to_canon[2];
@@ -739,6 +756,7 @@ pts_status pts_solver_query(pts_solver* s, const pts_step_c* path, int32_t npath
 }
 
 pts_status pts_solver_ranges_after(pts_solver* s, const pts_step_c* path, int32_t npath, float* out0, float* out1) {
+  std::lock_guard<std::recursive_mutex> lock_(s->mu);
   uint32_t id;
   std::vector<float> reach[2];
   std::vector<uint32_t> to_canon[2];
@@ -1071,6 +1089,7 @@ uint32_t pts_solver_num_leaves(pts_solver* s) {
 // regret_scale and strategy sums times sum_scale. Returns the nodes copied.
 pts_status pts_solver_copy_root_street(pts_solver* dst, pts_solver* src, float regret_scale, float sum_scale,
                                        uint32_t* copied) {
+  std::scoped_lock lock_(dst->mu, src->mu);
   NEED_ENGINE(dst);
   NEED_ENGINE(src);
   if (dst->hands.n(0) != src->hands.n(0) || dst->hands.n(1) != src->hands.n(1))
@@ -1105,6 +1124,7 @@ pts_status pts_solver_copy_root_street(pts_solver* dst, pts_solver* src, float r
 // normalised per hand, uniform where they sum to zero, cfr.rs:124-154), so
 // regret matching in dst plays exactly that strategy at those nodes.
 pts_status pts_solver_pin_root_street(pts_solver* dst, pts_solver* src, uint32_t* pinned) {
+  std::scoped_lock lock_(dst->mu, src->mu);
   NEED_ENGINE(dst);
   NEED_ENGINE(src);
   if (dst->hands.n(0) != src->hands.n(0) || dst->hands.n(1) != src->hands.n(1))

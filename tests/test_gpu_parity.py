@@ -511,3 +511,30 @@ def test_grouping_does_not_change_the_fast_solve():
     rb, sb = b.storage()
     assert same(ra, rb) and same(sa, sb)
     assert a.exploitability() == b.exploitability()
+
+
+def test_queries_from_reader_threads():
+    # study.rs:995-1051: queries from other threads after (or during) a
+    # solve; each call takes the solver's lock, so concurrent queries return
+    # what sequential ones do
+    import threading
+    cfg = spots.turn_spot(3)
+    g = Solver(cfg, precision=EXACT)
+    g.iterate(6)
+    q0 = [g.query([]).actions[0]]
+    paths = [[], q0] + [[("check"), ("check"), ("card", c)] for c in range(52) if c not in cfg.board][:6]
+    want = [g.query(p) for p in paths]
+    got = [None] * (len(paths) * 3)
+    def worker(k):
+        got[k] = g.query(paths[k % len(paths)])
+    threads = [threading.Thread(target=worker, args=(k,)) for k in range(len(got))]
+    for t in threads:
+        t.start()
+    solver_side = threading.Thread(target=lambda: g.exploitability())
+    solver_side.start()
+    for t in threads:
+        t.join()
+    solver_side.join()
+    for k, r in enumerate(got):
+        w = want[k % len(paths)]
+        assert same(r.strategy, w.strategy) and same(r.ev, w.ev) and same(r.reach, w.reach)
