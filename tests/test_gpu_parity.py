@@ -538,3 +538,41 @@ def test_queries_from_reader_threads():
     for k, r in enumerate(got):
         w = want[k % len(paths)]
         assert same(r.strategy, w.strategy) and same(r.ev, w.ev) and same(r.reach, w.reach)
+
+
+def full_range(board):
+    used = set(parse_cards(board))
+    w = np.zeros(1326, np.float32)
+    for c in range(1326):
+        hi = 1
+        while (hi + 1) * hi // 2 <= c:
+            hi += 1
+        lo = c - hi * (hi - 1) // 2
+        if hi not in used and lo not in used:
+            w[c] = 1.0
+    return w
+
+
+EXTREMES = [
+    # every combo the board leaves (1081 river hands, 1128 turn hands) on
+    # both sides, and one combo against all of them
+    ("river-full", "Kc8d5h3s2c", "full", "full", 8),
+    ("river-one-vs-full", "Kc8d5h3s2c", "AhAd", "full", 8),
+    ("turn-full", "Kc8d5h3s", "full", "full", 2),
+]
+
+
+@pytest.mark.parametrize("name,board,r0,r1,iters", EXTREMES, ids=[e[0] for e in EXTREMES])
+def test_extreme_range_sizes_bit_exact(name, board, r0, r1, iters):
+    ranges = [full_range(board) if r == "full" else r for r in (r0, r1)]
+    cfg = make_config(board, ranges[0], ranges[1], 1000, 4000, PostflopMenu.one(["50%", "allin"], ["3x"], max_raises=2))
+    g, c = Solver(cfg, precision=EXACT), O.OracleSolver(cfg)
+    n = [int(np.count_nonzero(x)) if not isinstance(x, str) else 1 for x in ranges]
+    assert list(g.n) == n
+    g.iterate(iters)
+    c.iterate(iters)
+    rg, sg = g.storage()
+    rc, sc = c.storage()
+    assert same(rg, rc), describe(rg, rc)
+    assert same(sg, sc), describe(sg, sc)
+    assert g.exploitability() == c.exploitability_f64()
