@@ -425,6 +425,51 @@ def test_fast_traversals_close_to_the_oracle(name, make):
         assert np.mean(rg == rc) > 0.999, describe(rg, rc)
 
 
+def _raked_dump():
+    from pts.dump import read_dump
+    from test_dump import TURN_FACING
+    return read_dump(TURN_FACING)[1]
+
+
+def _raked_chop():
+    # crates/solver/src/tests.rs:106-127 with rake: every showdown chops
+    from pts import BettingState
+    cfg = make_config("AsKsQsJsTs", "22,33,44", "55,66,AA", 1500, 5000, PostflopMenu.one(["100%"], ["3x"]),
+                      dcfr=REF, rake_rate=0.05, rake_cap=300.0)
+    cfg.root = BettingState(3, 1500, [0, 500], [5000, 4500], 500, 500, 1, 0, [True, False])
+    return cfg
+
+
+def _raked_river():
+    cfg = spots.river_spot(0)
+    cfg.rake_rate, cfg.rake_cap = 0.05, 300.0
+    return cfg
+
+
+RAKED = [("dump-turn", _raked_dump), ("chop-river", _raked_chop), ("river0", _raked_river)]
+
+
+@pytest.mark.parametrize("name,make", RAKED, ids=[s[0] for s in RAKED])
+def test_fast_traversals_with_rake(name, make):
+    # the fast terminal kernel's raked paths: fold payoffs less rake, and the
+    # tie sums (eval.rs:347-379) that only a raked showdown reaches
+    cfg = make()
+    o, t = oracle_tree(cfg)
+    r, s = random_storage(o.storage_len, 13)
+    o.set_storage(r, s)
+    e = Engine(t, r, s, precision=FAST)
+    for p in (0, 1):
+        for mode in (1, 2):
+            g = e.traverse(p, mode)
+            c = o.traverse_root_f64(p, mode)
+            assert np.allclose(g, c, rtol=1e-12, atol=1e-12), describe(g, c)
+    d = O.discount(3)
+    for p in (0, 1):
+        g = e.traverse(p, 0, (d[0], d[1], d[2], 1))
+        c = o.traverse_root(p, 0, 3)
+        assert np.allclose(g, c, rtol=1e-5, atol=1e-6), describe(g, c)
+
+
 def test_fast_solve_tracks_the_exact_solve():
     # crates/solver/src/gpu/engine/tests.rs:493-522 (fast vs exact after a
     # converged solve): the same convergence, close strategies
