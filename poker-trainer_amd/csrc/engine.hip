@@ -1126,36 +1126,64 @@ __global__ void __launch_bounds__(UP_T) chance_kernel(DPlan P, int p, uint32_t c
 }
 
 // ------------------------------------------------------------------ up
-// f32 walks keep a segment's member values in VGPRs (up_reg_kernel<CAP>:
-// up to 32 members in one vector register range, up to 48 at four waves per
-// SIMD, up to 64 at two); larger segments, and the f64 walks, keep them in
-// LDS (up_kernel).
+// f32 walks, and the f64 best-response walk, keep a segment's member values
+// in VGPRs (up_reg_kernel<V, MODE, CAP>: up to 32 members in one vector
+// register range, up to 48 at four waves per SIMD, up to 64 at two); larger
+// segments, and the other f64 walks, keep them in LDS (up_kernel).
 constexpr uint32_t REG_T = 32, REG_S = 48, REG_M = 64;
 constexpr uint64_t kNoRow = ~0ull;
 
-// CAP f32 values per lane in two vector registers (32 + CAP - 32 lanes of
+// CAP 32-bit values (T: f32 or u32) per lane in two vector registers (32 + CAP - 32 lanes of
 // the member index), read and written at a wave-uniform index (indexed
 // register moves: no LDS, no scratch). The vectors are separate local
 // variables of the kernel (each small enough for the compiler to keep in
 // registers under a variable index); indices are masked into range and the
 // selects take values, not addresses.
-template <int CAP>
+template <int CAP, typename T = float>
 struct MemberVec {
   static_assert(CAP == 32 || CAP == 48 || CAP == 64, "32, 48 or 64 members");
   static constexpr int W1 = CAP == 32 ? 2 : CAP - 32;  // (CAP 32: an unused pair)
-  typedef float lo __attribute__((ext_vector_type(32)));
-  typedef float hi __attribute__((ext_vector_type(W1)));
-  __device__ static __forceinline__ float get(const lo& r0, const hi& r1, uint32_t i) {
-    const float x0 = r0[i & 31];
+  typedef T lo __attribute__((ext_vector_type(32)));
+  typedef T hi __attribute__((ext_vector_type(W1)));
+  __device__ static __forceinline__ T get(const lo& r0, const hi& r1, uint32_t i) {
+    const T x0 = r0[i & 31];
     if (CAP == 32) return x0;
-    const float x1 = r1[i & (W1 - 1)];
+    const T x1 = r1[i & (W1 - 1)];
     return i < 32 ? x0 : x1;
   }
-  __device__ static __forceinline__ void set(lo& r0, hi& r1, uint32_t i, float v) {
+  __device__ static __forceinline__ void set(lo& r0, hi& r1, uint32_t i, T v) {
     if (CAP == 32 || i < 32) r0[i & 31] = v;
     else r1[i & (W1 - 1)] = v;
   }
 };
+
+// Member values of type V: f32 in one MemberVec pair (f0, f1); f64 as its
+// low and high words in two u32 pairs (w0, w1 and x0, x1): a variable index
+// into a vector of 64-bit elements is lowered through scratch memory, into
+// 32-bit ones by indexed register moves. The pairs V does not use are dead
+// and compiled away. (Separate locals, not a struct: a struct of vectors
+// under a variable index is kept in scratch memory.)
+template <typename V, int CAP>
+__device__ __forceinline__ V mval_get(const typename MemberVec<CAP>::lo& f0, const typename MemberVec<CAP>::hi& f1,
+                                      const typename MemberVec<CAP, uint32_t>::lo& w0,
+                                      const typename MemberVec<CAP, uint32_t>::hi& w1,
+                                      const typename MemberVec<CAP, uint32_t>::lo& x0,
+                                      const typename MemberVec<CAP, uint32_t>::hi& x1, uint32_t i) {
+  if constexpr (sizeof(V) == 4) return MemberVec<CAP>::get(f0, f1, i);
+  else return __hiloint2double((int)MemberVec<CAP, uint32_t>::get(x0, x1, i), (int)MemberVec<CAP, uint32_t>::get(w0, w1, i));
+}
+template <typename V, int CAP>
+__device__ __forceinline__ void mval_set(typename MemberVec<CAP>::lo& f0, typename MemberVec<CAP>::hi& f1,
+                                         typename MemberVec<CAP, uint32_t>::lo& w0, typename MemberVec<CAP, uint32_t>::hi& w1,
+                                         typename MemberVec<CAP, uint32_t>::lo& x0, typename MemberVec<CAP, uint32_t>::hi& x1,
+                                         uint32_t i, V v) {
+  if constexpr (sizeof(V) == 4) {
+    MemberVec<CAP>::set(f0, f1, i, v);
+  } else {
+    MemberVec<CAP, uint32_t>::set(w0, w1, i, (uint32_t)__double2loint(v));
+    MemberVec<CAP, uint32_t>::set(x0, x1, i, (uint32_t)__double2hiint(v));
+  }
+}
 
 __device__ __forceinline__ uint32_t lane_u32(uint32_t x, uint32_t m) { return (uint32_t)__builtin_amdgcn_readlane((int)x, (int)m); }
 __device__ __forceinline__ float lane_f32(float x, uint32_t m) { return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(x), (int)m)); }
@@ -1182,12 +1210,14 @@ constexpr uint32_t kFLive = 8, kFSpan = 16, kFRoot = 32;
 // largest |new regret| (atomically maxed over the hand tiles into rnx), 2 for
 // the same regrets encoded with that maximum as the node's new scale
 // (storage.rs:253-289: the scale is the whole block's maximum).
-template <int MODE, int CAP, int QP>
+//
+// V: the value type, f32, or f64 in best-response mode (no storage rows).
+template <typename V, int MODE, int CAP, int QP>
 __device__ __forceinline__ void up_reg_walk(const DPlan& P, int p, uint32_t seg0, const Span& span, const Bias& B,
-                                            float* __restrict__ vals, uint32_t rowlen, const uint32_t* __restrict__ alive,
-                                            const Store& sto, const IterParams* __restrict__ prm, float* __restrict__ qout,
+                                            V* __restrict__ vals, uint32_t rowlen, const uint32_t* __restrict__ alive,
+                                            const Store& sto, const IterParams* __restrict__ prm, V* __restrict__ qout,
                                             const uint32_t* __restrict__ list) {
-  typedef float V;
+  static_assert(std::is_same<V, float>::value || MODE == kBest, "f64 walks in registers: best response only");
   constexpr bool ROWS = MODE != kBest;
   constexpr bool Q = QP != 0;
   __shared__ uint64_t src_lds[CAP];
@@ -1231,13 +1261,16 @@ __device__ __forceinline__ void up_reg_walk(const DPlan& P, int p, uint32_t seg0
       if (Q) stp = stp_lds[lane];
     }
   }
-  typedef MemberVec<CAP> MV;
-  typename MV::lo l0 = typename MV::lo(0.0f), s0 = typename MV::lo(0.0f);  // member values; storage rows, by child
-  typename MV::hi l1 = typename MV::hi(0.0f), s1 = typename MV::hi(0.0f);
-#define LV_GET(i) MV::get(l0, l1, (i))
-#define LV_SET(i, x) MV::set(l0, l1, (i), (x))
-#define SL_GET(i) MV::get(s0, s1, (i))
-#define SL_SET(i, x) MV::set(s0, s1, (i), (x))
+  typedef MemberVec<CAP> SV;             // storage rows, by child; f32 member values
+  typedef MemberVec<CAP, uint32_t> WV;   // f64 member values: low and high words
+  typename SV::lo s0 = typename SV::lo(0.0f), l0 = typename SV::lo(0.0f);
+  typename SV::hi s1 = typename SV::hi(0.0f), l1 = typename SV::hi(0.0f);
+  typename WV::lo w0 = typename WV::lo(0u), x0 = typename WV::lo(0u);
+  typename WV::hi w1 = typename WV::hi(0u), x1 = typename WV::hi(0u);
+#define LV_GET(i) mval_get<V, CAP>(l0, l1, w0, w1, x0, x1, (i))
+#define LV_SET(i, x) mval_set<V, CAP>(l0, l1, w0, w1, x0, x1, (i), (x))
+#define SL_GET(i) SV::get(s0, s1, (i))
+#define SL_SET(i, x) SV::set(s0, s1, (i), (x))
   const float* store = MODE == kCfr ? sto.r : sto.s;
 #pragma unroll
   for (uint32_t m = 0; m < (uint32_t)CAP; m++) {  // unrolled: constant indices, all loads in flight
@@ -1277,7 +1310,7 @@ __device__ __forceinline__ void up_reg_walk(const DPlan& P, int p, uint32_t seg0
       if ((int)k == p) {
         // passes over the actions re-read the registers (few values live at
         // once); the operations and their order are up_kernel's
-        if (MODE == kCfr) {
+        if constexpr (MODE == kCfr) {
           float d = pos0(SL_GET(cl));
           for (uint32_t a = 1; a < na; a++) d += pos0(SL_GET(cl + a));
           float outv = 0.0f;
@@ -1313,6 +1346,14 @@ __device__ __forceinline__ void up_reg_walk(const DPlan& P, int p, uint32_t seg0
           }
 #undef UPDATED
           v = outv;
+        } else if constexpr (MODE == kBest) {
+          // up_kernel's best response: the larger child value, the first on ties
+          V b = LV_GET(cl);
+          for (uint32_t a = 1; a < na; a++) {
+            const V c = LV_GET(cl + a);
+            b = b > c ? b : (c > b ? c : b);
+          }
+          v = b;
         } else {
           float d = SL_GET(cl);
           for (uint32_t a = 1; a < na; a++) d += SL_GET(cl + a);
@@ -1339,26 +1380,26 @@ __device__ __forceinline__ void up_reg_walk(const DPlan& P, int p, uint32_t seg0
 #undef SL_SET
 }
 
-template <int MODE, int CAP, int QP>
-__global__ void __launch_bounds__(UP_T, CAP == 64 ? 2 : 4) up_reg_kernel(DPlan P, int p, uint32_t seg0, Span span, Bias B, float* __restrict__ vals,
+template <typename V, int MODE, int CAP, int QP>
+__global__ void __launch_bounds__(UP_T, CAP == 64 ? 2 : (sizeof(V) == 8 && CAP == 48 ? 3 : 4)) up_reg_kernel(DPlan P, int p, uint32_t seg0, Span span, Bias B, V* __restrict__ vals,
                                                       uint32_t rowlen, const uint32_t* __restrict__ alive, Store sto,
-                                                      const IterParams* __restrict__ prm, float* __restrict__ qout,
+                                                      const IterParams* __restrict__ prm, V* __restrict__ qout,
                                                       const uint32_t* __restrict__ list) {
-  up_reg_walk<MODE, CAP, QP>(P, p, seg0, span, B, vals, rowlen, alive, sto, prm, qout, list);
+  up_reg_walk<V, MODE, CAP, QP>(P, p, seg0, span, B, vals, rowlen, alive, sto, prm, qout, list);
 }
 
 // A range with many segments of 33 to 48 members: one launch, each
 // workgroup walking its segment with the 32-member code when it fits (one
 // indexed move per member read) and the 48-member code otherwise (at the
 // 48-member kernel's occupancy).
-template <int MODE, int QP>
-__global__ void __launch_bounds__(UP_T, 4) up_reg_dual_kernel(DPlan P, int p, uint32_t seg0, Span span, Bias B, float* __restrict__ vals,
+template <typename V, int MODE, int QP>
+__global__ void __launch_bounds__(UP_T, 4) up_reg_dual_kernel(DPlan P, int p, uint32_t seg0, Span span, Bias B, V* __restrict__ vals,
                                                           uint32_t rowlen, const uint32_t* __restrict__ alive, Store sto,
-                                                          const IterParams* __restrict__ prm, float* __restrict__ qout) {
+                                                          const IterParams* __restrict__ prm, V* __restrict__ qout) {
   if (P.seg_head[seg0 + blockIdx.x].y <= REG_T)
-    up_reg_walk<MODE, (int)REG_T, QP>(P, p, seg0, span, B, vals, rowlen, alive, sto, prm, qout, nullptr);
+    up_reg_walk<V, MODE, (int)REG_T, QP>(P, p, seg0, span, B, vals, rowlen, alive, sto, prm, qout, nullptr);
   else
-    up_reg_walk<MODE, (int)REG_S, QP>(P, p, seg0, span, B, vals, rowlen, alive, sto, prm, qout, nullptr);
+    up_reg_walk<V, MODE, (int)REG_S, QP>(P, p, seg0, span, B, vals, rowlen, alive, sto, prm, qout, nullptr);
 }
 
 // Values in LDS: the f64 walks, and f32 segments above REG_M members
@@ -2217,10 +2258,11 @@ static void launch_span(const DPlan& d, const Plan& P, hipStream_t st, int p, Sp
     // the walks of one storage pass over the segments [a, b)
     auto walks = [&](auto qp) {
       constexpr int QP = decltype(qp)::value;
-      if constexpr (std::is_same<V, float>::value) {
-        // f32 walks: segments of up to REG_T (or, when more than a quarter of
-        // them are larger, REG_S) members in registers over [a, b), then those
-        // of up to REG_S and REG_M among [a, b), then the larger ones in LDS
+      if constexpr (std::is_same<V, float>::value || UMODE == kBest) {
+        // f32 walks and the f64 best response: segments of up to REG_T (or,
+        // when more than a quarter of them are larger, REG_S) members in
+        // registers over [a, b), then those of up to REG_S and REG_M among
+        // [a, b), then the larger ones in LDS
         auto cut = [&](const uint32_t* l, uint32_t n, uint32_t& i0, uint32_t& i1) {
           i0 = (uint32_t)(std::lower_bound(l, l + n, a) - l);
           i1 = (uint32_t)(std::lower_bound(l, l + n, b) - l);
@@ -2230,23 +2272,23 @@ static void launch_span(const DPlan& d, const Plan& P, hipStream_t st, int p, Sp
         const bool wide = (uint64_t)(i1 - i0) * 4 > (uint64_t)(b - a);
         PROF_START(kKUp);
         if (wide)
-          hipLaunchKernelGGL((up_reg_dual_kernel<UMODE, QP>), dim3(b - a, tiles), dim3(UP_T), 0, st, d, p, a, span, B,
-                             (float*)vals, rowlen, (const uint32_t*)alive, sto, prm, (float*)qout);
+          hipLaunchKernelGGL((up_reg_dual_kernel<V, UMODE, QP>), dim3(b - a, tiles), dim3(UP_T), 0, st, d, p, a, span, B,
+                             (V*)vals, rowlen, (const uint32_t*)alive, sto, prm, qout);
         else
-          hipLaunchKernelGGL((up_reg_kernel<UMODE, (int)REG_T, QP>), dim3(b - a, tiles), dim3(UP_T), 0, st, d, p, a, span, B,
-                             (float*)vals, rowlen, (const uint32_t*)alive, sto, prm, (float*)qout, (const uint32_t*)nullptr);
+          hipLaunchKernelGGL((up_reg_kernel<V, UMODE, (int)REG_T, QP>), dim3(b - a, tiles), dim3(UP_T), 0, st, d, p, a, span, B,
+                             (V*)vals, rowlen, (const uint32_t*)alive, sto, prm, qout, (const uint32_t*)nullptr);
         PROF_STOP();
         if (!wide && i1 > i0) {
           PROF_START(kKUp);
-          hipLaunchKernelGGL((up_reg_kernel<UMODE, (int)REG_S, QP>), dim3(i1 - i0, tiles), dim3(UP_T), 0, st, d, p, i0, span, B,
-                             (float*)vals, rowlen, (const uint32_t*)alive, sto, prm, (float*)qout, d.m48);
+          hipLaunchKernelGGL((up_reg_kernel<V, UMODE, (int)REG_S, QP>), dim3(i1 - i0, tiles), dim3(UP_T), 0, st, d, p, i0, span, B,
+                             (V*)vals, rowlen, (const uint32_t*)alive, sto, prm, qout, d.m48);
           PROF_STOP();
         }
         cut(d.mid_host, d.num_mid, i0, i1);
         if (i1 > i0) {
           PROF_START(kKUp);
-          hipLaunchKernelGGL((up_reg_kernel<UMODE, (int)REG_M, QP>), dim3(i1 - i0, tiles), dim3(UP_T), 0, st, d, p, i0, span, B,
-                             (float*)vals, rowlen, (const uint32_t*)alive, sto, prm, (float*)qout, d.mid);
+          hipLaunchKernelGGL((up_reg_kernel<V, UMODE, (int)REG_M, QP>), dim3(i1 - i0, tiles), dim3(UP_T), 0, st, d, p, i0, span, B,
+                             (V*)vals, rowlen, (const uint32_t*)alive, sto, prm, qout, d.mid);
           PROF_STOP();
         }
         cut(d.big_host, d.num_big, i0, i1);
