@@ -95,12 +95,12 @@ struct DPlan {
   // per member: its alive flag for the current traverser (written by the
   // down pass, read by the up pass)
   uint8_t* malive;
-  // segments of more than REG_M members, of REG_S + 1 to REG_M members and
-  // of REG_T + 1 to REG_S members (ascending): device copies, and the host
-  // copies launches are cut from
-  const uint32_t *big, *mid, *m48;
-  const uint32_t *big_host, *mid_host, *m48_host;
-  uint32_t num_big, num_mid, num_m48;
+  // segments of more than REG_L members, of REG_M + 1 to REG_L, of REG_S + 1
+  // to REG_M and of REG_T + 1 to REG_S members (ascending): device copies,
+  // and the host copies launches are cut from
+  const uint32_t *big, *m80, *mid, *m48;
+  const uint32_t *big_host, *m80_host, *mid_host, *m48_host;
+  uint32_t num_big, num_m80, num_mid, num_m48;
   // per traverser p: each segment's opponent decision nodes in groups the
   // down pass runs together (one level of the segment, at most DG_A actions
   // in all): entries dg_ent[p][dg_off[p][seg] .. dg_off[p][seg + 1]), each
@@ -1128,12 +1128,12 @@ __global__ void __launch_bounds__(UP_T) chance_kernel(DPlan P, int p, uint32_t c
 // ------------------------------------------------------------------ up
 // f32 walks, and the f64 best-response walk, keep a segment's member values
 // in VGPRs (up_reg_kernel<V, MODE, CAP>: up to 32 members in one vector
-// register range, up to 48 at four waves per SIMD, up to 64 at two); larger
-// segments, and the other f64 walks, keep them in LDS (up_kernel).
-constexpr uint32_t REG_T = 32, REG_S = 48, REG_M = 64;
+// register range, up to 48 at four waves per SIMD, up to 64 and 80 at two);
+// larger segments, and the other f64 walks, keep them in LDS (up_kernel).
+constexpr uint32_t REG_T = 32, REG_S = 48, REG_M = 64, REG_L = 80;
 constexpr uint64_t kNoRow = ~0ull;
 
-// CAP 32-bit values (T: f32 or u32) per lane in two vector registers (32 + CAP - 32 lanes of
+// CAP 32-bit values (T: f32 or u32) per lane in up to three vector registers (32 + 32 + CAP - 64 lanes of
 // the member index), read and written at a wave-uniform index (indexed
 // register moves: no LDS, no scratch). The vectors are separate local
 // variables of the kernel (each small enough for the compiler to keep in
@@ -1141,47 +1141,58 @@ constexpr uint64_t kNoRow = ~0ull;
 // selects take values, not addresses.
 template <int CAP, typename T = float>
 struct MemberVec {
-  static_assert(CAP == 32 || CAP == 48 || CAP == 64, "32, 48 or 64 members");
-  static constexpr int W1 = CAP == 32 ? 2 : CAP - 32;  // (CAP 32: an unused pair)
+  static_assert(CAP == 32 || CAP == 48 || CAP == 64 || CAP == 80, "32, 48, 64 or 80 members");
+  static constexpr int W1 = CAP == 32 ? 2 : CAP >= 64 ? 32 : CAP - 32;  // (CAP 32: an unused pair)
+  static constexpr int W2 = CAP > 64 ? CAP - 64 : 2;                   // (CAP <= 64: an unused pair)
   typedef T lo __attribute__((ext_vector_type(32)));
-  typedef T hi __attribute__((ext_vector_type(W1)));
-  __device__ static __forceinline__ T get(const lo& r0, const hi& r1, uint32_t i) {
+  typedef T mi __attribute__((ext_vector_type(W1)));
+  typedef T hi __attribute__((ext_vector_type(W2)));
+  __device__ static __forceinline__ T get(const lo& r0, const mi& r1, const hi& r2, uint32_t i) {
     const T x0 = r0[i & 31];
     if (CAP == 32) return x0;
     const T x1 = r1[i & (W1 - 1)];
-    return i < 32 ? x0 : x1;
+    if (CAP <= 64) return i < 32 ? x0 : x1;
+    const T x2 = r2[i & (W2 - 1)];
+    return i < 32 ? x0 : i < 64 ? x1 : x2;
   }
-  __device__ static __forceinline__ void set(lo& r0, hi& r1, uint32_t i, T v) {
+  __device__ static __forceinline__ void set(lo& r0, mi& r1, hi& r2, uint32_t i, T v) {
     if (CAP == 32 || i < 32) r0[i & 31] = v;
-    else r1[i & (W1 - 1)] = v;
+    else if (CAP <= 64 || i < 64) r1[i & (W1 - 1)] = v;
+    else r2[i & (W2 - 1)] = v;
   }
 };
 
-// Member values of type V: f32 in one MemberVec pair (f0, f1); f64 as its
-// low and high words in two u32 pairs (w0, w1 and x0, x1): a variable index
+// Member values of type V: f32 in one MemberVec (f0, f1, f2); f64 as its
+// low and high words in two u32 ones (w0.. and x0..): a variable index
 // into a vector of 64-bit elements is lowered through scratch memory, into
 // 32-bit ones by indexed register moves. The pairs V does not use are dead
 // and compiled away. (Separate locals, not a struct: a struct of vectors
 // under a variable index is kept in scratch memory.)
 template <typename V, int CAP>
-__device__ __forceinline__ V mval_get(const typename MemberVec<CAP>::lo& f0, const typename MemberVec<CAP>::hi& f1,
+__device__ __forceinline__ V mval_get(const typename MemberVec<CAP>::lo& f0, const typename MemberVec<CAP>::mi& f1,
+                                      const typename MemberVec<CAP>::hi& f2,
                                       const typename MemberVec<CAP, uint32_t>::lo& w0,
-                                      const typename MemberVec<CAP, uint32_t>::hi& w1,
+                                      const typename MemberVec<CAP, uint32_t>::mi& w1,
+                                      const typename MemberVec<CAP, uint32_t>::hi& w2,
                                       const typename MemberVec<CAP, uint32_t>::lo& x0,
-                                      const typename MemberVec<CAP, uint32_t>::hi& x1, uint32_t i) {
-  if constexpr (sizeof(V) == 4) return MemberVec<CAP>::get(f0, f1, i);
-  else return __hiloint2double((int)MemberVec<CAP, uint32_t>::get(x0, x1, i), (int)MemberVec<CAP, uint32_t>::get(w0, w1, i));
+                                      const typename MemberVec<CAP, uint32_t>::mi& x1,
+                                      const typename MemberVec<CAP, uint32_t>::hi& x2, uint32_t i) {
+  typedef MemberVec<CAP, uint32_t> W;
+  if constexpr (sizeof(V) == 4) return MemberVec<CAP>::get(f0, f1, f2, i);
+  else return __hiloint2double((int)W::get(x0, x1, x2, i), (int)W::get(w0, w1, w2, i));
 }
 template <typename V, int CAP>
-__device__ __forceinline__ void mval_set(typename MemberVec<CAP>::lo& f0, typename MemberVec<CAP>::hi& f1,
-                                         typename MemberVec<CAP, uint32_t>::lo& w0, typename MemberVec<CAP, uint32_t>::hi& w1,
-                                         typename MemberVec<CAP, uint32_t>::lo& x0, typename MemberVec<CAP, uint32_t>::hi& x1,
-                                         uint32_t i, V v) {
+__device__ __forceinline__ void mval_set(typename MemberVec<CAP>::lo& f0, typename MemberVec<CAP>::mi& f1,
+                                         typename MemberVec<CAP>::hi& f2, typename MemberVec<CAP, uint32_t>::lo& w0,
+                                         typename MemberVec<CAP, uint32_t>::mi& w1, typename MemberVec<CAP, uint32_t>::hi& w2,
+                                         typename MemberVec<CAP, uint32_t>::lo& x0, typename MemberVec<CAP, uint32_t>::mi& x1,
+                                         typename MemberVec<CAP, uint32_t>::hi& x2, uint32_t i, V v) {
+  typedef MemberVec<CAP, uint32_t> W;
   if constexpr (sizeof(V) == 4) {
-    MemberVec<CAP>::set(f0, f1, i, v);
+    MemberVec<CAP>::set(f0, f1, f2, i, v);
   } else {
-    MemberVec<CAP, uint32_t>::set(w0, w1, i, (uint32_t)__double2loint(v));
-    MemberVec<CAP, uint32_t>::set(x0, x1, i, (uint32_t)__double2hiint(v));
+    W::set(w0, w1, w2, i, (uint32_t)__double2loint(v));
+    W::set(x0, x1, x2, i, (uint32_t)__double2hiint(v));
   }
 }
 
@@ -1202,8 +1213,10 @@ constexpr uint32_t kFLive = 8, kFSpan = 16, kFRoot = 32;
 // before the walk, all loads in flight together; the walk then runs on
 // registers, in reverse breadth-first order, with the arithmetic of
 // up_kernel below. CAP = 32 (or 48) runs over a range of segments and leaves
-// those above CAP members; CAP = 48 and 64 also run over the plan's lists of
-// segments of 33 to 48 and 49 to 64 members (list[seg0 + blockIdx.x]).
+// those above CAP members; CAP = 48, 64 and 80 also run over the plan's lists
+// of segments of 33 to 48, 49 to 64 and 65 to 80 members (list[seg0 +
+// blockIdx.x]). With CAP 80 each lane holds two records, members lane and
+// lane + 64.
 //
 // QP, the storage pass: 0 f32 storage; with 16-bit storage 3 reads (average
 // mode), and a CFR walk runs twice, 1 for the values and every own node's
@@ -1231,57 +1244,79 @@ __device__ __forceinline__ void up_reg_walk(const DPlan& P, int p, uint32_t seg0
   const uint32_t h = blockIdx.y * UP_T + lane;
   const bool act = h < np;
   const uint32_t hh = act ? h : 0;  // idle lanes load row element 0 and store nothing
-  uint32_t f = 0, vslot = 0, psr = 0;
-  uint64_t off = 0;
-  if (lane < mcnt) {
-    const MemRec& r = P.mrec[moff + lane];
-    const bool lm = P.malive[moff + lane] != 0;
-    f = (uint32_t)r.kind | (lm ? kFLive : 0u) | (span.has(r.node) ? kFSpan : 0u) | (r.node == span.root ? kFRoot : 0u) |
-        (uint32_t)r.na << 8 | (uint32_t)r.clocal << 16;
-    vslot = r.vslot;
-    off = r.offset;
-    psr = r.pslot;
-    if (ROWS) src_lds[lane] = kNoRow;
+  // member lane's record, and with CAP above 64 member lane + 64's as well
+  constexpr int H = CAP > 64 ? 2 : 1;
+  uint32_t f[H], vslot[H], psr[H];
+  uint64_t off[H], src[H];
+  float stp[H];
+#pragma unroll
+  for (int q = 0; q < H; q++) {
+    const uint32_t mm = lane + 64 * q;
+    f[q] = vslot[q] = psr[q] = 0;
+    off[q] = 0;
+    src[q] = kNoRow;
+    stp[q] = 0.0f;
+    if (mm < mcnt) {
+      const MemRec& r = P.mrec[moff + mm];
+      const bool lm = P.malive[moff + mm] != 0;
+      f[q] = (uint32_t)r.kind | (lm ? kFLive : 0u) | (span.has(r.node) ? kFSpan : 0u) | (r.node == span.root ? kFRoot : 0u) |
+             (uint32_t)r.na << 8 | (uint32_t)r.clocal << 16;
+      vslot[q] = r.vslot;
+      off[q] = r.offset;
+      psr[q] = r.pslot;
+      if (ROWS) src_lds[mm] = kNoRow;
+    }
   }
-  uint64_t src = kNoRow;
-  float stp = 0.0f;
   if (ROWS) {
     __syncthreads();
-    if (lane < mcnt && (f & (kFLive | kFSpan)) == (kFLive | kFSpan) && (int)(f & 7) == p) {
-      float step = 0.0f;
-      if (Q) step = MODE == kCfr ? sto.rsc[psr] / kRegMax : sto.ssc[psr] / kSumMax;
-      for (uint32_t a = 0; a < ((f >> 8) & 0xFF); a++) {
-        src_lds[(f >> 16) + a] = off + (uint64_t)a * np;
-        if (Q) stp_lds[(f >> 16) + a] = step;
+#pragma unroll
+    for (int q = 0; q < H; q++) {
+      const uint32_t fq = f[q];
+      if (lane + 64 * q < mcnt && (fq & (kFLive | kFSpan)) == (kFLive | kFSpan) && (int)(fq & 7) == p) {
+        float step = 0.0f;
+        if (Q) step = MODE == kCfr ? sto.rsc[psr[q]] / kRegMax : sto.ssc[psr[q]] / kSumMax;
+        for (uint32_t a = 0; a < ((fq >> 8) & 0xFF); a++) {
+          src_lds[(fq >> 16) + a] = off[q] + (uint64_t)a * np;
+          if (Q) stp_lds[(fq >> 16) + a] = step;
+        }
       }
     }
     __syncthreads();
-    if (lane < mcnt) {
-      src = src_lds[lane];
-      if (Q) stp = stp_lds[lane];
-    }
+#pragma unroll
+    for (int q = 0; q < H; q++)
+      if (lane + 64 * q < mcnt) {
+        src[q] = src_lds[lane + 64 * q];
+        if (Q) stp[q] = stp_lds[lane + 64 * q];
+      }
   }
+  // member m's record fields, read from the lane holding them (m is
+  // wave-uniform)
+#define LN32(x, m) (H == 1 || (m) < 64 ? lane_u32((x)[0], (m)) : lane_u32((x)[H - 1], (m) - 64))
+#define LN64(x, m) (H == 1 || (m) < 64 ? lane_u64((x)[0], (m)) : lane_u64((x)[H - 1], (m) - 64))
+#define LNF(x, m) (H == 1 || (m) < 64 ? lane_f32((x)[0], (m)) : lane_f32((x)[H - 1], (m) - 64))
   typedef MemberVec<CAP> SV;             // storage rows, by child; f32 member values
   typedef MemberVec<CAP, uint32_t> WV;   // f64 member values: low and high words
   typename SV::lo s0 = typename SV::lo(0.0f), l0 = typename SV::lo(0.0f);
-  typename SV::hi s1 = typename SV::hi(0.0f), l1 = typename SV::hi(0.0f);
+  typename SV::mi s1 = typename SV::mi(0.0f), l1 = typename SV::mi(0.0f);
+  typename SV::hi s2 = typename SV::hi(0.0f), l2 = typename SV::hi(0.0f);
   typename WV::lo w0 = typename WV::lo(0u), x0 = typename WV::lo(0u);
-  typename WV::hi w1 = typename WV::hi(0u), x1 = typename WV::hi(0u);
-#define LV_GET(i) mval_get<V, CAP>(l0, l1, w0, w1, x0, x1, (i))
-#define LV_SET(i, x) mval_set<V, CAP>(l0, l1, w0, w1, x0, x1, (i), (x))
-#define SL_GET(i) SV::get(s0, s1, (i))
-#define SL_SET(i, x) SV::set(s0, s1, (i), (x))
+  typename WV::mi w1 = typename WV::mi(0u), x1 = typename WV::mi(0u);
+  typename WV::hi w2 = typename WV::hi(0u), x2 = typename WV::hi(0u);
+#define LV_GET(i) mval_get<V, CAP>(l0, l1, l2, w0, w1, w2, x0, x1, x2, (i))
+#define LV_SET(i, x) mval_set<V, CAP>(l0, l1, l2, w0, w1, w2, x0, x1, x2, (i), (x))
+#define SL_GET(i) SV::get(s0, s1, s2, (i))
+#define SL_SET(i, x) SV::set(s0, s1, s2, (i), (x))
   const float* store = MODE == kCfr ? sto.r : sto.s;
 #pragma unroll
   for (uint32_t m = 0; m < (uint32_t)CAP; m++) {  // unrolled: constant indices, all loads in flight
     if (m < mcnt) {
-      const uint32_t fm = lane_u32(f, m);
+      const uint32_t fm = LN32(f, m);
       if ((fm & 7) >= kChance && (fm & (kFLive | kFSpan)) == (kFLive | kFSpan))
-        LV_SET(m, vals[(size_t)B.v(lane_u32(vslot, m)) * rowlen + hh]);
+        LV_SET(m, vals[(size_t)B.v(LN32(vslot, m)) * rowlen + hh]);
       if (ROWS) {
-        const uint64_t sm = lane_u64(src, m);
+        const uint64_t sm = LN64(src, m);
         if (sm != kNoRow) {
-          if (Q) SL_SET(m, (MODE == kCfr ? (float)sto.rq[sm + hh] : (float)sto.sq[sm + hh]) * lane_f32(stp, m));
+          if (Q) SL_SET(m, (MODE == kCfr ? (float)sto.rq[sm + hh] : (float)sto.sq[sm + hh]) * LNF(stp, m));
           else SL_SET(m, store[sm + hh]);
         }
       }
@@ -1293,7 +1328,7 @@ __device__ __forceinline__ void up_reg_walk(const DPlan& P, int p, uint32_t seg0
     neg = prm->neg;
   }
   for (uint32_t m = mcnt; m-- > 0;) {
-    const uint32_t fm = lane_u32(f, m);
+    const uint32_t fm = LN32(f, m);
     if (!(fm & kFSpan)) continue;
     const uint32_t k = fm & 7, na = (fm >> 8) & 0xFF, cl = fm >> 16;
     if (qout != nullptr && (fm & kFRoot)) {
@@ -1328,7 +1363,7 @@ __device__ __forceinline__ void up_reg_walk(const DPlan& P, int p, uint32_t seg0
 #define UPDATED(a) (SL_GET(cl + (a)) * (SL_GET(cl + (a)) > 0.0f ? pos : neg) + LV_GET(cl + (a)) - outv)
           if (QP == 0) {
             if (act)
-              for (uint32_t a = 0; a < na; a++) sto.r[lane_u64(src, cl + a) + h] = UPDATED(a);
+              for (uint32_t a = 0; a < na; a++) sto.r[LN64(src, cl + a) + h] = UPDATED(a);
           } else if (QP == 1) {
             float lm = 0.0f;
             for (uint32_t a = 0; a < na; a++) {
@@ -1336,13 +1371,13 @@ __device__ __forceinline__ void up_reg_walk(const DPlan& P, int p, uint32_t seg0
               if (act) lm = fmaxf(lm, fabsf(y));
             }
             lm = wave_max(lm);
-            if (lane == 0) atomicMax(&sto.rnx[lane_u32(psr, m)], __float_as_int(lm));
+            if (lane == 0) atomicMax(&sto.rnx[LN32(psr, m)], __float_as_int(lm));
           } else {
-            const float mx = __int_as_float(sto.rnx[lane_u32(psr, m)]);
+            const float mx = __int_as_float(sto.rnx[LN32(psr, m)]);
             const float inv = mx > 0.0f ? kRegMax / mx : 0.0f;
             if (act)
               for (uint32_t a = 0; a < na; a++)
-                sto.rq[lane_u64(src, cl + a) + h] = mx > 0.0f ? enc_regret(UPDATED(a), inv) : (int16_t)0;
+                sto.rq[LN64(src, cl + a) + h] = mx > 0.0f ? enc_regret(UPDATED(a), inv) : (int16_t)0;
           }
 #undef UPDATED
           v = outv;
@@ -1370,7 +1405,7 @@ __device__ __forceinline__ void up_reg_walk(const DPlan& P, int p, uint32_t seg0
   }
   const V top = LV_GET(0);
   if (act && QP != 2) {
-    vals[(size_t)B.v(lane_u32(vslot, 0)) * rowlen + h] = top;
+    vals[(size_t)B.v(LN32(vslot, 0u)) * rowlen + h] = top;
     if (span.init && P.seg_pslot[(size_t)p * P.num_segs + seg] == NONE)  // a spot's root: its result row
       ((V*)span.res)[((size_t)P.mrec[moff].spot * 2 + p) * rowlen + h] = top;
   }
@@ -1378,10 +1413,13 @@ __device__ __forceinline__ void up_reg_walk(const DPlan& P, int p, uint32_t seg0
 #undef LV_SET
 #undef SL_GET
 #undef SL_SET
+#undef LN32
+#undef LN64
+#undef LNF
 }
 
 template <typename V, int MODE, int CAP, int QP>
-__global__ void __launch_bounds__(UP_T, CAP == 64 ? 2 : (sizeof(V) == 8 && CAP == 48 ? 3 : 4)) up_reg_kernel(DPlan P, int p, uint32_t seg0, Span span, Bias B, V* __restrict__ vals,
+__global__ void __launch_bounds__(UP_T, CAP >= 64 ? 2 : (sizeof(V) == 8 && CAP == 48 ? 3 : 4)) up_reg_kernel(DPlan P, int p, uint32_t seg0, Span span, Bias B, V* __restrict__ vals,
                                                       uint32_t rowlen, const uint32_t* __restrict__ alive, Store sto,
                                                       const IterParams* __restrict__ prm, V* __restrict__ qout,
                                                       const uint32_t* __restrict__ list) {
@@ -1402,7 +1440,7 @@ __global__ void __launch_bounds__(UP_T, 4) up_reg_dual_kernel(DPlan P, int p, ui
     up_reg_walk<V, MODE, (int)REG_S, QP>(P, p, seg0, span, B, vals, rowlen, alive, sto, prm, qout, nullptr);
 }
 
-// Values in LDS: the f64 walks, and f32 segments above REG_M members
+// Values in LDS: the f64 average walk, and segments above REG_L members
 // (list[seg0 + blockIdx.x] when a list is given, else seg0 + blockIdx.x).
 // One wave per (segment, 64-hand tile), one hand per lane: values pulled up
 // the segment's members in reverse breadth-first order (no barriers after
@@ -1757,7 +1795,8 @@ __global__ void leaf_zero_kernel(Bias B, const LeafRec* __restrict__ leaves, V* 
 // ================================================================== engine
 struct Engine::DPlanHolder {
   DPlan d;
-  std::vector<uint32_t> big;  // segments above REG_M members, ascending
+  std::vector<uint32_t> big;  // segments above REG_L members, ascending
+  std::vector<uint32_t> m80;  // segments of REG_M + 1 to REG_L members, ascending
   std::vector<uint32_t> mid;  // segments of REG_S + 1 to REG_M members, ascending
   std::vector<uint32_t> m48;  // segments of REG_T + 1 to REG_S members, ascending
 };
@@ -1923,17 +1962,19 @@ Status Engine::create(int device, Plan&& plan, const float* regrets, const float
   }
   const size_t o_dgo0 = put(blob, dg_off[0]), o_dgo1 = put(blob, dg_off[1]);
   const size_t o_dge0 = put(blob, dg_ent[0]), o_dge1 = put(blob, dg_ent[1]);
-  std::vector<uint32_t> big, mid, m48;
+  std::vector<uint32_t> big, m80, mid, m48;
   std::vector<uint4> head(P.seg_mem_cnt.size());
   for (uint32_t sg = 0; sg < (uint32_t)P.seg_mem_cnt.size(); sg++) {
-    if (P.seg_mem_cnt[sg] > REG_M) big.push_back(sg);
+    if (P.seg_mem_cnt[sg] > REG_L) big.push_back(sg);
+    else if (P.seg_mem_cnt[sg] > REG_M) m80.push_back(sg);
     else if (P.seg_mem_cnt[sg] > REG_S) mid.push_back(sg);
     else if (P.seg_mem_cnt[sg] > REG_T) m48.push_back(sg);
     const MemRec& r0 = P.mrec[P.seg_mem_off[sg]];
     const Plan& T = P.spot(r0.spot);
     head[sg] = uint4{P.seg_mem_off[sg], P.seg_mem_cnt[sg], r0.node, T.n[0] | T.n[1] << 16};
   }
-  const size_t o_big = put(blob, big), o_mid = put(blob, mid), o_m48 = put(blob, m48), o_head = put(blob, head);
+  const size_t o_big = put(blob, big), o_m80 = put(blob, m80), o_mid = put(blob, mid), o_m48 = put(blob, m48),
+               o_head = put(blob, head);
   const size_t o_spots = (blob.size() + 255) & ~size_t(255);
   blob.resize(o_spots + K * sizeof(SpotTab));
   DeviceBuf b;
@@ -1985,9 +2026,11 @@ Status Engine::create(int device, Plan&& plan, const float* regrets, const float
   d.active = AT(uint32_t, o_act);
   e->active_ = (uint32_t*)(base + o_act);
   e->dp_->big = std::move(big);
+  e->dp_->m80 = std::move(m80);
   e->dp_->mid = std::move(mid);
   e->dp_->m48 = std::move(m48);
   d.big = AT(uint32_t, o_big);
+  d.m80 = AT(uint32_t, o_m80);
   d.mid = AT(uint32_t, o_mid);
   d.m48 = AT(uint32_t, o_m48);
   d.seg_head = AT(uint4, o_head);
@@ -1997,6 +2040,8 @@ Status Engine::create(int device, Plan&& plan, const float* regrets, const float
   d.dg_ent[1] = AT(uint32_t, o_dge1);
   d.big_host = e->dp_->big.data();
   d.num_big = (uint32_t)e->dp_->big.size();
+  d.m80_host = e->dp_->m80.data();
+  d.num_m80 = (uint32_t)e->dp_->m80.size();
   d.mid_host = e->dp_->mid.data();
   d.num_mid = (uint32_t)e->dp_->mid.size();
   d.m48_host = e->dp_->m48.data();
@@ -2261,8 +2306,8 @@ static void launch_span(const DPlan& d, const Plan& P, hipStream_t st, int p, Sp
       if constexpr (std::is_same<V, float>::value || UMODE == kBest) {
         // f32 walks and the f64 best response: segments of up to REG_T (or,
         // when more than a quarter of them are larger, REG_S) members in
-        // registers over [a, b), then those of up to REG_S and REG_M among
-        // [a, b), then the larger ones in LDS
+        // registers over [a, b), then those of up to REG_S, REG_M and REG_L
+        // among [a, b), then the larger ones in LDS
         auto cut = [&](const uint32_t* l, uint32_t n, uint32_t& i0, uint32_t& i1) {
           i0 = (uint32_t)(std::lower_bound(l, l + n, a) - l);
           i1 = (uint32_t)(std::lower_bound(l, l + n, b) - l);
@@ -2289,6 +2334,13 @@ static void launch_span(const DPlan& d, const Plan& P, hipStream_t st, int p, Sp
           PROF_START(kKUp);
           hipLaunchKernelGGL((up_reg_kernel<V, UMODE, (int)REG_M, QP>), dim3(i1 - i0, tiles), dim3(UP_T), 0, st, d, p, i0, span, B,
                              (V*)vals, rowlen, (const uint32_t*)alive, sto, prm, qout, d.mid);
+          PROF_STOP();
+        }
+        cut(d.m80_host, d.num_m80, i0, i1);
+        if (i1 > i0) {
+          PROF_START(kKUp);
+          hipLaunchKernelGGL((up_reg_kernel<V, UMODE, (int)REG_L, QP>), dim3(i1 - i0, tiles), dim3(UP_T), 0, st, d, p, i0, span, B,
+                             (V*)vals, rowlen, (const uint32_t*)alive, sto, prm, qout, d.m80);
           PROF_STOP();
         }
         cut(d.big_host, d.num_big, i0, i1);

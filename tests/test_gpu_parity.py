@@ -621,3 +621,26 @@ def test_extreme_range_sizes_bit_exact(name, board, r0, r1, iters):
     assert same(rg, rc), describe(rg, rc)
     assert same(sg, sc), describe(sg, sc)
     assert g.exploitability() == c.exploitability_f64()
+
+
+WIDE = [("river-93", lambda: spots.batch_spot("river", 0)), ("river-75", lambda: spots.batch_spot("river", 7)),
+        ("turn-69", lambda: spots.batch_spot("turn", 1))]
+
+
+@pytest.mark.parametrize("storage", [0, 1], ids=["f32", "16bit"])
+@pytest.mark.parametrize("name,make", WIDE, ids=[w[0] for w in WIDE])
+def test_wide_segments_bit_exact(name, make, storage):
+    # segments of 65 to 80 members: the register walk holding two member
+    # records per lane (CFR, the 16-bit passes, the FP64 best response and
+    # the root query); the 93-member segment takes the LDS walk
+    cfg = make()
+    cfg.storage_mode = storage
+    g, c = Solver(cfg, precision=EXACT), O.OracleSolver(cfg)
+    g.iterate(6)
+    c.iterate(6)
+    rg, sg = g.storage()
+    rc, sc = c.storage()
+    assert same(rg, rc) and same(sg, sc), describe(rg, rc)
+    assert g.exploitability() == c.exploitability_f64()
+    a, b = g.query([]), c.query([])
+    assert same(a.strategy, b.strategy) and same(a.ev, b.ev), describe(a.ev, b.ev)
