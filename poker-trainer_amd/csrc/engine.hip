@@ -118,7 +118,12 @@ struct Span {
   uint32_t root, lo, hi, seg;  // seg: segment holding `root`
   uint32_t init;
   void* res;
+  // a query's node's children [kid0, kid1): the query traverses each child on
+  // its own (lib.rs:854-863), so a terminal child is evaluated by itself, not
+  // as part of its river node's batch (cfr.rs:277-281)
+  uint32_t kid0 = 0, kid1 = 0;
   __device__ bool has(uint32_t id) const { return id == root || (id >= lo && id < hi); }
+  __device__ bool kid(uint32_t id) const { return id >= kid0 && id < kid1; }
 };
 
 // Reach slots and value rows of one chunk group: resident ones (below s0 /
@@ -569,7 +574,7 @@ __global__ void __launch_bounds__(TERM_T) terminal_kernel(DPlan P, int p, uint32
     }
   }
   __syncthreads();
-  const bool batched = (tr.tflags & kTermBatched) != 0;
+  const bool batched = (tr.tflags & kTermBatched) != 0 && !span.kid(tr.node);
   const double hp = S.half_pot, norm = S.norm, rake = (double)tr.rake;
   const int32_t inv_p = pick(tr.invest, p), inv_o = pick(tr.invest, o);
   const uint8_t* cp = S.cards[p];
@@ -580,7 +585,7 @@ __global__ void __launch_bounds__(TERM_T) terminal_kernel(DPlan P, int p, uint32
     const int f = tr.kind - kFold0;
     // cfr.rs:224-235
     const double pay = (f == p ? -(hp + (double)inv_p) : hp + (double)pick(tr.invest, f) - rake) * norm;
-    const bool paired = (tr.tflags & (p == 0 ? kTermPaired0 : kTermPaired1)) != 0;
+    const bool paired = batched && (tr.tflags & (p == 0 ? kTermPaired0 : kTermPaired1)) != 0;
     if (paired) {  // strength order of the sibling showdown's board
       const uint32_t b = tr.board;
       const uint32_t cnt = S.ent_cnt[o][b];
@@ -893,7 +898,7 @@ __global__ void __launch_bounds__(FT, sizeof(V) == 4 ? 6 : 5) terminal_fast_kern
       const double rake = (double)tr.rake;
       const int32_t inv_p = p ? tr.inv1 : tr.inv0, inv_o = p ? tr.inv0 : tr.inv1;
       const uint32_t kind = tr.kf & 0xFF;
-      const bool batched = ((tr.kf >> 8) & kTermBatched) != 0;
+      const bool batched = ((tr.kf >> 8) & kTermBatched) != 0 && !span.kid(tr.node);
       const bool showdown = kind == kShowdown;
       V* out = vals + (size_t)B.v(tr.vslot) * rowlen;
 
@@ -2786,6 +2791,8 @@ Status Engine::subtree_values(uint32_t node, int q, const std::vector<float>& op
   hipLaunchKernelGGL((begin_row_kernel<float>), dim3((nq + 255) / 256), dim3(256), 0, stream_, nq, (const float*)dreach,
                      (float*)reach_[0], slot, alive_[0], any ? 1u : 0u, rowlen_);
   Span span{node, P.first_child[node], P.subtree_end[node], P.seg_of[node]};
+  span.kid0 = P.first_child[node];
+  span.kid1 = P.first_child[node] + na;
   // depths from the node's segment on
   uint32_t seg = P.seg_of[node], first_depth = 0;
   while (first_depth + 1 < P.depth_off.size() && P.depth_off[first_depth + 1] <= seg) first_depth++;

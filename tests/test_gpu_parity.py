@@ -644,3 +644,27 @@ def test_wide_segments_bit_exact(name, make, storage):
     assert g.exploitability() == c.exploitability_f64()
     a, b = g.query([]), c.query([])
     assert same(a.strategy, b.strategy) and same(a.ev, b.ev), describe(a.ev, b.ev)
+    # below the root: a walk that starts inside the segment (the queried
+    # node's span), two actions deep
+    path = [b.actions[1]]  # a bet, then the largest raise
+    path = path + [c.query(path).actions[-1]]
+    for pth in (path[:1], path):
+        a, b = g.query(pth), c.query(pth)
+        assert a.actions == b.actions
+        assert same(a.strategy, b.strategy) and same(a.reach, b.reach)
+        assert same(a.ev, b.ev), describe(a.ev, b.ev)
+
+
+def test_query_facing_a_bet_on_the_river():
+    # lib.rs:854-863 traverses each child of the queried node on its own: a
+    # fold or showdown child is evaluated by itself (cfr.rs:277-281), not
+    # with its river node's batch (river.rs:163-213)
+    cfg = spots.river_spot(0)
+    g, c = Solver(cfg, precision=EXACT), O.OracleSolver(cfg)
+    g.iterate(10)
+    c.iterate(10)
+    root = c.query([])
+    for bet in root.actions[1:]:
+        a, b = g.query([bet]), c.query([bet])
+        assert a.actions == b.actions and a.actions[0] == "fold"
+        assert same(a.ev, b.ev), describe(a.ev, b.ev)
