@@ -12,7 +12,9 @@ of the traverser's nodes; terminal: 4 B per opponent hand per terminal) over
 its mean launch duration, both measured with HIP events on the engine's
 stream.
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload turn|river]
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
+                       [--workload turn|river|flop|batch|valuenet] [--precision fast|exact]
+                       [--storage f32|16bit]
 """
 import argparse
 import json
