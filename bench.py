@@ -1,6 +1,9 @@
 """Benchmark: time to 0.5%-of-pot exploitability of the exact HU DCFR solve on
 one MI355X, with the reference CPU solve (the oracle) timed on the host.
 
+Default workload: the flop spot (3-card board, 16.19 G storage cells, 1176
+hands per player), the configuration the headline metric is quoted on.
+
 A step is one DCFR iteration (both players' traversals) over the whole tree.
 `value` is the wall time of a fresh solve from zero storage until the first
 FP64 best-response check at or below the target (checks included; tree build
@@ -10,10 +13,11 @@ with the largest share of an iteration's event time: algorithmic bytes per
 launch (down: 12 B per storage cell of the opponent's nodes; up: 8 B per cell
 of the traverser's nodes; terminal: 4 B per opponent hand per terminal) over
 its mean launch duration, both measured with HIP events on the engine's
-stream.
+stream; `traffic` is the class's HBM bytes per launch as rocprofv3 counted
+them (profiles/traffic.json).
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
-                       [--workload turn|river|flop|batch|valuenet] [--precision fast|exact]
+                       [--workload flop|turn|river|batch|valuenet] [--precision fast|exact]
                        [--storage f32|16bit]
 """
 import argparse
@@ -26,6 +30,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "poker-trainer_amd"))
 
 METRIC = "time to 0.5%-pot exploitability, exact vs value-net DCFR, HU flop, 1 MI355X"
+STREET_NAME = {"river": "HU river", "turn": "HU turn", "flop": "HU flop"}
 HBM_PEAK_GBS = 8000.0
 
 WORKLOADS = {
@@ -87,18 +92,57 @@ def cpu_baseline(cfg, iters_to_target, checks, seconds):
                       f"iterations and {checks} checks"}
 
 
+def cpu_baseline_flop(cfg, iters_to_target, checks, seconds, flop_cells):
+    """The flop's oracle solve needs ~130 GB of storage and hours of host
+    time, so it is extrapolated: the oracle (parallel over chance children)
+    runs a bounded sample of DCFR iterations and one exploitability check on
+    the flop's largest turn subtree (the flop checked through: the flop
+    board plus its first free card, pot 600, stacks 9700, the same ranges
+    and menu), and the times are scaled by the flop's storage cells over the
+    subtree's, then to the GPU run's iteration and check counts."""
+    import copy
+    from pts.config import BettingState, TURN
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or os.cpu_count()
+    sub = copy.deepcopy(cfg)
+    turn_card = next(c for c in range(52) if c not in cfg.board)
+    sub.board = list(cfg.board) + [turn_card]
+    sub.root = BettingState.street_start(TURN, cfg.root.pot, cfg.root.stack[0], cfg.root.stack[1])
+    o = pyoracle.OracleSolver(sub, parallel=True, threads=threads)
+    sub_cells = o.storage_len
+    n = 0
+    t0 = time.perf_counter()
+    while True:
+        o.iterate(1)
+        n += 1
+        if time.perf_counter() - t0 > seconds:
+            break
+    t_iter = (time.perf_counter() - t0) / n
+    t1 = time.perf_counter()
+    o.exploitability()
+    t_check = time.perf_counter() - t1
+    scale = flop_cells / sub_cells
+    est = scale * (iters_to_target * t_iter + checks * t_check)
+    return {"value": round(est, 1), "unit": "s", "cores": threads, "kind": "port",
+            "sample": f"extrapolation: the oracle ran {n} DCFR iterations ({t_iter * 1e3:.1f} ms each) and one "
+                      f"best-response check ({t_check * 1e3:.1f} ms) of the flop's checked-through turn subtree "
+                      f"({sub_cells} cells), scaled by the flop's {flop_cells} cells ({scale:.1f}x) to the GPU run's "
+                      f"{iters_to_target} iterations and {checks} checks"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=50)
-    ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--workload", default="turn", choices=sorted(WORKLOADS) + ["batch", "valuenet"])
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--workload", default="flop", choices=sorted(WORKLOADS) + ["batch", "valuenet"])
     ap.add_argument("--subgames", type=int, default=256, help="valuenet: turn subgames solved for training")
     ap.add_argument("--classes", type=int, default=25, help="valuenet: hand classes per range")
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--target", type=float, default=0.5)
     ap.add_argument("--check-every", type=int, default=10)
-    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--max-iterations", type=int, default=0,
                     help="cap the solve at this many iterations (0: the workload's own count; a capped run that "
                          "stops short of the target says so in target_reached)")
@@ -191,11 +235,16 @@ def main():
     reached = pr.exploitability_pct <= args.target
     cpu = None
     if not args.no_cpu:
-        cpu = cpu_baseline(cfg, pr.iteration, len(hist), args.cpu_seconds)
+        if args.workload == "flop":
+            cpu = cpu_baseline_flop(cfg, pr.iteration, len(hist), args.cpu_seconds, cells)
+        else:
+            cpu = cpu_baseline(cfg, pr.iteration, len(hist), args.cpu_seconds)
         log(f"[bench] cpu baseline: {cpu}")
 
+    metric = METRIC if args.workload == "flop" else \
+        f"time to {args.target:g}%-pot exploitability, exact DCFR, {STREET_NAME[args.workload]}, 1 MI355X"
     line = {
-        "metric": METRIC, "value": round(solve_s, 4), "unit": "s", "n_gpus": 1, "steps": args.steps,
+        "metric": metric, "value": round(solve_s, 4), "unit": "s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": round(step_s * 1e3, 4), "higher_is_better": False, "scaling": "weak",
         "vs_baseline": None, "dtype": "f32", "data": "synthetic",
         "config": {"workload": desc, "seed": args.seed, "hands": [n0, n1], "decision_nodes": st.player_nodes,
@@ -204,7 +253,7 @@ def main():
                    "iterations_to_target": pr.iteration,
                    "target_reached": reached, "final_exploitability_pct": round(pr.exploitability_pct, 5),
                    "iterations_per_s": round(1.0 / step_s, 2), "build_s": round(build_s, 3),
-                   "parallelism": "1 GPU"},
+                   "checks": len(hist), "parallelism": "1 GPU"},
         "roofline": roofline, "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

@@ -67,6 +67,10 @@ size_t pts_last_error(char* buf, size_t len);
  * reference's GpuError::Driver and GpuError::OutOfMemory
  * (crates/solver/src/gpu/mod.rs:86-136). */
 void pts_last_error_detail(int32_t* hip_error, uint64_t* needed, uint64_t* available);
+/* evaluate_cards (crates/holdem/src/eval.rs:142-198) over `count` hands of
+ * `ncards` (5 to 7) distinct cards each (card = rank * 4 + suit), as the tree
+ * builder's strength tables use it: out[i] = category << 26 | payload. */
+pts_status pts_evaluate(const uint8_t* cards, uint32_t ncards, uint64_t count, uint32_t* out);
 /* Number of usable GPUs (crates/solver/src/gpu/mod.rs:242-282 opens device 0). */
 pts_status pts_device_count(int* out);
 /* Whether `backend` puts a solve rooted on `street` (1 flop, 2 turn, 3 river)

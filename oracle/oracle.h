@@ -74,6 +74,11 @@ size_t orc_config_size(void);
 
 /* primitives */
 uint32_t orc_evaluate(const int32_t* cards, int32_t n);
+/* evaluate_cards over `count` hands of `ncards` cards each */
+void orc_evaluate_batch(const uint8_t* cards, int32_t ncards, uint64_t count, uint32_t* out);
+/* The `count` ncards-card hands (cards ascending) from lexicographic rank
+ * `first` on, over the 52-card deck; returns how many were written. */
+uint64_t orc_enumerate_hands(int32_t ncards, uint64_t first, uint64_t count, uint8_t* out);
 int32_t orc_combo_index(int32_t a, int32_t b);
 void orc_combo_cards(int32_t idx, int32_t* hi, int32_t* lo);
 int32_t orc_combo_class(int32_t idx);

@@ -98,6 +98,50 @@ extern "C" {
 size_t orc_config_size(void) { return sizeof(orc_config_c); }
 
 uint32_t orc_evaluate(const int32_t* cards, int32_t n) { return evaluate_cards(cards, n); }
+
+void orc_evaluate_batch(const uint8_t* cards, int32_t ncards, uint64_t count, uint32_t* out) {
+#pragma omp parallel for schedule(static)
+  for (int64_t i = 0; i < (int64_t)count; i++) {
+    int c[7];
+    for (int k = 0; k < ncards; k++) c[k] = cards[i * ncards + k];
+    out[i] = evaluate_cards(c, ncards);
+  }
+}
+
+uint64_t orc_enumerate_hands(int32_t ncards, uint64_t first, uint64_t count, uint8_t* out) {
+  // unrank `first` (lexicographic order of ascending card tuples), then step
+  auto binom = [](int n, int k) -> uint64_t {
+    if (k < 0 || k > n) return 0;
+    uint64_t r = 1;
+    for (int i = 1; i <= k; i++) r = r * (uint64_t)(n - k + i) / (uint64_t)i;
+    return r;
+  };
+  const int k = ncards;
+  int c[7];
+  uint64_t rank = first;
+  int lo = 0;
+  for (int i = 0; i < k; i++) {
+    int x = lo;
+    for (;; x++) {
+      const uint64_t below = binom(51 - x, k - i - 1);  // tuples with card x here
+      if (x > 52 - (k - i)) return 0;                    // `first` beyond the last hand
+      if (rank < below) break;
+      rank -= below;
+    }
+    c[i] = x;
+    lo = x + 1;
+  }
+  uint64_t n = 0;
+  for (; n < count; n++) {
+    for (int i = 0; i < k; i++) out[n * k + i] = (uint8_t)c[i];
+    int i = k - 1;
+    while (i >= 0 && c[i] == 52 - k + i) i--;
+    if (i < 0) return n + 1;
+    c[i]++;
+    for (int j = i + 1; j < k; j++) c[j] = c[j - 1] + 1;
+  }
+  return n;
+}
 int32_t orc_combo_index(int32_t a, int32_t b) { return combo_index(a, b); }
 void orc_combo_cards(int32_t idx, int32_t* hi, int32_t* lo) {
   *hi = combo_cards(idx).hi;

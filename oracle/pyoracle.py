@@ -59,6 +59,9 @@ def _declare(L):
     L.orc_evaluate.restype = ctypes.c_uint32
     L.orc_evaluate.argtypes = [ctypes.POINTER(ctypes.c_int32), ctypes.c_int32]
     L.orc_combo_index.restype = ctypes.c_int32
+    L.orc_evaluate_batch.argtypes = [ctypes.c_void_p, ctypes.c_int32, ctypes.c_uint64, ctypes.c_void_p]
+    L.orc_enumerate_hands.restype = ctypes.c_uint64
+    L.orc_enumerate_hands.argtypes = [ctypes.c_int32, ctypes.c_uint64, ctypes.c_uint64, ctypes.c_void_p]
     L.orc_parse_range.argtypes = [ctypes.c_char_p, fp]
     L.orc_count_categories.argtypes = [ctypes.c_int32, ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_uint64)]
     L.orc_postflop_actions.restype = ctypes.c_int32
@@ -443,6 +446,22 @@ class OracleHandSet:
 
 def evaluate(cards):
     return lib().orc_evaluate(ints(cards), len(cards))
+
+
+def evaluate_batch(cards):
+    """evaluate_cards over the rows of a (count, ncards) uint8 array."""
+    cards = np.ascontiguousarray(cards, np.uint8)
+    out = np.zeros(cards.shape[0], np.uint32)
+    lib().orc_evaluate_batch(cards.ctypes.data, cards.shape[1], cards.shape[0], out.ctypes.data)
+    return out
+
+
+def enumerate_hands(ncards, first, count):
+    """Up to `count` ncards-card hands (ascending cards) from lexicographic
+    rank `first` on: a (k, ncards) uint8 array."""
+    out = np.zeros((count, ncards), np.uint8)
+    k = lib().orc_enumerate_hands(ncards, first, count, out.ctypes.data)
+    return out[:k]
 
 
 def count_categories(ncards):

@@ -195,6 +195,22 @@ size_t pts_last_error(char* buf, size_t len) {
   return e.size();
 }
 
+pts_status pts_evaluate(const uint8_t* cards, uint32_t ncards, uint64_t count, uint32_t* out) {  // eval.rs:142-198
+  if (ncards < 5 || ncards > 7) return bad("invalid input: a hand has 5 to 7 cards");
+  if (count && (!cards || !out)) return bad("invalid input: null buffer");
+  for (uint64_t i = 0; i < count; i++) {
+    const uint8_t* c = cards + i * ncards;
+    uint64_t seen = 0, lanes = 0;
+    for (uint32_t k = 0; k < ncards; k++) {
+      if (c[k] >= 52 || (seen >> c[k] & 1)) return bad("invalid input: hand " + std::to_string(i) + " repeats a card or names one above 51");
+      seen |= 1ull << c[k];
+      lanes |= lane_bit(c[k]);
+    }
+    out[i] = hand_strength(lanes);
+  }
+  return PTS_OK;
+}
+
 pts_status pts_device_count(int* out) {
   *out = 0;
   hipError_t e = hipGetDeviceCount(out);

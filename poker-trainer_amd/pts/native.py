@@ -72,7 +72,7 @@ EXPORTS = [
     "pts_net_get_params", "pts_net_get_grads", "pts_net_forward", "pts_net_gradients", "pts_net_train_step", "pts_net_profile",
     "pts_dl_new", "pts_solver_num_leaves", "pts_solver_copy_root_street", "pts_solver_pin_root_street", "pts_batch_root_rows",
     "pts_batch_num_combinations", "pts_solver_is_compressed", "pts_solver_get_storage_q", "pts_solver_set_storage_q",
-    "pts_engine_create_q", "pts_engine_download_q", "pts_last_error_detail",
+    "pts_engine_create_q", "pts_engine_download_q", "pts_last_error_detail", "pts_evaluate",
 ]
 
 
@@ -99,6 +99,7 @@ def _declare(L):
     L.pts_last_error_detail.argtypes = [ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_uint64),
                                         ctypes.POINTER(ctypes.c_uint64)]
     L.pts_device_count.argtypes = [ctypes.POINTER(ctypes.c_int)]
+    L.pts_evaluate.argtypes = [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint64, ctypes.c_void_p]
     L.pts_backend_wants_gpu.argtypes = [ctypes.c_int, ctypes.c_int]
     L.pts_engine_create.argtypes = [ctypes.c_int, ctypes.POINTER(TreeDescC), f32p, f32p, ctypes.c_int, ctypes.c_uint64,
                                     ctypes.POINTER(P)]
@@ -215,6 +216,15 @@ def last_error_detail():
 def check(status: int):
     if status != 0:
         raise PtsError(status, last_error(), *last_error_detail())
+
+
+def evaluate(cards) -> np.ndarray:
+    """pts_evaluate over the rows of a (count, ncards) array of cards
+    (evaluate_cards, crates/holdem/src/eval.rs:142-198)."""
+    c = np.ascontiguousarray(cards, np.uint8)
+    out = np.zeros(c.shape[0], np.uint32)
+    check(lib().pts_evaluate(c.ctypes.data, c.shape[1], c.shape[0], out.ctypes.data))
+    return out
 
 
 def device_count() -> int:

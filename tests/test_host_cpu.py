@@ -124,13 +124,6 @@ def test_estimate_matches_the_oracle():
             (b.player_nodes, b.chance_nodes, b.terminal_nodes, b.storage_cells, b.memory_bytes)
 
 
-def test_turn_workload_size():
-    # turn workload: ~5.8k decision nodes and ~10.5k terminals with the
-    # one-raise (cap 2) menu
-    st = O.OracleSolver.estimate(spots.turn_spot(0))
-    assert 4000 < st.player_nodes < 8000 and 8000 < st.terminal_nodes < 14000
-
-
 def test_rejections():
     cfg = spots.river_spot(0)
     cfg.max_memory_bytes = 1 << 10
