@@ -237,6 +237,36 @@ void pts_solver_stats(pts_solver* s, pts_stats_c* out);
  * progress of every exploitability check. */
 pts_status pts_solver_solve(pts_solver* s, uint32_t max_iterations, double target_pct, uint32_t check_every,
                             pts_progress_c* out, pts_progress_c* history, int32_t history_cap, int32_t* history_len);
+/* Called after every exploitability check with the progress so far
+ * (crates/solver/src/lib.rs:614-630, `on_progress`); returning 0 stops the
+ * solve. */
+typedef int32_t (*pts_progress_fn)(const pts_progress_c* progress, void* user);
+/* Solver::solve_cancellable (lib.rs:480-494): pts_solver_solve that reads
+ * *cancel (null: never set) with a relaxed atomic load before every
+ * iteration and stops within one iteration once it is nonzero (the layout of
+ * Rust's AtomicBool: one byte), and calls on_progress (may be null) after
+ * every check. A solve stopped either way returns PTS_CANCELLED with `out`
+ * holding the progress of its last check and finished = 1; the storage keeps
+ * every completed iteration, so a later solve continues from there. */
+pts_status pts_solver_solve_cancellable(pts_solver* s, uint32_t max_iterations, double target_pct, uint32_t check_every,
+                                        const uint8_t* cancel, pts_progress_fn on_progress, void* user,
+                                        pts_progress_c* out);
+/* The solve loop the two calls above share, for engine-layer callers that
+ * keep their own solver state (Solver::run_solve, lib.rs:576-642): runs
+ * iterate(user, t) for t = *iterations + 1, ... (a pts_status, 0 for OK;
+ * *iterations counts it first), measures the exploitability in chips with
+ * measure(user, &x) on the reference's adaptive schedule (next_check_gap,
+ * lib.rs:155-172: every check_every iterations until the trend predicts the
+ * target sooner) and at the last iteration, and stops at the first check at
+ * or below target_pct of `pot`, at max_iterations, or when cancelled as
+ * pts_solver_solve_cancellable is. *exploitability holds the last check. A
+ * status other than PTS_OK from iterate or measure ends the loop and is
+ * returned. */
+typedef int32_t (*pts_iterate_fn)(void* user, uint32_t iteration);
+typedef int32_t (*pts_measure_fn)(void* user, double* exploitability);
+pts_status pts_run_solve(uint32_t* iterations, double* exploitability, double pot, uint32_t max_iterations,
+                         double target_pct, uint32_t check_every, const uint8_t* cancel, pts_iterate_fn iterate,
+                         pts_measure_fn measure, pts_progress_fn on_progress, void* user, pts_progress_c* out);
 /* `count` DCFR iterations without exploitability checks. */
 pts_status pts_solver_iterate(pts_solver* s, uint32_t count);
 pts_status pts_solver_exploitability(pts_solver* s, double* out);

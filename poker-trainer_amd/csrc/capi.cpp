@@ -496,37 +496,50 @@ static pts_status need_engine(pts_solver* s) {
 
 static bool raked(const pts_solver* s) { return s->cfg.rake_rate > 0.0 && s->cfg.rake_cap > 0.0; }
 
-pts_status pts_solver_solve(pts_solver* s, uint32_t max_iterations, double target_pct, uint32_t check_every, pts_progress_c* out,
-                            pts_progress_c* history, int32_t history_cap, int32_t* history_len) {  // lib.rs:594-642
-  std::lock_guard<std::recursive_mutex> lock_(s->mu);
-  NEED_ENGINE(s);
-  double pot = (double)s->cfg.pot;
-  uint32_t ce = std::max(check_every, 1u);
-  pts_progress_c pr{s->iterations, max_iterations, s->exploitability, 100.0 * s->exploitability / pot, 0, 0};
-  double target = target_pct * pot / 100.0;
-  uint32_t next_check = s->iterations + ce;
+pts_status pts_run_solve(uint32_t* iterations, double* exploitability, double pot, uint32_t max_iterations,
+                         double target_pct, uint32_t check_every, const uint8_t* cancel, pts_iterate_fn iterate,
+                         pts_measure_fn measure, pts_progress_fn on_progress, void* user, pts_progress_c* out) {
+  // crates/solver/src/lib.rs:594-642 (run_solve)
+  if (!iterations || !exploitability || !iterate || !measure || !out) return bad("invalid input: null argument");
+  if (!(pot > 0.0)) return bad("invalid input: the pot must be positive");
+  const uint32_t ce = std::max(check_every, 1u);
+  pts_progress_c pr{*iterations, max_iterations, *exploitability, 100.0 * *exploitability / pot, 0, 0};
+  const double target = target_pct * pot / 100.0;
+  uint32_t next_check = *iterations + ce;
   bool has_last = false;
   uint32_t lt = 0;
   double le = 0.0;
-  int32_t nh = 0;
-  while (s->iterations < max_iterations) {
-    s->iterations += 1;
-    Status e = s->engine->cfr_iteration(discount(s->iterations, s->cfg));
-    if (e != kOk) return st(e);
-    bool last = s->iterations >= max_iterations;
-    if (s->iterations >= next_check || last) {
-      double x;
-      if ((e = s->engine->exploitability(raked(s), &x)) != kOk) return st(e);
-      s->exploitability = x;
-      pr.iteration = s->iterations;
+  while (*iterations < max_iterations) {
+    if (cancel && __atomic_load_n(cancel, __ATOMIC_RELAXED)) {
+      pr.finished = 1;
+      *out = pr;
+      set_error("cancelled");
+      return PTS_CANCELLED;
+    }
+    *iterations += 1;
+    int32_t rc = iterate(user, *iterations);
+    if (rc != PTS_OK) return (pts_status)rc;
+    const bool last = *iterations >= max_iterations;
+    if (*iterations >= next_check || last) {
+      double x = 0.0;
+      if ((rc = measure(user, &x)) != PTS_OK) return (pts_status)rc;
+      *exploitability = x;
+      pr.iteration = *iterations;
       pr.exploitability = x;
       pr.exploitability_pct = 100.0 * x / pot;
-      bool done = pr.exploitability_pct <= target_pct || last;
+      const bool done = pr.exploitability_pct <= target_pct || last;
       pr.finished = done;
-      if (history && nh < history_cap) history[nh] = pr;
-      nh++;
-      if (done) break;
-      uint32_t t = s->iterations;
+      const bool keep_going = on_progress ? on_progress(&pr, user) != 0 : true;
+      if (done || !keep_going) {
+        pr.finished = 1;
+        *out = pr;
+        if (!done) {
+          set_error("cancelled by the progress callback");
+          return PTS_CANCELLED;
+        }
+        return PTS_OK;
+      }
+      const uint32_t t = *iterations;
       next_check = t + check_gap(has_last, lt, le, t, x, target, ce);
       has_last = true;
       lt = t;
@@ -534,9 +547,59 @@ pts_status pts_solver_solve(pts_solver* s, uint32_t max_iterations, double targe
     }
   }
   pr.finished = 1;
-  if (history_len) *history_len = nh;
   *out = pr;
   return PTS_OK;
+}
+
+namespace {
+// The solver's engine under pts_run_solve, with the caller's callback and a
+// history of the checks.
+struct SolveCtx {
+  pts_solver* s;
+  pts_progress_fn user_fn;
+  void* user;
+  pts_progress_c* history;
+  int32_t cap, len;
+};
+int32_t solve_iterate(void* u, uint32_t t) {
+  SolveCtx* c = (SolveCtx*)u;
+  return (int32_t)st(c->s->engine->cfr_iteration(discount(t, c->s->cfg)));
+}
+int32_t solve_measure(void* u, double* x) {
+  SolveCtx* c = (SolveCtx*)u;
+  const bool raked = c->s->cfg.rake_rate > 0.0 && c->s->cfg.rake_cap > 0.0;
+  return (int32_t)st(c->s->engine->exploitability(raked, x));
+}
+int32_t solve_progress(const pts_progress_c* p, void* u) {
+  SolveCtx* c = (SolveCtx*)u;
+  if (c->history && c->len < c->cap) c->history[c->len] = *p;
+  c->len++;
+  return c->user_fn ? c->user_fn(p, c->user) : 1;
+}
+}  // namespace
+
+static pts_status solver_solve(pts_solver* s, uint32_t max_iterations, double target_pct, uint32_t check_every,
+                               const uint8_t* cancel, pts_progress_fn on_progress, void* user, pts_progress_c* out,
+                               pts_progress_c* history, int32_t history_cap, int32_t* history_len) {
+  std::lock_guard<std::recursive_mutex> lock_(s->mu);
+  NEED_ENGINE(s);
+  SolveCtx c{s, on_progress, user, history, std::max(history_cap, 0), 0};
+  pts_status r = pts_run_solve(&s->iterations, &s->exploitability, (double)s->cfg.pot, max_iterations, target_pct,
+                               check_every, cancel, solve_iterate, solve_measure, solve_progress, &c, out);
+  if (history_len) *history_len = c.len;
+  return r;
+}
+
+pts_status pts_solver_solve(pts_solver* s, uint32_t max_iterations, double target_pct, uint32_t check_every, pts_progress_c* out,
+                            pts_progress_c* history, int32_t history_cap, int32_t* history_len) {  // lib.rs:475-478
+  return solver_solve(s, max_iterations, target_pct, check_every, nullptr, nullptr, nullptr, out, history, history_cap,
+                      history_len);
+}
+
+pts_status pts_solver_solve_cancellable(pts_solver* s, uint32_t max_iterations, double target_pct, uint32_t check_every,
+                                        const uint8_t* cancel, pts_progress_fn on_progress, void* user,
+                                        pts_progress_c* out) {  // lib.rs:480-494
+  return solver_solve(s, max_iterations, target_pct, check_every, cancel, on_progress, user, out, nullptr, 0, nullptr);
 }
 
 pts_status pts_solver_iterate(pts_solver* s, uint32_t count) {
@@ -561,10 +624,19 @@ pts_status pts_solver_root_values(pts_solver* s, int mode, double out[2]) {
   if (mode != PTS_MODE_BEST_RESPONSE && mode != PTS_MODE_AVERAGE) return bad("invalid input: mode");
   return st(s->engine->root_values((Mode)mode, out));
 }
-uint32_t pts_solver_iterations(pts_solver* s) { return s->iterations; }
-void pts_solver_set_iterations(pts_solver* s, uint32_t t) { s->iterations = t; }
+uint32_t pts_solver_iterations(pts_solver* s) {
+  std::lock_guard<std::recursive_mutex> lock_(s->mu);
+  return s->iterations;
+}
+void pts_solver_set_iterations(pts_solver* s, uint32_t t) {
+  std::lock_guard<std::recursive_mutex> lock_(s->mu);
+  s->iterations = t;
+}
 uint64_t pts_solver_storage_len(pts_solver* s) { return s->tree.storage_len; }
-uint32_t pts_solver_num_groups(pts_solver* s, int f64) { return s->engine ? (uint32_t)s->engine->groups(f64 != 0).size() : 0u; }
+uint32_t pts_solver_num_groups(pts_solver* s, int f64) {
+  std::lock_guard<std::recursive_mutex> lock_(s->mu);
+  return s->engine ? (uint32_t)s->engine->groups(f64 != 0).size() : 0u;
+}
 
 pts_status pts_solver_plan_groups(pts_solver* s, uint64_t scratch_budget, int f64, uint32_t* num_groups,
                                   uint64_t* scratch_bytes) {

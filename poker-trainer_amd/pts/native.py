@@ -40,6 +40,11 @@ class ProgressC(ctypes.Structure):
                 ("exploitability_pct", ctypes.c_double), ("finished", ctypes.c_int32), ("_pad", ctypes.c_int32)]
 
 
+PROGRESS_FN = ctypes.CFUNCTYPE(ctypes.c_int32, ctypes.POINTER(ProgressC), ctypes.c_void_p)
+ITERATE_FN = ctypes.CFUNCTYPE(ctypes.c_int32, ctypes.c_void_p, ctypes.c_uint32)
+MEASURE_FN = ctypes.CFUNCTYPE(ctypes.c_int32, ctypes.c_void_p, ctypes.POINTER(ctypes.c_double))
+
+
 class StepC(ctypes.Structure):
     _fields_ = [("kind", ctypes.c_int32), ("act_kind", ctypes.c_int32), ("to", ctypes.c_int64), ("card", ctypes.c_int32),
                 ("_pad", ctypes.c_int32)]
@@ -73,6 +78,7 @@ EXPORTS = [
     "pts_dl_new", "pts_solver_num_leaves", "pts_solver_copy_root_street", "pts_solver_pin_root_street", "pts_batch_root_rows",
     "pts_batch_num_combinations", "pts_solver_is_compressed", "pts_solver_get_storage_q", "pts_solver_set_storage_q",
     "pts_engine_create_q", "pts_engine_download_q", "pts_last_error_detail", "pts_evaluate",
+    "pts_solver_solve_cancellable", "pts_run_solve",
 ]
 
 
@@ -165,6 +171,11 @@ def _declare(L):
     L.pts_solver_stats.argtypes = [P, ctypes.POINTER(StatsC)]
     L.pts_solver_solve.argtypes = [P, ctypes.c_uint32, ctypes.c_double, ctypes.c_uint32, ctypes.POINTER(ProgressC),
                                    ctypes.POINTER(ProgressC), ctypes.c_int32, ctypes.POINTER(ctypes.c_int32)]
+    L.pts_solver_solve_cancellable.argtypes = [P, ctypes.c_uint32, ctypes.c_double, ctypes.c_uint32, ctypes.c_void_p,
+                                               PROGRESS_FN, ctypes.c_void_p, ctypes.POINTER(ProgressC)]
+    L.pts_run_solve.argtypes = [ctypes.POINTER(ctypes.c_uint32), ctypes.POINTER(ctypes.c_double), ctypes.c_double,
+                                ctypes.c_uint32, ctypes.c_double, ctypes.c_uint32, ctypes.c_void_p, ITERATE_FN,
+                                MEASURE_FN, PROGRESS_FN, ctypes.c_void_p, ctypes.POINTER(ProgressC)]
     L.pts_solver_iterate.argtypes = [P, ctypes.c_uint32]
     L.pts_solver_exploitability.argtypes = [P, ctypes.POINTER(ctypes.c_double)]
     L.pts_solver_root_values.argtypes = [P, ctypes.c_int, f64p]

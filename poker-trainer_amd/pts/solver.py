@@ -15,7 +15,7 @@ from typing import List, Optional
 import numpy as np
 
 from .config import ConfigC, SolveConfig
-from .native import ProgressC, StatsC, TreeDescC, check, lib, steps_c
+from .native import PROGRESS_FN, ProgressC, StatsC, TreeDescC, check, lib, steps_c
 
 ACT_NAMES = {0: "fold", 1: "check", 2: "call"}
 
@@ -121,6 +121,25 @@ class Solver:
             history.extend(hist[i] for i in range(min(hl.value, 4096)))
         return out
 
+    def solve_cancellable(self, opts: SolveOptions, cancel: Optional[ctypes.c_uint8] = None, on_progress=None):
+        """Solver::solve_cancellable (lib.rs:480-494): `cancel` is a ctypes
+        c_uint8 another thread may set to 1 (the solve stops within one
+        iteration); `on_progress(progress)` runs after every check and stops
+        the solve by returning False. Returns (progress, cancelled)."""
+        out = ProgressC()
+
+        def cb(p, _user):
+            return 1 if on_progress is None or on_progress(p.contents) is not False else 0
+
+        fn = PROGRESS_FN(cb)
+        st = lib().pts_solver_solve_cancellable(self.h, opts.max_iterations, opts.target_exploitability_pct,
+                                                opts.check_every, ctypes.byref(cancel) if cancel is not None else None,
+                                                fn, None, ctypes.byref(out))
+        if st == 7:  # PTS_CANCELLED
+            return out, True
+        check(st)
+        return out, False
+
     def iterate(self, count: int = 1):
         check(lib().pts_solver_iterate(self.h, count))
 
@@ -209,6 +228,49 @@ class Solver:
         o1 = np.zeros(1326, np.float32)
         check(lib().pts_solver_ranges_after(self.h, steps_c(path), len(path), o0, o1))
         return o0, o1
+
+
+def run_solve(iterate, measure, pot: float, opts: SolveOptions, iterations: int = 0,
+              exploitability: float = float("inf"), cancel: Optional[ctypes.c_uint8] = None, on_progress=None):
+    """pts_run_solve (Solver::run_solve, lib.rs:576-642) over Python
+    callables: iterate(t) runs iteration t, measure() returns the
+    exploitability in chips. Returns (progress, cancelled, iterations,
+    exploitability)."""
+    from .native import ITERATE_FN, MEASURE_FN
+    errors = []
+
+    def it(_u, t):
+        try:
+            iterate(int(t))
+            return 0
+        except Exception as e:  # noqa: BLE001 - handed back after the C loop returns
+            errors.append(e)
+            return 5
+
+    def me(_u, x):
+        try:
+            x[0] = float(measure())
+            return 0
+        except Exception as e:  # noqa: BLE001
+            errors.append(e)
+            return 5
+
+    def pr(p, _u):
+        return 1 if on_progress is None or on_progress(p.contents) is not False else 0
+
+    fi, fm, fp = ITERATE_FN(it), MEASURE_FN(me), PROGRESS_FN(pr)
+    t = ctypes.c_uint32(iterations)
+    e = ctypes.c_double(exploitability)
+    out = ProgressC()
+    st = lib().pts_run_solve(ctypes.byref(t), ctypes.byref(e), pot, opts.max_iterations, opts.target_exploitability_pct,
+                             opts.check_every, ctypes.byref(cancel) if cancel is not None else None, fi, fm, fp, None,
+                             ctypes.byref(out))
+    if errors:
+        raise errors[0]
+    if st == 7:
+        return out, True, t.value, e.value
+    check(st)
+    return out, False, t.value, e.value
 
 
 def depth_limited(cfg: SolveConfig, net=None, device: int = 0, precision: int = 0) -> Solver:
