@@ -117,7 +117,8 @@ pts_status pts_engine_create(int device, const pts_tree_desc* tree, const float*
  * HostStorage::Compressed, gpu/engine.rs:292-343): storage_len i16 regrets
  * and u16 strategy sums; slot[i] is decision node i's scale slot (tree.rs:
  * 164-165, Node::slot; ignored for other nodes), num_slots regret scales and
- * num_slots strategy-sum scales. */
+ * num_slots strategy-sum scales; the four storage arrays may all be null
+ * (zeros). */
 pts_status pts_engine_create_q(int device, const pts_tree_desc* tree, const uint32_t* slot, uint32_t num_slots,
                                const int16_t* regrets, const uint16_t* sums, const float* regret_scale,
                                const float* strat_scale, int precision, uint64_t scratch_budget, pts_engine** out);

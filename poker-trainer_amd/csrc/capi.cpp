@@ -275,7 +275,10 @@ pts_status pts_engine_create_q(int device, const pts_tree_desc* t, const uint32_
                                const int16_t* regrets, const uint16_t* sums, const float* regret_scale,
                                const float* strat_scale, int precision, uint64_t scratch_budget, pts_engine** out) {
   *out = nullptr;
-  if (!t || !slot || !regrets || !sums || !regret_scale || !strat_scale) return bad("invalid input: null argument");
+  if (!t || !slot) return bad("invalid input: null argument");
+  const bool zeros = !regrets && !sums && !regret_scale && !strat_scale;
+  if (!zeros && (!regrets || !sums || !regret_scale || !strat_scale))
+    return bad("invalid input: give all four storage arrays, or none for zeros");
   if (num_slots == 0) return bad("invalid input: no player slots");
   PlanInput in = desc_input(t);
   in.pslot = slot;
@@ -286,7 +289,7 @@ pts_status pts_engine_create_q(int device, const pts_tree_desc* t, const uint32_
   std::unique_ptr<Engine> e;
   Status s = Engine::create(device, std::move(plan), nullptr, nullptr, precision, scratch_budget, e, true);
   if (s != kOk) return st(s);
-  if ((s = e->upload_q(regrets, sums, regret_scale, strat_scale)) != kOk) return st(s);
+  if (!zeros && (s = e->upload_q(regrets, sums, regret_scale, strat_scale)) != kOk) return st(s);
   *out = new pts_engine{std::move(e)};
   return PTS_OK;
 }

@@ -1883,11 +1883,17 @@ Status Engine::create(int device, Plan&& plan, const float* regrets, const float
   size_t fr = 0, tot = 0;
   HIPCHK(hipMemGetInfo(&fr, &tot));
   const size_t margin = 512ull << 20;
-  const size_t base_need = 2 * P.storage_len * sizeof(float) + P.num_nodes * 16 + (size_t)P.num_boards * (P.n[0] + P.n[1]) * 24;
+  // f32 storage: two 4-byte cells per storage cell; 16-bit storage: two
+  // 2-byte cells plus three 4-byte words per decision node (its two scales
+  // and the regrets' next scale), as the allocation below
+  const size_t base_need = 2 * P.storage_len * (q16 ? 2 : sizeof(float)) + (q16 ? (size_t)P.num_pslots * 12 : 0) +
+                           P.num_nodes * 16 + (size_t)P.num_boards * (P.n[0] + P.n[1]) * 24;
   if (base_need + margin > fr) {
+    const size_t avail = fr > margin ? fr - margin : 0;
     char m[200];
-    std::snprintf(m, sizeof m, "out of GPU memory: the storage needs %zu bytes, %zu available", base_need, fr);
+    std::snprintf(m, sizeof m, "out of GPU memory: the storage needs %zu bytes, %zu available", base_need, avail);
     set_error(m);
+    set_error_detail(ErrorDetail{0, (uint64_t)base_need, (uint64_t)avail});
     return kOutOfMemory;
   }
 

@@ -107,8 +107,8 @@ def _declare(L):
     L.pts_device_count.argtypes = [ctypes.POINTER(ctypes.c_int)]
     L.pts_evaluate.argtypes = [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint64, ctypes.c_void_p]
     L.pts_backend_wants_gpu.argtypes = [ctypes.c_int, ctypes.c_int]
-    L.pts_engine_create.argtypes = [ctypes.c_int, ctypes.POINTER(TreeDescC), f32p, f32p, ctypes.c_int, ctypes.c_uint64,
-                                    ctypes.POINTER(P)]
+    L.pts_engine_create.argtypes = [ctypes.c_int, ctypes.POINTER(TreeDescC), ctypes.c_void_p, ctypes.c_void_p,
+                                    ctypes.c_int, ctypes.c_uint64, ctypes.POINTER(P)]
     L.pts_engine_iterate.argtypes = [P, ctypes.c_float, ctypes.c_float, ctypes.c_float, ctypes.c_int]
     L.pts_engine_traverse.argtypes = [P, ctypes.c_int, ctypes.c_int, ctypes.c_float, ctypes.c_float, ctypes.c_float,
                                       ctypes.c_int, f64p, ctypes.c_size_t]
@@ -194,8 +194,8 @@ def _declare(L):
     L.pts_solver_get_storage_q.argtypes = [P, i16p, u16p, f32p, f32p]
     L.pts_solver_set_storage_q.argtypes = [P, i16p, u16p, f32p, f32p]
     L.pts_engine_create_q.argtypes = [ctypes.c_int, ctypes.c_void_p, np.ctypeslib.ndpointer(dtype=np.uint32),
-                                      ctypes.c_uint32, i16p, u16p, f32p, f32p, ctypes.c_int, ctypes.c_uint64,
-                                      ctypes.POINTER(ctypes.c_void_p)]
+                                      ctypes.c_uint32, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                      ctypes.c_void_p, ctypes.c_int, ctypes.c_uint64, ctypes.POINTER(ctypes.c_void_p)]
     L.pts_engine_download_q.argtypes = [P, i16p, u16p, f32p, f32p]
     L.pts_solver_profile_iteration.argtypes = [P, f64p, ctypes.POINTER(ctypes.c_uint32), ctypes.POINTER(ctypes.c_double)]
     L.pts_engine_profile_iteration.argtypes = [P, ctypes.c_float, ctypes.c_float, ctypes.c_float, ctypes.c_int, f64p,

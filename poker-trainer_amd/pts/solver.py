@@ -342,17 +342,22 @@ class Engine:
         self.tree = tree
         h = ctypes.c_void_p()
         if q16 is None:
-            check(lib().pts_engine_create(device, ctypes.byref(tree.desc), np.ascontiguousarray(regrets, np.float32),
-                                          np.ascontiguousarray(sums, np.float32), precision, scratch_budget,
-                                          ctypes.byref(h)))
+            # regrets / sums None: zeros
+            arrs = [None if x is None else np.ascontiguousarray(x, np.float32) for x in (regrets, sums)]
+            check(lib().pts_engine_create(device, ctypes.byref(tree.desc), *[None if a is None else a.ctypes.data
+                                                                             for a in arrs],
+                                          precision, scratch_budget, ctypes.byref(h)))
             self.slots = 0
         else:
             slot, r, s, rs, ss = q16
-            self.slots = len(rs)
+            self.slots = len(slot) if rs is None else len(rs)
+            arrs = None if r is None else [np.ascontiguousarray(r, np.int16), np.ascontiguousarray(s, np.uint16),
+                                           np.ascontiguousarray(rs, np.float32), np.ascontiguousarray(ss, np.float32)]
+            self._q = arrs  # kept alive across the call
+            ptrs = [None] * 4 if arrs is None else [a.ctypes.data for a in arrs]
             check(lib().pts_engine_create_q(device, ctypes.byref(tree.desc), np.ascontiguousarray(slot, np.uint32),
-                                            self.slots, np.ascontiguousarray(r, np.int16), np.ascontiguousarray(s, np.uint16),
-                                            np.ascontiguousarray(rs, np.float32), np.ascontiguousarray(ss, np.float32),
-                                            precision, scratch_budget, ctypes.byref(h)))
+                                            len(slot) if rs is None else len(rs), *ptrs, precision, scratch_budget,
+                                            ctypes.byref(h)))
         self.h = h
         self.storage_len = tree.desc.storage_len
         self.n = [tree.desc.num_hands[0], tree.desc.num_hands[1]]

@@ -164,7 +164,7 @@ def oracle_tree(cfg):
 def create_status(t, n):
     h = ctypes.c_void_p()
     z = np.zeros(n, np.float32)
-    st = lib().pts_engine_create(0, ctypes.byref(t.desc), z, z, 1, 0, ctypes.byref(h))
+    st = lib().pts_engine_create(0, ctypes.byref(t.desc), z.ctypes.data, z.ctypes.data, 1, 0, ctypes.byref(h))
     if st == 0:
         lib().pts_engine_destroy(h)
     return st
