@@ -259,7 +259,7 @@ static PlanInput desc_input(const pts_tree_desc* t) {
 pts_status pts_engine_create(int device, const pts_tree_desc* t, const float* regrets, const float* sums, int precision,
                              uint64_t scratch_budget, pts_engine** out) {
   *out = nullptr;
-  if (!t || !regrets || !sums) return bad("invalid input: null argument");
+  if (!t) return bad("invalid input: null argument");  // null regrets or sums: zeros
   PlanInput in = desc_input(t);
   Plan plan;
   std::string err = build_plan(in, plan);
