@@ -723,25 +723,53 @@ __global__ void __launch_bounds__(TERM_T) terminal_kernel(DPlan P, int p, uint32
 // instead of one lane's sequential chain: the sums differ from the
 // reference's only in summation order (a few f64 ulps), so an f32 value
 // occasionally differs by one rounding step, as the reference's own `Fast`
-// precision does (crates/solver/src/gpu/mod.rs:51-63). Prefix and suffix
-// sums are scanned separately, so a sum over terms that are all zero is an
-// exact zero, as in the sequential order.
+// precision does (crates/solver/src/gpu/mod.rs:51-63).
 //
-// One workgroup per run of terminals that share a river board (TermBatch):
-// the board's index lists and per-hand counts are loaded into registers
-// once, and each terminal's reach row is fetched while the previous
-// terminal is summed. Per terminal:
-//   the reach row to LDS; barrier;
-//   per-card prefix and suffix scans (8 lanes per card, shuffles) and the
-//   strength-order prefix and suffix scans (contiguous slices per lane,
-//   wave shuffles, wave totals); barrier; the cross-wave offsets; barrier;
+// Only prefix sums are stored, as the reference's GPU showdown does
+// (crates/solver/cuda/kernels/terminal.cu:307-384): a suffix is the total
+// less a prefix. A suffix whose terms are all zero is set to an exact zero
+// (the last nonzero strength position is kept per terminal), so a hand
+// nothing beats sums to zero as in the sequential order.
+//
+// One workgroup per run of terminals that share a river board (TermBatch).
+// Per run, every lane computes its LDS addresses once: the strength-order
+// and card-list entries it gathers, and per hand the prefix slots it reads.
+// Per terminal:
+//   the reach row to LDS (skipped when the previous terminal read the same
+//   row); barrier;
+//   waves 0-1: the opponent's strength-order exclusive prefix (11 entries
+//   per lane, wave scans of the lane totals; wave 1's prefixes exclude wave
+//   0's total, added at the read); waves 2-5: per-card exclusive prefixes,
+//   4 lanes per card, 13 positions each, at 52 slots per card, and the
+//   card totals; barrier;
 //   one value per hand; barrier.
-constexpr int FT = 512;                       // fast terminal workgroup
-constexpr int FK = (1326 + FT - 1) / FT;      // hands per lane (strided)
-constexpr int FE = (1326 + FT - 1) / FT;      // strength-order entries per lane (contiguous)
-constexpr int CG = 8;                         // lanes per card list
-constexpr int CL = (51 + CG - 1) / CG;        // card-list entries per lane (a list holds at most 51 hands)
-static_assert(52 * CG <= FT, "card groups fit the workgroup");
+// A fold reads only totals (its scans stop there), and a fold that follows
+// its sibling showdown on the same reach row (river.rs:163-213 pairs them)
+// reuses that showdown's sums.
+constexpr int FT = 512;                   // fast terminal workgroup
+constexpr int FK = (1326 + FT - 1) / FT;  // hands per lane (strided)
+constexpr int SLN = 128;                  // strength-order lanes (waves 0 and 1)
+constexpr int SE = 11;                    // strength-order entries per lane
+constexpr uint32_t SH = 64 * SE;          // wave 0's entries
+constexpr int CG = 4;                     // lanes per card list
+constexpr int CE = 13;                    // card-list positions per lane
+constexpr int CL0 = SLN;                  // first card lane
+constexpr int CR = CG * CE;               // prefix slots per card
+constexpr uint32_t ZR = 1344;             // the reach row's zero slot (rows hold at most 1326 hands)
+static_assert(SLN * SE >= 1326 + 1, "strength prefixes cover every position of a 1326-hand list");
+static_assert(CR >= 51 + 1, "card prefixes cover every position of a 51-hand list");
+static_assert(CL0 + 52 * CG <= FT && CL0 % 64 == 0, "card lanes fit the workgroup, groups inside waves");
+static_assert(ZR >= ((1326 + 31) & ~31), "the zero slot follows the longest row");
+
+template <typename V>
+struct TermLds {
+  V rv[ZR + 1];           // the opponent's reach row, hand order; rv[ZR] = 0
+  double pa[SLN * SE];    // strength-order exclusive prefixes (wave 1's without wave 0's total)
+  double ca[52 * CR];     // per card: exclusive prefixes of its list
+  double ct[52];          // per card: its list's total
+  double wt[2];           // the strength-order totals of waves 0 and 1
+  uint32_t lz[2];         // per wave: one past the last nonzero strength position (0: none)
+};
 
 // The fields of a terminal record one traversal of player p reads, as
 // wave-uniform scalars.
@@ -777,71 +805,90 @@ __device__ __forceinline__ double dpp_f64(double x) {
   const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(x), CTRL, ROW_MASK, 0xF, false);
   return __hiloint2double(hi, lo);
 }
-__device__ __forceinline__ double lane_f64(double x, int l) {
-  return __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(x), l), __builtin_amdgcn_readlane(__double2loint(x), l));
+// The wave's largest u32 (0 is the identity), in every lane: the steps of wave_max.
+__device__ __forceinline__ uint32_t wave_max_u32(uint32_t x) {
+  x = max(x, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x111, 0xF, 0xF, false));
+  x = max(x, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x112, 0xF, 0xF, false));
+  x = max(x, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x114, 0xF, 0xF, false));
+  x = max(x, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x118, 0xF, 0xF, false));
+  x = max(x, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x142, 0xA, 0xF, false));
+  x = max(x, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x143, 0xC, 0xF, false));
+  return (uint32_t)__builtin_amdgcn_readlane((int)x, 63);
 }
 
 template <typename V, int p>  // p: the traverser (a constant, so per-player tables are static fields)
 __global__ void __launch_bounds__(FT, sizeof(V) == 4 ? 6 : 5) terminal_fast_kernel(DPlan P, uint32_t b0, Span span, Bias B,
                                                               const V* __restrict__ reach, const uint32_t* __restrict__ alive,
                                                               V* __restrict__ vals, uint32_t rowlen) {
-  extern __shared__ double sh[];
-  __shared__ double tot[52];
-  __shared__ double wtot[FT / 64];
-  __shared__ uint32_t offs[53];
+  __shared__ TermLds<V> L;
   const TermBatch tb = P.tbatch[b0 + blockIdx.x];
   constexpr int o = p ^ 1;
   const SpotTab& S = P.spots[tb.spot];
   const uint32_t nq = S.n[o], np = S.n[p], tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const uint32_t b = tb.board;
-  const bool sorted = b != NONE;  // strength order of the board, else hand order
+  const bool sorted = b != NONE;  // strength order of the board, else hand order (folds only)
 
-  // per hand (strided): cards (c1 | c2 << 8) | same-combo index << 16
-  // (0xFFFF: none), and the board's lt / le counts
-  uint32_t hcs[FK];
-  uint4 hc[FK];
-  const uint16_t* cp2 = (const uint16_t*)S.cards[p];
-  const uint4* hcnt = S.hand_cnt[p] + (size_t)(sorted ? b : 0) * np;
+  // Per hand (strided): h0 = lt | le << 16 against the opponent's strength
+  // order (lt 0xFFFF: the board blocks the hand); h1, h2 = the card-prefix
+  // slots of its first and second card at lt(card) | le(card) << 16; h3 =
+  // c1 | c2 << 8 | the reach slot of the identical opponent combo << 16 (ZR: none).
+  uint32_t h0[FK], h1[FK], h2[FK], h3[FK];
+  {
+    const uint16_t* cp2 = (const uint16_t*)S.cards[p];
+    const uint4* hcnt = S.hand_cnt[p] + (size_t)(sorted ? b : 0) * np;
 #pragma unroll
-  for (int k = 0; k < FK; k++) {
-    const uint32_t i = tid + k * FT;
-    const uint32_t sm = i < np ? S.same[p][i] : NONE;
-    hcs[k] = (i < np ? (uint32_t)cp2[i] : 0u) | ((sm == NONE ? 0xFFFFu : sm) << 16);
-    hc[k] = (sorted && i < np) ? hcnt[i] : make_uint4(0, 0, 0, 0);
+    for (int k = 0; k < FK; k++) {
+      const uint32_t i = tid + k * FT;
+      uint32_t c1 = 0, c2 = 0, sm = ZR;
+      uint4 hc = make_uint4(0xFFFFu, 0, 0, 0);
+      if (i < np) {
+        const uint32_t cc = cp2[i];
+        c1 = cc & 0xFF;
+        c2 = cc >> 8;
+        const uint32_t s = S.same[p][i];
+        sm = s == NONE ? ZR : s;
+        if (sorted) hc = hcnt[i];
+      }
+      h0[k] = hc.x;
+      h1[k] = (c1 * CR + (hc.y & 0xFFFF)) | (c1 * CR + (hc.y >> 16)) << 16;
+      h2[k] = (c2 * CR + (hc.z & 0xFFFF)) | (c2 * CR + (hc.z >> 16)) << 16;
+      h3[k] = c1 | c2 << 8 | sm << 16;
+    }
   }
-  // the opponent's entries in strength order (or hand order): a contiguous
-  // slice per lane
-  const uint32_t nA = sorted ? S.ent_cnt[o][b] : nq;
-  const uint16_t* idxA = S.ent_idx[o] + (sorted ? S.ent_off[o][b] : 0u);
-  uint32_t ja[FE];
+  // The entries this lane gathers (reach slots, ZR past the list): waves 0-1
+  // a contiguous slice of the strength order; card lanes a slice of their
+  // card's list.
+  uint32_t src[(CE + 1) / 2];  // two 16-bit reach slots per register
+  const bool slane = tid < SLN, clane = tid >= CL0 && tid < CL0 + 52 * CG;
+  const uint32_t card = clane ? (tid - CL0) / CG : 0, cq = (tid - CL0) & (CG - 1);
+  {
+    const uint32_t nA = sorted ? S.ent_cnt[o][b] : nq;
+    uint32_t s1[CE];
 #pragma unroll
-  for (int k = 0; k < FE; k++) {
-    const uint32_t j = tid * FE + k;
-    ja[k] = j < nA ? (sorted ? (uint32_t)idxA[j] : j) : NONE;
-  }
-  // the opponent's hands holding card tid / CG, a slice per lane of the group
-  if (tid < 53) offs[tid] = sorted ? (uint32_t)S.cl_off[o][(size_t)b * 53 + tid] : S.bl_off[o][tid];
-  const uint16_t* idxC = sorted ? S.cl_idx[o] + S.cl_base[o][b] : S.bl_idx[o];
-  const uint32_t cq = tid & (CG - 1), card = tid / CG;
-  uint32_t cs = 0, ce = 0;
-  if (tid < 52 * CG) {
-    const uint32_t a = sorted ? (uint32_t)S.cl_off[o][(size_t)b * 53 + card] : S.bl_off[o][card];
-    const uint32_t e = sorted ? (uint32_t)S.cl_off[o][(size_t)b * 53 + card + 1] : S.bl_off[o][card + 1];
-    const uint32_t L = (e - a + CG - 1) / CG;
-    cs = min(a + cq * L, e);
-    ce = min(cs + L, e);
-  }
-  uint32_t jc[CL];
+    for (int k = 0; k < CE; k++) s1[k] = ZR;
+    if (slane) {
+      const uint16_t* idxA = S.ent_idx[o] + (sorted ? S.ent_off[o][b] : 0u);
 #pragma unroll
-  for (int k = 0; k < CL; k++) jc[k] = cs + k < ce ? (uint32_t)idxC[cs + k] : NONE;
-
-  double* Pa = sh;                // [rowlen] ascending prefix over entries
-  double* Sa = sh + rowlen;       // [rowlen] descending suffix
-  double* Ca = sh + 2 * rowlen;   // [2 rowlen] per-card ascending prefix
-  double* Cd = sh + 4 * rowlen;   // [2 rowlen] per-card descending suffix
-  V* rv = (V*)(sh + 6 * rowlen);  // [rowlen] the opponent's reach row, hand order
+      for (int k = 0; k < SE; k++) {
+        const uint32_t j = tid * SE + k;
+        if (j < nA) s1[k] = sorted ? (uint32_t)idxA[j] : j;
+      }
+    } else if (clane) {
+      const uint32_t a = sorted ? (uint32_t)S.cl_off[o][(size_t)b * 53 + card] : S.bl_off[o][card];
+      const uint32_t e = sorted ? (uint32_t)S.cl_off[o][(size_t)b * 53 + card + 1] : S.bl_off[o][card + 1];
+      const uint16_t* idx = sorted ? S.cl_idx[o] + S.cl_base[o][b] : S.bl_idx[o];
+#pragma unroll
+      for (int k = 0; k < CE; k++) {
+        const uint32_t x = a + cq * CE + k;
+        if (x < e) s1[k] = idx[x];
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < CE; k += 2) src[k / 2] = s1[k] | (k + 1 < CE ? s1[k + 1] : ZR) << 16;
+  }
+  auto slot_of = [&](int k) -> uint32_t { return (k & 1) ? src[k / 2] >> 16 : src[k / 2] & 0xFFFF; };
+  if (tid == 0) L.rv[ZR] = V(0);
   const double hp = S.half_pot, norm = S.norm;
-  const uint8_t* co = S.cards[o];
 
   // the first terminal's record, flag and reach row
   uint32_t t = tb.t0;
@@ -857,217 +904,173 @@ __global__ void __launch_bounds__(FT, sizeof(V) == 4 ? 6 : 5) terminal_fast_kern
       r[k] = j < nq ? R[j] : V(0);
     }
   }
+  uint32_t have_slot = NONE;  // the slot whose row is in L.rv
+  int have = 0;               // what L holds for it: 0 nothing, 1 totals, 2 prefixes and totals
   for (;;) {
-    // The per-lane lists are re-declared live each terminal (empty asm), so
-    // their range tests are recomputed here (a compare each) instead of
-    // being hoisted out of the loop as lane masks that spill.
+    // The per-lane tables are re-declared live each terminal (empty asm), so
+    // what is derived from them (addresses, range tests) is recomputed here
+    // instead of being hoisted out of the loop into registers that spill.
 #pragma unroll
-    for (int k = 0; k < CL; k++) asm volatile("" : "+v"(jc[k]));
+    for (int k = 0; k < FK; k++) asm volatile("" : "+v"(h0[k]), "+v"(h1[k]), "+v"(h2[k]), "+v"(h3[k]));
 #pragma unroll
-    for (int k = 0; k < FE; k++) asm volatile("" : "+v"(ja[k]));
-#pragma unroll
-    for (int k = 0; k < FK; k++) asm volatile("" : "+v"(hcs[k]), "+v"(hc[k].x), "+v"(hc[k].y), "+v"(hc[k].z));
-    asm volatile("" : "+v"(cs), "+v"(ce));
-    uint32_t tl = tid;  // the thread index, re-declared likewise
-    asm volatile("" : "+v"(tl));
-    const uint32_t ll = tl & 63, ql = tl & (CG - 1);
+    for (int k = 0; k < (CE + 1) / 2; k++) asm volatile("" : "+v"(src[k]));
     // fetch the next terminal while this one is summed (its slot is in
-    // range whether or not it is alive)
+    // range whether or not it is alive); a terminal on the same row reuses it
     const bool more = t + 1 < tend;
     TermFields trn = tr;
     uint32_t liven = 0;
     V rn[FK];
+#pragma unroll
+    for (int k = 0; k < FK; k++) rn[k] = r[k];
     if (more) {
       trn = term_fields<p>(P.trec + t + 1);
       const uint32_t sn = B.s(trn.slot);
       liven = span.has(trn.node) ? alive[sn] : 0u;
-      const V* R = reach + (size_t)sn * rowlen;
+      if (trn.slot != tr.slot) {
+        const V* R = reach + (size_t)sn * rowlen;
 #pragma unroll
-      for (int k = 0; k < FK; k++) {
-        const uint32_t j = tl + k * FT;
-        rn[k] = j < nq ? R[j] : V(0);
+        for (int k = 0; k < FK; k++) {
+          const uint32_t j = tid + k * FT;
+          rn[k] = j < nq ? R[j] : V(0);
+        }
       }
     }
     if (live) {  // a dead slot's value row is never read
+      const uint32_t kind = tr.kf & 0xFF;
+      const bool showdown = kind == kShowdown;
+      const int need = showdown ? 2 : 1;
+      if (tr.slot != have_slot) {
 #pragma unroll
-      for (int k = 0; k < FK; k++) {
-        const uint32_t j = tl + k * FT;
-        if (j < nq) rv[j] = r[k];
+        for (int k = 0; k < FK; k++) {
+          const uint32_t j = tid + k * FT;
+          if (j < nq) L.rv[j] = r[k];
+        }
+        have_slot = tr.slot;
+        have = 0;
+        __syncthreads();
       }
-      __syncthreads();
+      if (have < need) {
+        if (slane) {
+          V v[SE];
+          double tot = 0.0;
+          uint32_t last = 0;
+#pragma unroll
+          for (int k = 0; k < SE; k++) v[k] = L.rv[slot_of(k)];
+#pragma unroll
+          for (int k = 0; k < SE; k++) {
+            tot += (double)v[k];
+            if (v[k] != V(0)) last = tid * SE + k + 1;
+          }
+          // inclusive scan of the lane totals across the wave: DPP row
+          // shifts within rows of 16, then rows 0 and 2's last lanes into
+          // rows 1 and 3, then row 1's last lane into rows 2 and 3
+          double inc = tot;
+          inc += dpp_f64<0x111>(inc);
+          inc += dpp_f64<0x112>(inc);
+          inc += dpp_f64<0x114>(inc);
+          inc += dpp_f64<0x118>(inc);
+          inc += dpp_f64<0x142, 0xA>(inc);
+          inc += dpp_f64<0x143, 0xC>(inc);
+          if (need == 2) {
+            double run = dpp_f64<0x138>(inc);  // wave_shr:1
+            if (lane == 0) run = 0.0;
+#pragma unroll
+            for (int k = 0; k < SE; k++) {
+              L.pa[tid * SE + k] = run;
+              run += (double)v[k];
+            }
+            last = wave_max_u32(last);
+          }
+          if (lane == 63) {
+            L.wt[wave] = inc;
+            L.lz[wave] = last;
+          }
+        } else if (clane) {
+          V v[CE];
+          double tot = 0.0;
+#pragma unroll
+          for (int k = 0; k < CE; k++) v[k] = L.rv[slot_of(k)];
+#pragma unroll
+          for (int k = 0; k < CE; k++) tot += (double)v[k];
+          // inclusive scan over the card's 4 lanes (a value from the
+          // neighbouring group is masked out)
+          double inc = tot;
+          const double y1 = dpp_f64<0x111>(inc);
+          if (cq >= 1) inc += y1;
+          const double y2 = dpp_f64<0x112>(inc);
+          if (cq >= 2) inc += y2;
+          if (need == 2) {
+            double run = dpp_f64<0x111>(inc);
+            if (cq == 0) run = 0.0;
+#pragma unroll
+            for (int k = 0; k < CE; k++) {
+              L.ca[card * CR + cq * CE + k] = run;
+              run += (double)v[k];
+            }
+          }
+          if (cq == CG - 1) L.ct[card] = inc;
+        }
+        have = need;
+        __syncthreads();
+      }
       const double rake = (double)tr.rake;
       const int32_t inv_p = p ? tr.inv1 : tr.inv0, inv_o = p ? tr.inv0 : tr.inv1;
-      const uint32_t kind = tr.kf & 0xFF;
       const bool batched = ((tr.kf >> 8) & kTermBatched) != 0 && !span.kid(tr.node);
-      const bool showdown = kind == kShowdown;
       V* out = vals + (size_t)B.v(tr.vslot) * rowlen;
-
-      // per-card scans: 8 lanes per card, prefix and suffix
-      if (tl < 52 * CG) {
-        double v[CL];
-        double tsum = 0.0;
-#pragma unroll
-        for (int k = 0; k < CL; k++) {
-          v[k] = jc[k] != NONE ? (double)rv[jc[k]] : 0.0;
-          tsum += v[k];
-        }
-        // (a fold needs only the suffix: the card's total)
-        // inclusive suffix and prefix scans over the card's 8 lanes
-        // (Hillis-Steele steps 1, 2, 4 by DPP row shifts; a value from the
-        // neighbouring group is masked out)
-        double inc = tsum, incs = tsum;
-        {
-          const double z1 = dpp_f64<0x101>(incs);
-          if ((int)ql + 1 < CG) incs += z1;
-          const double z2 = dpp_f64<0x102>(incs);
-          if ((int)ql + 2 < CG) incs += z2;
-          const double z4 = dpp_f64<0x104>(incs);
-          if ((int)ql + 4 < CG) incs += z4;
-          if (showdown) {
-            const double y1 = dpp_f64<0x111>(inc);
-            if ((int)ql >= 1) inc += y1;
-            const double y2 = dpp_f64<0x112>(inc);
-            if ((int)ql >= 2) inc += y2;
-            const double y4 = dpp_f64<0x114>(inc);
-            if ((int)ql >= 4) inc += y4;
-          }
-        }
-        if (ql == 0) tot[card] = incs;  // the card's total
-        if (showdown) {
-          double ex = dpp_f64<0x111>(inc), exs = dpp_f64<0x101>(incs);
-          if (ql == 0) ex = 0.0;
-          if (ql == CG - 1) exs = 0.0;
-#pragma unroll
-          for (int k = 0; k < CL; k++) {
-            ex += v[k];
-            if (cs + k < ce) Ca[cs + k] = ex;
-          }
-#pragma unroll
-          for (int k = CL - 1; k >= 0; k--) {
-            exs += v[k];
-            if (cs + k < ce) Cd[cs + k] = exs;
-          }
-        }
-      }
-      // strength-order scans (showdown) or the total (fold)
-      double v[FE];
-      double tsum = 0.0;
-#pragma unroll
-      for (int k = 0; k < FE; k++) {
-        v[k] = ja[k] != NONE ? (double)rv[ja[k]] : 0.0;
-        tsum += v[k];
-      }
-      // (a fold needs only the prefix: the wave's total at lane 63).
-      // Inclusive prefix: DPP scans within rows of 16 lanes, then each row's
-      // last lane broadcast into the next rows. Inclusive suffix: scans
-      // within rows the other way, then the later rows' totals (their first
-      // lanes) added. Terms that are all zero sum to an exact zero either way.
-      double inc = tsum, incs = tsum;
-      inc += dpp_f64<0x111>(inc);
-      inc += dpp_f64<0x112>(inc);
-      inc += dpp_f64<0x114>(inc);
-      inc += dpp_f64<0x118>(inc);
-      inc += dpp_f64<0x142, 0xA>(inc);
-      inc += dpp_f64<0x143, 0xC>(inc);
-      if (showdown) {
-        incs += dpp_f64<0x101>(incs);
-        incs += dpp_f64<0x102>(incs);
-        incs += dpp_f64<0x104>(incs);
-        incs += dpp_f64<0x108>(incs);
-        const double t1 = lane_f64(incs, 16), t2 = lane_f64(incs, 32), t3 = lane_f64(incs, 48);
-        const double t23 = t2 + t3;
-        const uint32_t row = ll >> 4;
-        if (row < 3) incs += row == 2 ? t3 : row == 1 ? t23 : t1 + t23;
-      }
-      if (ll == 63) wtot[wave] = inc;
-      __syncthreads();
+      const double W0 = L.wt[0], total = W0 + L.wt[1];
       if (!showdown) {  // cfr.rs:224-235, eval.rs:39-64,170-206
-        double total = 0.0;
-#pragma unroll
-        for (int w = 0; w < FT / 64; w++) total += wtot[w];
         const int f = (int)kind - kFold0;
         const double pay = (f == p ? -(hp + (double)inv_p) : hp + (double)(f ? tr.inv1 : tr.inv0) - rake) * norm;
 #pragma unroll
         for (int k = 0; k < FK; k++) {
-          const uint32_t i = tl + k * FT;
-          if (i < np) {
-            const int a = hcs[k] & 0xFF, c = (hcs[k] >> 8) & 0xFF;
-            const uint32_t sm = hcs[k] >> 16;
-            const double sw = sm == 0xFFFF ? 0.0 : (double)rv[sm];
-            V x;
-            if (batched) x = (V)(pay * (total - tot[a] - tot[c] + sw));
-            else x = total == 0.0 ? V(0) : (V)(pay * (total - (tot[a] + tot[c]) + sw));
-            out[i] = x;
-          }
+          const uint32_t i = tid + k * FT;
+          if (i >= np) continue;
+          const double ta = L.ct[h3[k] & 0xFF], tc = L.ct[(h3[k] >> 8) & 0xFF];
+          const double sw = (double)L.rv[h3[k] >> 16];
+          V x;
+          if (batched) x = (V)(pay * (total - ta - tc + sw));
+          else x = total == 0.0 ? V(0) : (V)(pay * (total - (ta + tc) + sw));
+          out[i] = x;
         }
       } else {  // showdown: cfr.rs:238-246 payoffs, eval.rs:66-148,208-379 sweeps
-        double offp = 0.0, offs_ = 0.0;
-#pragma unroll
-        for (int w = 0; w < FT / 64; w++) {
-          const double x = wtot[w];
-          if (w < (int)wave) offp += x;
-          if (w > (int)wave) offs_ += x;
-        }
-        double ex = dpp_f64<0x138>(inc), exs = dpp_f64<0x130>(incs);  // wave_shr:1, wave_shl:1
-        if (ll == 0) ex = 0.0;
-        if (ll == 63) exs = 0.0;
-        double bp = offp + ex, bs = offs_ + exs;
-#pragma unroll
-        for (int k = 0; k < FE; k++) {
-          bp += v[k];
-          if (tl * FE + k < nA) Pa[tl * FE + k] = bp;
-        }
-#pragma unroll
-        for (int k = FE - 1; k >= 0; k--) {
-          bs += v[k];
-          if (tl * FE + k < nA) Sa[tl * FE + k] = bs;
-        }
-        __syncthreads();
+        const uint32_t lz = max(L.lz[0], L.lz[1]);  // one past the last nonzero strength position
         const double win = (hp + (double)inv_o - rake) * norm;
         const double lose = -(hp + (double)inv_p) * norm;
         const double tie = 0.5 * ((double)(inv_o - inv_p) - rake) * norm;
-        const uint32_t mo = nA;
 #pragma unroll
         for (int k = 0; k < FK; k++) {
-          const uint32_t i = tl + k * FT;
+          const uint32_t i = tid + k * FT;
           if (i >= np) continue;
-          const uint32_t lt = hc[k].x & 0xFFFF, le = hc[k].x >> 16;
+          const uint32_t lt = h0[k] & 0xFFFF, le = h0[k] >> 16;
           V x = V(0);
           if (lt != 0xFFFF) {
-            const int c1 = hcs[k] & 0xFF, c2 = (hcs[k] >> 8) & 0xFF;
-            const uint32_t l1 = hc[k].y & 0xFFFF, m1 = hc[k].y >> 16, l2 = hc[k].z & 0xFFFF, m2 = hc[k].z >> 16;
-            const uint32_t o1 = offs[c1], o2 = offs[c2];
-            const uint32_t len1 = offs[c1 + 1] - o1, len2 = offs[c2 + 1] - o2;
-            const double Pv = lt ? Pa[lt - 1] : 0.0;
-            const double P1 = l1 ? Ca[o1 + l1 - 1] : 0.0;
-            const double P2 = l2 ? Ca[o2 + l2 - 1] : 0.0;
-            const double Sv = le < mo ? Sa[le] : 0.0;
-            const double S1 = m1 < len1 ? Cd[o1 + m1] : 0.0;
-            const double S2 = m2 < len2 ? Cd[o2 + m2] : 0.0;
+            const double Pv = L.pa[lt] + (lt >= SH ? W0 : 0.0);
+            const double Pe = L.pa[le] + (le >= SH ? W0 : 0.0);
+            const double P1 = L.ca[h1[k] & 0xFFFF], Q1 = L.ca[h1[k] >> 16];
+            const double P2 = L.ca[h2[k] & 0xFFFF], Q2 = L.ca[h2[k] >> 16];
+            double Sv = 0.0, S1 = 0.0, S2 = 0.0;
+            if (le < lz) {  // something stronger has reach (else every stronger sum is an exact zero)
+              Sv = total - Pe;
+              S1 = L.ct[h3[k] & 0xFF] - Q1;
+              S2 = L.ct[(h3[k] >> 8) & 0xFF] - Q2;
+            }
             if (batched) {
               x = (V)(win * (Pv - P1 - P2) + lose * (Sv - S1 - S2));
             } else {
               x = (V)(win * (Pv - (P1 + P2)));
               x = x + (V)(lose * (Sv - (S1 + S2)));
             }
-            if (tie != 0.0) {  // eval.rs:347-379
-              double es = 0.0, e1 = 0.0, e2 = 0.0;
-              for (uint32_t j = lt; j < le; j++) {
-                const uint32_t idx = idxA[j];
-                const double w = (double)rv[idx];
-                es += w;
-                if (holds(co, idx, c1)) e1 += w;
-                if (holds(co, idx, c2)) e2 += w;
-              }
-              const uint32_t sm = hcs[k] >> 16;
-              const double sw = sm == 0xFFFF ? 0.0 : (double)rv[sm];
-              x = x + (V)(tie * (es - (e1 + e2) + sw));
+            if (tie != 0.0) {  // eval.rs:347-379: the tied block, less the hands holding a card, plus the same combo
+              const double sw = (double)L.rv[h3[k] >> 16];
+              x = x + (V)(tie * ((Pe - Pv) - ((Q1 - P1) + (Q2 - P2)) + sw));
             }
           }
           out[i] = x;
         }
       }
-      __syncthreads();  // the next terminal overwrites rv, the scans, tot and wtot
+      // the next terminal rewrites L unless it reads this row's sums as they are
+      const bool keep = more && liven && trn.slot == tr.slot && ((trn.kf & 0xFF) != kShowdown || have == 2);
+      if (!keep) __syncthreads();
     }
     if (!more) break;
     t++;
@@ -2113,8 +2116,8 @@ Status Engine::create(int device, Plan&& plan, const float* regrets, const float
   e->lds_down_ = mm * sizeof(MemRec) + ((mm + 15) & ~size_t(15)) + 4 * mm + (q16 ? (size_t)DG_A * rowlen * 4 : 0);
   for (int f64 = 0; f64 < 2; f64++) {
     size_t vs = f64 ? 8 : 4;
-    e->lds_term_[f64] = precision == kExact ? (size_t)rowlen * 3 * 8 + (size_t)rowlen * vs
-                                            : (size_t)rowlen * 6 * 8 + (size_t)rowlen * vs;
+    // the fast terminal kernel's LDS is static (TermLds)
+    e->lds_term_[f64] = precision == kExact ? (size_t)rowlen * 3 * 8 + (size_t)rowlen * vs : 0;
     e->lds_up_[f64] = mm * sizeof(MemRec) + ((mm + 15) & ~size_t(15)) + mm * UP_T * vs;
   }
   size_t lds_max = std::max({e->lds_term_[1], e->lds_up_[1], e->lds_down_});

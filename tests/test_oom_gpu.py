@@ -16,7 +16,7 @@ from test_host_cpu import oracle_tree
 
 pytestmark = pytest.mark.gpu
 
-HBM = 288 * 10 ** 9
+HBM = 288 << 30  # 288 GiB of HBM3E
 
 
 def tree_with_storage(cells):
