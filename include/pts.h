@@ -325,8 +325,11 @@ pts_status pts_batch_iterate(pts_batch* b, uint32_t count);
 /* Exploitability of every spot in chips (FP64 best responses, raked as Solver). */
 pts_status pts_batch_exploitability(pts_batch* b, double* out);
 /* Iterates until every spot's exploitability is at most target_pct of its
- * pot (checked every check_every iterations; a spot that reaches it is
- * frozen) or max_iterations; out[spot] is each spot's progress. */
+ * pot or max_iterations; out[spot] is each spot's progress. Each spot keeps
+ * the single solve's check schedule (lib.rs:155-172,594-642: every
+ * check_every iterations until its own trend predicts the target sooner);
+ * a check measures only the spots due, and a spot at its target is frozen
+ * (its traversals skipped). */
 pts_status pts_batch_solve(pts_batch* b, uint32_t max_iterations, double target_pct, uint32_t check_every,
                            pts_progress_c* out);
 /* Autotuner: times `iterations` iterations for every tile choice (terminals
@@ -390,7 +393,11 @@ pts_status pts_solver_copy_root_street(pts_solver* dst, pts_solver* src, float r
 /* Pins dst's root-street decision nodes to src's average strategy (no
  * reference counterpart; the value-net judge's re-solve of the later streets
  * under the depth-limited root strategy): sums copied, regrets set to the
- * average strategy's probabilities so that regret matching plays it. */
+ * average strategy's probabilities so that regret matching plays it, and
+ * those nodes frozen: DCFR iterations leave their regrets as written, so
+ * both traversals of every iteration play the pinned strategy (the strategy
+ * sums keep growing in the same proportions). The nodes stay frozen for the
+ * solver's life. */
 pts_status pts_solver_pin_root_street(pts_solver* dst, pts_solver* src, uint32_t* pinned);
 
 #ifdef __cplusplus

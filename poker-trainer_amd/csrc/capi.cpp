@@ -882,12 +882,12 @@ static bool same_dcfr(const pts_config_c& a, const pts_config_c& b) {
          a.restart_average == b.restart_average && a.unweighted_average == b.unweighted_average;
 }
 
-// Exploitability of every spot (chips), all spots active for the traversals.
-static pts_status batch_expl(pts_batch* b, std::vector<double>& out) {
+// Exploitability (chips) of the spots `mask` names (the others' traversals
+// are skipped and their entries left 0), then the solve's own active set back.
+static pts_status batch_expl(pts_batch* b, const std::vector<uint32_t>& mask, std::vector<double>& out) {
   const uint32_t K = (uint32_t)b->spots.size();
-  std::vector<uint32_t> all(K, 1u);
   Status e;
-  if ((e = b->engine->set_active(all)) != kOk) return st(e);
+  if ((e = b->engine->set_active(mask)) != kOk) return st(e);
   std::vector<double> br, ev;
   if ((e = b->engine->root_values_all(kBest, br)) != kOk) return st(e);
   bool any_raked = false;
@@ -895,7 +895,8 @@ static pts_status batch_expl(pts_batch* b, std::vector<double>& out) {
   if (any_raked && (e = b->engine->root_values_all(kAverage, ev)) != kOk) return st(e);
   out.assign(K, 0.0);
   for (uint32_t k = 0; k < K; k++)  // cfr.rs:494-502
-    out[k] = raked(b->spots[k].get()) ? ((br[2 * k] - ev[2 * k]) + (br[2 * k + 1] - ev[2 * k + 1])) / 2.0
+    if (mask[k])
+      out[k] = raked(b->spots[k].get()) ? ((br[2 * k] - ev[2 * k]) + (br[2 * k + 1] - ev[2 * k + 1])) / 2.0
                                       : (br[2 * k] + br[2 * k + 1]) / 2.0;
   if ((e = b->engine->set_active(b->active)) != kOk) return st(e);
   return PTS_OK;
@@ -976,7 +977,7 @@ pts_status pts_batch_iterate(pts_batch* b, uint32_t count) {
 pts_status pts_batch_exploitability(pts_batch* b, double* out) {
   NEED_BATCH_ENGINE(b);
   std::vector<double> x;
-  pts_status r = batch_expl(b, x);
+  pts_status r = batch_expl(b, std::vector<uint32_t>(b->spots.size(), 1u), x);
   if (r != PTS_OK) return r;
   std::memcpy(out, x.data(), x.size() * sizeof(double));
   return PTS_OK;
@@ -984,31 +985,48 @@ pts_status pts_batch_exploitability(pts_batch* b, double* out) {
 
 pts_status pts_batch_solve(pts_batch* b, uint32_t max_iterations, double target_pct, uint32_t check_every,
                            pts_progress_c* out) {
+  // every spot on its own copy of the single solve's check schedule
+  // (lib.rs:594-642): a check when the spot's next check is due, the gap
+  // from its own last two checks (next_check_gap); only the spots due are
+  // measured, and a spot at its target is frozen
   NEED_BATCH_ENGINE(b);
   const uint32_t K = (uint32_t)b->spots.size(), ce = std::max(check_every, 1u);
   for (uint32_t k = 0; k < K; k++)
     out[k] = {b->iterations, max_iterations, std::numeric_limits<double>::infinity(), std::numeric_limits<double>::infinity(),
               0, 0};
+  std::vector<uint32_t> next(K, b->iterations + ce), lt(K, 0);
+  std::vector<double> le(K, 0.0);
+  std::vector<bool> has_last(K, false);
   uint32_t left = 0;
   for (uint32_t k = 0; k < K; k++) left += b->active[k];
   while (b->iterations < max_iterations && left > 0) {
     pts_status r = pts_batch_iterate(b, 1);
     if (r != PTS_OK) return r;
-    const bool last = b->iterations >= max_iterations;
-    if (b->iterations % ce != 0 && !last) continue;
+    const uint32_t t = b->iterations;
+    const bool last = t >= max_iterations;
+    std::vector<uint32_t> due(K, 0u);
+    bool any = false;
+    for (uint32_t k = 0; k < K; k++)
+      if (b->active[k] && (t >= next[k] || last)) due[k] = 1u, any = true;
+    if (!any) continue;
     std::vector<double> x;
-    if ((r = batch_expl(b, x)) != PTS_OK) return r;
+    if ((r = batch_expl(b, due, x)) != PTS_OK) return r;
     for (uint32_t k = 0; k < K; k++) {
-      if (!b->active[k]) continue;
+      if (!due[k]) continue;
       const double pot = (double)b->spots[k]->cfg.pot, pct = 100.0 * x[k] / pot;
-      out[k].iteration = b->iterations;
+      out[k].iteration = t;
       out[k].exploitability = x[k];
       out[k].exploitability_pct = pct;
       if (pct <= target_pct || last) {
         out[k].finished = 1;
         b->active[k] = 0;  // a solved spot is frozen: its traversals are skipped
         left--;
+        continue;
       }
+      next[k] = t + check_gap(has_last[k], lt[k], le[k], t, x[k], target_pct * pot / 100.0, ce);
+      has_last[k] = true;
+      lt[k] = t;
+      le[k] = x[k];
     }
     Status e = b->engine->set_active(b->active);
     if (e != kOk) return st(e);
@@ -1224,6 +1242,7 @@ pts_status pts_solver_pin_root_street(pts_solver* dst, pts_solver* src, uint32_t
   const Tree& b = dst->tree;
   const uint8_t street = (uint8_t)src->spec.root_street;
   std::vector<std::pair<uint32_t, uint32_t>> stack{{0u, 0u}};
+  std::vector<uint32_t> frozen;
   uint32_t n = 0;
   while (!stack.empty()) {
     auto [x, y] = stack.back();
@@ -1241,9 +1260,14 @@ pts_status pts_solver_pin_root_street(pts_solver* dst, pts_solver* src, uint32_t
       for (size_t k = 0; k < na_; k++) r[k * nh + h] = d > 0.0 ? (float)(m[k * nh + h] / d) : (float)(1.0 / (double)na_);
     }
     if ((e = dst->engine->write_storage(nb.offset, len, r.data(), m.data())) != kOk) return st(e);
+    frozen.push_back(y);
     n++;
     for (uint32_t k = 0; k < na.num_children; k++) stack.push_back({na.first_child + k, nb.first_child + k});
   }
+  // the pinned regrets stay as written: both traversals of every iteration
+  // play the pinned strategy
+  Status e = dst->engine->set_frozen(frozen, true);
+  if (e != kOk) return st(e);
   *pinned = n;
   return PTS_OK;
 }

@@ -1211,7 +1211,7 @@ __device__ __forceinline__ uint64_t lane_u64(uint64_t x, uint32_t m) {
 }
 
 // Record flags of a member held by its lane
-constexpr uint32_t kFLive = 8, kFSpan = 16, kFRoot = 32;
+constexpr uint32_t kFLive = 8, kFSpan = 16, kFRoot = 32, kFFrozen = 64;
 
 // One wave per (segment, 64-hand tile), one hand per lane, the segment's
 // members in registers: lane m holds member m's record, read by every lane
@@ -1268,7 +1268,7 @@ __device__ __forceinline__ void up_reg_walk(const DPlan& P, int p, uint32_t seg0
       const MemRec& r = P.mrec[moff + mm];
       const bool lm = P.malive[moff + mm] != 0;
       f[q] = (uint32_t)r.kind | (lm ? kFLive : 0u) | (span.has(r.node) ? kFSpan : 0u) | (r.node == span.root ? kFRoot : 0u) |
-             (uint32_t)r.na << 8 | (uint32_t)r.clocal << 16;
+             ((r.tflags & kNodeFrozen) ? kFFrozen : 0u) | (uint32_t)r.na << 8 | (uint32_t)r.clocal << 16;
       vslot[q] = r.vslot;
       off[q] = r.offset;
       psr[q] = r.pslot;
@@ -1369,7 +1369,9 @@ __device__ __forceinline__ void up_reg_walk(const DPlan& P, int p, uint32_t seg0
           // child cl + a (a macro, not a lambda: the member vectors stay
           // plain values)
 #define UPDATED(a) (SL_GET(cl + (a)) * (SL_GET(cl + (a)) > 0.0f ? pos : neg) + LV_GET(cl + (a)) - outv)
-          if (QP == 0) {
+          if (fm & kFFrozen) {
+            // pinned regrets: no update (and, with 16-bit storage, no new scale)
+          } else if (QP == 0) {
             if (act)
               for (uint32_t a = 0; a < na; a++) sto.r[LN64(src, cl + a) + h] = UPDATED(a);
           } else if (QP == 1) {
@@ -1555,7 +1557,9 @@ __global__ void __launch_bounds__(UP_T) up_kernel(DPlan P, int p, uint32_t seg0,
               const float disc = x > 0.0f ? pos : neg;
               y[a] = x * disc + (float)cv[a] - outv;
             }
-          if (QP == 0) {
+          if (r.tflags & kNodeFrozen) {
+            // pinned regrets: no update (and, with 16-bit storage, no new scale)
+          } else if (QP == 0) {
 #pragma unroll
             for (int a = 0; a < MAXA; a++)
               if (a < (int)na) sto.r[off + (size_t)a * np + h] = y[a];
@@ -2749,7 +2753,28 @@ Status Engine::set_active(const std::vector<uint32_t>& active) {
     set_error("invalid input: one flag per spot");
     return kInvalidInput;
   }
+  HIPCHK(hipStreamSynchronize(stream_));
   HIPCHK(hipMemcpy(active_, active.data(), active.size() * 4, hipMemcpyHostToDevice));
+  return kOk;
+}
+
+Status Engine::set_frozen(const std::vector<uint32_t>& nodes, bool on) {
+  HIPCHK(hipStreamSynchronize(stream_));
+  for (uint32_t id : nodes) {
+    if (id >= plan_.num_nodes || plan_.kind[id] > kP1) {
+      set_error("invalid input: only decision nodes are frozen");
+      return kInvalidInput;
+    }
+    const size_t m = (size_t)plan_.seg_mem_off[plan_.seg_of[id]] + plan_.local[id];
+    if (m >= plan_.mrec.size() || plan_.members[m] != id) {
+      set_error("invalid input: node has no member record");
+      return kInvalidInput;
+    }
+    MemRec& r = plan_.mrec[m];
+    r.tflags = on ? (uint8_t)(r.tflags | kNodeFrozen) : (uint8_t)(r.tflags & ~kNodeFrozen);
+    HIPCHK(hipMemcpy((char*)dp_->d.mrec + m * sizeof(MemRec) + offsetof(MemRec, tflags), &r.tflags, 1,
+                     hipMemcpyHostToDevice));
+  }
   return kOk;
 }
 

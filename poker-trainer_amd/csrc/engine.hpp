@@ -96,6 +96,9 @@ class Engine {
   Status root_values_all(Mode mode, std::vector<double>& out);
   // Spots with active = 0 are skipped by every traversal (their roots are dead).
   Status set_active(const std::vector<uint32_t>& active);
+  // Freezes (or thaws) decision nodes: a frozen node's regrets are not
+  // updated by CFR traversals (its strategy stays what its regrets give).
+  Status set_frozen(const std::vector<uint32_t>& nodes, bool on);
   // Tiles: terminals per fast-terminal workgroup (run length, 1..16) and
   // opponent hands per down-kernel workgroup (256, 512 or 1024); rebuilds
   // the graphs. Same scratch groups (their rows do not depend on the runs).

@@ -32,6 +32,9 @@ namespace pts {
 constexpr uint8_t kTermBatched = 1;   // inside a river subtree (river.rs batched formulas)
 constexpr uint8_t kTermPaired0 = 2;   // fold read off the sibling showdown's sums (traverser 0)
 constexpr uint8_t kTermPaired1 = 4;   // (traverser 1)
+// Decision-node flag: its regrets are pinned (the up pass computes its value
+// and leaves its regrets as they are)
+constexpr uint8_t kNodeFrozen = 8;
 
 // One member of a segment, everything the down / up kernels read about it
 // (staged in LDS once per workgroup). Children of a decision node are

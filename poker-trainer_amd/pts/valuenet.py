@@ -235,11 +235,10 @@ def heldout_loss(net: ValueNet, X, Y, M):
 
 
 def resolve(cfg: SolveConfig, dl: Solver, target_pct: float = 0.5, max_iterations: int = 300, check_every: int = 10):
-    """The full tree with its flop decision nodes pinned to the depth-limited
-    solver's average strategy (re-pinned after every iteration) and the
-    later streets solved by exact DCFR, until the exact FP64 best response
-    of that profile is at or below the target. Returns (progress, seconds,
-    solver, reached)."""
+    """The full tree with its flop decision nodes pinned (frozen) to the
+    depth-limited solver's average strategy and the later streets solved by
+    exact DCFR, until the exact FP64 best response of that profile is at or
+    below the target. Returns (progress, seconds, solver, reached)."""
     full = Solver(cfg)
     t0 = time.perf_counter()
     pin_root_street(full, dl)
@@ -249,7 +248,6 @@ def resolve(cfg: SolveConfig, dl: Solver, target_pct: float = 0.5, max_iteration
     while t < max_iterations:
         full.iterate(1)
         t += 1
-        pin_root_street(full, dl)
         if t >= nxt or t == max_iterations:
             e = full.exploitability()
             hist.append((t, 100.0 * e / cfg.root.pot))

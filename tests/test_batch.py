@@ -8,7 +8,7 @@ import pytest
 import pyoracle as O
 from pts import spots
 from pts.native import EXACT, FAST
-from pts.solver import BatchSolver, Solver
+from pts.solver import BatchSolver, Solver, SolveOptions
 
 
 def small_batch():
@@ -65,18 +65,49 @@ def test_batch_iterations_bit_exact_per_spot():
 
 @pytest.mark.gpu
 def test_batch_solve_freezes_each_spot_at_its_target():
+    # every spot keeps the single solve's check schedule (lib.rs:155-172,
+    # 594-642), so it stops at the same check with the same exploitability
+    # as the same spot solved alone, and a frozen spot's storage is that of
+    # the spot alone after the same iterations
     cfgs = small_batch()
     b = BatchSolver(cfgs, precision=FAST)
     pr = b.solve(400, 1.0, 10)
     for k, p in enumerate(pr):
         assert p.finished
         assert p.exploitability_pct <= 1.0 or p.iteration == 400
-        # a frozen spot's storage is that of the same spot alone after the same iterations
         s = Solver(cfgs[k], precision=FAST)
-        s.iterate(p.iteration)
+        alone = s.solve(SolveOptions(400, 1.0, 10))
+        assert (p.iteration, p.exploitability) == (alone.iteration, alone.exploitability), k
         rb, sb = b.storage(k)
         rs, ss = s.storage()
         assert np.array_equal(rb, rs) and np.array_equal(sb, ss), k
+
+
+@pytest.mark.gpu
+def test_full_batch_workload_bit_exact():
+    # the batched workload at its size: 256 river and 64 turn spots in one
+    # plan, exact precision; every spot equals its own solve after the same
+    # iterations, and a sample (every 16th river spot, every 8th turn spot)
+    # equals the oracle bit for bit
+    cfgs = spots.batch_spots()
+    assert len(cfgs) == 320
+    b = BatchSolver(cfgs, precision=EXACT)
+    b.iterate(4)
+    xb = b.exploitability()
+    for k, c in enumerate(cfgs):
+        s = Solver(c, precision=EXACT)
+        s.iterate(4)
+        rb, sb = b.storage(k)
+        rs, ss = s.storage()
+        assert np.array_equal(rb, rs) and np.array_equal(sb, ss), k
+        assert xb[k] == s.exploitability(), k
+        if (k < 256 and k % 16 == 0) or (k >= 256 and (k - 256) % 8 == 0):
+            o = O.OracleSolver(c)
+            o.iterate(4)
+            rc, sc = o.storage()
+            assert np.array_equal(rb, rc) and np.array_equal(sb, sc), k
+            assert xb[k] == o.exploitability_f64(), k
+        del s
 
 
 @pytest.mark.gpu

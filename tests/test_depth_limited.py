@@ -140,3 +140,22 @@ def test_pinned_root_street_plays_the_depth_limited_average():
     want = np.where(tot > 0, sdb / np.where(tot > 0, tot, 1.0), 1.0 / na).astype(np.float32)
     assert np.allclose(rf[of:of + na * nh].reshape(na, nh), want, rtol=1e-6, atol=0)
     assert np.any(tot > 0)
+    # the pinned nodes are frozen: iterations of the full tree (both
+    # traversals) leave every pinned node's regrets as written, so the root
+    # keeps playing the pinned strategy and its average stays that strategy
+    pinned = [i for i in np.nonzero(nf["kind"] <= 1)[0] if nf["street"][i] == nf["street"][0]]
+    before = {i: rf[int(nf["offset"][i]):int(nf["offset"][i]) + int(nf["num_children"][i]) * full.n[int(nf["kind"][i])]].copy()
+              for i in pinned}
+    full.iterate(6)
+    r2, s2 = full.storage()
+    for i, blk in before.items():
+        o = int(nf["offset"][i])
+        assert np.array_equal(r2[o:o + len(blk)], blk), i
+    s2b = s2[of:of + na * nh].reshape(na, nh).astype(np.float64)
+    tot2 = s2b.sum(axis=0)
+    live = tot2 > 0
+    assert np.allclose((s2b / np.where(live, tot2, 1.0))[:, live], want[:, live], rtol=0, atol=1e-5)
+    # the later streets are not frozen: their regrets moved
+    later = [i for i in np.nonzero(nf["kind"] <= 1)[0] if nf["street"][i] != nf["street"][0]]
+    assert any(not np.array_equal(r2[int(nf["offset"][i]):int(nf["offset"][i]) + 4], rf[int(nf["offset"][i]):int(nf["offset"][i]) + 4])
+               for i in later)
