@@ -371,6 +371,15 @@ pts_status pts_net_get_params(pts_net* n, float* params);
 /* Gradients of the last pts_net_gradients / pts_net_train_step, packed alike. */
 pts_status pts_net_get_grads(pts_net* n, float* grads);
 pts_status pts_net_forward(pts_net* n, const float* x, uint32_t rows, float* y);
+/* pts_net_forward without allocating: no counterpart in the reference; for
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * serving loops, harnesses that forbid hipMalloc). x and y are device rows
+ * (rows x 2706 and rows x 2652 f32), rows a positive multiple of 64 up to
+ * 1024; the work is enqueued on `stream` (a hipStream_t; null: the
+ * network's own stream) and not waited for. The working rows live in one
+ * fixed device array of 20.8 MB per GPU (taken at pts_net_create), shared by
+ * every call on that GPU: calls that overlap on one GPU race. */
+pts_status pts_net_forward_noalloc(pts_net* n, const float* x, uint32_t rows, float* y, void* stream);
 pts_status pts_net_gradients(pts_net* n, const float* x, const float* target, const float* mask, uint32_t rows, double* loss);
 pts_status pts_net_train_step(pts_net* n, const float* x, const float* target, const float* mask, uint32_t rows, float lr,
                               double* loss);

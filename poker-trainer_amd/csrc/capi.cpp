@@ -1152,6 +1152,9 @@ pts_status pts_net_set_params(pts_net* n, const float* params) { return st(n->n-
 pts_status pts_net_get_params(pts_net* n, float* params) { return st(n->n->get_params(params)); }
 pts_status pts_net_get_grads(pts_net* n, float* grads) { return st(n->n->get_grads(grads)); }
 pts_status pts_net_forward(pts_net* n, const float* x, uint32_t rows, float* y) { return st(n->n->forward(x, rows, y)); }
+pts_status pts_net_forward_noalloc(pts_net* n, const float* x, uint32_t rows, float* y, void* stream) {
+  return st(n->n->forward_noalloc(x, rows, y, (hipStream_t)stream));
+}
 pts_status pts_net_gradients(pts_net* n, const float* x, const float* target, const float* mask, uint32_t rows, double* loss) {
   return st(n->n->gradients(x, target, mask, rows, loss));
 }

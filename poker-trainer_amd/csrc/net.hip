@@ -21,6 +21,7 @@
 #include <cstdio>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <random>
 #include <vector>
 
@@ -266,6 +267,37 @@ Net::~Net() {
   if (own_stream_ && stream_) (void)hipStreamDestroy(stream_);
 }
 
+// The working rows of the allocation-free forward (forward_noalloc): the
+// packed bf16 input, two bf16 hidden-row buffers used in turn and the padded
+// f32 output, for up to kScratchRows rows. Every GPU the library runs on
+// holds one copy until the process ends; all calls on a GPU share it, so
+// calls that overlap on one GPU race. Host code never names the array: its
+// device address is taken per device at Net::create (hipGetSymbolAddress,
+// the device current) and passed to the kernels.
+constexpr size_t kScrIn = (size_t)kScratchRows * IN_P * 2;
+constexpr size_t kScrHid = (size_t)kScratchRows * HID * 2;
+constexpr size_t kScrOut = (size_t)kScratchRows * OUT_P * 4;
+static __device__ __align__(256) unsigned char scratch[kScrIn + 2 * kScrHid + kScrOut];
+constexpr int kMaxDevices = 64;
+static unsigned char* g_scratch[kMaxDevices];
+static std::mutex g_scratch_mu;
+
+// The scratch array's address on `device` (current), taken once per device.
+static Status scratch_of(int device, unsigned char** out) {
+  if (device < 0 || device >= kMaxDevices) {
+    set_error("invalid input: device index beyond the scratch table");
+    return kInvalidInput;
+  }
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch[device]) {
+    void* ptr = nullptr;
+    NCHK(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+    g_scratch[device] = (unsigned char*)ptr;
+  }
+  *out = g_scratch[device];
+  return kOk;
+}
+
 Status Net::dalloc(void** p, size_t bytes) {
   hipError_t e = hipMalloc(p, bytes ? bytes : 1);
   if (e != hipSuccess) {
@@ -290,6 +322,8 @@ Status Net::create(int device, uint32_t batch_cap, std::unique_ptr<Net>& out) {
   NCHK(hipSetDevice(device));
   std::unique_ptr<Net> n(new Net());
   n->cap_ = batch_cap;
+  Status sst = scratch_of(device, &n->scratch_);
+  if (sst != kOk) return sst;
   NCHK(hipStreamCreateWithFlags(&n->stream_, hipStreamNonBlocking));
   n->own_stream_ = true;
   NCHK(hipEventCreate(&n->ev_[0]));
@@ -427,6 +461,37 @@ Status Net::forward_device(const bf16* x, uint32_t rows, float* y, hipStream_t s
   hipLaunchKernelGGL(unpack_output_kernel, dim3((unsigned)(((size_t)rows * OUT + 255) / 256)), dim3(256), 0, s, (const float*)y_, y,
                      (int)rows);
   stream_ = keep;
+  NCHK(hipGetLastError());
+  return kOk;
+}
+
+Status Net::forward_noalloc(const float* x, uint32_t rows, float* y, hipStream_t s) {
+  if (rows == 0 || rows % 64 != 0 || rows > kScratchRows) {
+    set_error("invalid input: the allocation-free forward takes a positive multiple of 64 rows, at most " +
+              std::to_string(kScratchRows));
+    return kInvalidInput;
+  }
+  if (!x || !y) {
+    set_error("invalid input: null rows");
+    return kInvalidInput;
+  }
+  if (!s) s = stream_;
+  bf16* in = (bf16*)scratch_;
+  bf16* hid[2] = {(bf16*)(scratch_ + kScrIn), (bf16*)(scratch_ + kScrIn + kScrHid)};
+  float* yf = (float*)(scratch_ + kScrIn + 2 * kScrHid);
+  const int B = (int)rows;
+  hipLaunchKernelGGL(pack_input_kernel, dim3((unsigned)(((size_t)rows * IN_P + 255) / 256)), dim3(256), 0, s, x, in, B);
+  const bf16* a = in;
+  for (int l = 0; l < kLayers; l++) {
+    if (l + 1 < kLayers) {
+      gemm<true, true, kEpiReluBf16>(s, a, kDin[l], wb_[l], kDin[l], hid[l & 1], kDout[l], B, kDout[l], kDin[l], b_[l]);
+      a = hid[l & 1];
+    } else {
+      gemm<true, true, kEpiBiasF32>(s, a, kDin[l], wb_[l], kDin[l], yf, kDout[l], B, kDout[l], kDin[l], b_[l]);
+    }
+  }
+  hipLaunchKernelGGL(unpack_output_kernel, dim3((unsigned)(((size_t)rows * OUT + 255) / 256)), dim3(256), 0, s, (const float*)yf, y,
+                     B);
   NCHK(hipGetLastError());
   return kOk;
 }

@@ -16,6 +16,7 @@ constexpr int IN = 2706, IN_P = 2752;   // 52 board + 2 pot/stack + 2 x 1326 ran
 constexpr int HID = 1024;
 constexpr int OUT = 2652, OUT_P = 2688;  // 2 x 1326 values; padded
 constexpr int kLayers = 5;
+constexpr uint32_t kScratchRows = 1024;  // rows of the allocation-free forward
 
 size_t num_params();
 double forward_flops_per_row();
@@ -35,6 +36,11 @@ class Net {
   Status get_grads(float* grads) const;
   // Host rows (rows x 2706) -> host rows (rows x 2652); rows a multiple of 64.
   Status forward(const float* x, uint32_t rows, float* y);
+  // Device rows (f32, rows x IN) -> device rows (f32, rows x OUT) on stream s
+  // (null: the network's), with the working rows in the per-GPU scratch
+  // array instead of the network's buffers: no allocation, no
+  // synchronization; rows a multiple of 64 up to kScratchRows.
+  Status forward_noalloc(const float* x, uint32_t rows, float* y, hipStream_t s);
   // Device rows already packed (bf16, rows x IN_P) -> device rows (f32, rows x OUT), on stream s.
   Status forward_device(const __bf16* x, uint32_t rows, float* y, hipStream_t s);
   // Masked Huber loss and its gradients for one batch (no update).
@@ -56,6 +62,7 @@ class Net {
   Status adam(float lr);
 
   uint32_t cap_ = 0;
+  unsigned char* scratch_ = nullptr;  // this device's scratch array (device address)
   hipStream_t stream_ = nullptr;
   bool own_stream_ = false;
   hipEvent_t ev_[2] = {nullptr, nullptr};

@@ -78,7 +78,7 @@ EXPORTS = [
     "pts_dl_new", "pts_solver_num_leaves", "pts_solver_copy_root_street", "pts_solver_pin_root_street", "pts_batch_root_rows",
     "pts_batch_num_combinations", "pts_solver_is_compressed", "pts_solver_get_storage_q", "pts_solver_set_storage_q",
     "pts_engine_create_q", "pts_engine_download_q", "pts_last_error_detail", "pts_evaluate",
-    "pts_solver_solve_cancellable", "pts_run_solve",
+    "pts_solver_solve_cancellable", "pts_run_solve", "pts_net_forward_noalloc",
 ]
 
 
@@ -154,6 +154,7 @@ def _declare(L):
     L.pts_net_get_params.argtypes = [P, f32p]
     L.pts_net_get_grads.argtypes = [P, f32p]
     L.pts_net_forward.argtypes = [P, f32p, ctypes.c_uint32, f32p]
+    L.pts_net_forward_noalloc.argtypes = [P, ctypes.c_void_p, ctypes.c_uint32, ctypes.c_void_p, ctypes.c_void_p]
     L.pts_net_gradients.argtypes = [P, f32p, f32p, f32p, ctypes.c_uint32, ctypes.POINTER(ctypes.c_double)]
     L.pts_net_train_step.argtypes = [P, f32p, f32p, f32p, ctypes.c_uint32, ctypes.c_float, ctypes.POINTER(ctypes.c_double)]
     L.pts_net_profile.argtypes = [P, ctypes.c_uint32, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]
