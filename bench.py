@@ -17,7 +17,7 @@ stream; `traffic` is the class's HBM bytes per launch as rocprofv3 counted
 them (profiles/traffic.json).
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
-                       [--workload flop|turn|river|batch|valuenet] [--precision fast|exact]
+                       [--workload flop|turn|river|batch|valuenet|river-auto] [--precision fast|exact]
                        [--storage f32|16bit]
 """
 import argparse
@@ -136,7 +136,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", default="flop", choices=sorted(WORKLOADS) + ["batch", "valuenet"])
+    ap.add_argument("--workload", default="flop", choices=sorted(WORKLOADS) + ["batch", "valuenet", "river-auto"])
     ap.add_argument("--subgames", type=int, default=256, help="valuenet: turn subgames solved for training")
     ap.add_argument("--classes", type=int, default=25, help="valuenet: hand classes per range")
     ap.add_argument("--seed", type=int, default=0)
@@ -180,6 +180,8 @@ def main():
 
     if args.workload == "batch":
         return bench_batch(args)
+    if args.workload == "river-auto":
+        return bench_river_auto(args)
     if args.workload == "valuenet":
         return bench_valuenet(args)
     desc, maker, max_iters = WORKLOADS[args.workload]
@@ -401,6 +403,42 @@ def bench_valuenet(args):
                    "target_pct": args.target, "parallelism": "1 GPU"},
         "roofline": mfma, "cpu_baseline": None,
     }
+    print(json.dumps(line), flush=True)
+
+
+def bench_river_auto(args):
+    """The measurement behind Backend::Auto (crates/solver/src/lib.rs:
+    496-503 keeps river roots on the CPU, from RTX 4060 timings): river
+    workload spots (seeds 0-4) solved to the target with checks every 10
+    iterations, wall time from building the tree to the end of the solve,
+    on the GPU (after one warm-up solve, so the HIP runtime's start-up is
+    not counted) and by the oracle on the host (one thread: a river tree has
+    no chance nodes to split over)."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle
+    from pts import spots
+    from pts.solver import Solver, SolveOptions
+    Solver(spots.river_spot(9)).solve(SolveOptions(50, args.target, 10))
+    rows = []
+    for seed in range(5):
+        cfg = spots.river_spot(seed)
+        t0 = time.perf_counter()
+        pr = Solver(cfg).solve(SolveOptions(1000, args.target, 10))
+        t1 = time.perf_counter()
+        pc, _ = pyoracle.OracleSolver(cfg).solve(1000, args.target, 10)
+        t2 = time.perf_counter()
+        rows.append({"seed": seed, "gpu_ms": round((t1 - t0) * 1e3, 3), "gpu_iterations": pr.iteration,
+                     "cpu_ms": round((t2 - t1) * 1e3, 3), "cpu_iterations": pc.iteration})
+        log(f"[bench] river seed {seed}: {rows[-1]}")
+    g = sorted(r["gpu_ms"] for r in rows)[2]
+    c = sorted(r["cpu_ms"] for r in rows)[2]
+    line = {"metric": "river root, build + solve to 0.5% pot, GPU vs host CPU (median of 5 spots)", "value": g,
+            "unit": "ms", "n_gpus": 1, "steps": 5, "warmup": 1, "ms_per_step": g, "higher_is_better": False,
+            "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+            "config": {"workload": WORKLOADS["river"][0], "spots": rows, "cpu_median_ms": c,
+                       "gpu_faster": g < c, "parallelism": "1 GPU"},
+            "roofline": None, "cpu_baseline": {"value": c, "unit": "ms", "cores": 1, "kind": "port",
+                                               "sample": "the same 5 spots, build + solve, oracle on one host thread"}}
     print(json.dumps(line), flush=True)
 
 

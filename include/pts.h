@@ -74,7 +74,9 @@ pts_status pts_evaluate(const uint8_t* cards, uint32_t ncards, uint64_t count, u
 /* Number of usable GPUs (crates/solver/src/gpu/mod.rs:242-282 opens device 0). */
 pts_status pts_device_count(int* out);
 /* Whether `backend` puts a solve rooted on `street` (1 flop, 2 turn, 3 river)
- * on the GPU (crates/solver/src/lib.rs:497-503). */
+ * on the GPU (crates/solver/src/lib.rs:497-503). Auto differs from the
+ * reference's: on MI355X river roots solve faster on the GPU too (build
+ * included), so Auto means the GPU for every street. */
 int pts_backend_wants_gpu(int backend, int street);
 
 /* ------------------------------------------------------------------ engine */

@@ -82,3 +82,15 @@ def gradients(params, x, target, mask, shp=None, quantize=True):
         if l > 0:
             dz = q((dz @ q(w)) * (acts[l] > 0))
     return loss, pack(grads)
+
+
+def adam(params, grads, m, v, t, lr, b1=0.9, b2=0.999, eps=1e-8):
+    """One Adam step (Kingma & Ba 2015, Algorithm 1): moments, bias-corrected
+    by 1 - b^t, and the update lr * m_hat / (sqrt(v_hat) + eps). Returns the
+    new (params, m, v) in float64."""
+    g = np.asarray(grads, np.float64)
+    m = b1 * m + (1.0 - b1) * g
+    v = b2 * v + (1.0 - b2) * g * g
+    mh = m / (1.0 - b1 ** t)
+    vh = v / (1.0 - b2 ** t)
+    return np.asarray(params, np.float64) - lr * mh / (np.sqrt(vh) + eps), m, v

@@ -223,9 +223,13 @@ pts_status pts_device_count(int* out) {
   return PTS_OK;
 }
 
+// lib.rs:496-503 keeps river roots on the CPU under Auto (an RTX 4060's GPU
+// solve of a river tree lost to the CPU). On MI355X a river root solves to
+// 0.5% of the pot faster on the GPU than on the host CPU, tree build included
+// (bench.py --workload river-auto), so Auto puts every street on the GPU.
 int pts_backend_wants_gpu(int backend, int street) {
   if (backend == PTS_BACKEND_GPU) return 1;
-  if (backend == PTS_BACKEND_AUTO) return street != kRiver;
+  if (backend == PTS_BACKEND_AUTO) return street >= kFlop && street <= kRiver;
   return 0;
 }
 

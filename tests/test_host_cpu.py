@@ -237,11 +237,13 @@ def test_error_details():
 
 
 def test_backend_switch():
-    # lib.rs:497-503
+    # lib.rs:497-503, with Auto set from the MI355X measurement (bench.py
+    # --workload river-auto): every street on the GPU
     L = lib()
-    assert L.pts_backend_wants_gpu(0, 2) == 0
+    assert L.pts_backend_wants_gpu(0, 2) == 0 and L.pts_backend_wants_gpu(0, 3) == 0
     assert L.pts_backend_wants_gpu(1, 3) == 1
-    assert L.pts_backend_wants_gpu(2, 3) == 0 and L.pts_backend_wants_gpu(2, 2) == 1
+    assert all(L.pts_backend_wants_gpu(2, s) == 1 for s in (1, 2, 3))
+    assert L.pts_backend_wants_gpu(2, 0) == 0 and L.pts_backend_wants_gpu(2, 4) == 0
 
 
 def test_spot_generators_are_deterministic():

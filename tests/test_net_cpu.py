@@ -42,3 +42,14 @@ def test_reference_shapes_match_the_network():
     shp = R.shapes()
     assert shp[0] == (1024, 2706) and shp[-1] == (2652, 1024) and len(shp) == 5
     assert sum(o * i + o for o, i in shp) == 8639068
+
+
+def test_reference_adam_closed_forms():
+    # t = 1 from zero moments: m_hat = g, v_hat = g^2, so the step is
+    # lr * g / (|g| + eps); at t = 2 with the same gradient the bias
+    # corrections make the step the same again
+    g = np.array([1e-9, 1e-8, 0.5, -2.0, 0.0])
+    w, m, v = R.adam(np.zeros(5), g, 0.0, 0.0, 1, 1e-3)
+    assert np.allclose(w, -1e-3 * g / (np.abs(g) + 1e-8), rtol=1e-12, atol=0)
+    w2, _, _ = R.adam(w, g, m, v, 2, 1e-3)
+    assert np.allclose(w2 - w, w, rtol=1e-12, atol=0)

@@ -70,3 +70,26 @@ def test_rows_must_be_a_multiple_of_64():
     net = ValueNet(64, seed=1)
     with pytest.raises(Exception):
         net.forward(np.zeros((10, R.IN), np.float32))
+
+
+def test_adam_steps_match_the_reference():
+    # two training steps; each Adam update against the textbook step
+    # (oracle/net_ref.py) on the gradients the GPU computed, from the same
+    # moments: the bias corrections 1 - b^t at t = 1 and 2, the moment
+    # recursions and eps's place in the denominator
+    net = ValueNet(64, seed=11)
+    x, t, m = batch(64, 12)
+    p = net.params().astype(np.float64)
+    mom, vel = np.zeros_like(p), np.zeros_like(p)
+    lr = 1e-3
+    for step in (1, 2):
+        net.train_step(x, t, m, lr)
+        g = net.grads().astype(np.float64)
+        want, mom, vel = R.adam(p, g, mom, vel, step, lr)
+        got = net.params().astype(np.float64)
+        du, dw = got - p, want - p
+        live = np.abs(dw) > 1e-3 * lr
+        assert np.count_nonzero(live) > 0.5 * len(p)
+        assert np.max(np.abs(du - dw)) <= 2e-3 * lr, (step, np.max(np.abs(du - dw)))
+        assert np.max(np.abs(du[live] - dw[live]) / np.abs(dw[live])) < 1e-2, step
+        p = got
