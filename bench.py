@@ -141,7 +141,9 @@ def main():
     ap.add_argument("--classes", type=int, default=25, help="valuenet: hand classes per range")
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--target", type=float, default=0.5)
-    ap.add_argument("--check-every", type=int, default=10)
+    ap.add_argument("--check-every", type=int, default=0,
+                    help="most iterations between exploitability checks (0: the reference trainer's rule, "
+                         "crates/trainer/src/review.rs:1065-1071: 20 for trees above 200,000 nodes, else 10)")
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--max-iterations", type=int, default=0,
                     help="cap the solve at this many iterations (0: the workload's own count; a capped run that "
@@ -212,6 +214,8 @@ def main():
     del s
 
     # the metric: a fresh solve to the target
+    if not args.check_every:
+        args.check_every = 20 if st.player_nodes + st.chance_nodes + st.terminal_nodes > 200_000 else 10
     s = Solver(cfg, precision=prec)
     hist = []
     t0 = time.perf_counter()
@@ -268,6 +272,7 @@ def bench_batch(args):
     from pts import spots
     from pts.solver import BatchSolver
     prec = 1 if args.precision == "exact" else 0
+    args.check_every = args.check_every or 10
     cfgs = spots.batch_spots()
     t0 = time.perf_counter()
     b = BatchSolver(cfgs, precision=prec)
@@ -353,6 +358,7 @@ def bench_valuenet(args):
                 "cpu_baseline": None}
         print(json.dumps(line), flush=True)
         return
+    args.check_every = args.check_every or 20  # review.rs:1065-1071: flop trees are big
     cfg = V.eval_flop(args.seed, args.classes)
     pots = V.leaf_pots(cfg)
     X, Y, M, info = V.generate(args.subgames, args.classes, pots, batch=64, log=log, base=cfg.ranges)
