@@ -137,8 +137,11 @@ def main():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default="flop", choices=sorted(WORKLOADS) + ["batch", "valuenet", "river-auto"])
-    ap.add_argument("--subgames", type=int, default=256, help="valuenet: turn subgames solved for training")
-    ap.add_argument("--classes", type=int, default=25, help="valuenet: hand classes per range")
+    ap.add_argument("--subgames", type=int, default=1024, help="valuenet: turn subgames solved for training")
+    ap.add_argument("--flops", type=int, default=2, help="valuenet: flops of the flop workload judged (seeds 0..)")
+    ap.add_argument("--train-steps", type=int, default=600, help="valuenet: Adam steps of 1024 rows")
+    ap.add_argument("--resolve-iterations", type=int, default=300,
+                    help="valuenet: most iterations of the later streets' re-solve before the exact fallback")
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--target", type=float, default=0.5)
     ap.add_argument("--check-every", type=int, default=0,
@@ -330,17 +333,27 @@ def bench_batch(args):
 
 
 def bench_valuenet(args):
-    """The value-network path against the exact solve on one flop: turn
-    subgames solved by the batched GPU DCFR to 0.2% of the pot, the network
-    trained on them by its own kernels, the depth-limited flop solve (300
-    iterations), then the exact full-tree DCFR warm-started from its flop
-    regrets until the FP64 best response says <= target (the judge and
-    fallback). Training data are reused across spots and not counted. The
-    comparison is the exact solve from scratch."""
+    """The value-network path against the exact solve on flops of the flop
+    workload (its tree and 169-class ranges; seeds 0..flops-1).
+
+    Training (not counted in the per-flop times, as it serves every flop):
+    `subgames` turn subgames, each a random turn card under one of the
+    evaluated flops' leaf pots and that flop's ranges scaled per class,
+    solved by the batched GPU DCFR to 0.2% of their pot; the network trained
+    on them by its own kernels (Adam, batches of 1024 rows, suits relabelled
+    at random).
+
+    Per flop, the value-net path: the depth-limited solve (300 iterations,
+    leaves valued by the network), then the judge: the full tree with the
+    flop pinned to the depth-limited average strategy and the later streets
+    re-solved by exact DCFR until the exact FP64 best response of that whole
+    profile is at or below the target. The exact path: the same tree solved
+    from scratch to the same target."""
     import numpy as np
+    from pts import spots
     from pts import valuenet as V
     from pts.net import ValueNet
-    from pts.solver import depth_limited
+    from pts.solver import Solver, SolveOptions, depth_limited
     if args.net_only:
         net = ValueNet(1024, seed=1)
         net.profile(1024)
@@ -358,17 +371,23 @@ def bench_valuenet(args):
                 "cpu_baseline": None}
         print(json.dumps(line), flush=True)
         return
-    args.check_every = args.check_every or 20  # review.rs:1065-1071: flop trees are big
-    cfg = V.eval_flop(args.seed, args.classes)
-    pots = V.leaf_pots(cfg)
-    X, Y, M, info = V.generate(args.subgames, args.classes, pots, batch=64, log=log, base=cfg.ranges)
-    log(f"[bench] {args.subgames} turn subgames solved in {info['seconds']:.1f}s "
-        f"(mean {np.mean(info['iterations']):.0f} iterations to 0.2% pot)")
+    cfgs = [spots.flop_spot(k) for k in range(args.flops)]
+    # training rows: subgames spread over the evaluated flops
+    X, Y, M, its, gen_s = [], [], [], [], 0.0
+    per = -(-args.subgames // len(cfgs))
+    for k, cfg in enumerate(cfgs):
+        x, y, m, info = V.generate(per, 0, V.leaf_pots(cfg), batch=64, base=cfg.ranges, seed0=k * 1_000_000,
+                                   log=log)
+        X.append(x), Y.append(y), M.append(m)
+        its += info["iterations"]
+        gen_s += info["seconds"]
+    X, Y, M = np.concatenate(X), np.concatenate(Y), np.concatenate(M)
+    log(f"[bench] {len(X)} turn subgames solved in {gen_s:.1f}s (mean {np.mean(its):.0f} iterations to 0.2% pot)")
     n = X.shape[0]
     ntr = n * 7 // 8
     net = ValueNet(1024, seed=1)
     t0 = time.perf_counter()
-    losses = V.train(net, X[:ntr], Y[:ntr], M[:ntr], 800, batch=256, lr=1e-3)
+    V.train(net, X[:ntr], Y[:ntr], M[:ntr], args.train_steps, batch=1024, lr=1e-3)
     train_s = time.perf_counter() - t0
     held = V.heldout_loss(net, X[ntr:], Y[ntr:], M[ntr:])
     zero = float(np.sum(M[ntr:] * np.where(np.abs(Y[ntr:]) <= 1, 0.5 * Y[ntr:] ** 2, np.abs(Y[ntr:]) - 0.5)) /
@@ -379,72 +398,46 @@ def bench_valuenet(args):
             "frac": round(flops / (fwd_ms * 1e-3) / 1e12 / 2500.0, 5), "traffic": None,
             "kernel": "gemm_kernel (forward, 1024 rows)", "forward_ms": round(fwd_ms, 4),
             "train_step_tflops": round(3 * flops / (train_ms * 1e-3) / 1e12, 2)}
-    log(f"[bench] network: train {train_s:.1f}s, held-out loss {held:.5f} (zero output {zero:.5f}), forward "
-        f"{fwd_ms:.3f} ms / 1024 rows = {mfma['achieved']} TFLOP/s")
-    t0 = time.perf_counter()
-    dl = depth_limited(cfg, net)
-    dl.iterate(300)
-    dl_s = time.perf_counter() - t0
-    pr_w, warm_s, _ = V.judge(cfg, dl, args.target, 600, args.check_every)
-    warm_it, warm_pct = pr_w.iteration, pr_w.exploitability_pct
-    log(f"[bench] value-net path: depth-limited {dl_s:.2f}s + exact continuation {warm_it} iterations, "
-        f"{warm_pct:.4f}% pot, {warm_s:.1f}s")
-    pr_c, cold_s, _ = V.judge(cfg, None, args.target, 600, args.check_every)
-    log(f"[bench] exact from scratch: {pr_c.iteration} iterations, {pr_c.exploitability_pct:.4f}% pot, {cold_s:.1f}s")
-    total = dl_s + warm_s
+    log(f"[bench] network: {args.train_steps} steps of 1024 rows in {train_s:.1f}s, held-out loss {held:.5f} "
+        f"(zero output {zero:.5f}), forward {fwd_ms:.3f} ms / 1024 rows = {mfma['achieved']} TFLOP/s")
+    per_flop = []
+    for k, cfg in enumerate(cfgs):
+        t0 = time.perf_counter()
+        dl = depth_limited(cfg, net)
+        dl.iterate(300)
+        dl_s = time.perf_counter() - t0
+        hist, rs_s, full, reached = V.resolve(cfg, dl, args.target, args.resolve_iterations, args.check_every)
+        del full
+        t0 = time.perf_counter()
+        ex = Solver(cfg)
+        pr = ex.solve(SolveOptions(1000, args.target, args.check_every))
+        ex_s = time.perf_counter() - t0
+        del ex
+        # a profile that misses the target falls back to the exact solve
+        vn_s = dl_s + rs_s + (0.0 if reached else ex_s)
+        r = {"flop": k, "value_net_s": round(vn_s, 2), "depth_limited_s": round(dl_s, 2), "resolve_s": round(rs_s, 2),
+             "resolve_iterations": hist[-1][0], "value_net_exploitability_pct": round(hist[-1][1], 4),
+             "reached": reached, "fell_back": not reached, "exact_s": round(ex_s, 2), "exact_iterations": pr.iteration,
+             "exact_exploitability_pct": round(pr.exploitability_pct, 4), "speedup": round(ex_s / vn_s, 3)}
+        per_flop.append(r)
+        log(f"[bench] flop {k}: {r}")
+    vn = sum(r["value_net_s"] for r in per_flop)
+    ex = sum(r["exact_s"] for r in per_flop)
     line = {
-        "metric": METRIC, "value": round(total, 3), "unit": "s", "n_gpus": 1, "steps": warm_it, "warmup": 0,
-        "ms_per_step": None, "higher_is_better": False, "scaling": "weak", "vs_baseline": None, "dtype": "bf16+f32",
-        "data": "synthetic",
-        "config": {"workload": f"value-net depth-limited flop solve judged by the exact FP64 best response: flop seed "
-                               f"{args.seed}, flop-workload tree (5.06M decision nodes) with {args.classes}-class ranges; "
-                               f"network 2706-4x1024-2652 trained on {args.subgames} GPU-solved turn subgames",
-                   "value_net_path_s": round(total, 3), "depth_limited_s": round(dl_s, 3),
-                   "exact_continuation_s": round(warm_s, 3), "exact_continuation_iterations": warm_it,
-                   "value_net_path_exploitability_pct": round(warm_pct, 5),
-                   "exact_from_scratch_s": round(cold_s, 3), "exact_from_scratch_iterations": pr_c.iteration,
-                   "exact_from_scratch_exploitability_pct": round(pr_c.exploitability_pct, 5),
-                   "speedup_vs_exact": round(cold_s / total, 3), "training_data_s": round(info["seconds"], 2),
-                   "training_s": round(train_s, 2), "heldout_loss": round(held, 6), "zero_output_loss": round(zero, 6),
-                   "target_pct": args.target, "parallelism": "1 GPU"},
+        "metric": METRIC, "value": round(vn / len(per_flop), 3), "unit": "s", "n_gpus": 1, "steps": len(per_flop),
+        "warmup": 0, "ms_per_step": None, "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
+        "dtype": "bf16+f32", "data": "synthetic",
+        "config": {"workload": f"value-net depth-limited flop solve judged by the exact FP64 best response of the whole "
+                               f"profile (flop pinned to the depth-limited average, later streets re-solved by exact "
+                               f"DCFR): {len(per_flop)} flops of the flop workload (seeds 0-{len(per_flop) - 1}); "
+                               f"network 2706-4x1024-2652 trained on {n} GPU-solved turn subgames, batches of 1024",
+                   "per_flop": per_flop, "value_net_mean_s": round(vn / len(per_flop), 3),
+                   "exact_mean_s": round(ex / len(per_flop), 3), "speedup_vs_exact": round(ex / vn, 3),
+                   "target_pct": args.target, "check_every": args.check_every, "training_subgames": n,
+                   "training_data_s": round(gen_s, 2), "training_s": round(train_s, 2),
+                   "heldout_loss": round(held, 6), "zero_output_loss": round(zero, 6), "parallelism": "1 GPU"},
         "roofline": mfma, "cpu_baseline": None,
     }
-    print(json.dumps(line), flush=True)
-
-
-def bench_river_auto(args):
-    """The measurement behind Backend::Auto (crates/solver/src/lib.rs:
-    496-503 keeps river roots on the CPU, from RTX 4060 timings): river
-    workload spots (seeds 0-4) solved to the target with checks every 10
-    iterations, wall time from building the tree to the end of the solve,
-    on the GPU (after one warm-up solve, so the HIP runtime's start-up is
-    not counted) and by the oracle on the host (one thread: a river tree has
-    no chance nodes to split over)."""
-    sys.path.insert(0, os.path.join(ROOT, "oracle"))
-    import pyoracle
-    from pts import spots
-    from pts.solver import Solver, SolveOptions
-    Solver(spots.river_spot(9)).solve(SolveOptions(50, args.target, 10))
-    rows = []
-    for seed in range(5):
-        cfg = spots.river_spot(seed)
-        t0 = time.perf_counter()
-        pr = Solver(cfg).solve(SolveOptions(1000, args.target, 10))
-        t1 = time.perf_counter()
-        pc, _ = pyoracle.OracleSolver(cfg).solve(1000, args.target, 10)
-        t2 = time.perf_counter()
-        rows.append({"seed": seed, "gpu_ms": round((t1 - t0) * 1e3, 3), "gpu_iterations": pr.iteration,
-                     "cpu_ms": round((t2 - t1) * 1e3, 3), "cpu_iterations": pc.iteration})
-        log(f"[bench] river seed {seed}: {rows[-1]}")
-    g = sorted(r["gpu_ms"] for r in rows)[2]
-    c = sorted(r["cpu_ms"] for r in rows)[2]
-    line = {"metric": "river root, build + solve to 0.5% pot, GPU vs host CPU (median of 5 spots)", "value": g,
-            "unit": "ms", "n_gpus": 1, "steps": 5, "warmup": 1, "ms_per_step": g, "higher_is_better": False,
-            "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
-            "config": {"workload": WORKLOADS["river"][0], "spots": rows, "cpu_median_ms": c,
-                       "gpu_faster": g < c, "parallelism": "1 GPU"},
-            "roofline": None, "cpu_baseline": {"value": c, "unit": "ms", "cores": 1, "kind": "port",
-                                               "sample": "the same 5 spots, build + solve, oracle on one host thread"}}
     print(json.dumps(line), flush=True)
 
 

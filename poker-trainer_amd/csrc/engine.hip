@@ -1315,12 +1315,21 @@ __device__ __forceinline__ void up_reg_walk(const DPlan& P, int p, uint32_t seg0
 #define SL_GET(i) SV::get(s0, s1, s2, (i))
 #define SL_SET(i, x) SV::set(s0, s1, s2, (i), (x))
   const float* store = MODE == kCfr ? sto.r : sto.s;
+  // per member (in its own lane): its value row with the chunk group's shift
+  // applied, the top bit set unless the walk preloads it (a live terminal or
+  // chance node in the span), so the unrolled loop below reads one word per
+  // member
+#pragma unroll
+  for (int q = 0; q < H; q++)
+    vslot[q] = B.v(vslot[q]) |
+               ((lane + 64 * q < mcnt && (f[q] & 7) >= kChance && (f[q] & (kFLive | kFSpan)) == (kFLive | kFSpan))
+                    ? 0u
+                    : 0x80000000u);
 #pragma unroll
   for (uint32_t m = 0; m < (uint32_t)CAP; m++) {  // unrolled: constant indices, all loads in flight
     if (m < mcnt) {
-      const uint32_t fm = LN32(f, m);
-      if ((fm & 7) >= kChance && (fm & (kFLive | kFSpan)) == (kFLive | kFSpan))
-        LV_SET(m, vals[(size_t)B.v(LN32(vslot, m)) * rowlen + hh]);
+      const uint32_t vr = LN32(vslot, m);
+      if (!(vr >> 31)) LV_SET(m, vals[(size_t)vr * rowlen + hh]);
       if (ROWS) {
         const uint64_t sm = LN64(src, m);
         if (sm != kNoRow) {
@@ -1354,10 +1363,15 @@ __device__ __forceinline__ void up_reg_walk(const DPlan& P, int p, uint32_t seg0
         // passes over the actions re-read the registers (few values live at
         // once); the operations and their order are up_kernel's
         if constexpr (MODE == kCfr) {
-          float d = pos0(SL_GET(cl));
-          for (uint32_t a = 1; a < na; a++) d += pos0(SL_GET(cl + a));
-          float outv = 0.0f;
-          for (uint32_t a = 0; a < na; a++) outv += pos0(SL_GET(cl + a)) * LV_GET(cl + a);
+          // one pass over the children for both sums (each accumulated in
+          // action order, as up_kernel's separate loops do): the positive
+          // regrets and the regret-weighted values
+          float d = 0.0f, outv = 0.0f;
+          for (uint32_t a = 0; a < na; a++) {
+            const float r = pos0(SL_GET(cl + a));
+            d += r;
+            outv += r * LV_GET(cl + a);
+          }
           const float inv = recip(d);
           outv *= inv;
           if (inv == 0.0f) {
@@ -1366,14 +1380,15 @@ __device__ __forceinline__ void up_reg_walk(const DPlan& P, int p, uint32_t seg0
             outv = sm * (1.0f / (float)na);
           }
           // cfr.rs:374-379; the regret row of action a is the storage row of
-          // child cl + a (a macro, not a lambda: the member vectors stay
-          // plain values)
+          // child cl + a, at the node's offset + a * np (a macro, not a
+          // lambda: the member vectors stay plain values)
 #define UPDATED(a) (SL_GET(cl + (a)) * (SL_GET(cl + (a)) > 0.0f ? pos : neg) + LV_GET(cl + (a)) - outv)
+          const uint64_t row0 = LN64(off, m);  // wave-uniform
           if (fm & kFFrozen) {
             // pinned regrets: no update (and, with 16-bit storage, no new scale)
           } else if (QP == 0) {
             if (act)
-              for (uint32_t a = 0; a < na; a++) sto.r[LN64(src, cl + a) + h] = UPDATED(a);
+              for (uint32_t a = 0; a < na; a++) sto.r[row0 + (uint64_t)a * np + h] = UPDATED(a);
           } else if (QP == 1) {
             float lm = 0.0f;
             for (uint32_t a = 0; a < na; a++) {
@@ -1387,7 +1402,7 @@ __device__ __forceinline__ void up_reg_walk(const DPlan& P, int p, uint32_t seg0
             const float inv = mx > 0.0f ? kRegMax / mx : 0.0f;
             if (act)
               for (uint32_t a = 0; a < na; a++)
-                sto.rq[LN64(src, cl + a) + h] = mx > 0.0f ? enc_regret(UPDATED(a), inv) : (int16_t)0;
+                sto.rq[row0 + (uint64_t)a * np + h] = mx > 0.0f ? enc_regret(UPDATED(a), inv) : (int16_t)0;
           }
 #undef UPDATED
           v = outv;
@@ -1415,7 +1430,7 @@ __device__ __forceinline__ void up_reg_walk(const DPlan& P, int p, uint32_t seg0
   }
   const V top = LV_GET(0);
   if (act && QP != 2) {
-    vals[(size_t)B.v(LN32(vslot, 0u)) * rowlen + h] = top;
+    vals[(size_t)(LN32(vslot, 0u) & 0x7FFFFFFFu) * rowlen + h] = top;
     if (span.init && P.seg_pslot[(size_t)p * P.num_segs + seg] == NONE)  // a spot's root: its result row
       ((V*)span.res)[((size_t)P.mrec[moff].spot * 2 + p) * rowlen + h] = top;
   }
@@ -1429,7 +1444,7 @@ __device__ __forceinline__ void up_reg_walk(const DPlan& P, int p, uint32_t seg0
 }
 
 template <typename V, int MODE, int CAP, int QP>
-__global__ void __launch_bounds__(UP_T, CAP >= 64 ? 2 : (sizeof(V) == 8 && CAP == 48 ? 3 : 4)) up_reg_kernel(DPlan P, int p, uint32_t seg0, Span span, Bias B, V* __restrict__ vals,
+__global__ void __launch_bounds__(UP_T, CAP >= 64 ? 2 : CAP == 32 && QP != 2 ? 6 : (sizeof(V) == 8 && CAP == 48 ? 3 : 4)) up_reg_kernel(DPlan P, int p, uint32_t seg0, Span span, Bias B, V* __restrict__ vals,
                                                       uint32_t rowlen, const uint32_t* __restrict__ alive, Store sto,
                                                       const IterParams* __restrict__ prm, V* __restrict__ qout,
                                                       const uint32_t* __restrict__ list) {

@@ -47,3 +47,32 @@ def test_trained_network_beats_no_network_and_drives_the_solve():
     dl.iterate(100)
     pr, sec, full = V.judge(cfg, dl, 5.0, 200, 10)
     assert pr.exploitability_pct <= 5.0
+
+
+def test_value_net_workload_shapes_on_the_full_flop():
+    # The value-net workload's shapes: a flop of the flop workload (its tree
+    # and 169-class ranges, 1176 hands per player), the 2706-4x1024-2652
+    # network trained with batches of 1024 rows on GPU-solved turn subgames
+    # that start from that flop's ranges, the depth-limited solve, and the
+    # judge: the full 16.19 G-cell tree with the flop pinned to the
+    # depth-limited average strategy and the later streets solved by exact
+    # DCFR, whose exploitability the exact FP64 best response measures.
+    from pts import spots
+    cfg = spots.flop_spot(1)
+    X, Y, M, info = V.generate(128, 0, V.leaf_pots(cfg), batch=64, base=cfg.ranges, target_pct=1.0)
+    assert X.shape == (128, 2706) and Y.shape == (128, 2652) and np.all(np.isfinite(Y))
+    net = ValueNet(1024, seed=3)
+    losses = V.train(net, X, Y, M, 30, batch=1024, lr=1e-3)
+    assert np.all(np.isfinite(losses)) and losses[-1] < losses[0]
+    dl = depth_limited(cfg, net)
+    dl.iterate(50)
+    assert np.isfinite(dl.exploitability()) and dl.exploitability() >= 0.0
+    hist, sec, full, reached = V.resolve(cfg, dl, target_pct=0.0, max_iterations=12, check_every=6)
+    assert full.stats().storage_cells == 16188966528
+    assert [t for t, _ in hist] == [6, 12]
+    assert 0.0 <= hist[1][1] < hist[0][1] and np.isfinite(hist[0][1])
+    # the pinned flop root plays the depth-limited solve's average strategy
+    qd, qf = dl.query([]), full.query([])
+    assert qd.actions == qf.actions
+    live = qd.reach > 0
+    assert np.allclose(qf.strategy[:, live], qd.strategy[:, live], rtol=0, atol=1e-5)
