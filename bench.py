@@ -52,6 +52,21 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+def solve_logged(s, opts, hist=None):
+    """Solver.solve with a line per exploitability check on stderr (the
+    progress callback of solve_cancellable; it runs only at the checks)."""
+    seen = hist if hist is not None else []
+
+    def progress(p):
+        seen.append(type(p).from_buffer_copy(p))
+        log(f"[bench] check: iteration {p.iteration}, {p.exploitability_pct:.4f}% pot")
+        return True
+
+    pr, cancelled = s.solve_cancellable(opts, on_progress=progress)
+    assert not cancelled
+    return pr
+
+
 def measured_traffic(workload, precision, kernel):
     """HBM bytes per launch of a kernel class as rocprofv3 counted them
     (FETCH_SIZE and WRITE_SIZE, scaled by a plain copy's counts), recorded in
@@ -222,7 +237,7 @@ def main():
     s = Solver(cfg, precision=prec)
     hist = []
     t0 = time.perf_counter()
-    pr = s.solve(SolveOptions(max_iters, args.target, args.check_every), hist)
+    pr = solve_logged(s, SolveOptions(max_iters, args.target, args.check_every), hist)
     solve_s = time.perf_counter() - t0
     log(f"[bench] solve: {pr.iteration} iterations, {pr.exploitability_pct:.4f}% pot, {solve_s:.3f}s, "
         f"{len(hist)} checks")
@@ -387,7 +402,7 @@ def bench_valuenet(args):
     ntr = n * 7 // 8
     net = ValueNet(1024, seed=1)
     t0 = time.perf_counter()
-    V.train(net, X[:ntr], Y[:ntr], M[:ntr], args.train_steps, batch=1024, lr=1e-3)
+    V.train(net, X[:ntr], Y[:ntr], M[:ntr], args.train_steps, batch=1024, lr=1e-3, log=log)
     train_s = time.perf_counter() - t0
     held = V.heldout_loss(net, X[ntr:], Y[ntr:], M[ntr:])
     zero = float(np.sum(M[ntr:] * np.where(np.abs(Y[ntr:]) <= 1, 0.5 * Y[ntr:] ** 2, np.abs(Y[ntr:]) - 0.5)) /
@@ -406,11 +421,12 @@ def bench_valuenet(args):
         dl = depth_limited(cfg, net)
         dl.iterate(300)
         dl_s = time.perf_counter() - t0
-        hist, rs_s, full, reached = V.resolve(cfg, dl, args.target, args.resolve_iterations, args.check_every)
+        hist, rs_s, full, reached = V.resolve(cfg, dl, args.target, args.resolve_iterations, args.check_every, log=log)
         del full
+        log(f"[bench] flop {k}: depth-limited {dl_s:.1f}s, re-solve {rs_s:.1f}s to {hist[-1][1]:.4f}% pot; exact solve next")
         t0 = time.perf_counter()
         ex = Solver(cfg)
-        pr = ex.solve(SolveOptions(1000, args.target, args.check_every))
+        pr = solve_logged(ex, SolveOptions(1000, args.target, args.check_every))
         ex_s = time.perf_counter() - t0
         del ex
         # a profile that misses the target falls back to the exact solve

@@ -234,7 +234,8 @@ def heldout_loss(net: ValueNet, X, Y, M):
     return num / max(den, 1e-30)
 
 
-def resolve(cfg: SolveConfig, dl: Solver, target_pct: float = 0.5, max_iterations: int = 300, check_every: int = 10):
+def resolve(cfg: SolveConfig, dl: Solver, target_pct: float = 0.5, max_iterations: int = 300, check_every: int = 10,
+            log=None):
     """The full tree with its flop decision nodes pinned (frozen) to the
     depth-limited solver's average strategy and the later streets solved by
     exact DCFR, until the exact FP64 best response of that profile is at or
@@ -251,6 +252,8 @@ def resolve(cfg: SolveConfig, dl: Solver, target_pct: float = 0.5, max_iteration
         if t >= nxt or t == max_iterations:
             e = full.exploitability()
             hist.append((t, 100.0 * e / cfg.root.pot))
+            if log:
+                log(f"[valuenet] resolve: iteration {t}, {hist[-1][1]:.4f}% pot")
             if hist[-1][1] <= target_pct:
                 break
             nxt = t + check_every
