@@ -426,19 +426,28 @@ def bench_valuenet(args):
         log(f"[bench] flop {k}: depth-limited {dl_s:.1f}s, re-solve {rs_s:.1f}s to {hist[-1][1]:.4f}% pot; exact solve next")
         t0 = time.perf_counter()
         ex = Solver(cfg)
-        pr = solve_logged(ex, SolveOptions(1000, args.target, args.check_every))
+        ex_hist = []
+        pr = solve_logged(ex, SolveOptions(1000, args.target, args.check_every), ex_hist)
         ex_s = time.perf_counter() - t0
+        cells = ex.stats().storage_cells
         del ex
         # a profile that misses the target falls back to the exact solve
         vn_s = dl_s + rs_s + (0.0 if reached else ex_s)
         r = {"flop": k, "value_net_s": round(vn_s, 2), "depth_limited_s": round(dl_s, 2), "resolve_s": round(rs_s, 2),
              "resolve_iterations": hist[-1][0], "value_net_exploitability_pct": round(hist[-1][1], 4),
              "reached": reached, "fell_back": not reached, "exact_s": round(ex_s, 2), "exact_iterations": pr.iteration,
-             "exact_exploitability_pct": round(pr.exploitability_pct, 4), "speedup": round(ex_s / vn_s, 3)}
+             "exact_exploitability_pct": round(pr.exploitability_pct, 4), "exact_checks": len(ex_hist),
+             "cells": cells, "speedup": round(ex_s / vn_s, 3)}
         per_flop.append(r)
         log(f"[bench] flop {k}: {r}")
     vn = sum(r["value_net_s"] for r in per_flop)
     ex = sum(r["exact_s"] for r in per_flop)
+    cpu = None
+    if not args.no_cpu:
+        # the exact solve's CPU baseline, on the first flop
+        r0 = per_flop[0]
+        cpu = cpu_baseline_flop(cfgs[0], r0["exact_iterations"], r0["exact_checks"], args.cpu_seconds, r0["cells"])
+        log(f"[bench] cpu baseline (exact solve of flop 0): {cpu}")
     line = {
         "metric": METRIC, "value": round(vn / len(per_flop), 3), "unit": "s", "n_gpus": 1, "steps": len(per_flop),
         "warmup": 0, "ms_per_step": None, "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
@@ -452,7 +461,7 @@ def bench_valuenet(args):
                    "target_pct": args.target, "check_every": args.check_every, "training_subgames": n,
                    "training_data_s": round(gen_s, 2), "training_s": round(train_s, 2),
                    "heldout_loss": round(held, 6), "zero_output_loss": round(zero, 6), "parallelism": "1 GPU"},
-        "roofline": mfma, "cpu_baseline": None,
+        "roofline": mfma, "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
 
