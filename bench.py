@@ -387,6 +387,10 @@ def bench_valuenet(args):
         print(json.dumps(line), flush=True)
         return
     cfgs = [spots.flop_spot(k) for k in range(args.flops)]
+    if not args.check_every:
+        # the reference trainer's rule, as for the flop workload
+        st = Solver.estimate(cfgs[0])
+        args.check_every = 20 if st.player_nodes + st.chance_nodes + st.terminal_nodes > 200_000 else 10
     # training rows: subgames spread over the evaluated flops
     X, Y, M, its, gen_s = [], [], [], [], 0.0
     per = -(-args.subgames // len(cfgs))

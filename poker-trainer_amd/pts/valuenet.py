@@ -234,12 +234,26 @@ def heldout_loss(net: ValueNet, X, Y, M):
     return num / max(den, 1e-30)
 
 
+def projects_beyond(hist, target_pct: float, budget: int) -> bool:
+    """True when the last two checks (iteration, % pot) fall too slowly to
+    reach `target_pct` within `budget` iterations at their rate, or do not
+    fall at all."""
+    if len(hist) < 2:
+        return False
+    (t1, e1), (t2, e2) = hist[-2], hist[-1]
+    rate = (e1 - e2) / (t2 - t1)
+    return rate <= 0.0 or t2 + (e2 - target_pct) / rate > budget
+
+
 def resolve(cfg: SolveConfig, dl: Solver, target_pct: float = 0.5, max_iterations: int = 300, check_every: int = 10,
             log=None):
     """The full tree with its flop decision nodes pinned (frozen) to the
     depth-limited solver's average strategy and the later streets solved by
     exact DCFR, until the exact FP64 best response of that profile is at or
-    below the target. Returns (progress, seconds, solver, reached)."""
+    below the target. The re-solve gives up early when the last two checks'
+    rate of fall projects the target beyond `max_iterations` (the pinned flop
+    strategy bounds how far the later streets can bring the profile down).
+    Returns (progress, seconds, solver, reached)."""
     full = Solver(cfg)
     t0 = time.perf_counter()
     pin_root_street(full, dl)
@@ -255,6 +269,10 @@ def resolve(cfg: SolveConfig, dl: Solver, target_pct: float = 0.5, max_iteration
             if log:
                 log(f"[valuenet] resolve: iteration {t}, {hist[-1][1]:.4f}% pot")
             if hist[-1][1] <= target_pct:
+                break
+            if projects_beyond(hist, target_pct, max_iterations):
+                if log:
+                    log(f"[valuenet] resolve: the target projects beyond {max_iterations} iterations; giving up")
                 break
             nxt = t + check_every
     reached = bool(hist) and hist[-1][1] <= target_pct

@@ -53,3 +53,16 @@ def test_reference_adam_closed_forms():
     assert np.allclose(w, -1e-3 * g / (np.abs(g) + 1e-8), rtol=1e-12, atol=0)
     w2, _, _ = R.adam(w, g, m, v, 2, 1e-3)
     assert np.allclose(w2 - w, w, rtol=1e-12, atol=0)
+
+
+def test_resolve_gives_up_when_the_target_projects_beyond_the_budget():
+    from pts.valuenet import projects_beyond
+    assert not projects_beyond([(20, 130.0)], 0.5, 300)  # one check: no rate yet
+    # 130 -> 124 over 20 iterations: 0.3 % pot per iteration, 412 more to 0.5%
+    assert projects_beyond([(20, 130.0), (40, 124.0)], 0.5, 300)
+    assert not projects_beyond([(20, 130.0), (40, 124.0)], 0.5, 500)
+    # 3.0 -> 1.0 over 20: 0.1 per iteration, 5 more to 0.5%
+    assert not projects_beyond([(20, 3.0), (40, 1.0)], 0.5, 50)
+    assert projects_beyond([(20, 3.0), (40, 1.0)], 0.5, 44)
+    assert projects_beyond([(20, 3.0), (40, 3.0)], 0.5, 10 ** 6)  # flat
+    assert projects_beyond([(20, 3.0), (40, 3.5)], 0.5, 10 ** 6)  # rising
