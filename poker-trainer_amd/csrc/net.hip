@@ -9,11 +9,13 @@
 // fractions of the pot.
 //
 // Every product is one hand-written GEMM kernel on gfx950 MFMA
-// (v_mfma_f32_16x16x32_bf16): 64x64 output tiles per 256-thread workgroup,
-// each wave a 32x32 quadrant (2x2 MFMA blocks), K in steps of 64 staged
-// through LDS (the next four steps' tiles in flight in registers). Widths are
-// padded (input 2752, output 2688) so every dimension is a multiple of 64;
-// padded weights stay zero.
+// (v_mfma_f32_16x16x32_bf16): 64x64 output tiles per 512-thread workgroup,
+// two groups of four waves on alternate K steps of 64, each wave a 32x32
+// quadrant (2x2 MFMA blocks), tiles staged through double-buffered LDS (the
+// next four steps' tiles in flight in registers), tiles dealt to the XCDs in
+// compact blocks. Widths are padded
+// (input 2752, output 2688) so every dimension is a multiple of 64; padded
+// weights stay zero.
 #include <hip/hip_runtime.h>
 
 #include <chrono>
@@ -44,18 +46,51 @@ enum Epi : int {
   kEpiMaskBf16 = 3,  // C (bf16) = (A B) * [mask > 0]
 };
 
+// The output tile of workgroup `bid` on a grid of tn_count x tm_count tiles.
+// Workgroups go to the 8 XCDs in turn (bid % 8), and each XCD has its own
+// L2: so the tiles are dealt so that the workgroups of one XCD cover a
+// compact block of rows and columns (groups of 4 tile rows, walked column
+// by column), and each XCD fetches a fraction of A and B instead of most of
+// both. Grids whose tile count is not a multiple of 8 keep row-major order.
+__device__ __forceinline__ void tile_of(int bid, int tn_count, int tm_count, int& tm, int& tn) {
+  const int nb = tn_count * tm_count;
+  int lin = bid;
+  if ((nb & 7) == 0) lin = (bid & 7) * (nb >> 3) + (bid >> 3);
+  constexpr int G = 4;
+  const int full = tm_count / G * G;  // tile rows in whole groups of G
+  if (lin < full * tn_count) {
+    const int g = lin / (G * tn_count), r = lin % (G * tn_count);
+    tm = g * G + r % G;
+    tn = r / G;
+  } else {  // the last rows (fewer than G): row-major
+    const int r = lin - full * tn_count;
+    tm = full + r / tn_count;
+    tn = r % tn_count;
+  }
+}
+
 // C[M x N] = A[M x K] B[K x N]. A is K-contiguous (A[m * lda + k]) when AK,
 // else M-contiguous (A[k * lda + m]); B is given K-contiguous as Bt[n * ldb +
 // k] when BK, else N-contiguous (B[k * ldb + n]). M, N and K are multiples of
-// 64 (the host checks); grid (N / 64, M / 64).
+// 64 (the host checks); a 1-D grid of (M / 64) (N / 64) workgroups.
+//
+// 512 threads: two groups of four waves share the output tile and split its
+// K steps (group g takes steps i with i % 2 == g), so each SIMD runs two
+// waves whose loads, LDS traffic and MFMAs interleave; group 1's partial
+// tile is added into group 0's through LDS before the epilogue.
+constexpr int GT = 512;
 template <bool AK, bool BK, int EPI>
-__global__ void __launch_bounds__(256) gemm_kernel(const bf16* __restrict__ A, int lda, const bf16* __restrict__ B, int ldb,
-                                                   void* __restrict__ C, int ldc, int K, const float* __restrict__ bias,
-                                                   const bf16* __restrict__ mask, int ldm) {
-  __shared__ __attribute__((aligned(16))) bf16 As[TM][LDS_K];
-  __shared__ __attribute__((aligned(16))) bf16 Bs[TN][LDS_K];
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int m0 = blockIdx.y * TM, n0 = blockIdx.x * TN;
+__global__ void __launch_bounds__(GT) gemm_kernel(const bf16* __restrict__ A, int lda, const bf16* __restrict__ B, int ldb,
+                                                  void* __restrict__ C, int ldc, int M, int N, int K,
+                                                  const float* __restrict__ bias, const bf16* __restrict__ mask, int ldm) {
+  // per group: double-buffered A and B tiles
+  __shared__ __attribute__((aligned(16))) bf16 As[2][2][TM][LDS_K];
+  __shared__ __attribute__((aligned(16))) bf16 Bs[2][2][TN][LDS_K];
+  const int lane = threadIdx.x & 63, wave = (threadIdx.x >> 6) & 3, grp = threadIdx.x >> 8;
+  const int tid = threadIdx.x & 255;  // thread within the group
+  int tm, tn;
+  tile_of(blockIdx.x, N / TN, M / TM, tm, tn);
+  const int m0 = tm * TM, n0 = tn * TN;
   const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
   f32x4 acc[2][2];
 #pragma unroll
@@ -75,73 +110,119 @@ __global__ void __launch_bounds__(256) gemm_kernel(const bf16* __restrict__ A, i
     if (BK) return *(const bf16x8*)(B + (size_t)(n0 + (c >> 3)) * ldb + k0 + (c & 7) * 8);
     return *(const bf16x8*)(B + (size_t)(k0 + (c >> 3)) * ldb + n0 + (c & 7) * 8);
   };
-  auto store_a = [&](bf16x8 v, int q) {
+  auto store_a = [&](int buf, bf16x8 v, int q) {
     const int c = tid + 256 * q;
     if (AK) {
-      *(bf16x8*)&As[c >> 3][(c & 7) * 8] = v;
+      *(bf16x8*)&As[grp][buf][c >> 3][(c & 7) * 8] = v;
     } else {
 #pragma unroll
-      for (int j = 0; j < 8; j++) As[(c & 7) * 8 + j][c >> 3] = v[j];
+      for (int j = 0; j < 8; j++) As[grp][buf][(c & 7) * 8 + j][c >> 3] = v[j];
     }
   };
-  auto store_b = [&](bf16x8 v, int q) {
+  auto store_b = [&](int buf, bf16x8 v, int q) {
     const int c = tid + 256 * q;
     if (BK) {
-      *(bf16x8*)&Bs[c >> 3][(c & 7) * 8] = v;
+      *(bf16x8*)&Bs[grp][buf][c >> 3][(c & 7) * 8] = v;
     } else {
 #pragma unroll
-      for (int j = 0; j < 8; j++) Bs[(c & 7) * 8 + j][c >> 3] = v[j];
+      for (int j = 0; j < 8; j++) Bs[grp][buf][(c & 7) * 8 + j][c >> 3] = v[j];
     }
   };
 
-  // KS K steps in flight: the tiles of step k + KS are loaded into the
-  // registers step k has just stored to LDS (the stage index is a constant
-  // in the unrolled inner loop, so the stages stay in registers)
+  // The group's own steps j = 0, 1, ... are K steps 2 j + grp. Their tiles
+  // are loaded into register stage j % KS, KS - 1 steps ahead of their store
+  // to LDS; LDS is double-buffered (step j in buffer j % 2), so step j + 1's
+  // tile is stored while step j is multiplied, and one barrier per round
+  // separates a buffer's reads from its next writes. Both groups run the
+  // same rounds (a group without a step in a round only meets the barrier).
+  // Stage and buffer indices are constants in the unrolled loop (KS is
+  // even), so the stages stay in registers.
   constexpr int KS = 4;
+  const int nk = K / TK;
+  const int rounds = (nk + 1) / 2;
+  const int mine = (nk - grp + 1) / 2;  // this group's steps
+  auto kof = [&](int j) { return (2 * j + grp) * TK; };
   bf16x8 pa[KS][2], pb[KS][2];
 #pragma unroll
   for (int q = 0; q < KS; q++)
-    if (q * TK < K) {
-      pa[q][0] = load_a(q * TK, 0);
-      pa[q][1] = load_a(q * TK, 1);
-      pb[q][0] = load_b(q * TK, 0);
-      pb[q][1] = load_b(q * TK, 1);
+    if (q < mine) {
+      pa[q][0] = load_a(kof(q), 0);
+      pa[q][1] = load_a(kof(q), 1);
+      pb[q][0] = load_b(kof(q), 0);
+      pb[q][1] = load_b(kof(q), 1);
     }
+  if (mine > 0) {
+    store_a(0, pa[0][0], 0);
+    store_a(0, pa[0][1], 1);
+    store_b(0, pb[0][0], 0);
+    store_b(0, pb[0][1], 1);
+    if (KS < mine) {
+      pa[0][0] = load_a(kof(KS), 0);
+      pa[0][1] = load_a(kof(KS), 1);
+      pb[0][0] = load_b(kof(KS), 0);
+      pb[0][1] = load_b(kof(KS), 1);
+    }
+  }
+  __syncthreads();
   const int r = lane & 15, kg = (lane >> 4) * 8;
-  for (int k0 = 0; k0 < K; k0 += KS * TK) {
+  for (int j0 = 0; j0 < rounds; j0 += KS) {
 #pragma unroll
     for (int q = 0; q < KS; q++) {
-      const int k = k0 + q * TK;
-      if (k < K) {
-        store_a(pa[q][0], 0);
-        store_a(pa[q][1], 1);
-        store_b(pb[q][0], 0);
-        store_b(pb[q][1], 1);
-        __syncthreads();
-        if (k + KS * TK < K) {
-          pa[q][0] = load_a(k + KS * TK, 0);
-          pa[q][1] = load_a(k + KS * TK, 1);
-          pb[q][0] = load_b(k + KS * TK, 0);
-          pb[q][1] = load_b(k + KS * TK, 1);
-        }
+      const int j = j0 + q;
+      if (j < rounds) {
+        if (j < mine) {
+          const int cur = q & 1, nxt = cur ^ 1, st = (q + 1) % KS;
+          if (j + 1 < mine) {
+            store_a(nxt, pa[st][0], 0);
+            store_a(nxt, pa[st][1], 1);
+            store_b(nxt, pb[st][0], 0);
+            store_b(nxt, pb[st][1], 1);
+            if (j + 1 + KS < mine) {
+              const int k = kof(j + 1 + KS);
+              pa[st][0] = load_a(k, 0);
+              pa[st][1] = load_a(k, 1);
+              pb[st][0] = load_b(k, 0);
+              pb[st][1] = load_b(k, 1);
+            }
+          }
 #pragma unroll
-        for (int s = 0; s < TK; s += 32) {
-          bf16x8 fa[2], fb[2];
+          for (int s = 0; s < TK; s += 32) {
+            bf16x8 fa[2], fb[2];
 #pragma unroll
-          for (int i = 0; i < 2; i++) fa[i] = *(const bf16x8*)&As[wm + i * 16 + r][s + kg];
+            for (int ii = 0; ii < 2; ii++) fa[ii] = *(const bf16x8*)&As[grp][cur][wm + ii * 16 + r][s + kg];
 #pragma unroll
-          for (int j = 0; j < 2; j++) fb[j] = *(const bf16x8*)&Bs[wn + j * 16 + r][s + kg];
+            for (int jj = 0; jj < 2; jj++) fb[jj] = *(const bf16x8*)&Bs[grp][cur][wn + jj * 16 + r][s + kg];
 #pragma unroll
-          for (int i = 0; i < 2; i++)
+            for (int ii = 0; ii < 2; ii++)
 #pragma unroll
-            for (int j = 0; j < 2; j++)
-              acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa[i], fb[j], acc[i][j], 0, 0, 0);
+              for (int jj = 0; jj < 2; jj++)
+                acc[ii][jj] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa[ii], fb[jj], acc[ii][jj], 0, 0, 0);
+          }
         }
         __syncthreads();
       }
     }
   }
-  // D[4 * (lane / 16) + e][lane % 16] of every 16x16 block
+  // group 1's partial tile into group 0's: D[4 * (lane / 16) + e][lane % 16]
+  // of every 16x16 block, through LDS (the A tiles, free after the last barrier)
+  float* red = (float*)&As[0][0][0][0];
+  static_assert(sizeof(As) >= 4 * 2 * 2 * 4 * 64 * sizeof(float), "the reduction fits the A tiles");
+  if (grp == 1) {
+#pragma unroll
+    for (int i = 0; i < 2; i++)
+#pragma unroll
+      for (int j = 0; j < 2; j++)
+#pragma unroll
+        for (int e = 0; e < 4; e++) red[(((wave * 2 + i) * 2 + j) * 4 + e) * 64 + lane] = acc[i][j][e];
+  }
+  __syncthreads();
+  if (grp == 1) return;
+#pragma unroll
+  for (int i = 0; i < 2; i++)
+#pragma unroll
+    for (int j = 0; j < 2; j++)
+#pragma unroll
+      for (int e = 0; e < 4; e++) acc[i][j][e] += red[(((wave * 2 + i) * 2 + j) * 4 + e) * 64 + lane];
   const int col = lane & 15, rowb = (lane >> 4) * 4;
 #pragma unroll
   for (int i = 0; i < 2; i++)
@@ -428,8 +509,8 @@ Status Net::get_grads(float* grads) const {
 template <bool AK, bool BK, int EPI>
 static void gemm(hipStream_t s, const bf16* A, int lda, const bf16* B, int ldb, void* C, int ldc, int M, int N, int K,
                  const float* bias = nullptr, const bf16* mask = nullptr, int ldm = 0) {
-  hipLaunchKernelGGL((gemm_kernel<AK, BK, EPI>), dim3(N / TN, M / TM), dim3(256), 0, s, A, lda, B, ldb, C, ldc, K, bias, mask,
-                     ldm);
+  hipLaunchKernelGGL((gemm_kernel<AK, BK, EPI>), dim3((N / TN) * (M / TM)), dim3(GT), 0, s, A, lda, B, ldb, C, ldc, M, N, K,
+                     bias, mask, ldm);
 }
 
 Status Net::check_rows(uint32_t rows) const {
