@@ -1163,10 +1163,22 @@ struct MemberVec {
     const T x2 = r2[i & (W2 - 1)];
     return i < 32 ? x0 : i < 64 ? x1 : x2;
   }
+  // Every vector is written (its element kept where i is not in it), with
+  // no branch on i: a branch would make the compiler merge the vectors of
+  // both paths by copying a whole vector per write.
   __device__ static __forceinline__ void set(lo& r0, mi& r1, hi& r2, uint32_t i, T v) {
-    if (CAP == 32 || i < 32) r0[i & 31] = v;
-    else if (CAP <= 64 || i < 64) r1[i & (W1 - 1)] = v;
-    else r2[i & (W2 - 1)] = v;
+    if (CAP == 32) {
+      r0[i & 31] = v;
+      return;
+    }
+    const T o0 = r0[i & 31];
+    r0[i & 31] = i < 32 ? v : o0;
+    const T o1 = r1[i & (W1 - 1)];
+    r1[i & (W1 - 1)] = (i >= 32 && (CAP <= 64 || i < 64)) ? v : o1;
+    if (CAP > 64) {
+      const T o2 = r2[i & (W2 - 1)];
+      r2[i & (W2 - 1)] = i >= 64 ? v : o2;
+    }
   }
 };
 
