@@ -96,11 +96,14 @@ struct DPlan {
   // down pass, read by the up pass)
   uint8_t* malive;
   // segments of more than REG_L members, of REG_M + 1 to REG_L, of REG_S + 1
-  // to REG_M and of REG_T + 1 to REG_S members (ascending): device copies,
-  // and the host copies launches are cut from
-  const uint32_t *big, *m80, *mid, *m48;
-  const uint32_t *big_host, *m80_host, *mid_host, *m48_host;
-  uint32_t num_big, num_m80, num_mid, num_m48;
+  // to REG_M and of REG_T + 1 to REG_S members, and the segments with a walk
+  // of up to REG_T and up to REG_S members (ascending): device copies, and
+  // the host copies launches are cut from. A chance child whose one member
+  // is a terminal or a chance node has no walk: that member writes the
+  // segment root's row itself, and a chance node reads a dead child as zero.
+  const uint32_t *big, *m80, *mid, *m48, *w32, *w48;
+  const uint32_t *big_host, *m80_host, *mid_host, *m48_host, *w32_host, *w48_host;
+  uint32_t num_big, num_m80, num_mid, num_m48, num_w32, num_w48;
   // per traverser p: each segment's opponent decision nodes in groups the
   // down pass runs together (one level of the segment, at most DG_A actions
   // in all): entries dg_ent[p][dg_off[p][seg] .. dg_off[p][seg + 1]), each
@@ -1084,8 +1087,9 @@ __global__ void __launch_bounds__(FT, sizeof(V) == 4 ? 6 : 5) terminal_fast_kern
 // ------------------------------------------------------------------ chance
 // Chance nodes of one segment depth, after the deeper segments' up pass:
 // the f64 sum over dealt cards of the child rows, zero for own hands holding
-// the card (cfr.rs:457-478); dead nodes get a zero row. One lane per hand,
-// 16 child rows in flight per lane.
+// the card (cfr.rs:457-478); dead nodes get a zero row, and a dead child adds
+// an exact zero (its row is not written when it is one terminal or chance
+// node). One lane per hand, 16 child rows in flight per lane.
 template <typename V>
 __global__ void __launch_bounds__(UP_T) chance_kernel(DPlan P, int p, uint32_t c0, Span span, Bias B, V* __restrict__ vals,
                                                       uint32_t rowlen, const uint32_t* __restrict__ alive) {
@@ -1102,6 +1106,7 @@ __global__ void __launch_bounds__(UP_T) chance_kernel(DPlan P, int p, uint32_t c
   }
   const uint32_t na = r.na;
   const V* row = vals + (size_t)B.v(P.chance_child_vslot[ci]) * rowlen + h;
+  const uint32_t* calive = alive + B.s(r.cslot[p]);  // the children's alive flags (consecutive slots)
   const int8_t* cards = P.card + r.offset;
   const int c1 = S.cards[p][2 * h], c2 = S.cards[p][2 * h + 1];
   double acc = 0.0;
@@ -1109,7 +1114,7 @@ __global__ void __launch_bounds__(UP_T) chance_kernel(DPlan P, int p, uint32_t c
   for (; a + 16 <= na; a += 16) {
     V x[16];
 #pragma unroll
-    for (int k = 0; k < 16; k++) x[k] = row[(size_t)(a + k) * rowlen];
+    for (int k = 0; k < 16; k++) x[k] = calive[a + k] ? row[(size_t)(a + k) * rowlen] : V(0);
 #pragma unroll
     for (int k = 0; k < 16; k++) {
       const int c = cards[a + k];
@@ -1118,7 +1123,7 @@ __global__ void __launch_bounds__(UP_T) chance_kernel(DPlan P, int p, uint32_t c
   }
   for (; a < na; a++) {
     const int c = cards[a];
-    const V x = row[(size_t)a * rowlen];
+    const V x = calive[a] ? row[(size_t)a * rowlen] : V(0);
     acc += (c == c1 || c == c2) ? 0.0 : (double)x;
   }
   // cards merged by suit isomorphism, after the canonical ones: the
@@ -1127,7 +1132,7 @@ __global__ void __launch_bounds__(UP_T) chance_kernel(DPlan P, int p, uint32_t c
   for (uint32_t i = P.ciso_off[ci]; i < P.ciso_off[ci + 1]; i++) {
     const IsoRef e = P.ciso[i];
     const uint32_t hp = S.perm[p][(size_t)e.pair * np + h];
-    const V x = base[(size_t)e.child * rowlen + hp];
+    const V x = calive[e.child] ? base[(size_t)e.child * rowlen + hp] : V(0);
     acc += ((int)e.card == c1 || (int)e.card == c2) ? 0.0 : (double)x;
   }
   *out = (V)acc;
@@ -1463,18 +1468,19 @@ __global__ void __launch_bounds__(UP_T, CAP >= 64 ? 2 : CAP == 32 && QP != 2 ? 6
   up_reg_walk<V, MODE, CAP, QP>(P, p, seg0, span, B, vals, rowlen, alive, sto, prm, qout, list);
 }
 
-// A range with many segments of 33 to 48 members: one launch, each
-// workgroup walking its segment with the 32-member code when it fits (one
-// indexed move per member read) and the 48-member code otherwise (at the
-// 48-member kernel's occupancy).
+// A range with many segments of 33 to 48 members: one launch over the list
+// of segments of up to 48 members, each workgroup walking its segment with
+// the 32-member code when it fits (one indexed move per member read) and the
+// 48-member code otherwise (at the 48-member kernel's occupancy).
 template <typename V, int MODE, int QP>
 __global__ void __launch_bounds__(UP_T, 4) up_reg_dual_kernel(DPlan P, int p, uint32_t seg0, Span span, Bias B, V* __restrict__ vals,
                                                           uint32_t rowlen, const uint32_t* __restrict__ alive, Store sto,
-                                                          const IterParams* __restrict__ prm, V* __restrict__ qout) {
-  if (P.seg_head[seg0 + blockIdx.x].y <= REG_T)
-    up_reg_walk<V, MODE, (int)REG_T, QP>(P, p, seg0, span, B, vals, rowlen, alive, sto, prm, qout, nullptr);
+                                                          const IterParams* __restrict__ prm, V* __restrict__ qout,
+                                                          const uint32_t* __restrict__ list) {
+  if (P.seg_head[list[seg0 + blockIdx.x]].y <= REG_T)
+    up_reg_walk<V, MODE, (int)REG_T, QP>(P, p, seg0, span, B, vals, rowlen, alive, sto, prm, qout, list);
   else
-    up_reg_walk<V, MODE, (int)REG_S, QP>(P, p, seg0, span, B, vals, rowlen, alive, sto, prm, qout, nullptr);
+    up_reg_walk<V, MODE, (int)REG_S, QP>(P, p, seg0, span, B, vals, rowlen, alive, sto, prm, qout, list);
 }
 
 // Values in LDS: the f64 average walk, and segments above REG_L members
@@ -1838,6 +1844,8 @@ struct Engine::DPlanHolder {
   std::vector<uint32_t> m80;  // segments of REG_M + 1 to REG_L members, ascending
   std::vector<uint32_t> mid;  // segments of REG_S + 1 to REG_M members, ascending
   std::vector<uint32_t> m48;  // segments of REG_T + 1 to REG_S members, ascending
+  std::vector<uint32_t> w32;  // segments with a walk, of up to REG_T members, ascending
+  std::vector<uint32_t> w48;  // segments with a walk, of up to REG_S members, ascending
 };
 
 Engine::~Engine() {
@@ -2007,19 +2015,24 @@ Status Engine::create(int device, Plan&& plan, const float* regrets, const float
   }
   const size_t o_dgo0 = put(blob, dg_off[0]), o_dgo1 = put(blob, dg_off[1]);
   const size_t o_dge0 = put(blob, dg_ent[0]), o_dge1 = put(blob, dg_ent[1]);
-  std::vector<uint32_t> big, m80, mid, m48;
+  std::vector<uint32_t> big, m80, mid, m48, w32, w48;
   std::vector<uint4> head(P.seg_mem_cnt.size());
   for (uint32_t sg = 0; sg < (uint32_t)P.seg_mem_cnt.size(); sg++) {
+    const MemRec& r0 = P.mrec[P.seg_mem_off[sg]];
+    // a chance child (it has a parent reach slot) of one terminal or chance
+    // node: that member, the segment root, writes the root row itself
+    const bool walk = !(P.seg_mem_cnt[sg] == 1 && P.seg_pslot[sg] != kNone && (r0.kind == kChance || is_terminal(r0.kind)));
     if (P.seg_mem_cnt[sg] > REG_L) big.push_back(sg);
     else if (P.seg_mem_cnt[sg] > REG_M) m80.push_back(sg);
     else if (P.seg_mem_cnt[sg] > REG_S) mid.push_back(sg);
     else if (P.seg_mem_cnt[sg] > REG_T) m48.push_back(sg);
-    const MemRec& r0 = P.mrec[P.seg_mem_off[sg]];
+    if (walk && P.seg_mem_cnt[sg] <= REG_T) w32.push_back(sg);
+    if (walk && P.seg_mem_cnt[sg] <= REG_S) w48.push_back(sg);
     const Plan& T = P.spot(r0.spot);
     head[sg] = uint4{P.seg_mem_off[sg], P.seg_mem_cnt[sg], r0.node, T.n[0] | T.n[1] << 16};
   }
   const size_t o_big = put(blob, big), o_m80 = put(blob, m80), o_mid = put(blob, mid), o_m48 = put(blob, m48),
-               o_head = put(blob, head);
+               o_w32 = put(blob, w32), o_w48 = put(blob, w48), o_head = put(blob, head);
   const size_t o_spots = (blob.size() + 255) & ~size_t(255);
   blob.resize(o_spots + K * sizeof(SpotTab));
   DeviceBuf b;
@@ -2074,6 +2087,14 @@ Status Engine::create(int device, Plan&& plan, const float* regrets, const float
   e->dp_->m80 = std::move(m80);
   e->dp_->mid = std::move(mid);
   e->dp_->m48 = std::move(m48);
+  e->dp_->w32 = std::move(w32);
+  e->dp_->w48 = std::move(w48);
+  d.w32 = AT(uint32_t, o_w32);
+  d.w48 = AT(uint32_t, o_w48);
+  d.w32_host = e->dp_->w32.data();
+  d.num_w32 = (uint32_t)e->dp_->w32.size();
+  d.w48_host = e->dp_->w48.data();
+  d.num_w48 = (uint32_t)e->dp_->w48.size();
   d.big = AT(uint32_t, o_big);
   d.m80 = AT(uint32_t, o_m80);
   d.mid = AT(uint32_t, o_mid);
@@ -2349,25 +2370,32 @@ static void launch_span(const DPlan& d, const Plan& P, hipStream_t st, int p, Sp
     auto walks = [&](auto qp) {
       constexpr int QP = decltype(qp)::value;
       if constexpr (std::is_same<V, float>::value || UMODE == kBest) {
-        // f32 walks and the f64 best response: segments of up to REG_T (or,
-        // when more than a quarter of them are larger, REG_S) members in
-        // registers over [a, b), then those of up to REG_S, REG_M and REG_L
-        // among [a, b), then the larger ones in LDS
+        // f32 walks and the f64 best response, each over a list of the
+        // segments among [a, b): those of up to REG_T members (or, when more
+        // than a quarter of the walks of up to REG_S members are larger, of
+        // up to REG_S) in registers, then those of up to REG_S, REG_M and
+        // REG_L, then the larger ones in LDS
         auto cut = [&](const uint32_t* l, uint32_t n, uint32_t& i0, uint32_t& i1) {
           i0 = (uint32_t)(std::lower_bound(l, l + n, a) - l);
           i1 = (uint32_t)(std::lower_bound(l, l + n, b) - l);
         };
-        uint32_t i0, i1;
+        uint32_t i0, i1, j0, j1;
         cut(d.m48_host, d.num_m48, i0, i1);
-        const bool wide = (uint64_t)(i1 - i0) * 4 > (uint64_t)(b - a);
-        PROF_START(kKUp);
-        if (wide)
-          hipLaunchKernelGGL((up_reg_dual_kernel<V, UMODE, QP>), dim3(b - a, tiles), dim3(UP_T), 0, st, d, p, a, span, B,
-                             (V*)vals, rowlen, (const uint32_t*)alive, sto, prm, qout);
-        else
-          hipLaunchKernelGGL((up_reg_kernel<V, UMODE, (int)REG_T, QP>), dim3(b - a, tiles), dim3(UP_T), 0, st, d, p, a, span, B,
-                             (V*)vals, rowlen, (const uint32_t*)alive, sto, prm, qout, (const uint32_t*)nullptr);
-        PROF_STOP();
+        cut(d.w48_host, d.num_w48, j0, j1);
+        const bool wide = (uint64_t)(i1 - i0) * 4 > (uint64_t)(j1 - j0);
+        if (wide && j1 > j0) {
+          PROF_START(kKUp);
+          hipLaunchKernelGGL((up_reg_dual_kernel<V, UMODE, QP>), dim3(j1 - j0, tiles), dim3(UP_T), 0, st, d, p, j0, span, B,
+                             (V*)vals, rowlen, (const uint32_t*)alive, sto, prm, qout, d.w48);
+          PROF_STOP();
+        }
+        cut(d.w32_host, d.num_w32, j0, j1);
+        if (!wide && j1 > j0) {
+          PROF_START(kKUp);
+          hipLaunchKernelGGL((up_reg_kernel<V, UMODE, (int)REG_T, QP>), dim3(j1 - j0, tiles), dim3(UP_T), 0, st, d, p, j0, span, B,
+                             (V*)vals, rowlen, (const uint32_t*)alive, sto, prm, qout, d.w32);
+          PROF_STOP();
+        }
         if (!wide && i1 > i0) {
           PROF_START(kKUp);
           hipLaunchKernelGGL((up_reg_kernel<V, UMODE, (int)REG_S, QP>), dim3(i1 - i0, tiles), dim3(UP_T), 0, st, d, p, i0, span, B,

@@ -425,6 +425,19 @@ std::string build_plan(const PlanInput& in, Plan& P) {
       if (P.kind[id] != kShowdown && P.tboard[id] == kNone && in.street[parent[id]] == kRiver)
         P.tboard[id] = seg_board[P.seg_of[id]];
   }
+  // Within the top segment's terminals and within each chunk's, the ones on
+  // one river board together (in tree order otherwise), so the fast terminal
+  // kernel's runs load a board's tables once for up to kBatchMax terminals:
+  // all-in runouts deal one showdown per segment, and without this each
+  // would be a run of its own. Siblings on one board stay adjacent.
+  {
+    auto by_board = [&](uint32_t a, uint32_t b) {
+      std::stable_sort(P.terminals.begin() + a, P.terminals.begin() + b,
+                       [&](uint32_t x, uint32_t y) { return P.tboard[x] < P.tboard[y]; });
+    };
+    by_board(0, C ? P.term_coff[0] : (uint32_t)P.terminals.size());
+    for (uint32_t c = 0; c < C; c++) by_board(P.term_coff[c], P.term_coff[c + 1]);
+  }
   P.trec.assign(P.terminals.size(), TermRec{});
   for (size_t t = 0; t < P.terminals.size(); t++) {
     uint32_t id = P.terminals[t];
