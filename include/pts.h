@@ -156,6 +156,11 @@ pts_status pts_engine_profile_iteration(pts_engine* e, float pos, float neg, flo
 uint64_t pts_engine_device_bytes(pts_engine* e);
 /* Chunk groups one traversal runs in (f64 = 0: DCFR rows, 1: FP64 best-response rows). */
 uint32_t pts_engine_num_groups(pts_engine* e, int f64);
+/* Segments whose DCFR up walk runs in a kernel specialised to the segment's
+ * shape, compiled at creation for the shapes of at least PTS_JIT_MIN_SEGMENTS
+ * segments (default 1024; PTS_JIT=0 turns them off). No reference
+ * counterpart: the reference's walk is per level (gpu/engine.rs:600-749). */
+uint64_t pts_engine_jit_segments(pts_engine* e);
 void pts_engine_destroy(pts_engine* e);
 
 /* ------------------------------------------------------------------ solver */
@@ -228,6 +233,8 @@ pts_status pts_solver_estimate(const pts_config_c* cfg, pts_stats_c* out);
 pts_status pts_solver_new(const pts_config_c* cfg, int device, int precision, uint64_t scratch_budget, pts_solver** out);
 /* Chunk groups of the solver's engine (as pts_engine_num_groups). */
 uint32_t pts_solver_num_groups(pts_solver* s, int f64);
+/* Shape-specialised walks of the solver's engine (as pts_engine_jit_segments). */
+uint64_t pts_solver_jit_segments(pts_solver* s);
 /* Host-only solvers (device -1): the chunk groups a scratch budget gives and
  * the scratch bytes they need, after checking that every slot and row a
  * grouped traversal touches maps inside that scratch (Plan::build_with's

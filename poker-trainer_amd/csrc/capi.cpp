@@ -347,6 +347,7 @@ pts_status pts_engine_subtree_values(pts_engine* e, uint32_t node, int player, c
 
 uint64_t pts_engine_device_bytes(pts_engine* e) { return e->e->device_bytes(); }
 uint32_t pts_engine_num_groups(pts_engine* e, int f64) { return (uint32_t)e->e->groups(f64 != 0).size(); }
+uint64_t pts_engine_jit_segments(pts_engine* e) { return e->e->jit_segments(); }
 
 pts_status pts_engine_profile_iteration(pts_engine* e, float pos, float neg, float strat, int rw, double ms[4],
                                         uint32_t launches[4], double* wall_ms) {
@@ -643,6 +644,10 @@ uint64_t pts_solver_storage_len(pts_solver* s) { return s->tree.storage_len; }
 uint32_t pts_solver_num_groups(pts_solver* s, int f64) {
   std::lock_guard<std::recursive_mutex> lock_(s->mu);
   return s->engine ? (uint32_t)s->engine->groups(f64 != 0).size() : 0u;
+}
+uint64_t pts_solver_jit_segments(pts_solver* s) {
+  std::lock_guard<std::recursive_mutex> lock_(s->mu);
+  return s->engine ? s->engine->jit_segments() : 0u;
 }
 
 pts_status pts_solver_plan_groups(pts_solver* s, uint64_t scratch_budget, int f64, uint32_t* num_groups,

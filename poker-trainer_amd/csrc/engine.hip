@@ -18,8 +18,10 @@
 #include <cstdlib>
 #include <cstring>
 #include <functional>
+#include <map>
 
 #include "engine.hpp"
+#include "jit.hpp"
 #include "net.hpp"
 
 namespace pts {
@@ -1838,6 +1840,15 @@ __global__ void leaf_zero_kernel(Bias B, const LeafRec* __restrict__ leaves, V* 
 }
 
 // ================================================================== engine
+// Shape-specialised CFR walks (jit.hpp): per shape, its two kernels (one per
+// traverser) and its segments, ascending (host copy; device copy).
+struct JitLaunch {
+  std::vector<hipFunction_t> fn;
+  std::vector<std::vector<uint32_t>> lists;
+  std::vector<const uint32_t*> dev;
+  const float* zrow = nullptr;  // MAXA * rowlen zeros, read by dead members
+};
+
 struct Engine::DPlanHolder {
   DPlan d;
   std::vector<uint32_t> big;  // segments above REG_L members, ascending
@@ -1846,6 +1857,12 @@ struct Engine::DPlanHolder {
   std::vector<uint32_t> m48;  // segments of REG_T + 1 to REG_S members, ascending
   std::vector<uint32_t> w32;  // segments with a walk, of up to REG_T members, ascending
   std::vector<uint32_t> w48;  // segments with a walk, of up to REG_S members, ascending
+  // DCFR and best-response traversals with specialised walks: d with those
+  // walks' segments taken out of the generic walk lists (cfr_* below), and
+  // the walks
+  DPlan dcfr;
+  std::vector<uint32_t> cfr_m80, cfr_mid, cfr_m48, cfr_w32, cfr_w48;
+  JitLaunch jit;
 };
 
 Engine::~Engine() {
@@ -1882,6 +1899,116 @@ static size_t put(std::vector<unsigned char>& blob, const std::vector<T>& v) {
 }
 
 Store Engine::store() const { return Store{regrets_, sums_, rq_, sq_, rsc_, ssc_, rnx_}; }
+
+// Specialised walks (jit.hpp) for the frequent segment shapes, f32 storage
+// only: the f32 DCFR walk and the FP64 best response. A shape is every
+// member's kind (0 or 1 for a decision node, 2 for a member read from its
+// row), action count and first child. Shapes of at least
+// PTS_JIT_MIN_SEGMENTS segments (default 1024) get their own kernels
+// (PTS_JIT=0: none); a spot's root segment, which also writes the spot's
+// result row, keeps the generic walk. The other segments, the average-
+// strategy walks and queries use the generic kernels.
+Status Engine::setup_jit(bool q16) {
+  DPlanHolder& H = *dp_;
+  const Plan& P = plan_;
+  H.dcfr = H.d;
+  const char* jo = std::getenv("PTS_JIT");
+  if (q16 || (jo && jo[0] == '0')) return kOk;
+  uint32_t min_segs = 1024;
+  if (const char* m = std::getenv("PTS_JIT_MIN_SEGMENTS")) min_segs = (uint32_t)std::max(1, std::atoi(m));
+  std::map<std::string, std::vector<uint32_t>> by_shape;
+  std::map<std::string, jit::Shape> shape_of;
+  for (uint32_t sg = 0; sg < (uint32_t)P.seg_mem_cnt.size(); sg++) {
+    const uint32_t mc = P.seg_mem_cnt[sg];
+    if (mc < 2 || mc > jit::kMaxMembers || P.seg_pslot[sg] == kNone) continue;
+    const MemRec* rr = P.mrec.data() + P.seg_mem_off[sg];
+    jit::Shape S(mc);
+    std::string key(mc * 4, '\0');
+    for (uint32_t m = 0; m < mc; m++) {
+      const bool dec = is_player(rr[m].kind);
+      S[m] = jit::Member{dec ? rr[m].kind : (uint8_t)2, dec ? rr[m].na : (uint8_t)0, dec ? rr[m].clocal : (uint16_t)0};
+      std::memcpy(&key[m * 4], &S[m], 4);
+    }
+    auto& v = by_shape[key];
+    if (v.empty()) shape_of[key] = S;
+    v.push_back(sg);
+  }
+  std::vector<std::pair<size_t, std::string>> pick;
+  for (const auto& kv : by_shape)
+    if (kv.second.size() >= min_segs) pick.push_back({kv.second.size(), kv.first});
+  std::sort(pick.begin(), pick.end(), [](const auto& x, const auto& y) { return x.first > y.first || (x.first == y.first && x.second < y.second); });
+  if (pick.size() > 32) pick.resize(32);
+  if (pick.empty()) return kOk;
+  std::vector<jit::Shape> shapes;
+  std::vector<uint8_t> taken(P.seg_mem_cnt.size(), 0);
+  JitLaunch& J = H.jit;
+  for (const auto& x : pick) {
+    shapes.push_back(shape_of[x.second]);
+    J.lists.push_back(std::move(by_shape[x.second]));
+    for (uint32_t sg : J.lists.back()) taken[sg] = 1;
+    jit_segments_ += J.lists.back().size();
+  }
+  Status st = jit::compile_walks(shapes, J.fn, &jit_seconds_);
+  if (st != kOk) return st;
+  auto rest = [&](const std::vector<uint32_t>& l, std::vector<uint32_t>& out) {
+    for (uint32_t sg : l)
+      if (!taken[sg]) out.push_back(sg);
+  };
+  rest(H.m80, H.cfr_m80);
+  rest(H.mid, H.cfr_mid);
+  rest(H.m48, H.cfr_m48);
+  rest(H.w32, H.cfr_w32);
+  rest(H.w48, H.cfr_w48);
+  DeviceBuf z;
+  if ((st = alloc(z, (size_t)MAXA * rowlen_ * sizeof(float))) != kOk) return st;
+  HIPCHK(hipMemset(z.p, 0, z.bytes));
+  J.zrow = (const float*)z.p;
+  // one buffer: the shapes' lists, then the reduced generic lists
+  std::vector<const std::vector<uint32_t>*> all;
+  for (const auto& l : J.lists) all.push_back(&l);
+  all.push_back(&H.cfr_m80);
+  all.push_back(&H.cfr_mid);
+  all.push_back(&H.cfr_m48);
+  all.push_back(&H.cfr_w32);
+  all.push_back(&H.cfr_w48);
+  size_t n = 0;
+  for (const auto* l : all) n += l->size() + 64;  // each list on its own 256-byte boundary
+  DeviceBuf b;
+  if ((st = alloc(b, n * 4)) != kOk) return st;
+  std::vector<uint32_t> host(n, 0);
+  std::vector<const uint32_t*> dev;
+  size_t at = 0;
+  for (const auto* l : all) {
+    std::copy(l->begin(), l->end(), host.begin() + at);
+    dev.push_back((const uint32_t*)b.p + at);
+    at += (l->size() + 63) & ~size_t(63);
+  }
+  HIPCHK(hipMemcpy(b.p, host.data(), n * 4, hipMemcpyHostToDevice));
+  const size_t k = J.lists.size();
+  J.dev.assign(dev.begin(), dev.begin() + k);
+  DPlan& c = H.dcfr;
+  c.m80 = dev[k + 0];
+  c.mid = dev[k + 1];
+  c.m48 = dev[k + 2];
+  c.w32 = dev[k + 3];
+  c.w48 = dev[k + 4];
+  c.m80_host = H.cfr_m80.data();
+  c.mid_host = H.cfr_mid.data();
+  c.m48_host = H.cfr_m48.data();
+  c.w32_host = H.cfr_w32.data();
+  c.w48_host = H.cfr_w48.data();
+  c.num_m80 = (uint32_t)H.cfr_m80.size();
+  c.num_mid = (uint32_t)H.cfr_mid.size();
+  c.num_m48 = (uint32_t)H.cfr_m48.size();
+  c.num_w32 = (uint32_t)H.cfr_w32.size();
+  c.num_w48 = (uint32_t)H.cfr_w48.size();
+  jit_shapes_ = (uint32_t)k;
+  if (const char* vb = std::getenv("PTS_JIT_VERBOSE"))
+    if (vb[0] == '1')
+      std::fprintf(stderr, "[pts] specialised walks: %u shapes, %llu segments, compiled in %.2f s\n", jit_shapes_,
+                   (unsigned long long)jit_segments_, jit_seconds_);
+  return kOk;
+}
 
 Status Engine::create(int device, Plan&& plan, const float* regrets, const float* sums, int precision,
                       uint64_t scratch_budget, std::unique_ptr<Engine>& out, bool q16) {
@@ -2209,6 +2336,8 @@ Status Engine::create(int device, Plan&& plan, const float* regrets, const float
               (st = lds_attr((const void*)down_kernel<double, kAverage, true>, e->lds_down_)) != kOk ||
               (st = lds_attr((const void*)down_kernel<float, kAverage, true>, e->lds_down_)) != kOk))
     return st;
+  // the specialised walks take a copy of the finished device view (d)
+  if ((st = e->setup_jit(q16)) != kOk) return st;
   const char* ng = std::getenv("PTS_NO_GRAPH");
   e->use_graph_ = !(ng && ng[0] == '1');
   if (e->use_graph_ && (st = e->build_graphs()) != kOk) return st;
@@ -2331,7 +2460,7 @@ static void launch_span(const DPlan& d, const Plan& P, hipStream_t st, int p, Sp
                         const std::vector<ChunkGroup>& groups, void* reach, uint32_t* alive, const Store& sto,
                         const IterParams* prm, void* vals, uint32_t rowlen, size_t lds_down, size_t lds_term, size_t lds_up,
                         V* qout, bool fast, uint32_t down_t, ProfRec* prof = nullptr,
-                        const std::function<void()>* leaf_hook = nullptr) {
+                        const std::function<void()>* leaf_hook = nullptr, const JitLaunch* jit = nullptr) {
   const uint32_t D = P.depths(), C = P.num_chunks;
   const uint32_t tiles = (P.n[p] + UP_T - 1) / UP_T;
   const Bias B0{P.slot_s0[p], 0, P.val_v0, 0};
@@ -2423,6 +2552,23 @@ static void launch_span(const DPlan& d, const Plan& P, hipStream_t st, int p, Sp
                              rowlen, (const uint32_t*)alive, sto, prm, qout, d.big);
           PROF_STOP();
         }
+        // the shape-specialised walks (f32 DCFR and FP64 best-response
+        // traversals; d's generic lists then leave their segments out)
+        if constexpr (((std::is_same<V, float>::value && UMODE == kCfr) || UMODE == kBest) && QP == 0) {
+          if (jit != nullptr)
+            for (size_t k = 0; k < jit->lists.size(); k++) {
+              const std::vector<uint32_t>& L = jit->lists[k];
+              cut(L.data(), (uint32_t)L.size(), i0, i1);
+              if (i1 <= i0) continue;
+              jit::WalkArgs A{jit->dev[k], d.seg_head, (const unsigned char*)d.mrec, d.malive, (float*)vals, sto.r,
+                              (const float*)prm, jit->zrow, i0, rowlen, B.v0, B.vb};
+              void* args[] = {&A};
+              PROF_START(kKUp);
+              (void)hipModuleLaunchKernel(jit->fn[4 * k + (UMODE == kBest ? 2 : 0) + p], i1 - i0, tiles, 1, UP_T, 1, 1, 0, st,
+                                          args, nullptr);
+              PROF_STOP();
+            }
+        }
       } else {
         PROF_START(kKUp);
         hipLaunchKernelGGL((up_kernel<V, UMODE, QP>), dim3(b - a, tiles), dim3(UP_T), lds_up, st, d, p, a, span, B, (V*)vals, rowlen,
@@ -2487,12 +2633,13 @@ Status Engine::launch_traversal(int p, Mode mode, bool f64, ProfRec* prof) {
   auto run = [&](auto qc) {
     constexpr bool Q = decltype(qc)::value;
     if (mode == kCfr)
-      launch_span<float, kCfr, kCfr, Q>(d, P, stream_, p, span, 0, groups_[0], reach, alive, sto, params_, vals, rowlen_, lds_down_,
-                                        lds_term_[0], lds_up_[0], (float*)nullptr, precision_ == kFast, down_threads_, prof, hk);
+      launch_span<float, kCfr, kCfr, Q>(Q ? d : dp_->dcfr, P, stream_, p, span, 0, groups_[0], reach, alive, sto, params_, vals,
+                                        rowlen_, lds_down_, lds_term_[0], lds_up_[0], (float*)nullptr, precision_ == kFast,
+                                        down_threads_, prof, hk, Q ? nullptr : &dp_->jit);
     else if (mode == kBest)
-      launch_span<double, kAverage, kBest, Q>(d, P, stream_, p, span, 0, groups_[1], reach, alive, sto, params_, vals, rowlen_,
-                                              lds_down_, lds_term_[1], lds_up_[1], (double*)nullptr, precision_ == kFast,
-                                              down_threads_, nullptr, hk);
+      launch_span<double, kAverage, kBest, Q>(Q ? d : dp_->dcfr, P, stream_, p, span, 0, groups_[1], reach, alive, sto, params_,
+                                              vals, rowlen_, lds_down_, lds_term_[1], lds_up_[1], (double*)nullptr,
+                                              precision_ == kFast, down_threads_, nullptr, hk, Q ? nullptr : &dp_->jit);
     else if (f64)
       launch_span<double, kAverage, kAverage, Q>(d, P, stream_, p, span, 0, groups_[1], reach, alive, sto, params_, vals, rowlen_,
                                                  lds_down_, lds_term_[1], lds_up_[1], (double*)nullptr, precision_ == kFast,

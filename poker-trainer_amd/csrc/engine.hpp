@@ -133,12 +133,18 @@ class Engine {
   size_t device_bytes() const { return bytes_; }
   // Chunk groups of a traversal, for f32 (CFR) or f64 rows.
   const std::vector<ChunkGroup>& groups(bool f64) const { return groups_[f64 ? 1 : 0]; }
+  // Segments whose CFR walks run in shape-specialised kernels, their shapes,
+  // and the seconds their compilation took at creation (0 when cached).
+  uint64_t jit_segments() const { return jit_segments_; }
+  uint32_t jit_shapes() const { return jit_shapes_; }
+  double jit_seconds() const { return jit_seconds_; }
 
  private:
   Engine() = default;
   Status alloc(DeviceBuf& b, size_t bytes);
 
   Status launch_traversal(int p, Mode mode, bool f64, struct ProfRec* prof = nullptr);
+  Status setup_jit(bool q16);
   void launch_leaves(int p, Mode mode, bool f64, void* reach, void* vals, uint32_t* alive);
   Status build_graphs();
   Status run_graph(int which);
@@ -184,6 +190,9 @@ class Engine {
   bool use_graph_ = true;
   int precision_ = kFast;
   std::vector<ChunkGroup> groups_[2];
+  uint64_t jit_segments_ = 0;
+  uint32_t jit_shapes_ = 0;
+  double jit_seconds_ = 0;
   struct DPlanHolder;
   std::unique_ptr<DPlanHolder> dp_;
 };

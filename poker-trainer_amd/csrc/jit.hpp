@@ -1,0 +1,64 @@
+// Walks specialised to a segment shape, compiled at engine creation.
+//
+// The generic up walk (up_reg_kernel) reads a segment's member records at
+// run time and keeps the member values in register vectors indexed by a
+// wave-uniform member number (indexed register moves, a few scalar
+// instructions and a branch per access). A tree such as the flop spot's has
+// hundreds of thousands of river segments but only a handful of distinct
+// shapes (the same betting tree under each river card), so the walk of each
+// frequent shape is generated as straight-line HIP with every member a named
+// value, compiled once with hiprtc for gfx950, and launched over the list of
+// segments of that shape. The arithmetic is the generic walk's, operation for
+// operation (the same compiler flags: no contraction, no packed FP32), so
+// results are bit-identical to it; only the addressing is resolved at
+// compile time.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "engine.hpp"
+
+namespace pts {
+namespace jit {
+
+// One member of a segment shape: a decision node (kind 0 or 1, its action
+// count and the local index of its first child), or a member whose value is
+// read from its row (kind 2: a chance node, terminal or leaf).
+struct Member {
+  uint8_t kind, na;
+  uint16_t cl;
+  bool operator==(const Member& o) const { return kind == o.kind && na == o.na && cl == o.cl; }
+};
+using Shape = std::vector<Member>;
+
+// The kernels' one argument (the generated source declares the same layout).
+struct WalkArgs {
+  const uint32_t* list;        // segment ids, walked list[i0 + blockIdx.x]
+  const uint4* seg_head;       // per segment: first member, member count, root node, hands of both players
+  const unsigned char* mrec;   // MemRec array (48 B each)
+  const uint8_t* malive;       // per member: alive for the current traverser
+  float* vals;                 // value rows (f64 for the best response)
+  float* regrets;              // f32 regret storage (DCFR walk)
+  const float* prm;            // IterParams: pos, neg (DCFR walk)
+  const float* zrow;           // 8 * rowlen f32 zeros: what a dead member reads
+  uint32_t i0, rowlen, v0, vb; // list offset, row length, value-row bias (Bias::v)
+};
+
+// The most members a specialised walk takes.
+constexpr uint32_t kMaxMembers = 80;
+
+// Compiles the walks of `shapes` for both traversers on the current device
+// (or takes them from this process's cache): fn[4 * i + p] is shape i's f32
+// DCFR walk for traverser p, fn[4 * i + 2 + p] its FP64 best response.
+// `seconds` gets the compile time (0 when cached).
+Status compile_walks(const std::vector<Shape>& shapes, std::vector<hipFunction_t>& fn, double* seconds);
+
+// The generated HIP source for `shapes` (for inspection and tests).
+std::string walk_source(const std::vector<Shape>& shapes);
+
+}  // namespace jit
+}  // namespace pts

@@ -78,7 +78,8 @@ EXPORTS = [
     "pts_dl_new", "pts_solver_num_leaves", "pts_solver_copy_root_street", "pts_solver_pin_root_street", "pts_batch_root_rows",
     "pts_batch_num_combinations", "pts_solver_is_compressed", "pts_solver_get_storage_q", "pts_solver_set_storage_q",
     "pts_engine_create_q", "pts_engine_download_q", "pts_last_error_detail", "pts_evaluate",
-    "pts_solver_solve_cancellable", "pts_run_solve", "pts_net_forward_noalloc",
+    "pts_solver_solve_cancellable", "pts_run_solve", "pts_net_forward_noalloc", "pts_engine_jit_segments",
+    "pts_solver_jit_segments",
 ]
 
 
@@ -168,6 +169,10 @@ def _declare(L):
     L.pts_batch_num_combinations.argtypes = [P, ctypes.c_uint32]
     L.pts_engine_num_groups.restype = ctypes.c_uint32
     L.pts_engine_num_groups.argtypes = [P, ctypes.c_int]
+    L.pts_engine_jit_segments.argtypes = [P]
+    L.pts_engine_jit_segments.restype = ctypes.c_uint64
+    L.pts_solver_jit_segments.argtypes = [P]
+    L.pts_solver_jit_segments.restype = ctypes.c_uint64
     L.pts_solver_free.argtypes = [P]
     L.pts_solver_stats.argtypes = [P, ctypes.POINTER(StatsC)]
     L.pts_solver_solve.argtypes = [P, ctypes.c_uint32, ctypes.c_double, ctypes.c_uint32, ctypes.POINTER(ProgressC),

@@ -103,6 +103,12 @@ class Solver:
         """Chunk groups a traversal runs in (DCFR rows, or FP64 best-response rows)."""
         return lib().pts_solver_num_groups(self.h, int(f64))
 
+    def jit_segments(self) -> int:
+        """Segments whose DCFR up walk runs in a shape-specialised kernel
+        (compiled at creation for shapes of at least PTS_JIT_MIN_SEGMENTS
+        segments; PTS_JIT=0 turns them off)."""
+        return int(lib().pts_solver_jit_segments(self.h))
+
     @property
     def iterations(self):
         return lib().pts_solver_iterations(self.h)
@@ -412,6 +418,9 @@ class Engine:
 
     def num_groups(self, f64: bool = False) -> int:
         return lib().pts_engine_num_groups(self.h, int(f64))
+
+    def jit_segments(self) -> int:
+        return int(lib().pts_engine_jit_segments(self.h))
 
 
 class BatchSolver:

@@ -50,6 +50,10 @@ def test_full_flop_properties():
     pot = cfg.root.pot
     s = Solver(cfg, precision=FAST)
     assert s.stats().storage_cells == 16188966528
+    # every segment with a walk but the root (671,448 segments: the root,
+    # 319,872 one-member chance children, 351,575 walks of 7 shapes) runs in
+    # a shape-specialised kernel
+    assert s.jit_segments() == 351085
     s.iterate(4)
     e4 = s.exploitability()
     s.iterate(12)
