@@ -777,7 +777,11 @@ struct TermLds {
 };
 
 // The fields of a terminal record one traversal of player p reads, as
-// wave-uniform scalars.
+// wave-uniform scalars. Records and alive flags are read through the
+// constant address space, so they are scalar loads counted by lgkmcnt: a
+// vector load here would make each terminal wait (vmcnt) for the value-row
+// stores of the one before it. (Both arrays are read-only in this kernel.)
+typedef const __attribute__((address_space(4))) uint32_t* cu32p;
 struct TermFields {
   uint32_t node, slot, vslot, kf;  // kf: kind | tflags << 8
   int32_t inv0, inv1;
@@ -785,15 +789,15 @@ struct TermFields {
 };
 template <int p>
 __device__ __forceinline__ TermFields term_fields(const TermRec* __restrict__ r) {
-  const uint32_t* w = (const uint32_t*)r;  // node, slot[2], vslot, board, invest[2], rake, kind | tflags << 8
+  cu32p w = (cu32p)r;  // node, slot[2], vslot, board, invest[2], rake, kind | tflags << 8
   TermFields f;
-  f.node = __builtin_amdgcn_readfirstlane(w[0]);
-  f.slot = __builtin_amdgcn_readfirstlane(w[1 + p]);
-  f.vslot = __builtin_amdgcn_readfirstlane(w[3]);
-  f.inv0 = (int32_t)__builtin_amdgcn_readfirstlane(w[5]);
-  f.inv1 = (int32_t)__builtin_amdgcn_readfirstlane(w[6]);
-  f.rake = __uint_as_float(__builtin_amdgcn_readfirstlane(w[7]));
-  f.kf = __builtin_amdgcn_readfirstlane(w[8]) & 0xFFFFu;
+  f.node = w[0];
+  f.slot = w[1 + p];
+  f.vslot = w[3];
+  f.inv0 = (int32_t)w[5];
+  f.inv1 = (int32_t)w[6];
+  f.rake = __uint_as_float(w[7]);
+  f.kf = w[8] & 0xFFFFu;
   return f;
 }
 static_assert(offsetof(TermRec, invest) == 20 && offsetof(TermRec, rake) == 28 && offsetof(TermRec, kind) == 32 &&
@@ -899,7 +903,7 @@ __global__ void __launch_bounds__(FT, sizeof(V) == 4 ? 6 : 5) terminal_fast_kern
   uint32_t t = tb.t0;
   const uint32_t tend = tb.t0 + tb.cnt;
   TermFields tr = term_fields<p>(P.trec + t);
-  uint32_t live = span.has(tr.node) ? alive[B.s(tr.slot)] : 0u;
+  uint32_t live = span.has(tr.node) ? ((cu32p)alive)[B.s(tr.slot)] : 0u;
   V r[FK];
   {
     const V* R = reach + (size_t)B.s(tr.slot) * rowlen;
@@ -930,7 +934,7 @@ __global__ void __launch_bounds__(FT, sizeof(V) == 4 ? 6 : 5) terminal_fast_kern
     if (more) {
       trn = term_fields<p>(P.trec + t + 1);
       const uint32_t sn = B.s(trn.slot);
-      liven = span.has(trn.node) ? alive[sn] : 0u;
+      liven = span.has(trn.node) ? ((cu32p)alive)[sn] : 0u;
       if (trn.slot != tr.slot) {
         const V* R = reach + (size_t)sn * rowlen;
 #pragma unroll
