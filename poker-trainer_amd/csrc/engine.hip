@@ -365,10 +365,20 @@ __global__ void __launch_bounds__(1024) down_kernel(DPlan P, int p, uint32_t seg
             xs[k][a] = Q ? (float)sto.sq[i] * sstep : sto.s[i];
           }
       }
+      // f32 DCFR rounds compute every output before the first global store:
+      // a store ahead of a read of a loaded value would make that read wait
+      // for the store's completion (the other modes store as they go, which
+      // needs fewer registers)
+      constexpr bool DEFER = std::is_same<V, float>::value && MODE == kCfr && !Q;
+      float yv[UB][MAXA];
+      V cv[UB][MAXA];
+      uint32_t wbk[UB];
+      float lmx[UB];
 #pragma unroll
       for (int k = 0; k < UB; k++) {
+        wbk[k] = 0;
+        lmx[k] = 0.0f;
         if (!on[k]) continue;
-        const MemRec& r = *rp[k];
         const uint32_t n = na[k], h = hx[k];
         V sa[MAXA];
         float lmax = 0.0f;
@@ -394,6 +404,8 @@ __global__ void __launch_bounds__(1024) down_kernel(DPlan P, int p, uint32_t seg
                   lmax = fmaxf(lmax, fabsf(y));
                   ybuf[(yo + a) * nq + h] = y;
                 }
+              } else if (DEFER) {
+                yv[k][a] = y;
               } else if (act[k]) {
                 sto.s[off[k] + (size_t)a * nq + h] = y;
               }
@@ -412,18 +424,36 @@ __global__ void __launch_bounds__(1024) down_kernel(DPlan P, int p, uint32_t seg
           for (int a = 0; a < MAXA; a++)
             if (a < (int)n) sa[a] = inv == V(0) ? uni : sv[k][a] * inv;
         }
-        V* crow = reach + (size_t)B.s(r.cslot[p]) * rowlen;
         uint32_t wb = 0;
+        V* crow = reach + (size_t)B.s(rp[k]->cslot[p]) * rowlen;
 #pragma unroll
         for (int a = 0; a < MAXA; a++)
           if (a < (int)n) {
             const V v = sa[a] * rr[k];
-            if (act[k]) crow[(size_t)a * rowlen + h] = v;
+            if (DEFER) cv[k][a] = v;
+            else if (act[k]) crow[(size_t)a * rowlen + h] = v;
             if (__ballot(act[k] && v != V(0))) wb |= 1u << a;
           }
-        if (lane == 0 && wb) atomicOr(&gor[gi][jj[k]], wb);
+        wbk[k] = wb;
+        lmx[k] = lmax;
+      }
+#pragma unroll
+      for (int k = 0; k < UB; k++) {
+        if (!on[k]) continue;
+        const MemRec& r = *rp[k];
+        const uint32_t n = na[k], h = hx[k];
+        if (DEFER && act[k]) {
+          V* crow = reach + (size_t)B.s(r.cslot[p]) * rowlen;
+#pragma unroll
+          for (int a = 0; a < MAXA; a++)
+            if (a < (int)n) {
+              sto.s[off[k] + (size_t)a * nq + h] = yv[k][a];
+              crow[(size_t)a * rowlen + h] = cv[k][a];
+            }
+        }
+        if (lane == 0 && wbk[k]) atomicOr(&gor[gi][jj[k]], wbk[k]);
         if (Q && MODE == kCfr) {
-          const float wm = wave_max(lmax);
+          const float wm = wave_max(lmx[k]);
           if (lane == 0) atomicMax(&gmx[gi][jj[k]], __float_as_int(wm));
         }
       }

@@ -123,9 +123,12 @@ static void emit_kernel(std::ostringstream& s, const Shape& S, size_t idx, int p
     if (S[m].kind >= 2) s << "  const " << T << " " << v(m) << " = " << v(m) << "l;\n";
   // the walk, members in reverse breadth-first order; a dead decision node
   // is 0 and updates nothing
+  std::ostringstream tail;
   for (uint32_t m = M; m-- > 0;) {
     const Member& x = S[m];
     if (x.kind >= 2) continue;
+    if (!best && (int)x.kind == p)
+      for (uint32_t a = 0; a < x.na; a++) s << "  float u" << m << "_" << a << ";\n";
     s << "  " << T << " " << v(m) << ";\n  {\n";
     if ((int)x.kind != p) {  // opponent node: the sum of its children
       s << "    " << T << " t = " << zero << ";\n";
@@ -148,18 +151,21 @@ static void emit_kernel(std::ostringstream& s, const Shape& S, size_t idx, int p
       for (uint32_t a = 0; a < x.na; a++) s << "    sm += " << v(x.cl + a) << ";\n";
       s << "    const float uo = sm * (1.0f / (float)" << (uint32_t)x.na << "u);\n";
       s << "    o = inv == 0.0f ? uo : o;\n";
-      s << "    if (" << live(m) << " && !" << frozen(m) << " && act) {\n";
-      s << "      float* w = A.regrets + " << of(m) << ";\n";
       for (uint32_t a = 0; a < x.na; a++) {
         const std::string r = "r" + std::to_string(m) + "_" + std::to_string(a);
-        s << "      w[(u64)" << a << "u * np + h] = " << r << " * (" << r << " > 0.0f ? pos : neg) + " << v(x.cl + a)
-          << " - o;\n";
+        s << "    u" << m << "_" << a << " = " << r << " * (" << r << " > 0.0f ? pos : neg) + " << v(x.cl + a) << " - o;\n";
       }
-      s << "    }\n";
       s << "    " << v(m) << " = " << live(m) << " ? o : 0.0f;\n";
+      // the updated regrets are stored after the walk: a store followed by
+      // a read of a loaded row would make that read wait for the store
+      tail << "  if (" << live(m) << " && !" << frozen(m) << " && act) {\n";
+      tail << "    float* w = A.regrets + " << of(m) << ";\n";
+      for (uint32_t a = 0; a < x.na; a++) tail << "    w[(u64)" << a << "u * np + h] = u" << m << "_" << a << ";\n";
+      tail << "  }\n";
     }
     s << "  }\n";
   }
+  s << tail.str();
   s << "  if (act) vals[(u64)" << vs(0) << " * A.rowlen + h] = v0;\n";
   s << "}\n";
 }
