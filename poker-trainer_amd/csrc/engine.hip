@@ -2595,11 +2595,11 @@ static void launch_span(const DPlan& d, const Plan& P, hipStream_t st, int p, Sp
               cut(L.data(), (uint32_t)L.size(), i0, i1);
               if (i1 <= i0) continue;
               jit::WalkArgs A{jit->dev[k], d.seg_head, (const unsigned char*)d.mrec, d.malive, (float*)vals, sto.r,
-                              (const float*)prm, jit->zrow, i0, rowlen, B.v0, B.vb};
+                              (const float*)prm, jit->zrow, i0, i1 - i0, tiles, rowlen, B.v0, B.vb};
               void* args[] = {&A};
               PROF_START(kKUp);
-              (void)hipModuleLaunchKernel(jit->fn[4 * k + (UMODE == kBest ? 2 : 0) + p], i1 - i0, tiles, 1, UP_T, 1, 1, 0, st,
-                                          args, nullptr);
+              (void)hipModuleLaunchKernel(jit->fn[4 * k + (UMODE == kBest ? 2 : 0) + p], jit::walk_groups(i1 - i0, tiles), 1, 1,
+                                          UP_T, 1, 1, 0, st, args, nullptr);
               PROF_STOP();
             }
         }

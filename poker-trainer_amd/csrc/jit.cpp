@@ -18,7 +18,7 @@ static_assert(offsetof(MemRec, offset) == 0 && offsetof(MemRec, vslot) == 12 && 
               "the generated walks read MemRec at these byte offsets");
 static_assert(kNodeFrozen == 8, "the generated walks test bit 3 of tflags");
 static_assert(offsetof(IterParams, pos) == 0 && offsetof(IterParams, neg) == 4, "the generated walks read pos, neg");
-static_assert(sizeof(WalkArgs) == 8 * sizeof(void*) + 16, "WalkArgs layout");
+static_assert(sizeof(WalkArgs) == 8 * sizeof(void*) + 24, "WalkArgs layout");
 
 // Declarations every generated kernel uses. The operations follow
 // up_reg_walk in engine.hip: pos0 and recip are its helpers
@@ -38,7 +38,7 @@ struct WalkArgs {
   float* regrets;
   const float* prm;
   const float* zrow;
-  u32 i0, rowlen, v0, vb;
+  u32 i0, nseg, tiles, rowlen, v0, vb;
 };
 __device__ __forceinline__ float pos0(float x) { return x > 0.0f ? x : 0.0f; }
 __device__ __forceinline__ float recip(float d) {
@@ -77,12 +77,19 @@ static void emit_kernel(std::ostringstream& s, const Shape& S, size_t idx, int p
   };
   auto v = [&](uint32_t m) { return "v" + std::to_string(m); };
   s << "extern \"C\" __global__ void __launch_bounds__(64, 4) " << kernel_name(idx, p, best) << "(WalkArgs A) {\n";
-  s << "  const u32 seg = A.list[A.i0 + blockIdx.x];\n";
+  // workgroup L runs on XCD L % 8; its slot L / 8 there names (segment,
+  // hand tile) with a segment's tiles consecutive, so one XCD walks all of a
+  // segment's tiles and its member records come from that XCD's L2
+  s << "  const u32 xcd = blockIdx.x & 7u, slot = blockIdx.x >> 3;\n";
+  s << "  const u32 sl = slot / A.tiles, tile = slot - sl * A.tiles;\n";
+  s << "  const u32 si = sl * 8u + xcd;\n";
+  s << "  if (si >= A.nseg) return;\n";
+  s << "  const u32 seg = A.list[A.i0 + si];\n";
   s << "  const H4 head = A.seg_head[seg];\n";
   s << "  const u32 moff = head.x;\n";
   s << "  const u32 np = " << (p ? "head.w >> 16" : "head.w & 0xFFFFu") << ";\n";
   s << "  const u32 lane = threadIdx.x;\n";
-  s << "  const u32 h = blockIdx.y * 64u + lane;\n";
+  s << "  const u32 h = tile * 64u + lane;\n";
   s << "  const bool act = h < np;\n";
   s << "  const u32 hh = act ? h : 0u;\n";
   s << "  " << T << "* const vals = (" << T << "*)A.vals;\n";

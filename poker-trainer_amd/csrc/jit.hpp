@@ -37,7 +37,7 @@ using Shape = std::vector<Member>;
 
 // The kernels' one argument (the generated source declares the same layout).
 struct WalkArgs {
-  const uint32_t* list;        // segment ids, walked list[i0 + blockIdx.x]
+  const uint32_t* list;        // segment ids list[i0 .. i0 + nseg)
   const uint4* seg_head;       // per segment: first member, member count, root node, hands of both players
   const unsigned char* mrec;   // MemRec array (48 B each)
   const uint8_t* malive;       // per member: alive for the current traverser
@@ -45,11 +45,17 @@ struct WalkArgs {
   float* regrets;              // f32 regret storage (DCFR walk)
   const float* prm;            // IterParams: pos, neg (DCFR walk)
   const float* zrow;           // 8 * rowlen f32 zeros: what a dead member reads
-  uint32_t i0, rowlen, v0, vb; // list offset, row length, value-row bias (Bias::v)
+  uint32_t i0, nseg, tiles;    // the segments walked; 64-hand tiles per segment
+  uint32_t rowlen, v0, vb;     // row length, value-row bias (Bias::v)
 };
 
 // The most members a specialised walk takes.
 constexpr uint32_t kMaxMembers = 80;
+
+// Launch grid of a walk over n segments of t hand tiles: one 64-thread
+// workgroup per (segment, tile), dealt to the 8 XCDs in turn, padded to a
+// multiple of 8 segments.
+inline uint32_t walk_groups(uint32_t n, uint32_t t) { return (n + 7u) / 8u * 8u * t; }
 
 // Compiles the walks of `shapes` for both traversers on the current device
 // (or takes them from this process's cache): fn[4 * i + p] is shape i's f32
