@@ -343,7 +343,7 @@ pts_status pts_batch_solve(pts_batch* b, uint32_t max_iterations, double target_
                            pts_progress_c* out);
 /* Autotuner: times `iterations` iterations for every tile choice (terminals
  * per terminal workgroup 1..16, opponent hands per down workgroup
- * 256/512/1024), keeps the fastest (best[0], best[1], best_ms per
+ * 128/256/512/1024), keeps the fastest (best[0], best[1], best_ms per
  * iteration) and restarts the solve from zero storage. */
 pts_status pts_batch_autotune(pts_batch* b, uint32_t iterations, uint32_t best[2], double* best_ms);
 pts_status pts_batch_set_tiles(pts_batch* b, uint32_t run_length, uint32_t down_threads);

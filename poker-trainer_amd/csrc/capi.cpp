@@ -1048,7 +1048,7 @@ pts_status pts_batch_solve(pts_batch* b, uint32_t max_iterations, double target_
 pts_status pts_batch_autotune(pts_batch* b, uint32_t iterations, uint32_t best[2], double* best_ms) {
   NEED_BATCH_ENGINE(b);
   static const uint32_t runs[] = {1, 2, 4, 8, 16};
-  static const uint32_t downs[] = {256, 512, 1024};
+  static const uint32_t downs[] = {128, 256, 512, 1024};
   double bm = std::numeric_limits<double>::infinity();
   uint32_t br = b->engine->run_length(), bd = b->engine->down_threads();
   const uint32_t it = std::max(iterations, 1u);

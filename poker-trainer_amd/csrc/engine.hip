@@ -2242,6 +2242,16 @@ Status Engine::create(int device, Plan&& plan, const float* regrets, const float
   d.spots = AT(SpotTab, o_spots);
   e->tbatch_dev_ = (TermBatch*)(base + o_tb);
   e->run_length_ = default_run_length(P);
+  // down workgroups: when a depth has many segments (the flop's river level
+  // has 670,320), workgroups of 128 threads keep more segments in flight per
+  // CU (flop down pass 67 -> 55 ms per iteration against 512); a few hundred
+  // segments (a turn's river level) run best at 512 (the turn: 0.38 against
+  // 0.48 ms per iteration at 128)
+  {
+    uint32_t widest = 0;
+    for (size_t dd = 0; dd + 1 < P.depth_off.size(); dd++) widest = std::max(widest, P.depth_off[dd + 1] - P.depth_off[dd]);
+    e->down_threads_ = widest >= 16384 ? 128u : 512u;
+  }
   d.active = AT(uint32_t, o_act);
   e->active_ = (uint32_t*)(base + o_act);
   e->dp_->big = std::move(big);
@@ -2934,7 +2944,7 @@ Status Engine::root_values_all(Mode mode, std::vector<double>& res) {
 }
 
 Status Engine::retune(uint32_t run_length, uint32_t down_threads) {
-  if (down_threads != 256 && down_threads != 512 && down_threads != 1024) {
+  if (down_threads != 128 && down_threads != 256 && down_threads != 512 && down_threads != 1024) {
     set_error("invalid input: down-kernel workgroup size");
     return kInvalidInput;
   }

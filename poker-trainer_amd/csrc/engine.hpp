@@ -100,7 +100,7 @@ class Engine {
   // updated by CFR traversals (its strategy stays what its regrets give).
   Status set_frozen(const std::vector<uint32_t>& nodes, bool on);
   // Tiles: terminals per fast-terminal workgroup (run length, 1..16) and
-  // opponent hands per down-kernel workgroup (256, 512 or 1024); rebuilds
+  // opponent hands per down-kernel workgroup (128, 256, 512 or 1024); rebuilds
   // the graphs. Same scratch groups (their rows do not depend on the runs).
   Status retune(uint32_t run_length, uint32_t down_threads);
   uint32_t run_length() const { return run_length_; }

@@ -115,7 +115,7 @@ def test_autotune_keeps_the_arithmetic():
     cfgs = small_batch()
     a, b = BatchSolver(cfgs, precision=FAST), BatchSolver(cfgs, precision=FAST)
     (rl, dt), ms = b.autotune(2)
-    assert rl in (1, 2, 4, 8, 16) and dt in (256, 512, 1024) and ms > 0
+    assert rl in (1, 2, 4, 8, 16) and dt in (128, 256, 512, 1024) and ms > 0
     assert b.iterations == 0
     a.iterate(6)
     b.iterate(6)
@@ -123,3 +123,23 @@ def test_autotune_keeps_the_arithmetic():
         ra, sa = a.storage(k)
         rb, sb = b.storage(k)
         assert np.array_equal(ra, rb) and np.array_equal(sa, sb), k
+
+
+@pytest.mark.gpu
+def test_down_workgroup_sizes_keep_the_exact_bits():
+    # the down pass's workgroup size (128 threads for levels of many
+    # segments, such as the flop's river level) changes only how the
+    # (node, hand tile) units are dealt to waves
+    cfgs = small_batch()
+    ref = None
+    for dt in (128, 256, 512, 1024):
+        b = BatchSolver(cfgs, precision=EXACT)
+        b.set_tiles(4, dt)
+        b.iterate(5)
+        got = [b.storage(k) for k in range(len(cfgs))]
+        if ref is None:
+            ref = got
+        else:
+            for k, ((ra, sa), (rb, sb)) in enumerate(zip(ref, got)):
+                assert np.array_equal(ra, rb) and np.array_equal(sa, sb), (dt, k)
+        del b
