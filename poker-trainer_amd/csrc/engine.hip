@@ -2555,7 +2555,8 @@ static void launch_span(const DPlan& d, const Plan& P, hipStream_t st, int p, Sp
         uint32_t i0, i1, j0, j1;
         cut(d.m48_host, d.num_m48, i0, i1);
         cut(d.w48_host, d.num_w48, j0, j1);
-        const bool wide = (uint64_t)(i1 - i0) * 4 > (uint64_t)(j1 - j0);
+        // (f64 walks keep the two kernels: the dual kernel's f64 form spills)
+        const bool wide = sizeof(V) == 4 && (uint64_t)(i1 - i0) * 4 > (uint64_t)(j1 - j0);
         if (wide && j1 > j0) {
           PROF_START(kKUp);
           hipLaunchKernelGGL((up_reg_dual_kernel<V, UMODE, QP>), dim3(j1 - j0, tiles), dim3(UP_T), 0, st, d, p, j0, span, B,
