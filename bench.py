@@ -14,7 +14,10 @@ launch (down: 12 B per storage cell of the opponent's nodes; up: 8 B per cell
 of the traverser's nodes; terminal: 4 B per opponent hand per terminal) over
 its mean launch duration, both measured with HIP events on the engine's
 stream; `traffic` is the class's HBM bytes per launch as rocprofv3 counted
-them (profiles/traffic.json).
+them (profiles/traffic.json). `roofline.iteration` and `roofline.solve` give
+the whole path's algorithmic bytes (20 B per storage cell plus 4 B per
+opponent hand per terminal per iteration; 4 B per cell plus the terminal term
+per FP64 check) over the timed iteration and over the solve.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
                        [--workload flop|turn|river|batch|valuenet|river-auto] [--precision fast|exact]
@@ -255,6 +258,19 @@ def main():
                 "launches_per_iteration": launches[dom], "avg_launch_ms": round(ms[dom] / max(launches[dom], 1), 5),
                 "algorithmic_bytes_per_launch": round(bytes_cls[dom] / max(launches[dom], 1)),
                 "iteration_ms_by_class": {k: round(v, 4) for k, v in ms.items()}}
+    # the whole path's algorithmic bytes: 20 B per cell (10 with 16-bit
+    # storage) + 4 B per opponent hand per terminal per iteration; 4 B (2) per
+    # cell + the same terminal term per FP64 check
+    term_b = 4.0 * st.terminal_nodes * (n0 + n1)
+    b_iter = 5 * cb * cells + term_b
+    b_br = cb * cells + term_b
+    solve_b = b_iter * pr.iteration + b_br * len(hist)
+    roofline["iteration"] = {"algorithmic_bytes": round(b_iter), "ms": round(step_s * 1e3, 4),
+                             "achieved": round(b_iter / step_s / 1e9, 2),
+                             "frac": round(b_iter / step_s / 1e9 / HBM_PEAK_GBS, 5)}
+    roofline["solve"] = {"algorithmic_bytes": round(solve_b), "s": round(solve_s, 4),
+                         "achieved": round(solve_b / solve_s / 1e9, 2),
+                         "frac": round(solve_b / solve_s / 1e9 / HBM_PEAK_GBS, 5)}
 
     reached = pr.exploitability_pct <= args.target
     cpu = None
