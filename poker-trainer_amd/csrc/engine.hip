@@ -1983,7 +1983,15 @@ Status Engine::setup_jit(bool q16) {
     jit_segments_ += J.lists.back().size();
   }
   Status st = jit::compile_walks(shapes, J.fn, &jit_seconds_);
-  if (st != kOk) return st;
+  if (st != kOk) {
+    // the generic walks serve every segment, with the same results
+    std::fprintf(stderr, "[pts] specialised walks unavailable, the generic walks run instead: %s\n", last_error().c_str());
+    J = JitLaunch();
+    jit_segments_ = 0;
+    jit_seconds_ = 0;
+    set_error("");
+    return kOk;
+  }
   auto rest = [&](const std::vector<uint32_t>& l, std::vector<uint32_t>& out) {
     for (uint32_t sg : l)
       if (!taken[sg]) out.push_back(sg);
