@@ -157,9 +157,11 @@ uint64_t pts_engine_device_bytes(pts_engine* e);
 /* Chunk groups one traversal runs in (f64 = 0: DCFR rows, 1: FP64 best-response rows). */
 uint32_t pts_engine_num_groups(pts_engine* e, int f64);
 /* Segments whose DCFR up walk runs in a kernel specialised to the segment's
- * shape, compiled at creation for the shapes of at least PTS_JIT_MIN_SEGMENTS
- * segments (default 1024; PTS_JIT=0 turns them off). No reference
- * counterpart: the reference's walk is per level (gpu/engine.rs:600-749). */
+ * shape, compiled at creation: in a plan with at least 8192 segments that
+ * have a walk, for every shape of at least 64 of them (none in a smaller
+ * plan). PTS_JIT_MIN_SEGMENTS=n sets the per-shape count and drops the
+ * plan-size rule; PTS_JIT=0 turns them off. No reference counterpart: the
+ * reference's walk is per level (gpu/engine.rs:600-749). */
 uint64_t pts_engine_jit_segments(pts_engine* e);
 void pts_engine_destroy(pts_engine* e);
 

@@ -1937,21 +1937,29 @@ Store Engine::store() const { return Store{regrets_, sums_, rq_, sq_, rsc_, ssc_
 // Specialised walks (jit.hpp) for the frequent segment shapes, f32 storage
 // only: the f32 DCFR walk and the FP64 best response. A shape is every
 // member's kind (0 or 1 for a decision node, 2 for a member read from its
-// row), action count and first child. Shapes of at least
-// PTS_JIT_MIN_SEGMENTS segments (default 1024) get their own kernels
-// (PTS_JIT=0: none); a spot's root segment, which also writes the spot's
-// result row, keeps the generic walk. The other segments, the average-
-// strategy walks and queries use the generic kernels.
+// row), action count and first child. A plan with at least kJitPlanSegments
+// segments that have a walk compiles every shape of at least kJitShapeSegments
+// of them (a smaller plan compiles none: its solve would not repay the
+// compilation); PTS_JIT_MIN_SEGMENTS=n sets the per-shape count and drops the
+// plan-size rule, PTS_JIT=0 turns them off. A spot's root segment, which also
+// writes the spot's result row, keeps the generic walk. The other segments,
+// the average-strategy walks and queries use the generic kernels.
+constexpr uint32_t kJitPlanSegments = 8192, kJitShapeSegments = 64;
 Status Engine::setup_jit(bool q16) {
   DPlanHolder& H = *dp_;
   const Plan& P = plan_;
   H.dcfr = H.d;
   const char* jo = std::getenv("PTS_JIT");
   if (q16 || (jo && jo[0] == '0')) return kOk;
-  uint32_t min_segs = 1024;
-  if (const char* m = std::getenv("PTS_JIT_MIN_SEGMENTS")) min_segs = (uint32_t)std::max(1, std::atoi(m));
+  uint32_t min_segs = kJitShapeSegments;
+  bool forced = false;
+  if (const char* m = std::getenv("PTS_JIT_MIN_SEGMENTS")) {
+    min_segs = (uint32_t)std::max(1, std::atoi(m));
+    forced = true;
+  }
   std::map<std::string, std::vector<uint32_t>> by_shape;
   std::map<std::string, jit::Shape> shape_of;
+  size_t walk_segs = 0;
   for (uint32_t sg = 0; sg < (uint32_t)P.seg_mem_cnt.size(); sg++) {
     const uint32_t mc = P.seg_mem_cnt[sg];
     if (mc < 2 || mc > jit::kMaxMembers || P.seg_pslot[sg] == kNone) continue;
@@ -1966,7 +1974,9 @@ Status Engine::setup_jit(bool q16) {
     auto& v = by_shape[key];
     if (v.empty()) shape_of[key] = S;
     v.push_back(sg);
+    walk_segs++;
   }
+  if (!forced && walk_segs < kJitPlanSegments) return kOk;
   std::vector<std::pair<size_t, std::string>> pick;
   for (const auto& kv : by_shape)
     if (kv.second.size() >= min_segs) pick.push_back({kv.second.size(), kv.first});

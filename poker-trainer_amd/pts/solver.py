@@ -105,8 +105,9 @@ class Solver:
 
     def jit_segments(self) -> int:
         """Segments whose DCFR up walk runs in a shape-specialised kernel
-        (compiled at creation for shapes of at least PTS_JIT_MIN_SEGMENTS
-        segments; PTS_JIT=0 turns them off)."""
+        (compiled at creation, in a plan of at least 8192 walk segments, for
+        shapes of at least 64 of them; PTS_JIT_MIN_SEGMENTS=n sets the
+        per-shape count for any plan; PTS_JIT=0 turns them off)."""
         return int(lib().pts_solver_jit_segments(self.h))
 
     @property

@@ -76,3 +76,13 @@ def test_specialised_walks_equal_the_generic_walk(monkeypatch):
     assert same(sa, sb), describe(sa, sb)
     assert a.exploitability() == b.exploitability()
 
+
+
+def test_small_plans_keep_the_generic_walks(monkeypatch):
+    # the turn spot has 721 segments with a walk, under the 8192 a plan needs
+    # before its shapes are compiled; forcing a per-shape count compiles them
+    monkeypatch.delenv("PTS_JIT", raising=False)
+    monkeypatch.delenv("PTS_JIT_MIN_SEGMENTS", raising=False)
+    assert Solver(spots.turn_spot(0), precision=FAST).jit_segments() == 0
+    monkeypatch.setenv("PTS_JIT_MIN_SEGMENTS", "64")
+    assert Solver(spots.turn_spot(0), precision=FAST).jit_segments() > 0
