@@ -79,6 +79,7 @@ struct SpotTab {
 struct DPlan {
   uint32_t num_nodes, num_segs, num_spots;
   uint32_t max_entries, max_members;
+  uint32_t max_dec;  // the most decision members in a segment
   const int8_t* card;
   const uint32_t *seg_mem_off, *seg_mem_cnt, *seg_pslot;
   const float* seg_factor;
@@ -1543,34 +1544,40 @@ __global__ void __launch_bounds__(UP_T) up_kernel(DPlan P, int p, uint32_t seg0,
   const uint32_t h = blockIdx.y * UP_T + tid;
   const uint32_t root = P.mrec[moff].node;
   if (!span.has(root) && seg != span.seg) return;
+  // LDS: the records, their alive flags, each decision member's value slot,
+  // and one value per hand for each decision member (a terminal or chance
+  // member is read from its row when its parent is walked, so the LDS per
+  // wave scales with the decision members only)
   MemRec* rec = (MemRec*)smem_up;
   uint8_t* live = (uint8_t*)(rec + P.max_members);
-  V* lv = (V*)(live + ((P.max_members + 15) & ~15u));
+  uint16_t* dslot = (uint16_t*)(live + ((P.max_members + 15) & ~15u));
+  V* lv = (V*)(dslot + ((P.max_members + 7) & ~7u));
   stage_records(P.mrec + moff, mcnt, rec);
   __syncthreads();
   for (uint32_t m = tid; m < mcnt; m += UP_T) live[m] = alive[B.s(rec[m].slot[p])] != 0;
+  {
+    uint32_t base = 0;
+    for (uint32_t m0 = 0; m0 < mcnt; m0 += UP_T) {
+      const uint32_t m = m0 + tid;
+      const bool dec = m < mcnt && rec[m].kind < kChance;
+      const uint64_t bal = __ballot(dec);
+      if (dec) dslot[m] = (uint16_t)(base + __popcll(bal & ((1ull << tid) - 1ull)));
+      base += __popcll(bal);
+    }
+  }
   __syncthreads();
   // the max pass keeps every lane for its wave reductions: lanes past the
   // hands read hand 0 and store nothing
   const bool act = h < np;
   if (QP != 1 && !act) return;
   const uint32_t hc = act ? h : 0;
-  // leaves first: terminal and chance values, 8 rows in flight per lane
-  for (uint32_t m0 = 0; m0 < mcnt; m0 += 8) {
-    V x[8];
-#pragma unroll
-    for (int k = 0; k < 8; k++) {
-      const uint32_t m = m0 + k;
-      x[k] = V(0);
-      if (m < mcnt && rec[m].kind >= kChance && live[m] && span.has(rec[m].node))
-        x[k] = vals[(size_t)B.v(rec[m].vslot) * rowlen + hc];
-    }
-#pragma unroll
-    for (int k = 0; k < 8; k++) {
-      const uint32_t m = m0 + k;
-      if (m < mcnt && rec[m].kind >= kChance) lv[(size_t)m * UP_T + tid] = x[k];
-    }
-  }
+  // member c's value: a decision member's from LDS; a terminal or chance
+  // member's from its row (0 when dead or outside the span)
+  auto value = [&](uint32_t c) -> V {
+    const MemRec& q = rec[c];
+    if (q.kind >= kChance) return (live[c] && span.has(q.node)) ? vals[(size_t)B.v(q.vslot) * rowlen + hc] : V(0);
+    return lv[(size_t)dslot[c] * UP_T + tid];
+  };
   float pos = 0.0f, neg = 0.0f;
   if (MODE == kCfr) {
     pos = prm->pos;
@@ -1583,17 +1590,17 @@ __global__ void __launch_bounds__(UP_T) up_kernel(DPlan P, int p, uint32_t seg0,
     const uint32_t na = r.na, cl = r.clocal;
     if (QP != 1 && qout != nullptr && r.node == span.root) {
       // query: the values of the decision node's children (lib.rs:854-863)
-      for (uint32_t a = 0; a < na; a++) qout[(size_t)a * np + h] = lv[(size_t)(cl + a) * UP_T + tid];
+      for (uint32_t a = 0; a < na; a++) qout[(size_t)a * np + h] = value(cl + a);
       return;
     }
-    if (k >= kChance) continue;  // prefetched
+    if (k >= kChance) continue;  // read by its parent
     V v = V(0);
     if (live[m]) {
       if ((int)k == p) {
         V cv[MAXA];
 #pragma unroll
         for (int a = 0; a < MAXA; a++)
-          if (a < (int)na) cv[a] = lv[(size_t)(cl + a) * UP_T + tid];
+          if (a < (int)na) cv[a] = value(cl + a);
         const uint64_t off = r.offset;
         if (MODE == kCfr) {
           float rg[MAXA];
@@ -1671,15 +1678,22 @@ __global__ void __launch_bounds__(UP_T) up_kernel(DPlan P, int p, uint32_t seg0,
           v = acc;
         }
       } else {
-        for (uint32_t a = 0; a < na; a++) v += lv[(size_t)(cl + a) * UP_T + tid];
+        V cv[MAXA];
+#pragma unroll
+        for (int a = 0; a < MAXA; a++)
+          if (a < (int)na) cv[a] = value(cl + a);
+#pragma unroll
+        for (int a = 0; a < MAXA; a++)
+          if (a < (int)na) v += cv[a];
       }
     }
-    lv[(size_t)m * UP_T + tid] = v;
+    lv[(size_t)dslot[m] * UP_T + tid] = v;
   }
   if (act && QP != 2) {
-    vals[(size_t)B.v(rec[0].vslot) * rowlen + h] = lv[tid];
+    const V v0 = value(0);
+    vals[(size_t)B.v(rec[0].vslot) * rowlen + h] = v0;
     if (span.init && P.seg_pslot[(size_t)p * P.num_segs + seg] == NONE)  // a spot's root: its result row
-      ((V*)span.res)[((size_t)rec[0].spot * 2 + p) * rowlen + h] = lv[tid];
+      ((V*)span.res)[((size_t)rec[0].spot * 2 + p) * rowlen + h] = v0;
   }
 }
 
@@ -2252,6 +2266,15 @@ Status Engine::create(int device, Plan&& plan, const float* regrets, const float
   d.num_spots = K;
   d.max_entries = std::max(P.max_entries, 1u);
   d.max_members = std::max(P.max_members, 1u);
+  {
+    uint32_t md = 1;
+    for (size_t g = 0; g < P.seg_mem_cnt.size(); g++) {
+      uint32_t c = 0;
+      for (uint32_t m = 0; m < P.seg_mem_cnt[g]; m++) c += is_player(P.mrec[P.seg_mem_off[g] + m].kind) ? 1u : 0u;
+      md = std::max(md, c);
+    }
+    d.max_dec = md;
+  }
   d.card = AT(int8_t, o_card);
   d.seg_mem_off = AT(uint32_t, o_smo); d.seg_mem_cnt = AT(uint32_t, o_smc); d.seg_pslot = AT(uint32_t, o_sps);
   d.seg_factor = AT(float, o_sf); d.mrec = AT(MemRec, o_mr); d.trec = AT(TermRec, o_tr);
@@ -2359,7 +2382,9 @@ Status Engine::create(int device, Plan&& plan, const float* regrets, const float
     size_t vs = f64 ? 8 : 4;
     // the fast terminal kernel's LDS is static (TermLds)
     e->lds_term_[f64] = precision == kExact ? (size_t)rowlen * 3 * 8 + (size_t)rowlen * vs : 0;
-    e->lds_up_[f64] = mm * sizeof(MemRec) + ((mm + 15) & ~size_t(15)) + mm * UP_T * vs;
+    // records, alive flags, value slots, and one value row per decision member
+    e->lds_up_[f64] = mm * sizeof(MemRec) + ((mm + 15) & ~size_t(15)) + ((mm + 7) & ~size_t(7)) * 2 +
+                      (size_t)d.max_dec * UP_T * vs;
   }
   size_t lds_max = std::max({e->lds_term_[1], e->lds_up_[1], e->lds_down_});
   if (lds_max > 159 * 1024) {  // 160 KiB per workgroup, less the kernels' static arrays
