@@ -1047,7 +1047,7 @@ pts_status pts_batch_solve(pts_batch* b, uint32_t max_iterations, double target_
 // workgroup size, keeps the fastest, then restarts the solve from zero.
 pts_status pts_batch_autotune(pts_batch* b, uint32_t iterations, uint32_t best[2], double* best_ms) {
   NEED_BATCH_ENGINE(b);
-  static const uint32_t runs[] = {1, 2, 4, 8, 16};
+  static const uint32_t runs[] = {1, 2, 4, 8, 16, 32, 64, 256};
   static const uint32_t downs[] = {128, 256, 512, 1024};
   double bm = std::numeric_limits<double>::infinity();
   uint32_t br = b->engine->run_length(), bd = b->engine->down_threads();

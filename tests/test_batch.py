@@ -115,7 +115,7 @@ def test_autotune_keeps_the_arithmetic():
     cfgs = small_batch()
     a, b = BatchSolver(cfgs, precision=FAST), BatchSolver(cfgs, precision=FAST)
     (rl, dt), ms = b.autotune(2)
-    assert rl in (1, 2, 4, 8, 16) and dt in (128, 256, 512, 1024) and ms > 0
+    assert rl in (1, 2, 4, 8, 16, 32, 64, 256) and dt in (128, 256, 512, 1024) and ms > 0
     assert b.iterations == 0
     a.iterate(6)
     b.iterate(6)
@@ -142,4 +142,24 @@ def test_down_workgroup_sizes_keep_the_exact_bits():
         else:
             for k, ((ra, sa), (rb, sb)) in enumerate(zip(ref, got)):
                 assert np.array_equal(ra, rb) and np.array_equal(sa, sb), (dt, k)
+        del b
+
+
+@pytest.mark.gpu
+def test_terminal_run_lengths_keep_the_fast_bits():
+    # a run only decides which terminals one workgroup sums in sequence (and
+    # loads a board's tables for): the fast values are the same bits for runs
+    # of 1, 16 and 256 terminals
+    cfgs = small_batch()
+    ref = None
+    for rl in (1, 16, 256):
+        b = BatchSolver(cfgs, precision=FAST)
+        b.set_tiles(rl, 256)
+        b.iterate(6)
+        got = [b.storage(k) for k in range(len(cfgs))]
+        if ref is None:
+            ref = got
+        else:
+            for k, ((ra, sa), (rb, sb)) in enumerate(zip(ref, got)):
+                assert np.array_equal(ra, rb) and np.array_equal(sa, sb), (rl, k)
         del b
