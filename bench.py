@@ -349,7 +349,8 @@ def bench_batch(args):
     dom = max(("down", "terminal", "up"), key=lambda k: ms[k])
     achieved = bytes_cls[dom] / (ms[dom] * 1e-3) / 1e9 if ms[dom] > 0 else 0.0
     roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None, "kernel": dom,
+                "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": measured_traffic("batch", args.precision, dom),
+                "kernel": dom,
                 "launches_per_iteration": launches[dom], "avg_launch_ms": round(ms[dom] / max(launches[dom], 1), 5),
                 "algorithmic_bytes_per_launch": round(bytes_cls[dom] / max(launches[dom], 1)),
                 "iteration_ms_by_class": {k: round(v, 4) for k, v in ms.items()}}
