@@ -111,6 +111,10 @@ int32_t orc_solver_is_compressed(void* s);
 /* rand 0.9.5's StdRng pieces and sampling.rs's card sample (test hooks) */
 void orc_chacha_block(int32_t rounds, const uint32_t* in, uint32_t* out);
 void orc_stdrng_u64(uint64_t seed, uint64_t n, uint64_t* out);
+/* PCG32 words as rand_core's seed_from_u64 draws them, from a state and increment. */
+void orc_pcg32_words(uint64_t state, uint64_t inc, uint64_t n, uint32_t* out);
+/* StdRng's stream from a key and round count: n next_u32 words, n / 2 next_u64 draws. */
+void orc_stream_words(const uint32_t* key, int32_t rounds, uint64_t n, uint32_t* out, uint64_t* out64);
 int32_t orc_sample_cards(const int32_t* available, int32_t n, int32_t k, uint64_t seed, int32_t* out);
 /* one block through the 16-bit codec from zero storage (rq: regrets, sq: sums) */
 void orc_encode_block(int32_t sums, const float* v, uint64_t n, int16_t* rq, uint16_t* sq, float* scale);

@@ -303,6 +303,14 @@ void orc_stdrng_u64(uint64_t seed, uint64_t n, uint64_t* out) {
   StdRng r(seed);
   for (uint64_t i = 0; i < n; i++) out[i] = r.next_u64();
 }
+void orc_pcg32_words(uint64_t state, uint64_t inc, uint64_t n, uint32_t* out) {
+  for (uint64_t i = 0; i < n; i++) out[i] = pcg32_next(state, inc);
+}
+void orc_stream_words(const uint32_t* key, int32_t rounds, uint64_t n, uint32_t* out, uint64_t* out64) {
+  StdRng a(key, rounds), b(key, rounds);
+  for (uint64_t i = 0; i < n; i++) out[i] = a.next_u32();
+  for (uint64_t i = 0; i < n / 2; i++) out64[i] = b.next_u64();
+}
 int32_t orc_sample_cards(const int32_t* available, int32_t n, int32_t k, uint64_t seed, int32_t* out) {
   StdRng r(seed);
   std::vector<int> av(available, available + n);

@@ -517,20 +517,22 @@ void chacha_block(int rounds, const uint32_t in[16], uint32_t out[16]) {
   for (int i = 0; i < 16; i++) out[i] = x[i] + in[i];
 }
 
-StdRng::StdRng(uint64_t state) {
-  const uint64_t MUL = 6364136223846793005ull, INC = 11634580027462260723ull;
-  for (int i = 0; i < 8; i++) {
-    state = state * MUL + INC;
-    const uint32_t xs = (uint32_t)(((state >> 18) ^ state) >> 27);
-    const uint32_t rot = (uint32_t)(state >> 59);
-    key[i] = (xs >> rot) | (xs << ((32 - rot) & 31));
-  }
+uint32_t pcg32_next(uint64_t& state, uint64_t inc) {
+  state = state * 6364136223846793005ull + inc;
+  const uint32_t xs = (uint32_t)(((state >> 18) ^ state) >> 27);
+  const uint32_t rot = (uint32_t)(state >> 59);
+  return (xs >> rot) | (xs << ((32 - rot) & 31));
 }
+StdRng::StdRng(uint64_t state) {
+  const uint64_t INC = 11634580027462260723ull;
+  for (int i = 0; i < 8; i++) key[i] = pcg32_next(state, INC);
+}
+StdRng::StdRng(const uint32_t k[8], int r) : rounds(r) { std::memcpy(key, k, sizeof key); }
 void StdRng::refill() {
   for (int b = 0; b < 4; b++) {
     const uint32_t in[16] = {0x61707865u, 0x3320646eu, 0x79622d32u, 0x6b206574u, key[0], key[1], key[2], key[3],
                              key[4], key[5], key[6], key[7], (uint32_t)counter, (uint32_t)(counter >> 32), 0u, 0u};
-    chacha_block(12, in, buf + 16 * b);
+    chacha_block(rounds, in, buf + 16 * b);
     counter++;
   }
 }

@@ -296,12 +296,19 @@ struct Solver {
 // rand 0.9.5's StdRng (ChaCha12Rng seeded by seed_from_u64) and the
 // reference's stratified card sample (sampling.rs)
 void chacha_block(int rounds, const uint32_t in[16], uint32_t out[16]);
+// one PCG32 word as rand_core's seed_from_u64 draws it: the state advanced
+// (state * 6364136223846793005 + inc), then the XSH RR output of the new state
+uint32_t pcg32_next(uint64_t& state, uint64_t inc);
 struct StdRng {
   uint32_t key[8];
   uint64_t counter = 0;
   uint32_t buf[64];
   size_t idx = 64;
+  int rounds = 12;
   explicit StdRng(uint64_t seed);
+  // the same stream from a given key and round count (ChaCha20 for the
+  // published keystream vectors)
+  StdRng(const uint32_t k[8], int rounds);
   void refill();
   uint32_t next_u32();
   uint64_t next_u64();

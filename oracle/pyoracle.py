@@ -103,6 +103,8 @@ def _declare(L):
     u32p, u64p, i32p = (np.ctypeslib.ndpointer(dtype=t) for t in (np.uint32, np.uint64, np.int32))
     L.orc_chacha_block.argtypes = [ctypes.c_int32, u32p, u32p]
     L.orc_stdrng_u64.argtypes = [ctypes.c_uint64, ctypes.c_uint64, u64p]
+    L.orc_pcg32_words.argtypes = [ctypes.c_uint64, ctypes.c_uint64, ctypes.c_uint64, u32p]
+    L.orc_stream_words.argtypes = [u32p, ctypes.c_int32, ctypes.c_uint64, u32p, u64p]
     L.orc_sample_cards.restype = ctypes.c_int32
     L.orc_sample_cards.argtypes = [i32p, ctypes.c_int32, ctypes.c_int32, ctypes.c_uint64, i32p]
     L.orc_solver_iterations.restype = ctypes.c_uint32
@@ -386,6 +388,23 @@ def stdrng_u64(seed, n):
     out = np.zeros(n, np.uint64)
     lib().orc_stdrng_u64(seed, n, out)
     return out
+
+
+def pcg32_words(state, inc, n):
+    """n PCG32 words as seed_from_u64 draws them (state advanced, then the
+    XSH RR output of the new state), from the given state and increment."""
+    out = np.zeros(n, np.uint32)
+    lib().orc_pcg32_words(state & (2**64 - 1), inc & (2**64 - 1), n, out)
+    return out
+
+
+def stream_words(key, rounds, n):
+    """StdRng's stream from a given 8-word key and round count: the first n
+    next_u32 words, and the first n // 2 next_u64 draws of a fresh copy."""
+    w = np.zeros(n, np.uint32)
+    w64 = np.zeros(max(n // 2, 1), np.uint64)
+    lib().orc_stream_words(np.ascontiguousarray(key, np.uint32), rounds, n, w, w64)
+    return w, w64[:n // 2]
 
 
 def sample_cards(available, k, seed):

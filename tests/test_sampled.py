@@ -1,11 +1,16 @@
 """The chance-sampled mode (crates/solver/src/sampling.rs, tree.rs:128-136,
 314-334). Its cards come from rand 0.9.5's StdRng (rand_chacha 0.9.0's
 ChaCha12Rng, seeded by rand_core 0.9.5's seed_from_u64), a crate absent from
-the reference tree; the oracle restates its published algorithm. Pinned
-here: the ChaCha block function by RFC 8439's known answer, the sample by
-the reference's own test and by the stratification it promises. Which cards
-a given seed draws (the PCG32 seeding and the word order of rand's stream)
-has no known answer here: parity unpinned beyond product == oracle."""
+the reference tree; the oracle restates its published algorithm, and the
+product's is checked against the oracle's. Pinned here by published known
+answers: the ChaCha block function (RFC 8439 section 2.3.2), the stream's
+layout (key, 64-bit block counter from 0, block and word order, next_u64 low
+word first) by the keystream of RFC 8439 appendix A.1 test vectors 1 and 2
+run through the oracle's StdRng with 20 rounds, and the seed expansion's
+PCG32 step and XSH RR output by the PCG reference generator's first outputs
+(seed 42, stream 54). The 12-round count and the increment
+11634580027462260723 are the crates' constants; the sample is pinned by the
+reference's own test and by the stratification it promises."""
 import numpy as np
 import pytest
 
@@ -25,6 +30,30 @@ def test_chacha20_block_rfc8439():
             0x466482d2, 0x09aa9f07, 0x05d7c214, 0xa2028bd9,
             0xd19c12b5, 0xb94e16de, 0xe883d0cb, 0x4e3c50a2]
     assert O.chacha_block(state, 20).tolist() == want
+
+
+def test_stream_layout_by_the_rfc8439_keystream():
+    # RFC 8439 appendix A.1, test vectors 1 and 2: key and nonce all zero,
+    # block counters 0 and 1; StdRng's stream with 20 rounds and a zero key
+    # must give these 128 bytes in order, and next_u64 the words in pairs,
+    # low word first
+    tv1 = ("76b8e0ada0f13d90405d6ae55386bd28bdd219b8a08ded1aa836efcc8b770dc7"
+           "da41597c5157488d7724e03fb8d84a376a43b8f41518a11cc387b669b2ee6586")
+    tv2 = ("9f07e7be5551387a98ba977c732d080dcb0f29a048e3656912c6533e32ee7aed"
+           "29b721769ce64e43d57133b074d839d531ed1f28510afb45ace10a1f4b794d6f")
+    w, w64 = O.stream_words(np.zeros(8, np.uint32), 20, 32)
+    assert w.astype("<u4").tobytes() == bytes.fromhex(tv1 + tv2)
+    assert w64.tolist() == [int(w[2 * i]) | int(w[2 * i + 1]) << 32 for i in range(16)]
+
+
+def test_seed_expansion_by_the_pcg32_reference_outputs():
+    # the PCG reference generator (M. O'Neill, pcg32_srandom_r(42, 54)):
+    # increment 54 * 2 + 1, its state before the first output (0 -> step ->
+    # + 42) is 151, and its first outputs are these six words; seed_from_u64
+    # draws its eight key words with the same step and output
+    inc = 54 * 2 + 1
+    assert O.pcg32_words(151, inc, 6).tolist() == [0xa15c02b7, 0x7b47f409, 0xba1d3330,
+                                                   0x83d2f293, 0xbfa4784b, 0xcbed606e]
 
 
 def test_stream_is_deterministic_and_uniform():
