@@ -384,7 +384,7 @@ def bench_valuenet(args):
     at random).
 
     Per flop, the value-net path: the depth-limited solve (300 iterations,
-    leaves valued by the network), then the judge: the full tree with the
+    leaves valued by the network), then the exact check: the full tree with the
     flop pinned to the depth-limited average strategy and the later streets
     re-solved by exact DCFR until the exact FP64 best response of that whole
     profile is at or below the target. The exact path: the same tree solved
