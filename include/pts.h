@@ -344,10 +344,13 @@ pts_status pts_batch_exploitability(pts_batch* b, double* out);
 pts_status pts_batch_solve(pts_batch* b, uint32_t max_iterations, double target_pct, uint32_t check_every,
                            pts_progress_c* out);
 /* Autotuner: times `iterations` iterations for every tile choice (terminals
- * per terminal workgroup 1..16, opponent hands per down workgroup
- * 128/256/512/1024), keeps the fastest (best[0], best[1], best_ms per
- * iteration) and restarts the solve from zero storage. */
+ * per terminal workgroup 1, 2, 4, 8, 16, 32, 64 or 256; opponent hands per
+ * down workgroup 128/256/512/1024), keeps the fastest (best[0], best[1],
+ * best_ms per iteration) and restarts the solve from zero storage. */
 pts_status pts_batch_autotune(pts_batch* b, uint32_t iterations, uint32_t best[2], double* best_ms);
+/* Sets the tiles: run_length 1..256 terminals per terminal workgroup (values
+ * above 256 are clamped), down_threads 128/256/512/1024. The tiles change
+ * launch shapes only, not the arithmetic. */
 pts_status pts_batch_set_tiles(pts_batch* b, uint32_t run_length, uint32_t down_threads);
 /* Every spot's per-hand root values (FP64 traversal under `mode`: best
  * response or average) in combo order: out[(spot * 2 + player) * 1326 +
