@@ -2287,11 +2287,14 @@ Status Engine::create(int device, Plan&& plan, const float* regrets, const float
   // has 670,320), workgroups of 128 threads keep more segments in flight per
   // CU (flop down pass 67 -> 55 ms per iteration against 512); a few hundred
   // segments (a turn's river level) run best at 512 (the turn: 0.38 against
-  // 0.48 ms per iteration at 128)
+  // 0.48 ms per iteration at 128). 16-bit storage keeps 512: its down pass
+  // stages a group's new sums in LDS (up to 16 rows per workgroup), so small
+  // workgroups would leave a CU a few hundred threads (the flop's 16-bit down
+  // pass: 392 ms at 128 against 131 at 512)
   {
     uint32_t widest = 0;
     for (size_t dd = 0; dd + 1 < P.depth_off.size(); dd++) widest = std::max(widest, P.depth_off[dd + 1] - P.depth_off[dd]);
-    e->down_threads_ = widest >= 16384 ? 128u : 512u;
+    e->down_threads_ = widest >= 16384 && !q16 ? 128u : 512u;
   }
   d.active = AT(uint32_t, o_act);
   e->active_ = (uint32_t*)(base + o_act);
