@@ -1948,8 +1948,8 @@ static size_t put(std::vector<unsigned char>& blob, const std::vector<T>& v) {
 
 Store Engine::store() const { return Store{regrets_, sums_, rq_, sq_, rsc_, ssc_, rnx_}; }
 
-// Specialised walks (jit.hpp) for the frequent segment shapes, f32 storage
-// only: the f32 DCFR walk and the FP64 best response. A shape is every
+// Specialised walks (jit.hpp) for the frequent segment shapes: the DCFR walk
+// (f32, or the two passes of 16-bit storage) and the FP64 best response. A shape is every
 // member's kind (0 or 1 for a decision node, 2 for a member read from its
 // row), action count and first child. A plan with at least kJitPlanSegments
 // segments that have a walk compiles every shape of at least kJitShapeSegments
@@ -1964,7 +1964,7 @@ Status Engine::setup_jit(bool q16) {
   const Plan& P = plan_;
   H.dcfr = H.d;
   const char* jo = std::getenv("PTS_JIT");
-  if (q16 || (jo && jo[0] == '0')) return kOk;
+  if (jo && jo[0] == '0') return kOk;
   uint32_t min_segs = kJitShapeSegments;
   bool forced = false;
   if (const char* m = std::getenv("PTS_JIT_MIN_SEGMENTS")) {
@@ -2006,7 +2006,7 @@ Status Engine::setup_jit(bool q16) {
     for (uint32_t sg : J.lists.back()) taken[sg] = 1;
     jit_segments_ += J.lists.back().size();
   }
-  Status st = jit::compile_walks(shapes, J.fn, &jit_seconds_);
+  Status st = jit::compile_walks(shapes, q16, J.fn, &jit_seconds_);
   if (st != kOk) {
     // the generic walks serve every segment, with the same results
     std::fprintf(stderr, "[pts] specialised walks unavailable, the generic walks run instead: %s\n", last_error().c_str());
@@ -2645,20 +2645,26 @@ static void launch_span(const DPlan& d, const Plan& P, hipStream_t st, int p, Sp
                              rowlen, (const uint32_t*)alive, sto, prm, qout, d.big);
           PROF_STOP();
         }
-        // the shape-specialised walks (f32 DCFR and FP64 best-response
-        // traversals; d's generic lists then leave their segments out)
-        if constexpr (((std::is_same<V, float>::value && UMODE == kCfr) || UMODE == kBest) && QP == 0) {
+        // the shape-specialised walks (the f32 DCFR walk or, with 16-bit
+        // storage, its two passes, and the FP64 best response; d's generic
+        // lists then leave their segments out)
+        constexpr int kind = UMODE == kBest ? jit::kWalkBest
+                             : QP == 1    ? jit::kWalkMax16
+                             : QP == 2    ? jit::kWalkEnc16
+                                          : jit::kWalkF32;
+        if constexpr ((std::is_same<V, float>::value && UMODE == kCfr && QP != 3) || (UMODE == kBest && (QP == 0 || QP == 3))) {
           if (jit != nullptr)
             for (size_t k = 0; k < jit->lists.size(); k++) {
               const std::vector<uint32_t>& L = jit->lists[k];
               cut(L.data(), (uint32_t)L.size(), i0, i1);
               if (i1 <= i0) continue;
+              hipFunction_t fn = jit->fn[jit::walk_fn(k, kind, p)];
+              if (fn == nullptr) continue;  // (not compiled for this storage: the engine never asks for it)
               jit::WalkArgs A{jit->dev[k], d.seg_head, (const unsigned char*)d.mrec, d.malive, (float*)vals, sto.r,
-                              (const float*)prm, jit->zrow, i0, i1 - i0, tiles, rowlen, B.v0, B.vb};
+                              (const float*)prm, jit->zrow, sto.rq, sto.rsc, sto.rnx, i0, i1 - i0, tiles, rowlen, B.v0, B.vb};
               void* args[] = {&A};
               PROF_START(kKUp);
-              (void)hipModuleLaunchKernel(jit->fn[4 * k + (UMODE == kBest ? 2 : 0) + p], jit::walk_groups(i1 - i0, tiles), 1, 1,
-                                          UP_T, 1, 1, 0, st, args, nullptr);
+              (void)hipModuleLaunchKernel(fn, jit::walk_groups(i1 - i0, tiles), 1, 1, UP_T, 1, 1, 0, st, args, nullptr);
               PROF_STOP();
             }
         }
@@ -2726,13 +2732,13 @@ Status Engine::launch_traversal(int p, Mode mode, bool f64, ProfRec* prof) {
   auto run = [&](auto qc) {
     constexpr bool Q = decltype(qc)::value;
     if (mode == kCfr)
-      launch_span<float, kCfr, kCfr, Q>(Q ? d : dp_->dcfr, P, stream_, p, span, 0, groups_[0], reach, alive, sto, params_, vals,
-                                        rowlen_, lds_down_, lds_term_[0], lds_up_[0], (float*)nullptr, precision_ == kFast,
-                                        down_threads_, prof, hk, Q ? nullptr : &dp_->jit);
+      launch_span<float, kCfr, kCfr, Q>(dp_->dcfr, P, stream_, p, span, 0, groups_[0], reach, alive, sto, params_, vals, rowlen_,
+                                        lds_down_, lds_term_[0], lds_up_[0], (float*)nullptr, precision_ == kFast, down_threads_,
+                                        prof, hk, &dp_->jit);
     else if (mode == kBest)
-      launch_span<double, kAverage, kBest, Q>(Q ? d : dp_->dcfr, P, stream_, p, span, 0, groups_[1], reach, alive, sto, params_,
-                                              vals, rowlen_, lds_down_, lds_term_[1], lds_up_[1], (double*)nullptr,
-                                              precision_ == kFast, down_threads_, nullptr, hk, Q ? nullptr : &dp_->jit);
+      launch_span<double, kAverage, kBest, Q>(dp_->dcfr, P, stream_, p, span, 0, groups_[1], reach, alive, sto, params_, vals,
+                                              rowlen_, lds_down_, lds_term_[1], lds_up_[1], (double*)nullptr, precision_ == kFast,
+                                              down_threads_, nullptr, hk, &dp_->jit);
     else if (f64)
       launch_span<double, kAverage, kAverage, Q>(d, P, stream_, p, span, 0, groups_[1], reach, alive, sto, params_, vals, rowlen_,
                                                  lds_down_, lds_term_[1], lds_up_[1], (double*)nullptr, precision_ == kFast,

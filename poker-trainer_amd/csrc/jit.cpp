@@ -14,15 +14,17 @@ namespace pts {
 namespace jit {
 
 static_assert(offsetof(MemRec, offset) == 0 && offsetof(MemRec, vslot) == 12 && offsetof(MemRec, tflags) == 41 &&
-                  sizeof(MemRec) == 48,
+                  offsetof(MemRec, pslot) == 44 && sizeof(MemRec) == 48,
               "the generated walks read MemRec at these byte offsets");
 static_assert(kNodeFrozen == 8, "the generated walks test bit 3 of tflags");
 static_assert(offsetof(IterParams, pos) == 0 && offsetof(IterParams, neg) == 4, "the generated walks read pos, neg");
-static_assert(sizeof(WalkArgs) == 8 * sizeof(void*) + 24, "WalkArgs layout");
+static_assert(sizeof(WalkArgs) == 11 * sizeof(void*) + 24, "WalkArgs layout");
 
 // Declarations every generated kernel uses. The operations follow
 // up_reg_walk in engine.hip: pos0 and recip are its helpers
-// (crates/solver/src/cfr.rs:113-121), readlane its record reads.
+// (crates/solver/src/cfr.rs:113-121), readlane its record reads; enc16 is
+// its regret encoding (storage.rs:274-281: y = v * (MAX / max), rounded half
+// away from zero, Rust's saturating `as i16`) and wmax its wave maximum.
 static const char* kPreamble = R"(#ifndef __HIPCC_RTC__
 #include <hip/hip_runtime.h>
 #endif
@@ -38,6 +40,9 @@ struct WalkArgs {
   float* regrets;
   const float* prm;
   const float* zrow;
+  short* rq;
+  const float* rsc;
+  int* rnx;
   u32 i0, nseg, tiles, rowlen, v0, vb;
 };
 __device__ __forceinline__ float pos0(float x) { return x > 0.0f ? x : 0.0f; }
@@ -47,20 +52,40 @@ __device__ __forceinline__ float recip(float d) {
 }
 __device__ __forceinline__ u32 rl32(u32 x, u32 m) { return (u32)__builtin_amdgcn_readlane((int)x, (int)m); }
 __device__ __forceinline__ u64 rl64(u64 x, u32 m) { return (u64)rl32((u32)x, m) | (u64)rl32((u32)(x >> 32), m) << 32; }
+__device__ __forceinline__ short enc16(float v, float inv) {
+  const float y = v * inv;
+  const float z = y + __builtin_copysignf(0.5f, y);
+  if (!(z == z)) return 0;
+  return (short)(int)__builtin_fminf(__builtin_fmaxf(z, -32768.0f), 32767.0f);
+}
+__device__ __forceinline__ float dmax(float x, int src) { return __builtin_fmaxf(x, __builtin_bit_cast(float, src)); }
+__device__ __forceinline__ float wmax(float x) {
+  x = dmax(x, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, x), 0x111, 0xF, 0xF, false));
+  x = dmax(x, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, x), 0x112, 0xF, 0xF, false));
+  x = dmax(x, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, x), 0x114, 0xF, 0xF, false));
+  x = dmax(x, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, x), 0x118, 0xF, 0xF, false));
+  x = dmax(x, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, x), 0x142, 0xA, 0xF, false));
+  x = dmax(x, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, x), 0x143, 0xC, 0xF, false));
+  return __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, x), 63));
+}
 )";
 
-static std::string kernel_name(size_t i, int p, bool best) {
-  return std::string(best ? "pts_best_" : "pts_walk_") + std::to_string(i) + "_p" + std::to_string(p);
+static std::string kernel_name(size_t i, int p, int kind) {
+  static const char* pre[kWalkKinds] = {"pts_walk_", "pts_best_", "pts_wmax_", "pts_wenc_"};
+  return std::string(pre[kind]) + std::to_string(i) + "_p" + std::to_string(p);
 }
 
-// One kernel: the walk of shape S for traverser p, the f32 DCFR walk (regret
-// update) or, with `best`, the FP64 best response (no storage). Member m's
-// value is v<m>; own node m's regret row for action a is r<m>_<a>. Lane m
-// (and lane m - 64 of the second set) holds member m's record; LIVE, FROZEN,
-// VS and OF read member m's flags, value row and storage offset.
-static void emit_kernel(std::ostringstream& s, const Shape& S, size_t idx, int p, bool best) {
+// One kernel: the walk of shape S for traverser p, of the given kind: the
+// f32 DCFR walk (regret update), the FP64 best response (no storage), or one
+// of the two 16-bit DCFR passes (the largest new regret of each own node,
+// then the new regrets encoded with the new scale). Member m's value is v<m>;
+// own node m's regret row for action a is r<m>_<a>. Lane m (and lane m - 64
+// of the second set) holds member m's record; LIVE, FROZEN, VS, OF and PS
+// read member m's flags, value row, storage offset and scale slot.
+static void emit_kernel(std::ostringstream& s, const Shape& S, size_t idx, int p, int kind) {
   const uint32_t M = (uint32_t)S.size();
   const bool two = M > 64;
+  const bool best = kind == kWalkBest, q16 = kind == kWalkMax16 || kind == kWalkEnc16;
   const char* T = best ? "double" : "float";
   const char* zero = best ? "0.0" : "0.0f";
   auto live = [&](uint32_t m) {
@@ -75,8 +100,11 @@ static void emit_kernel(std::ostringstream& s, const Shape& S, size_t idx, int p
   auto of = [&](uint32_t m) {
     return m < 64 ? "rl64(of0, " + std::to_string(m) + "u)" : "rl64(of1, " + std::to_string(m - 64) + "u)";
   };
+  auto ps = [&](uint32_t m) {
+    return m < 64 ? "rl32(ps0, " + std::to_string(m) + "u)" : "rl32(ps1, " + std::to_string(m - 64) + "u)";
+  };
   auto v = [&](uint32_t m) { return "v" + std::to_string(m); };
-  s << "extern \"C\" __global__ void __launch_bounds__(64, 4) " << kernel_name(idx, p, best) << "(WalkArgs A) {\n";
+  s << "extern \"C\" __global__ void __launch_bounds__(64, 4) " << kernel_name(idx, p, kind) << "(WalkArgs A) {\n";
   // workgroup L runs on XCD L % 8; its slot L / 8 there names (segment,
   // hand tile) with a segment's tiles consecutive, so one XCD walks all of a
   // segment's tiles and its member records come from that XCD's L2
@@ -97,13 +125,14 @@ static void emit_kernel(std::ostringstream& s, const Shape& S, size_t idx, int p
   if (!best) s << "  const float pos = A.prm[0], neg = A.prm[1];\n";
   for (int q = 0; q < (two ? 2 : 1); q++) {
     const uint32_t cnt = q == 0 ? (M < 64 ? M : 64) : M - 64;
-    s << "  u32 vs" << q << " = 0u, fz" << q << " = 0u, lv" << q << " = 0u;\n";
+    s << "  u32 vs" << q << " = 0u, fz" << q << " = 0u, lv" << q << " = 0u, ps" << q << " = 0u;\n";
     s << "  u64 of" << q << " = 0ull;\n";
     s << "  if (lane < " << cnt << "u) {\n";
     s << "    const u32 mm = moff + " << 64 * q << "u + lane;\n";
     s << "    const unsigned char* r = A.mrec + (u64)mm * 48u;\n";
     s << "    of" << q << " = *(const u64*)r;\n";
     s << "    vs" << q << " = *(const u32*)(r + 12);\n";
+    if (q16) s << "    ps" << q << " = *(const u32*)(r + 44);\n";
     s << "    fz" << q << " = r[41];\n";
     s << "    lv" << q << " = A.malive[mm];\n";
     s << "  }\n";
@@ -113,17 +142,22 @@ static void emit_kernel(std::ostringstream& s, const Shape& S, size_t idx, int p
   }
   // every row the walk reads, all loads issued before the walk and none
   // behind a branch: the values of chance / terminal / leaf members and, in
-  // the DCFR walk, the regret rows of own nodes (up_reg_walk's preload); a
-  // dead member reads the zero row instead, so it holds 0 as in the generic
-  // walk
+  // the DCFR walks, the regret rows of own nodes (up_reg_walk's preload,
+  // 16-bit regrets decoded as i16 * (scale / 32767)); a dead member reads
+  // the zero row instead, so it holds 0 as in the generic walk
   for (uint32_t m = 0; m < M; m++) {
     const Member& x = S[m];
     if (x.kind >= 2) {
       s << "  const " << T << " " << v(m) << "l = (" << live(m) << " ? vals + (u64)" << vs(m) << " * A.rowlen : zrow)[hh];\n";
-    } else if ((int)x.kind == p && !best) {
+    } else if ((int)x.kind == p && !best && !q16) {
       s << "  const float* q" << m << " = " << live(m) << " ? A.regrets + " << of(m) << " : A.zrow;\n";
       for (uint32_t a = 0; a < x.na; a++)
         s << "  const float r" << m << "_" << a << " = q" << m << "[(u64)" << a << "u * np + hh];\n";
+    } else if ((int)x.kind == p && q16) {
+      s << "  const short* q" << m << " = " << live(m) << " ? A.rq + " << of(m) << " : (const short*)A.zrow;\n";
+      s << "  const float st" << m << " = " << live(m) << " ? A.rsc[" << ps(m) << "] / 32767.0f : 0.0f;\n";
+      for (uint32_t a = 0; a < x.na; a++)
+        s << "  const float r" << m << "_" << a << " = (float)q" << m << "[(u64)" << a << "u * np + hh] * st" << m << ";\n";
     }
   }
   for (uint32_t m = 0; m < M; m++)
@@ -165,24 +199,49 @@ static void emit_kernel(std::ostringstream& s, const Shape& S, size_t idx, int p
       s << "    " << v(m) << " = " << live(m) << " ? o : 0.0f;\n";
       // the updated regrets are stored after the walk: a store followed by
       // a read of a loaded row would make that read wait for the store
-      tail << "  if (" << live(m) << " && !" << frozen(m) << " && act) {\n";
-      tail << "    float* w = A.regrets + " << of(m) << ";\n";
-      for (uint32_t a = 0; a < x.na; a++) tail << "    w[(u64)" << a << "u * np + h] = u" << m << "_" << a << ";\n";
-      tail << "  }\n";
+      if (kind == kWalkF32) {
+        tail << "  if (" << live(m) << " && !" << frozen(m) << " && act) {\n";
+        tail << "    float* w = A.regrets + " << of(m) << ";\n";
+        for (uint32_t a = 0; a < x.na; a++) tail << "    w[(u64)" << a << "u * np + h] = u" << m << "_" << a << ";\n";
+        tail << "  }\n";
+      } else if (kind == kWalkMax16) {  // the node's largest |new regret| over its hands, as float bits
+        tail << "  if (" << live(m) << " && !" << frozen(m) << ") {\n";
+        tail << "    float lm = 0.0f;\n";
+        for (uint32_t a = 0; a < x.na; a++) tail << "    if (act) lm = __builtin_fmaxf(lm, __builtin_fabsf(u" << m << "_" << a << "));\n";
+        tail << "    lm = wmax(lm);\n";
+        tail << "    if (lane == 0u) atomicMax(A.rnx + " << ps(m) << ", __builtin_bit_cast(int, lm));\n";
+        tail << "  }\n";
+      } else {  // kWalkEnc16: the new regrets encoded with the node's new scale
+        tail << "  if (" << live(m) << " && !" << frozen(m) << ") {\n";
+        tail << "    const float mx = __builtin_bit_cast(float, A.rnx[" << ps(m) << "]);\n";
+        tail << "    const float inv = mx > 0.0f ? 32767.0f / mx : 0.0f;\n";
+        tail << "    if (act) {\n";
+        tail << "      short* w = A.rq + " << of(m) << ";\n";
+        for (uint32_t a = 0; a < x.na; a++)
+          tail << "      w[(u64)" << a << "u * np + h] = mx > 0.0f ? enc16(u" << m << "_" << a << ", inv) : (short)0;\n";
+        tail << "    }\n";
+        tail << "  }\n";
+      }
     }
     s << "  }\n";
   }
   s << tail.str();
-  s << "  if (act) vals[(u64)" << vs(0) << " * A.rowlen + h] = v0;\n";
+  // the root's value row (the 16-bit encoding pass leaves it to the first pass)
+  if (kind != kWalkEnc16) s << "  if (act) vals[(u64)" << vs(0) << " * A.rowlen + h] = v0;\n";
   s << "}\n";
 }
 
-std::string walk_source(const std::vector<Shape>& shapes) {
+// the kinds an engine launches: the f32 walk and the best response, or with
+// 16-bit storage the best response and the two 16-bit passes
+static bool kind_used(int kind, bool q16) { return q16 ? kind != kWalkF32 : kind == kWalkF32 || kind == kWalkBest; }
+
+std::string walk_source(const std::vector<Shape>& shapes, bool q16) {
   std::ostringstream s;
   s << kPreamble;
   for (size_t i = 0; i < shapes.size(); i++)
-    for (int best = 0; best < 2; best++)
-      for (int p = 0; p < 2; p++) emit_kernel(s, shapes[i], i, p, best != 0);
+    for (int kind = 0; kind < kWalkKinds; kind++)
+      if (kind_used(kind, q16))
+        for (int p = 0; p < 2; p++) emit_kernel(s, shapes[i], i, p, kind);
   return s.str();
 }
 
@@ -201,7 +260,7 @@ std::map<std::string, hipModule_t> g_modules;  // (device, source) -> module, ke
     }                                                                                           \
   } while (0)
 
-Status compile_walks(const std::vector<Shape>& shapes, std::vector<hipFunction_t>& fn, double* seconds) {
+Status compile_walks(const std::vector<Shape>& shapes, bool q16, std::vector<hipFunction_t>& fn, double* seconds) {
   if (seconds) *seconds = 0.0;
   fn.clear();
   if (shapes.empty()) return kOk;
@@ -223,7 +282,7 @@ Status compile_walks(const std::vector<Shape>& shapes, std::vector<hipFunction_t
   std::string arch = prop.gcnArchName;
   const size_t colon = arch.find(':');
   if (colon != std::string::npos) arch = arch.substr(0, colon);
-  const std::string src = walk_source(shapes);
+  const std::string src = walk_source(shapes, q16);
   const std::string key = std::to_string(dev) + "\n" + arch + "\n" + src;
   std::lock_guard<std::mutex> lk(g_mu);
   hipModule_t mod = nullptr;
@@ -261,11 +320,12 @@ Status compile_walks(const std::vector<Shape>& shapes, std::vector<hipFunction_t
     g_modules[key] = mod;
     if (seconds) *seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
   }
-  fn.resize(shapes.size() * 4);
+  fn.assign(shapes.size() * kWalkKinds * 2, nullptr);
   for (size_t i = 0; i < shapes.size(); i++)
-    for (int best = 0; best < 2; best++)
-      for (int p = 0; p < 2; p++)
-        JITCHK(hipModuleGetFunction(&fn[4 * i + 2 * best + p], mod, kernel_name(i, p, best != 0).c_str()));
+    for (int kind = 0; kind < kWalkKinds; kind++)
+      if (kind_used(kind, q16))
+        for (int p = 0; p < 2; p++)
+          JITCHK(hipModuleGetFunction(&fn[walk_fn(i, kind, p)], mod, kernel_name(i, p, kind).c_str()));
   return kOk;
 }
 

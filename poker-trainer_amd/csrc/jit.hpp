@@ -45,9 +45,19 @@ struct WalkArgs {
   float* regrets;              // f32 regret storage (DCFR walk)
   const float* prm;            // IterParams: pos, neg (DCFR walk)
   const float* zrow;           // 8 * rowlen f32 zeros: what a dead member reads
+  int16_t* rq;                 // 16-bit regret storage (16-bit walks)
+  const float* rsc;            // per player node: the regrets' scale (16-bit walks)
+  int32_t* rnx;                // per player node: the new scale's float bits (16-bit walks)
   uint32_t i0, nseg, tiles;    // the segments walked; 64-hand tiles per segment
   uint32_t rowlen, v0, vb;     // row length, value-row bias (Bias::v)
 };
+
+// The kernels of one shape for one traverser: the f32 DCFR walk, the FP64
+// best response, and with 16-bit storage the DCFR walk's two passes (the
+// largest new regret of every own node, then the new regrets encoded with it).
+enum WalkKind { kWalkF32 = 0, kWalkBest = 1, kWalkMax16 = 2, kWalkEnc16 = 3 };
+constexpr int kWalkKinds = 4;
+inline size_t walk_fn(size_t shape, int kind, int p) { return (shape * kWalkKinds + (size_t)kind) * 2 + (size_t)p; }
 
 // The most members a specialised walk takes.
 constexpr uint32_t kMaxMembers = 80;
@@ -58,13 +68,15 @@ constexpr uint32_t kMaxMembers = 80;
 inline uint32_t walk_groups(uint32_t n, uint32_t t) { return (n + 7u) / 8u * 8u * t; }
 
 // Compiles the walks of `shapes` for both traversers on the current device
-// (or takes them from this process's cache): fn[4 * i + p] is shape i's f32
-// DCFR walk for traverser p, fn[4 * i + 2 + p] its FP64 best response.
-// `seconds` gets the compile time (0 when cached).
-Status compile_walks(const std::vector<Shape>& shapes, std::vector<hipFunction_t>& fn, double* seconds);
+// (or takes them from this process's cache): fn[walk_fn(i, kind, p)] is shape
+// i's kernel of that kind for traverser p. With `q16` the kinds are the best
+// response and the two 16-bit passes, else the f32 walk and the best
+// response; the others are null. `seconds` gets the compile time (0 when
+// cached).
+Status compile_walks(const std::vector<Shape>& shapes, bool q16, std::vector<hipFunction_t>& fn, double* seconds);
 
 // The generated HIP source for `shapes` (for inspection and tests).
-std::string walk_source(const std::vector<Shape>& shapes);
+std::string walk_source(const std::vector<Shape>& shapes, bool q16);
 
 }  // namespace jit
 }  // namespace pts

@@ -86,3 +86,54 @@ def test_small_plans_keep_the_generic_walks(monkeypatch):
     assert Solver(spots.turn_spot(0), precision=FAST).jit_segments() == 0
     monkeypatch.setenv("PTS_JIT_MIN_SEGMENTS", "64")
     assert Solver(spots.turn_spot(0), precision=FAST).jit_segments() > 0
+
+
+def compressed(cfg):
+    # 16-bit storage (storage_mode 1: i16 regrets, u16 sums, per-node scales)
+    cfg.storage_mode = 1
+    return cfg
+
+
+# 16-bit storage runs each DCFR up walk twice (the largest new regret of every
+# own node, then the new regrets encoded with it); the specialised kernels of
+# both passes and of the best response from 16-bit storage, against the
+# oracle's 16-bit solve bit for bit, from zero storage and from random 16-bit
+# storage (every path of the codec: integers over their whole range, some
+# scales 0, frozen nodes none)
+@pytest.mark.parametrize("name,make,iters", [("turn0", lambda: spots.turn_spot(0), 6),
+                                             ("small-flop", lambda: spots.small_flop_spot(0), 3)],
+                         ids=["turn0", "small-flop"])
+def test_specialised_16bit_walks_bit_exact(monkeypatch, name, make, iters):
+    specialise(monkeypatch, True)
+    cfg = compressed(make())
+    g, c = Solver(cfg, precision=EXACT), O.OracleSolver(cfg)
+    assert g.compressed and c.compressed and g.jit_segments() > 0
+    g.iterate(iters)
+    c.iterate(iters)
+    for x, y, what in zip(g.storage_q(), c.storage_q(), ("regrets", "sums", "regret scales", "sum scales")):
+        assert same(x, y), f"{what}: {describe(x, y)}"
+    assert g.exploitability() == c.exploitability_f64()
+
+
+def test_specialised_16bit_walks_from_random_storage(monkeypatch):
+    specialise(monkeypatch, True)
+    cfg = compressed(spots.turn_spot(1))
+    g, c = Solver(cfg, precision=EXACT), O.OracleSolver(cfg)
+    assert g.jit_segments() > 0
+    n, k = c.storage_len, c.stats().player_nodes
+    rng = np.random.default_rng(11)
+    r = rng.integers(-32768, 32768, n).astype(np.int16)
+    s = rng.integers(0, 65536, n).astype(np.uint16)
+    rs = (rng.random(k) * 50).astype(np.float32)
+    ss = (rng.random(k) * 20).astype(np.float32)
+    rs[rng.random(k) < 0.1] = 0.0
+    ss[rng.random(k) < 0.1] = 0.0
+    g.set_storage_q(r, s, rs, ss)
+    c.set_storage_q(r, s, rs, ss)
+    assert g.exploitability() == c.exploitability_f64()
+    g.iterations = 5
+    c.iterations = 5
+    g.iterate(2)
+    c.iterate(2)
+    for x, y, what in zip(g.storage_q(), c.storage_q(), ("regrets", "sums", "regret scales", "sum scales")):
+        assert same(x, y), f"{what}: {describe(x, y)}"
