@@ -53,7 +53,13 @@ constexpr int TERM_T = 64;    // terminal workgroup: one wave
 constexpr int UP_T = 64;      // up workgroup: one wave, one hand per lane
 constexpr int DOWN_T = 512;   // down workgroup
 constexpr int MAX_K = 21;     // ceil(1326 / 64): hands per lane in the terminal kernel
-constexpr int DG_A = 16;      // actions per down-pass group (the 16-bit pass stages DG_A rows in LDS)
+constexpr int DG_A = 16;      // actions per down-pass group
+// actions per down-pass group with 16-bit storage, whose down pass stages a
+// group's new sums in LDS (one row per action): 8 rows let three 512-thread
+// workgroups share a CU where 16 let two (the flop's 16-bit down pass 131.6 ->
+// 101.2 ms); a node has at most MAXA = 8 actions, so one node always fits
+constexpr int DG_A16 = 8;
+static_assert(DG_A16 >= MAXA && DG_A16 <= DG_A, "one node's actions fit a 16-bit group's LDS rows");
 constexpr int DG_N = 8;       // nodes per down-pass group
 
 // One spot's hands and strength tables on the device (a batch holds many
@@ -2171,7 +2177,7 @@ Status Engine::create(int device, Plan&& plan, const float* regrets, const float
   // down-pass groups: per segment and traverser, the opponent's decision
   // nodes by level (the number of the opponent's decision nodes above them
   // in the segment: a level's nodes read only rows of earlier levels), cut
-  // into groups of at most DG_N nodes and DG_A actions
+  // into groups of at most DG_N nodes and DG_A actions (DG_A16 with 16-bit storage)
   std::vector<uint32_t> dg_off[2], dg_ent[2];
   for (int q = 0; q < 2; q++) {
     const int opp = q ^ 1;
@@ -2194,7 +2200,7 @@ Status Engine::create(int device, Plan&& plan, const float* regrets, const float
       uint32_t grp = 0, acts = 0, nodes = 0, cur = ~0u;
       for (const auto& x : ent) {
         const uint32_t na = rr[x.second].na;
-        if (nodes > 0 && (x.first != cur || acts + na > (uint32_t)DG_A || nodes == (uint32_t)DG_N)) {
+        if (nodes > 0 && (x.first != cur || acts + na > (uint32_t)(q16 ? DG_A16 : DG_A) || nodes == (uint32_t)DG_N)) {
           grp++;
           acts = nodes = 0;
         }
@@ -2380,7 +2386,7 @@ Status Engine::create(int device, Plan&& plan, const float* regrets, const float
 
   // LDS sizes
   const size_t mm = std::max<size_t>(P.max_members, 1);
-  e->lds_down_ = mm * sizeof(MemRec) + ((mm + 15) & ~size_t(15)) + 4 * mm + (q16 ? (size_t)DG_A * rowlen * 4 : 0);
+  e->lds_down_ = mm * sizeof(MemRec) + ((mm + 15) & ~size_t(15)) + 4 * mm + (q16 ? (size_t)DG_A16 * rowlen * 4 : 0);
   for (int f64 = 0; f64 < 2; f64++) {
     size_t vs = f64 ? 8 : 4;
     // the fast terminal kernel's LDS is static (TermLds)
