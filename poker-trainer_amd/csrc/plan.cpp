@@ -553,8 +553,10 @@ std::string build_plan(const PlanInput& in, Plan& P) {
 
 uint32_t default_run_length(const Plan& P) {
   // long runs amortize the tables; short ones keep a small tree's
-  // workgroups spread over the chip (about 4096 runs at least)
-  return std::min<uint32_t>(kBatchMax, std::max<uint32_t>(1, (uint32_t)(P.trec.size() / 4096)));
+  // workgroups spread over the chip (about 2048 runs at least: the turn's
+  // 10,526 terminals in runs of 5, terminal pass 0.108 -> 0.098 ms per
+  // iteration against runs of 2; runs of 10 gain nothing)
+  return std::min<uint32_t>(kBatchMax, std::max<uint32_t>(1, (uint32_t)(P.trec.size() / 2048)));
 }
 
 void build_runs(Plan& P, uint32_t bmax) {
