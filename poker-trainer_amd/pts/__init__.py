@@ -7,9 +7,10 @@ solve runs in hand-written HIP kernels for gfx950.
 """
 from .config import (BetSize, BettingState, DcfrParams, PostflopMenu, SolveConfig, StreetSizes, make_config,
                      parse_card, parse_cards, card_str, combo_index, combo_cards, FLOP, TURN, RIVER,
-                     BACKEND_CPU, BACKEND_GPU, BACKEND_AUTO)
+                     BACKEND_CPU, BACKEND_GPU, BACKEND_AUTO, STORAGE_F32, STORAGE_COMPRESSED, STORAGE_AUTO)
 from .ranges import parse_range, HandClass
 
 __all__ = ["BetSize", "BettingState", "DcfrParams", "PostflopMenu", "SolveConfig", "StreetSizes", "make_config",
            "parse_card", "parse_cards", "card_str", "combo_index", "combo_cards", "parse_range", "HandClass",
-           "FLOP", "TURN", "RIVER", "BACKEND_CPU", "BACKEND_GPU", "BACKEND_AUTO"]
+           "FLOP", "TURN", "RIVER", "BACKEND_CPU", "BACKEND_GPU", "BACKEND_AUTO", "STORAGE_F32",
+           "STORAGE_COMPRESSED", "STORAGE_AUTO"]

@@ -151,10 +151,19 @@ class DcfrParams:
 
 BACKEND_CPU, BACKEND_GPU, BACKEND_AUTO = 0, 1, 2
 
+# storage.rs:80-85: f32, 16-bit ("compressed"), automatic (16-bit when the f32
+# tree would exceed 1 GB).
+STORAGE_F32, STORAGE_COMPRESSED, STORAGE_AUTO = 0, 1, 2
+
 
 @dataclass
 class SolveConfig:
-    """lib.rs:73-135."""
+    """lib.rs:73-135.
+
+    The field defaults ask for f32 storage and no isomorphism, the plainest
+    tree. `SolveConfig.new` gives the defaults of the reference's
+    `SolveConfig::new` (lib.rs:107-134): automatic storage and isomorphism on.
+    """
     board: List[int]
     ranges: List[List[float]]
     root: BettingState
@@ -171,6 +180,14 @@ class SolveConfig:
     backend: int = BACKEND_GPU
     # tree.rs:128-136: None (Exact) or (turn cards, river cards, seed) sampled per branch
     sampled: Optional[tuple] = None
+
+    @classmethod
+    def new(cls, board: List[int], ranges: List[List[float]], root: BettingState,
+            menu: PostflopMenu) -> "SolveConfig":
+        """lib.rs:107-134: exact runouts, no rake, automatic storage,
+        isomorphism on, an 8 GB limit; the solve runs on the GPU."""
+        return cls(board=list(board), ranges=[list(r) for r in ranges], root=root, menu=menu,
+                   storage_mode=STORAGE_AUTO, isomorphism=True)
 
     def to_c(self) -> "ConfigC":
         c = ConfigC()
@@ -259,7 +276,10 @@ def _menu_to_c(m: PostflopMenu, out: MenuC):
 
 def make_config(board: str, oop, ip, pot: int, stack: int, menu: PostflopMenu, **kw) -> SolveConfig:
     """The reference tests' `config` helper (crates/solver/src/tests.rs:19-28):
-    ranges are notation strings or 1326-weight lists."""
+    ranges are notation strings or 1326-weight lists. It keeps the field
+    defaults (f32 storage, isomorphism off) unless `kw` asks otherwise; the
+    reference helper builds through `SolveConfig::new`, whose isomorphism is
+    on, so a test that follows one of its cases passes `isomorphism=True`."""
     from .ranges import parse_range
     cards = parse_cards(board)
     street = STREET_OF_BOARD[len(cards)]

@@ -124,6 +124,25 @@ def test_estimate_matches_the_oracle():
             (b.player_nodes, b.chance_nodes, b.terminal_nodes, b.storage_cells, b.memory_bytes)
 
 
+def test_solve_config_new_takes_the_reference_defaults():
+    """lib.rs:107-134: automatic storage, isomorphism on, 8 GB, no rake.
+    Automatic storage keeps f32 (8 bytes a cell) under 1 GB of f32 tree and
+    takes 16-bit storage (4 bytes a cell) above it (lib.rs:346-352,
+    tree.rs:264-272); every node costs 56 bytes either way."""
+    from pts import SolveConfig, STORAGE_AUTO
+    from pts.solver import Solver
+    for make, cell_bytes in ((lambda: spots.river_spot(2), 8), (lambda: spots.flop_spot(0), 4)):
+        c = make()
+        n = SolveConfig.new(c.board, c.ranges, c.root, c.menu)
+        assert n.storage_mode == STORAGE_AUTO and n.isomorphism
+        assert (n.max_memory_bytes, n.big_blind, n.prune, n.rake_rate, n.rake_cap) == (8 << 30, 100, 0.0, 0.0, 0.0)
+        assert n.future_menu is None and n.sampled is None
+        st = Solver.estimate(n)
+        nodes = st.player_nodes + st.chance_nodes + st.terminal_nodes
+        assert st.memory_bytes == st.storage_cells * cell_bytes + nodes * 56
+        assert (st.storage_cells * 8 + nodes * 56 > (1 << 30)) == (cell_bytes == 4)
+
+
 def test_rejections():
     cfg = spots.river_spot(0)
     cfg.max_memory_bytes = 1 << 10
