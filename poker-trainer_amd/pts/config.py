@@ -162,7 +162,8 @@ class SolveConfig:
 
     The field defaults ask for f32 storage and no isomorphism, the plainest
     tree. `SolveConfig.new` gives the defaults of the reference's
-    `SolveConfig::new` (lib.rs:107-134): automatic storage and isomorphism on.
+    `SolveConfig::new` (lib.rs:107-134): automatic storage, isomorphism on and
+    the reference's default DCFR parameters.
     """
     board: List[int]
     ranges: List[List[float]]
@@ -185,9 +186,11 @@ class SolveConfig:
     def new(cls, board: List[int], ranges: List[List[float]], root: BettingState,
             menu: PostflopMenu) -> "SolveConfig":
         """lib.rs:107-134: exact runouts, no rake, automatic storage,
-        isomorphism on, an 8 GB limit; the solve runs on the GPU."""
+        isomorphism on, an 8 GB limit, the reference's default DCFR
+        parameters (cfr.rs:65-81: gamma 3, unweighted sums, restarts); the
+        solve runs on the GPU."""
         return cls(board=list(board), ranges=[list(r) for r in ranges], root=root, menu=menu,
-                   storage_mode=STORAGE_AUTO, isomorphism=True)
+                   storage_mode=STORAGE_AUTO, isomorphism=True, dcfr=DcfrParams.reference_default())
 
     def to_c(self) -> "ConfigC":
         c = ConfigC()

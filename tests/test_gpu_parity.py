@@ -381,6 +381,27 @@ def test_all_in_equity_on_the_gpu():
     assert abs(ev[1] - (2.0 / 44.0 * 3000.0 - 1000.0)) < 0.5 and abs(ev[0]) < 1e-3
 
 
+def test_sampled_flop_solve_on_the_gpu():
+    # crates/solver/src/tests.rs:150-172: 8 turn and 6 river cards per branch
+    # (seed 3) under the default menu, one 66% bet and 3x raise after the flop;
+    # the solve converges and the root's range strategy is a distribution
+    cfg = make_config("Kh7d2c", "AK, KQ, 77, 22, AJs, QJs, T9s, 65s, A5s", "AK, KJ, QQ, JJ, 88, A7s, 98s, JTs, 76s",
+                      600, 9700, PostflopMenu(), future_menu=PostflopMenu.one(["66%"], ["3x"]), isomorphism=True,
+                      dcfr=REF)
+    cfg.sampled = (8, 6, 3)
+    assert Solver.estimate(cfg).player_nodes > 0
+    s, c = Solver(cfg, precision=EXACT), O.OracleSolver(cfg)
+    hg = []
+    p = s.solve(SolveOptions(300, 0.5, 25), hg)
+    assert p.exploitability_pct < 1.0, p.exploitability_pct
+    root = s.query([])
+    assert abs(float(np.sum(root.range_strategy())) - 1.0) < 1e-4
+    # the exact solve is the oracle's, check for check
+    _, hc = c.solve(300, 0.5, 25)
+    assert [h.iteration for h in hg] == [h.iteration for h in hc]
+    assert same(root.strategy, c.query([]).strategy)
+
+
 def test_ties_and_rake_on_the_gpu():
     # crates/solver/src/tests.rs:106-127,305-328
     from pts import BettingState

@@ -137,6 +137,8 @@ def test_solve_config_new_takes_the_reference_defaults():
         assert n.storage_mode == STORAGE_AUTO and n.isomorphism
         assert (n.max_memory_bytes, n.big_blind, n.prune, n.rake_rate, n.rake_cap) == (8 << 30, 100, 0.0, 0.0, 0.0)
         assert n.future_menu is None and n.sampled is None
+        d = n.dcfr  # cfr.rs:65-81
+        assert (d.alpha, d.beta, d.gamma, d.restart_average, d.unweighted_average) == (1.5, 0.0, 3.0, True, True)
         st = Solver.estimate(n)
         nodes = st.player_nodes + st.chance_nodes + st.terminal_nodes
         assert st.memory_bytes == st.storage_cells * cell_bytes + nodes * 56
