@@ -294,6 +294,31 @@ def test_solve_loop_matches_the_oracle():
         assert abs(a.exploitability - b.exploitability) < 1e-4 * cfg.root.pot
 
 
+def test_solves_continue_across_calls_and_from_the_oracle():
+    # gpu/engine/tests.rs:700-711: solving to 3 and then to 6 iterations leaves
+    # the storage of one solve to 6; gpu/engine/tests.rs:274-286: a GPU solve
+    # continues a CPU solve (here the oracle's, bit for bit)
+    cfg = spots.turn_spot(1)
+    once, twice = Solver(cfg, precision=EXACT), Solver(cfg, precision=EXACT)
+    once.solve(SolveOptions(6, 0.0, 25))
+    twice.solve(SolveOptions(3, 0.0, 25))
+    p = twice.solve(SolveOptions(6, 0.0, 25))
+    assert p.iteration == 6 and twice.iterations == 6
+    (r1, s1), (r2, s2) = once.storage(), twice.storage()
+    assert same(r1, r2) and same(s1, s2), describe(r1, r2)
+    c = O.OracleSolver(cfg)
+    c.iterate(3)
+    g = Solver(cfg, precision=EXACT)
+    g.set_storage(*c.storage())
+    g.iterations = 3
+    g.solve(SolveOptions(6, 0.0, 25))
+    c.iterate(3)
+    (rg, sg), (rc, sc) = g.storage(), c.storage()
+    assert g.iterations == 6
+    assert same(rg, rc) and same(sg, sc), describe(rg, rc)
+    assert same(rg, r1) and same(sg, s1)
+
+
 def test_queries_match_the_oracle():
     cfg = spots.turn_spot(2)
     g, c = Solver(cfg, precision=EXACT), O.OracleSolver(cfg)
