@@ -294,6 +294,43 @@ def test_solve_loop_matches_the_oracle():
         assert abs(a.exploitability - b.exploitability) < 1e-4 * cfg.root.pot
 
 
+@pytest.mark.parametrize("name,make", SPOTS[::2], ids=[s[0] for s in SPOTS[::2]])
+def test_traversals_from_extreme_storage(name, make):
+    # gpu/tests.rs:101-120,143-150: regret matching and averaging where a
+    # hand's positive regrets (or sums) are all zero, sum to a subnormal whose
+    # reciprocal overflows, to a subnormal that still inverts, or overflow to
+    # infinity; the guard (cfr.rs:113-121) makes each of these uniform or exact.
+    # An own node's value is the sum of r+ * cfv times the reciprocal
+    # (cfr.rs:315-382), so an f32::MAX regret overflows it to infinity and
+    # the regret update then writes infinities and NaNs: both sides must agree
+    # on those too.
+    cfg = make()
+    o, t = oracle_tree(cfg)
+    n = o.storage_len
+    rng = np.random.default_rng(5)
+    sub, sub2, big = (np.array([1, 0x00400000, 0x7F7FFFFF], np.uint32).view(np.float32))
+    rpool = np.array([-1.0, -0.0, 0.0, sub, sub2, 1.0, big, 1e30, 1e-40, -1e-30, -3.0, 2.0], np.float32)
+    spool = np.array([0.0, sub, sub2, big, 1.0, 1e30, 3.0], np.float32)
+    r, s = rpool[rng.integers(0, len(rpool), n)], spool[rng.integers(0, len(spool), n)]
+    o.set_storage(r, s)
+    e = Engine(t, r, s, precision=EXACT)
+
+    def eq(a, b):
+        a, b = np.asarray(a), np.asarray(b)
+        return a.shape == b.shape and np.array_equal(a, b, equal_nan=True)
+    for p in (0, 1):
+        for mode in (1, 2):  # FP64 best response and average
+            g, c = e.traverse(p, mode), o.traverse_root_f64(p, mode)
+            assert eq(g, c), f"p{p} mode{mode}: {describe(g, c)}"
+    d = O.discount(3)
+    for p in (0, 1):
+        g = e.traverse(p, 0, (d[0], d[1], d[2], 1))
+        c = o.traverse_root(p, 0, 3)
+        assert eq(g.astype(np.float32), c), f"cfr p{p}: {describe(g, c)}"
+        (rg, sg), (rc, sc) = e.download(), o.storage()
+        assert eq(rg, rc) and eq(sg, sc), f"storage after p{p}: {describe(rg, rc)}"
+
+
 def test_solves_continue_across_calls_and_from_the_oracle():
     # gpu/engine/tests.rs:700-711: solving to 3 and then to 6 iterations leaves
     # the storage of one solve to 6; gpu/engine/tests.rs:274-286: a GPU solve
